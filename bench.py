@@ -1,0 +1,177 @@
+"""Benchmark of the ray-scene hot path on one MI355X.
+
+Workload (the headline configuration): a random soup of 1,000,000 FP32
+triangles in the unit cube (990,000 Lambertian with vertex-colour albedo
+U(0.2, 0.8), 10,000 emissive area lights with radiance 5), 1920x1080,
+PathTracer at 16 samples per pixel, maximum depth 6 with Russian roulette
+from depth 3, per-pixel Halton pixel sampler, BVH accelerator, FP32.
+
+One step = one Renderer::renderFrame of the full 16-spp frame.  The scene,
+the BVH and the output buffers are resident in HBM before the timed region.
+Mrays/s counts every traced camera, extension, reflection/transmission and
+shadow ray (rays the reference traces beyond the maximum depth, whose result
+it discards, are not traced and not counted).
+
+Prints one JSON line (see the repository's benchmark contract).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+W, H, SPP = 1920, 1080, 16
+N_TRIS, N_LIGHTS = 1_000_000, 10_000
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_desc(mrt, abi, cam, spp=SPP, instrument=False):
+    return mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam,
+                           pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 5),
+                           rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 6),
+                           light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 7),
+                           hemi=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 8),
+                           depth_min=2, depth_max=6, instrument=instrument)
+
+
+def soup_camera(mrt, abi):
+    ratio = np.float32(W) / np.float32(H)
+    return mrt.make_camera(abi.CAMERA_PERSPECTIVE, (0.5, 0.5, -1.0), (0.5, 0.5, 0.5), (0, 1, 0),
+                           float(np.float32(45.0) * ratio), 45.0)
+
+
+def cpu_baseline(data, cam, abi, mrt) -> dict:
+    """The oracle (reference-faithful CPU restatement) on a bounded sample of
+    the same workload: the 16 central tiles of the 16x16 tile grid, the first
+    2 sample passes, on min(16, cores) threads."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    desc = make_desc(mrt, abi, cam, spp=2)
+    sc = oracle.OracleScene(data.desc())
+    p = abi.AccelParams()
+    p.leaf_size = 2
+    sc.build(abi.BVH, p)
+    vals = np.frombuffer(bytes(desc.tile_values), np.float32)
+    tiles = np.rint(vals * 256).astype(int)
+    central = {ty * 16 + tx for ty in range(6, 10) for tx in range(6, 10)}
+    mask = np.array([1 if t in central else 0 for t in tiles], np.uint8)
+    bm, counters, secs = sc.render(desc, threads=threads, tile_mask=mask)
+    rays = int(counters.sum())
+    return {"value": rays / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle render, 16 central tiles x 2 spp of the 1920x1080 frame, {rays} rays in {secs:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the frame would shard by tiles or samples, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import mobileraytracer_amd as mrt
+    from mobileraytracer_amd import abi
+
+    t0 = time.time()
+    data = mrt.soup_scene(2, N_TRIS, N_LIGHTS, 5.0, 0.005, abi.SAMPLER_RANDOM)
+    cam = soup_camera(mrt, abi)
+    log(f"scene generated in {time.time() - t0:.1f} s")
+    ctx = mrt.Context(0)
+    scene = mrt.Scene(ctx, data)
+    t0 = time.time()
+    scene.build(abi.BVH)
+    log(f"BVH built and uploaded in {time.time() - t0:.1f} s, footprint {scene.footprint() / 1e6:.1f} MB")
+
+    dev = torch.device("cuda:0")
+    bitmap = torch.zeros((H, W), dtype=torch.int32, device=dev)
+    accum = torch.zeros((H, W, 3), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    # Instrumented frame (1 spp): node visits and primitive tests per ray.
+    _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True), bitmap=bitmap, accum=accum,
+                         mem=abi.MEM_DEVICE)
+    rays1 = st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
+    closest1 = st.rays_camera + st.rays_extension + st.rays_specular
+    pairs_per_ray = st.node_visits / rays1
+    prims_per_ray = st.prim_tests / rays1
+    log(f"instrumented pass: {rays1} rays, {pairs_per_ray:.1f} node pairs and {prims_per_ray:.1f} "
+        f"primitive tests per ray")
+
+    desc = make_desc(mrt, abi, cam)
+    for _ in range(args.warmup):
+        scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    rays = 0
+    closest = 0
+    kernel_ms = 0.0
+    for _ in range(args.steps):
+        _, st = scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
+        rays += st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
+        closest += st.rays_camera + st.rays_extension + st.rays_specular
+        kernel_ms += st.ms_kernel
+        log(f"step: {st.ms_render:.1f} ms, {st.rays_camera} camera / {st.rays_extension} extension / "
+            f"{st.rays_shadow} shadow rays")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    mrays = rays / elapsed / 1e6
+
+    # Roofline of the render-pass kernel: algorithmic bytes per ray (ray in 32,
+    # hit out 16, 64 per node pair, 36 per primitive test, 256 of shading per
+    # closest-hit ray) x rays per launch / average launch duration.
+    launches = args.steps * SPP
+    bytes_per_ray = 48 + 64 * pairs_per_ray + 36 * prims_per_ray
+    bytes_per_launch = (rays / launches) * bytes_per_ray + (closest / launches) * 256
+    avg_launch_s = (kernel_ms / 1000.0) / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    out = {
+        "metric": "Mrays/s (1M-tri path trace, 16 spp) vs CPU; p50/p99 render-request latency",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000.0 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 1M-triangle soup (10k emissive), deterministic per-pixel sampling",
+        "config": {"workload": "soup-1M 1920x1080 PathTracer 16spp depth6 RR3 Halton BVH megakernel",
+                   "triangles": N_TRIS, "lights": N_LIGHTS, "width": W, "height": H, "spp": SPP,
+                   "rays_per_step": rays // args.steps},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_ray": round(bytes_per_ray, 1)},
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt)
+        log(f"cpu baseline took {time.time() - t0:.1f} s")
+        out["vs_cpu"] = round(mrays / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,310 @@
+/*
+ * mobilert_gpu.h - C ABI of the MI355X (gfx950) ray-scene library.
+ *
+ * Plain C linkage, plain pointers and sizes, opaque handles, int status codes
+ * (0 = OK, < 0 = error; mrt_last_error() gives the message).  No exception
+ * crosses this boundary.  Every compute entry point runs HIP kernels on the
+ * device the context was created on; there is no CPU fallback: when no GPU is
+ * present the context cannot be created and every call fails with
+ * MRT_E_NODEVICE.
+ *
+ * Reference interfaces replaced (paths relative to the MobileRT source tree):
+ *   Renderer::renderFrame / stopRender / getSample  app/MobileRT/Renderer.hpp:54-58
+ *   Shader::rayTrace / shadowTrace                   app/MobileRT/Shader.hpp:73-75
+ *   Shader::initializeAccelerators                   app/MobileRT/Shader.hpp:53
+ *   BVH<T> / RegularGrid<T> / Naive<T> trace, shadowTrace
+ *                                                    app/MobileRT/Accelerators/BVH.hpp:56-58,
+ *                                                    RegularGrid.hpp:44-46, Naive.hpp:39-41, 55-74
+ *   Triangle/Sphere/Plane::intersect                 app/MobileRT/Shapes/Triangle.hpp:55-57,
+ *                                                    Sphere.hpp:35-37, Plane.hpp
+ *   Camera::generateRay                              app/MobileRT/Camera.hpp:230-233
+ *   extern "C" RayTrace                              app/System_dependent/Linux/c_wrapper.h:10-15
+ *
+ * Struct layouts mirror the reference classes field for field so that a C++
+ * caller holding std::vector<Triangle> etc. can pass .data() directly.
+ */
+#ifndef MOBILERT_GPU_H
+#define MOBILERT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRT_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------ */
+enum mrt_status {
+    MRT_OK = 0,
+    MRT_E_INVALID = -1,     /* bad argument / size / handle                  */
+    MRT_E_HIP = -2,         /* HIP runtime error                             */
+    MRT_E_NOMEM = -3,       /* device allocation failed                      */
+    MRT_E_UNSUPPORTED = -4, /* configuration not supported by the kernels    */
+    MRT_E_BUDGET = -5,      /* admission refused: HBM budget exceeded        */
+    MRT_E_STOPPED = -6,     /* render stopped by mrt_render_stop             */
+    MRT_E_NODEVICE = -7     /* no HIP device available                       */
+};
+
+/* ---- enums that keep the reference's integer values -------------------- */
+/* Shader::Accelerator, app/MobileRT/Shader.hpp:34-38 */
+enum mrt_accelerator { MRT_NAIVE = 0, MRT_REGULAR_GRID = 1, MRT_BVH = 2 };
+/* shader selector of the Linux CLI, app/System_dependent/Linux/c_wrapper.cpp:444-486 */
+enum mrt_shader {
+    MRT_SHADER_NOSHADOWS = 0,
+    MRT_SHADER_WHITTED = 1,
+    MRT_SHADER_PATHTRACER = 2,
+    MRT_SHADER_DEPTHMAP = 3,
+    MRT_SHADER_DIFFUSE = 4
+};
+/* Primitive categories in the order Shader::rayTrace visits them
+ * (app/MobileRT/Shader.cpp:144-148 planes, spheres, triangles, then lights). */
+enum mrt_category { MRT_CAT_PLANE = 0, MRT_CAT_SPHERE = 1, MRT_CAT_TRIANGLE = 2, MRT_CAT_LIGHT = 3 };
+#define MRT_ID_NONE 0xFFFFFFFFu
+#define MRT_MAKE_ID(cat, idx) ((((uint32_t)(cat)) << 30) | ((uint32_t)(idx)))
+#define MRT_ID_CAT(id) ((uint32_t)(id) >> 30)
+#define MRT_ID_INDEX(id) ((uint32_t)(id) & 0x3FFFFFFFu)
+
+/* ---- scene data (reference class layouts) ------------------------------- */
+/* Triangle stored form, app/MobileRT/Shapes/Triangle.hpp:16-25 (112 bytes). */
+typedef struct mrt_triangle {
+    float ac[3], ab[3], a[3];
+    float na[3], nb[3], nc[3];
+    float ca[3], cb[3], cc[3];
+    int32_t material_id;
+} mrt_triangle;
+
+/* Sphere, app/MobileRT/Shapes/Sphere.hpp:16-18 (20 bytes). */
+typedef struct mrt_sphere {
+    float center[3];
+    float sq_radius;
+    int32_t material_id;
+} mrt_sphere;
+
+/* Plane, app/MobileRT/Shapes/Plane.hpp:16-18 (28 bytes). */
+typedef struct mrt_plane {
+    float normal[3];
+    float point[3];
+    int32_t material_id;
+} mrt_plane;
+
+/* Material, app/MobileRT/Material.hpp:13-17 (52 bytes). */
+typedef struct mrt_material {
+    float le[3], kd[3], ks[3], kt[3];
+    float ior;
+} mrt_material;
+
+/* Sampler descriptor.  The reference samplers are host objects with atomic
+ * counters or thread_local mt19937 streams (app/Components/Samplers,
+ * app/MobileRT/Shader.cpp:193-236); on the GPU every draw is a pure function
+ * of (seed, pixel, sample, dimension) so results do not depend on scheduling:
+ *   CONSTANT          value (Components::Constant)
+ *   RANDOM            counter-based hash stream (replaces MersenneTwister)
+ *   HALTON            per-pixel scrambled radical inverse over the sample index
+ *   STRATIFIED        per-pixel stratified over the sample index
+ *   HALTON_COMPAT     the reference HaltonSeq counter: draw c of the frame, in
+ *                     single-thread render order, is haltonSequence(c, 2) while
+ *                     c < domain_size and 1.0 afterwards (HaltonSeq.cpp:14-22)
+ *   STRATIFIED_COMPAT same with c / domain_size (Stratified.cpp:14-23)
+ * The *_COMPAT kinds are accepted only for the pixel sampler. */
+enum mrt_sampler_kind {
+    MRT_SAMPLER_CONSTANT = 0,
+    MRT_SAMPLER_RANDOM = 1,
+    MRT_SAMPLER_HALTON = 2,
+    MRT_SAMPLER_STRATIFIED = 3,
+    MRT_SAMPLER_HALTON_COMPAT = 4,
+    MRT_SAMPLER_STRATIFIED_COMPAT = 5
+};
+typedef struct mrt_sampler {
+    int32_t kind;
+    float value;          /* CONSTANT value                                  */
+    uint32_t seed;        /* stream seed (RANDOM/HALTON/STRATIFIED)          */
+    uint32_t domain_size; /* *_COMPAT: Sampler::domainSize_ (Sampler.cpp:9-13) */
+} mrt_sampler;
+
+/* Light: AreaLight (triangle emitter sampled with its own sampler,
+ * app/Components/Lights/AreaLight.cpp:22-43) or PointLight (PointLight.cpp). */
+enum mrt_light_kind { MRT_LIGHT_AREA = 0, MRT_LIGHT_POINT = 1 };
+typedef struct mrt_light {
+    int32_t kind;
+    mrt_material radiance;  /* Light::radiance_, app/MobileRT/Light.hpp     */
+    mrt_triangle triangle;  /* AreaLight::triangle_ (material_id -1)          */
+    float position[3];      /* PointLight::position_                          */
+    mrt_sampler sampler;    /* AreaLight::samplerPointLight_                  */
+} mrt_light;
+
+typedef struct mrt_scene_desc {
+    const mrt_triangle* triangles; uint32_t n_triangles;
+    const mrt_sphere* spheres;     uint32_t n_spheres;
+    const mrt_plane* planes;       uint32_t n_planes;
+    const mrt_material* materials; uint32_t n_materials;
+    const mrt_light* lights;       uint32_t n_lights;
+} mrt_scene_desc;
+
+/* ---- rays and hits ------------------------------------------------------ */
+/* Ray (app/MobileRT/Ray.hpp:14-17) in 32 bytes: the excluded primitive is an
+ * id (category << 30 | index) instead of a pointer; tmax is the initial
+ * Intersection::length_ (1e30 for rayTrace) or the shadowTrace distance. */
+typedef struct mrt_ray {
+    float org[3];
+    float dir[3];
+    float tmax;
+    uint32_t exclude;
+} mrt_ray;
+
+/* Nearest hit: t, id (MRT_ID_NONE on a miss), barycentrics u (towards B) and
+ * v (towards C) for triangles and area lights, 0 otherwise. */
+typedef struct mrt_hit {
+    float t;
+    uint32_t id;
+    float u;
+    float v;
+} mrt_hit;
+
+/* ---- camera and render descriptors ------------------------------------- */
+enum mrt_camera_kind { MRT_CAMERA_PERSPECTIVE = 0, MRT_CAMERA_ORTHOGRAPHIC = 1 };
+/* Camera fields after construction (app/MobileRT/Camera.cpp:47-54):
+ * perspective: fov_h/fov_v in radians (Perspective.cpp:12-18);
+ * orthographic: half sizes sizeH_/sizeV_ (Orthographic.cpp:11-17). */
+typedef struct mrt_camera {
+    int32_t kind;
+    float position[3], direction[3], right[3], up[3];
+    float fov_h, fov_v;
+} mrt_camera;
+
+/* Kernel shape for mrt_render. */
+enum mrt_kernel { MRT_KERNEL_AUTO = 0, MRT_KERNEL_MEGAKERNEL = 1, MRT_KERNEL_WAVEFRONT = 2 };
+
+typedef struct mrt_render_desc {
+    int32_t shader;            /* enum mrt_shader                              */
+    uint32_t width, height;    /* Renderer width_/height_                      */
+    uint32_t samples_pixel;    /* Renderer samplesPixel_                       */
+    uint32_t samples_light;    /* Shader samplesLight_                         */
+    int32_t ray_depth_min;     /* RayDepthMin (4 in Utils.hpp:35)              */
+    int32_t ray_depth_max;     /* RayDepthMax (8 in Utils.hpp:36)              */
+    mrt_camera camera;
+    mrt_sampler sampler_pixel; /* Renderer samplerPixel_                       */
+    mrt_sampler sampler_rr;    /* PathTracer samplerRussianRoulette_           */
+    mrt_sampler sampler_light; /* Shader::getLight draw                        */
+    mrt_sampler sampler_hemi;  /* Shader::getCosineSampleHemisphere draws      */
+    float tile_values[256];    /* Camera VALUES table in claim order           */
+    int32_t kernel;            /* enum mrt_kernel                              */
+    uint32_t first_sample;     /* first sample pass (progressive resume)       */
+    uint32_t instrument;       /* 1: also count node visits / primitive tests  */
+    uint32_t reserved[5];
+} mrt_render_desc;
+
+typedef struct mrt_render_stats {
+    uint64_t rays_camera;      /* primary rays traced                          */
+    uint64_t rays_extension;   /* diffuse continuation rays traced             */
+    uint64_t rays_shadow;      /* shadowTrace calls                            */
+    uint64_t rays_specular;    /* reflection + transmission rays traced        */
+    uint64_t node_visits;      /* interior node pairs fetched (instrumented)   */
+    uint64_t prim_tests;       /* primitive tests (instrumented)               */
+    uint32_t samples_done;     /* completed sample passes                      */
+    uint32_t covered_pixels;   /* pixels inside the 16x16 tile grid            */
+    float ms_render;           /* device time of the render, milliseconds      */
+    float ms_kernel;           /* summed duration of the dominant kernel       */
+} mrt_render_stats;
+
+/* Accelerator build parameters (Shader::initializeAccelerators inputs). */
+typedef struct mrt_accel_params {
+    float camera_min[3];       /* Camera::getAABB(), joins the grid bounds     */
+    float camera_max[3];       /* (app/MobileRT/Shader.cpp:52-62)              */
+    uint32_t leaf_size;        /* BVH maxLeafSize (2 in BVH.hpp:19)            */
+    uint32_t reserved[3];
+} mrt_accel_params;
+
+typedef struct mrt_ctx mrt_ctx;
+typedef struct mrt_scene mrt_scene;
+
+/* Memory location of the ray/hit/bitmap/accumulation arguments. */
+enum mrt_mem { MRT_MEM_HOST = 0, MRT_MEM_DEVICE = 1 };
+
+/* ---- context ------------------------------------------------------------ */
+MRT_API const char* mrt_last_error(void);
+MRT_API int mrt_device_count(int* count);
+MRT_API int mrt_ctx_create(int device, mrt_ctx** out);
+MRT_API int mrt_ctx_destroy(mrt_ctx* ctx);
+/* HBM budget for scene admission, bytes (0 = unlimited). Mirrors the memory
+ * admission of app/System_dependent/Android_JNI/DrawView.cpp:373-416. */
+MRT_API int mrt_ctx_set_budget(mrt_ctx* ctx, uint64_t bytes);
+MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx);
+
+/* ---- scene -------------------------------------------------------------- */
+/* Copies the scene to the device (Shader(Scene&&, ...), Shader.cpp:18-25). */
+MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* desc, mrt_scene** out);
+MRT_API int mrt_scene_destroy(mrt_scene* scene);
+/* Builds the accelerator of every category (Shader::initializeAccelerators,
+ * Shader.cpp:31-122): BVH with the reference in-order SAH builder
+ * (BVH.hpp:64-155, 274-311), RegularGrid with the reference sizing and cell
+ * registration (Shader.cpp:40-113, RegularGrid.hpp:50-154), or Naive. */
+MRT_API int mrt_scene_build(mrt_scene* scene, int accelerator, const mrt_accel_params* params);
+/* Device bytes held by the scene (geometry + accelerator). */
+MRT_API int mrt_scene_footprint(const mrt_scene* scene, uint64_t* bytes);
+
+/* ---- traversal entry points -------------------------------------------- */
+/* Shader::rayTrace's intersection step (Shader.cpp:124-160): nearest hit over
+ * planes, spheres, triangles (through the built accelerator) and lights
+ * (Scene::traceLights, Scene.cpp:31-37). */
+MRT_API int mrt_trace_closest(mrt_scene* scene, const mrt_ray* rays, uint32_t n, mrt_hit* hits, int mem);
+/* Shader::shadowTrace (Shader.cpp:162-187): any hit in (1e-6, tmax), lights excluded. */
+MRT_API int mrt_trace_any(mrt_scene* scene, const mrt_ray* rays, uint32_t n, uint8_t* occluded, int mem);
+/* Shader::rayTrace including shade(): radiance and the "hit light" flag for a
+ * batch of rays, with the render descriptor's shader, depths and samplers;
+ * ray i uses pixel index i and sample `sample` for its sampler draws. */
+MRT_API int mrt_shade_rays(mrt_scene* scene, const mrt_render_desc* desc, const mrt_ray* rays,
+                           const int32_t* depths, uint32_t n, uint32_t sample,
+                           float* rgb, uint8_t* hit_light, int mem);
+/* Camera::generateRay for a batch of (u, v, deviationU, deviationV). */
+MRT_API int mrt_camera_rays(mrt_ctx* ctx, const mrt_camera* camera, const float* uvdd, uint32_t n,
+                            mrt_ray* rays, int mem);
+/* Pairwise primitive tests: ray i against primitive i (KAT entry points for
+ * Triangle::intersect (both forms) and Plane::intersect, Sphere::intersect). */
+MRT_API int mrt_intersect_pairs(mrt_ctx* ctx, int category, const void* prims, const mrt_ray* rays,
+                                uint32_t n, mrt_hit* closest, uint8_t* any, int mem);
+
+/* ---- render ------------------------------------------------------------- */
+/* Renderer::renderFrame (Renderer.cpp:31-59, 67-116): renders desc->samples_pixel
+ * passes over the 16x16 tile grid, folding every pass into the RGBA8 bitmap
+ * with the reference 8-bit running mean (Utils.cpp:48-71), and, when accum is
+ * not NULL, adding the FP32 radiance of every pass to accum (3 floats per
+ * pixel, stride width).  stride_bytes is the bitmap row stride. */
+MRT_API int mrt_render(mrt_scene* scene, const mrt_render_desc* desc, uint32_t* bitmap,
+                       uint32_t stride_bytes, float* accum, int mem, mrt_render_stats* stats);
+/* Renderer::stopRender (Renderer.cpp:61-65): callable from any thread; the
+ * running mrt_render returns MRT_E_STOPPED after the pass in flight. */
+MRT_API int mrt_render_stop(mrt_scene* scene);
+/* Renderer::getSample (Renderer.cpp:118-120). */
+MRT_API int mrt_render_progress(const mrt_scene* scene, uint32_t* samples_done);
+
+/* ---- host-side construction helpers (no device work) ------------------- */
+/* Triangle constructors (Triangle.cpp:12-55): normals/colors may be NULL. */
+MRT_API int mrt_triangle_make(const float a[3], const float b[3], const float c[3], int32_t material_id,
+                              const float* normals9, const float* colors9, mrt_triangle* out);
+/* Camera constructor basis (Camera.cpp:47-54) and Perspective/Orthographic fields. */
+MRT_API int mrt_camera_make(int kind, const float position[3], const float look_at[3], const float up[3],
+                            float fov_h_or_size_h, float fov_v_or_size_v, mrt_camera* out);
+/* Camera tile table (Camera.cpp:16-26): haltonSequence(i, 2) for i < 256,
+ * shuffled by a seeded Fisher-Yates (seed 0 keeps the unshuffled order). */
+MRT_API int mrt_tile_values(uint32_t seed, float out[256]);
+/* Triangle::intersectBox (Triangle.cpp:154-246), used by the grid builder. */
+MRT_API int mrt_triangle_intersect_box(const mrt_triangle* tri, const float box_min[3], const float box_max[3]);
+
+/* ---- synthetic scenes ---------------------------------------------------- */
+/* Procedural Cornell box: 10 wall triangles + 24 box triangles, 2 area lights. */
+#define MRT_CORNELL_TRIANGLES 34u
+#define MRT_CORNELL_MATERIALS 7u
+#define MRT_CORNELL_LIGHTS 2u
+MRT_API int mrt_scene_cornell(uint32_t seed, mrt_triangle* tris, mrt_material* mats, mrt_light* lights);
+/* Random soup of n triangles in the unit cube (Morton ordered), n_emissive of
+ * them emitted as area lights with radiance le; the rest go to tris
+ * (n - n_emissive entries) with vertex colours U(0.2,0.8) and material -1. */
+MRT_API int mrt_scene_soup(uint32_t seed, uint32_t n, uint32_t n_emissive, float le, float half_extent,
+                           mrt_triangle* tris, mrt_light* lights, int light_sampler_kind);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOBILERT_GPU_H */

@@ -1,0 +1,219 @@
+"""MI355X (gfx950) ray-scene path of MobileRT behind a C ABI.
+
+Python-side conveniences over libmrt_gpu.so (include/mobilert_gpu.h): scene
+generation, upload, accelerator builds, traversal and the tile renderer.
+Every compute call runs HIP kernels; when the library or the GPU is missing
+the calls raise instead of computing anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import (BVH, NAIVE, REGULAR_GRID, MEM_DEVICE, MEM_HOST, SHADER_PATHTRACER, SHADER_WHITTED,  # noqa: F401
+                  check, lib, ptr)
+
+__all__ = ["Context", "Scene", "cornell_scene", "soup_scene", "make_camera", "tile_values", "render_desc",
+           "sampler", "abi"]
+
+
+def sampler(kind: int = abi.SAMPLER_CONSTANT, value: float = 0.5, seed: int = 0, domain_size: int = 0) -> abi.Sampler:
+    return abi.Sampler(kind, value, seed, domain_size)
+
+
+def make_camera(kind, position, look_at, up, a, b) -> abi.Camera:
+    """Camera ctor fields (perspective: fovs in degrees; orthographic: sizes)."""
+    cam = abi.Camera()
+    p, l, u = (np.asarray(v, np.float32) for v in (position, look_at, up))
+    check(lib().mrt_camera_make(kind, p.ctypes.data, l.ctypes.data, u.ctypes.data, a, b, C.byref(cam)),
+          "mrt_camera_make")
+    return cam
+
+
+def camera_aabb(cam: abi.Camera) -> tuple[np.ndarray, np.ndarray]:
+    """Camera::getAABB for a perspective camera (Camera.cpp:39-45): its position."""
+    p = np.array(cam.position, np.float32)
+    return p, p.copy()
+
+
+def tile_values(seed: int = 0) -> np.ndarray:
+    out = np.zeros(256, np.float32)
+    check(lib().mrt_tile_values(seed, out.ctypes.data), "mrt_tile_values")
+    return out
+
+
+def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Camera, *, samples_light: int = 1,
+                depth_min: int = 4, depth_max: int = 8, pixel=None, rr=None, light=None, hemi=None,
+                tile_seed: int = 0, kernel: int = abi.KERNEL_MEGAKERNEL, first_sample: int = 0,
+                instrument: bool = False) -> abi.RenderDesc:
+    d = abi.RenderDesc()
+    d.shader = shader
+    d.width, d.height, d.samples_pixel, d.samples_light = width, height, spp, samples_light
+    d.ray_depth_min, d.ray_depth_max = depth_min, depth_max
+    d.camera = camera
+    d.sampler_pixel = pixel or sampler()
+    d.sampler_rr = rr or sampler()
+    d.sampler_light = light or sampler()
+    d.sampler_hemi = hemi or sampler()
+    vals = tile_values(tile_seed)
+    C.memmove(d.tile_values, vals.ctypes.data, 256 * 4)
+    d.kernel = kernel
+    d.first_sample = first_sample
+    d.instrument = 1 if instrument else 0
+    return d
+
+
+class SceneData:
+    """Host arrays of a scene in the C ABI layouts."""
+
+    def __init__(self, triangles=None, materials=None, lights=None, spheres=None, planes=None):
+        self.triangles = triangles if triangles is not None else np.zeros(0, abi.TRIANGLE_DTYPE)
+        self.materials = materials if materials is not None else np.zeros(0, abi.MATERIAL_DTYPE)
+        self.lights = lights if lights is not None else np.zeros(0, abi.LIGHT_DTYPE)
+        self.spheres = spheres if spheres is not None else np.zeros(0, abi.SPHERE_DTYPE)
+        self.planes = planes if planes is not None else np.zeros(0, abi.PLANE_DTYPE)
+
+    def desc(self) -> abi.SceneDesc:
+        d = abi.SceneDesc()
+
+        def put(arr, name):
+            setattr(d, name, arr.ctypes.data if len(arr) else None)
+            setattr(d, "n_" + name, len(arr))
+        put(self.triangles, "triangles")
+        put(self.spheres, "spheres")
+        put(self.planes, "planes")
+        put(self.materials, "materials")
+        put(self.lights, "lights")
+        return d
+
+
+def cornell_scene(seed: int = 1) -> SceneData:
+    tris = np.zeros(abi.CORNELL_TRIANGLES, abi.TRIANGLE_DTYPE)
+    mats = np.zeros(abi.CORNELL_MATERIALS, abi.MATERIAL_DTYPE)
+    lights = np.zeros(abi.CORNELL_LIGHTS, abi.LIGHT_DTYPE)
+    check(lib().mrt_scene_cornell(seed, tris.ctypes.data, mats.ctypes.data, lights.ctypes.data), "mrt_scene_cornell")
+    return SceneData(tris, mats, lights)
+
+
+def soup_scene(seed: int, n: int, n_emissive: int, le: float = 5.0, half_extent: float = 0.005,
+               light_sampler: int = abi.SAMPLER_RANDOM) -> SceneData:
+    tris = np.zeros(n - n_emissive, abi.TRIANGLE_DTYPE)
+    lights = np.zeros(n_emissive, abi.LIGHT_DTYPE)
+    check(lib().mrt_scene_soup(seed, n, n_emissive, le, half_extent, tris.ctypes.data,
+                               lights.ctypes.data if n_emissive else None, light_sampler), "mrt_scene_soup")
+    return SceneData(tris, None, lights)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().mrt_ctx_create(device, C.byref(h)), "mrt_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mrt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_budget(self, nbytes: int) -> None:
+        check(lib().mrt_ctx_set_budget(self.h, nbytes), "mrt_ctx_set_budget")
+
+    def camera_rays(self, cam: abi.Camera, uvdd: np.ndarray) -> np.ndarray:
+        uvdd = np.ascontiguousarray(uvdd, np.float32)
+        out = np.zeros(len(uvdd), abi.RAY_DTYPE)
+        check(lib().mrt_camera_rays(self.h, C.byref(cam), uvdd.ctypes.data, len(uvdd), out.ctypes.data, MEM_HOST),
+              "mrt_camera_rays")
+        return out
+
+    def intersect_pairs(self, category: int, prims: np.ndarray, rays: np.ndarray):
+        n = len(rays)
+        hits = np.zeros(n, abi.HIT_DTYPE)
+        anyh = np.zeros(n, np.uint8)
+        check(lib().mrt_intersect_pairs(self.h, category, prims.ctypes.data, rays.ctypes.data, n, hits.ctypes.data,
+                                        anyh.ctypes.data, MEM_HOST), "mrt_intersect_pairs")
+        return hits, anyh
+
+
+class Scene:
+    def __init__(self, ctx: Context, data: SceneData):
+        self.ctx = ctx
+        self.data = data
+        h = C.c_void_p()
+        desc = data.desc()
+        check(lib().mrt_scene_create(ctx.h, C.byref(desc), C.byref(h)), "mrt_scene_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mrt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, accelerator: int, camera_min=(0, 0, 0), camera_max=(0, 0, 0), leaf_size: int = 2) -> None:
+        p = abi.AccelParams()
+        p.camera_min[:] = list(camera_min)
+        p.camera_max[:] = list(camera_max)
+        p.leaf_size = leaf_size
+        check(lib().mrt_scene_build(self.h, accelerator, C.byref(p)), "mrt_scene_build")
+
+    def footprint(self) -> int:
+        v = C.c_uint64()
+        check(lib().mrt_scene_footprint(self.h, C.byref(v)), "mrt_scene_footprint")
+        return v.value
+
+    def trace_closest(self, rays, mem: int = MEM_HOST, out=None):
+        n = len(rays) if mem == MEM_HOST else rays.shape[0]
+        if out is None:
+            out = np.zeros(n, abi.HIT_DTYPE)
+        check(lib().mrt_trace_closest(self.h, ptr(rays), n, ptr(out), mem), "mrt_trace_closest")
+        return out
+
+    def trace_any(self, rays, mem: int = MEM_HOST, out=None):
+        n = len(rays) if mem == MEM_HOST else rays.shape[0]
+        if out is None:
+            out = np.zeros(n, np.uint8)
+        check(lib().mrt_trace_any(self.h, ptr(rays), n, ptr(out), mem), "mrt_trace_any")
+        return out
+
+    def shade_rays(self, desc: abi.RenderDesc, rays: np.ndarray, depths=None, sample: int = 0):
+        n = len(rays)
+        rgb = np.zeros((n, 3), np.float32)
+        flag = np.zeros(n, np.uint8)
+        dp = None if depths is None else np.ascontiguousarray(depths, np.int32)
+        check(lib().mrt_shade_rays(self.h, C.byref(desc), rays.ctypes.data, ptr(dp), n, sample, rgb.ctypes.data,
+                                   flag.ctypes.data, MEM_HOST), "mrt_shade_rays")
+        return rgb, flag
+
+    def render(self, desc: abi.RenderDesc, bitmap=None, accum=None, stride_bytes: int | None = None,
+               mem: int = MEM_HOST):
+        """Renderer::renderFrame.  Host numpy buffers by default; with
+        mem=MEM_DEVICE `bitmap`/`accum` are device tensors."""
+        if bitmap is None:
+            bitmap = np.zeros((desc.height, desc.width), np.uint32)
+        if stride_bytes is None:
+            stride_bytes = desc.width * 4
+        stats = abi.RenderStats()
+        check(lib().mrt_render(self.h, C.byref(desc), ptr(bitmap), stride_bytes, ptr(accum), mem, C.byref(stats)),
+              "mrt_render")
+        return bitmap, stats
+
+    def stop(self) -> None:
+        check(lib().mrt_render_stop(self.h), "mrt_render_stop")
+
+    def progress(self) -> int:
+        v = C.c_uint32()
+        check(lib().mrt_render_progress(self.h, C.byref(v)), "mrt_render_progress")
+        return v.value

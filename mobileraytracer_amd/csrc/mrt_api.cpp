@@ -1,0 +1,787 @@
+// C ABI of the gfx950 ray-scene library: contexts, scene upload, accelerator
+// builds, traversal / shading entry points and the tile renderer.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mrt_build.hpp"
+#include "mrt_kernels.hpp"
+#include "mrt_path.hpp"
+
+namespace {
+
+using namespace mrt;
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define MRT_HIP(expr)                                                                                        \
+    do {                                                                                                     \
+        const hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(MRT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// Device allocation owned by a scene or context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n) {
+        if (n <= bytes && p) return MRT_OK;
+        release();
+        if (n == 0) return MRT_OK;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            return fail(MRT_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+        return MRT_OK;
+    }
+    int upload(const void* src, size_t n, hipStream_t s) {
+        const int rc = alloc(n);
+        if (rc != MRT_OK) return rc;
+        if (n) MRT_HIP(hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s));
+        return MRT_OK;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr uint32_t kStackLds = 16;     // entries per lane in LDS (mrt_kernels.hip)
+constexpr uint32_t kLinearLights = 16; // up to this many area lights are scanned linearly
+
+}  // namespace
+
+struct mrt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t num_cu = 0;
+    uint64_t budget = 0;
+    std::mutex mu;
+};
+
+struct mrt_scene {
+    mrt_ctx* ctx = nullptr;
+    std::vector<mrt_triangle> tris;
+    std::vector<mrt_sphere> sphs;
+    std::vector<mrt_plane> plas;
+    std::vector<mrt_material> mats;   // scene materials then one per light
+    std::vector<mrt_light> lights;
+    uint32_t n_scene_mats = 0;
+    DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
+    DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3];
+    DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank;
+    SceneDev dev{};
+    bool built = false;
+    bool generic = false;
+    uint32_t depth_need = 0;
+    std::atomic<bool> stop{false};
+    std::atomic<uint32_t> progress{0};
+};
+
+namespace {
+
+uint64_t buffers_bytes(const mrt_scene* s) {
+    uint64_t b = s->tri_geo.bytes + s->tri_shade.bytes + s->sph.bytes + s->sph_mat.bytes + s->pla.bytes +
+                 s->mat.bytes + s->light_geo.bytes + s->light_shade.bytes + s->light_info.bytes;
+    for (int i = 0; i < 4; ++i) b += s->nodes[i].bytes + s->perm[i].bytes;
+    for (int i = 0; i < 3; ++i) b += s->cell_off[i].bytes + s->cell_ref[i].bytes;
+    return b;
+}
+
+void pack_geo(const mrt_triangle& t, float4* g) {
+    g[0] = make_float4(t.a[0], t.a[1], t.a[2], t.ab[0]);
+    g[1] = make_float4(t.ab[1], t.ab[2], t.ac[0], t.ac[1]);
+    g[2] = make_float4(t.ac[2], 0.0f, 0.0f, 0.0f);
+}
+void pack_shade(const mrt_triangle& t, float4* s) {
+    float mid;
+    std::memcpy(&mid, &t.material_id, 4);
+    s[0] = make_float4(t.na[0], t.na[1], t.na[2], t.nb[0]);
+    s[1] = make_float4(t.nb[1], t.nb[2], t.nc[0], t.nc[1]);
+    s[2] = make_float4(t.nc[2], t.ca[0], t.ca[1], t.ca[2]);
+    s[3] = make_float4(t.cb[0], t.cb[1], t.cb[2], t.cc[0]);
+    s[4] = make_float4(t.cc[1], t.cc[2], mid, 0.0f);
+}
+
+void fill_accel_bvh(Accel* a, const BvhOut& b, const DevBuf& nodes, const DevBuf* perm, uint32_t count) {
+    a->count = count;
+    a->kind = MRT_BVH;
+    a->nodes = nodes.as<BNode>();
+    a->perm = perm ? perm->as<uint32_t>() : nullptr;
+    st3(a->root_mn, b.root.mn);
+    st3(a->root_mx, b.root.mx);
+    a->root_ref = b.root_ref;
+    a->root_count = b.root_count;
+    a->depth = b.depth;
+}
+
+int upload_bvh(mrt_scene* s, int slot, const BvhOut& b, hipStream_t st) {
+    if (b.pairs.empty()) return MRT_OK;
+    return s->nodes[slot].upload(b.pairs.data(), b.pairs.size() * sizeof(BNode), st);
+}
+
+// Render parameters common to mrt_render and mrt_shade_rays.
+RenderParams params_from(const mrt_render_desc& d) {
+    RenderParams p{};
+    p.shader = d.shader;
+    p.width = d.width;
+    p.height = d.height;
+    p.spp = d.samples_pixel;
+    p.samples_light = d.samples_light;
+    p.depth_min = d.ray_depth_min;
+    p.depth_max = d.ray_depth_max;
+    p.cam = d.camera;
+    p.s_pixel = d.sampler_pixel;
+    p.s_rr = d.sampler_rr;
+    p.s_light = d.sampler_light;
+    p.s_hemi = d.sampler_hemi;
+    return p;
+}
+
+int check_sampler(const mrt_sampler& s, bool pixel, const char* what) {
+    const bool compat = s.kind == MRT_SAMPLER_HALTON_COMPAT || s.kind == MRT_SAMPLER_STRATIFIED_COMPAT;
+    if (s.kind < MRT_SAMPLER_CONSTANT || s.kind > MRT_SAMPLER_STRATIFIED_COMPAT || (compat && !pixel))
+        return fail(MRT_E_INVALID, std::string("invalid sampler kind for ") + what);
+    if (compat && s.domain_size == 0) return fail(MRT_E_INVALID, std::string("compat sampler needs domain_size: ") + what);
+    return MRT_OK;
+}
+
+int check_desc(const mrt_scene* s, const mrt_render_desc* d) {
+    if (!d) return fail(MRT_E_INVALID, "null render descriptor");
+    if (d->shader != MRT_SHADER_WHITTED && d->shader != MRT_SHADER_PATHTRACER)
+        return fail(MRT_E_UNSUPPORTED, "shader " + std::to_string(d->shader) + " is not implemented on the GPU");
+    if (d->ray_depth_max < 0 || d->ray_depth_max > 15) return fail(MRT_E_UNSUPPORTED, "ray_depth_max must be 0..15");
+    if (d->samples_light > 255) return fail(MRT_E_UNSUPPORTED, "samples_light must be <= 255");
+    int rc;
+    if ((rc = check_sampler(d->sampler_pixel, true, "pixel")) != MRT_OK) return rc;
+    if ((rc = check_sampler(d->sampler_rr, false, "russian roulette")) != MRT_OK) return rc;
+    if ((rc = check_sampler(d->sampler_light, false, "light choice")) != MRT_OK) return rc;
+    if ((rc = check_sampler(d->sampler_hemi, false, "hemisphere")) != MRT_OK) return rc;
+    for (const mrt_light& l : s->lights)
+        if ((rc = check_sampler(l.sampler, false, "area light")) != MRT_OK) return rc;
+    if (d->camera.kind != MRT_CAMERA_PERSPECTIVE && d->camera.kind != MRT_CAMERA_ORTHOGRAPHIC)
+        return fail(MRT_E_INVALID, "camera kind");
+    return MRT_OK;
+}
+
+// Grows the spill area to cover `lanes` lanes; fills the stack config.
+int ensure_spill(mrt_scene* s, uint32_t lanes, StackCfg* sc) {
+    const uint32_t need = s->depth_need + 1u;
+    sc->spill = nullptr;
+    sc->spill_stride = lanes;
+    sc->cap_spill = 0;
+    if (need <= kStackLds) return MRT_OK;
+    sc->cap_spill = need - kStackLds;
+    const size_t bytes = static_cast<size_t>(sc->cap_spill) * lanes * sizeof(uint32_t);
+    const int rc = s->spill.alloc(bytes);
+    if (rc != MRT_OK) return rc;
+    sc->spill = s->spill.as<uint32_t>();
+    return MRT_OK;
+}
+
+int prepare_aux(mrt_scene* s) {
+    int rc;
+    if ((rc = s->counters.alloc(8 * sizeof(unsigned long long))) != MRT_OK) return rc;
+    if ((rc = s->err.alloc(sizeof(uint32_t))) != MRT_OK) return rc;
+    MRT_HIP(hipMemsetAsync(s->counters.p, 0, 8 * sizeof(unsigned long long), s->ctx->stream));
+    MRT_HIP(hipMemsetAsync(s->err.p, 0, sizeof(uint32_t), s->ctx->stream));
+    return MRT_OK;
+}
+
+int check_err(mrt_scene* s) {
+    uint32_t e = 0;
+    MRT_HIP(hipMemcpyAsync(&e, s->err.p, sizeof(e), hipMemcpyDeviceToHost, s->ctx->stream));
+    MRT_HIP(hipStreamSynchronize(s->ctx->stream));
+    if (e & 1u) return fail(MRT_E_INVALID, "traversal stack overflow (BVH deeper than the stack budget)");
+    if (e & 2u) return fail(MRT_E_UNSUPPORTED, "too many pending reflection/transmission rays");
+    return MRT_OK;
+}
+
+// Copies `bytes` from host or device memory into a staging buffer (host) or
+// returns the device pointer as is.
+int stage(DevBuf& buf, const void* src, size_t bytes, int mem, hipStream_t st, const void** out) {
+    if (mem == MRT_MEM_DEVICE) {
+        *out = src;
+        return MRT_OK;
+    }
+    const int rc = buf.upload(src, bytes, st);
+    if (rc != MRT_OK) return rc;
+    *out = buf.p;
+    return MRT_OK;
+}
+
+uint32_t cap_blocks(const mrt_ctx* c) { return (c->num_cu ? c->num_cu : 256u) * 16u; }
+
+}  // namespace
+
+extern "C" {
+
+MRT_API const char* mrt_last_error(void) { return g_error.c_str(); }
+
+MRT_API int mrt_device_count(int* count) {
+    if (!count) return MRT_E_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return MRT_OK;
+}
+
+MRT_API int mrt_ctx_create(int device, mrt_ctx** out) {
+    if (!out) return fail(MRT_E_INVALID, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(MRT_E_NODEVICE, "no HIP device");
+    if (device < 0 || device >= n) return fail(MRT_E_INVALID, "device index out of range");
+    MRT_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MRT_HIP(hipGetDeviceProperties(&prop, device));
+    auto* c = new mrt_ctx;
+    c->device = device;
+    c->num_cu = static_cast<uint32_t>(prop.multiProcessorCount);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(MRT_E_HIP, "hipStreamCreate failed");
+    }
+    *out = c;
+    return MRT_OK;
+}
+
+MRT_API int mrt_ctx_destroy(mrt_ctx* ctx) {
+    if (!ctx) return MRT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MRT_OK;
+}
+
+MRT_API int mrt_ctx_set_budget(mrt_ctx* ctx, uint64_t bytes) {
+    if (!ctx) return fail(MRT_E_INVALID, "null context");
+    ctx->budget = bytes;
+    return MRT_OK;
+}
+
+MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx) {
+    if (!ctx) return fail(MRT_E_INVALID, "null context");
+    MRT_HIP(hipSetDevice(ctx->device));
+    MRT_HIP(hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
+MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** out) {
+    if (!ctx || !d || !out) return fail(MRT_E_INVALID, "null argument");
+    *out = nullptr;
+    if ((d->n_triangles && !d->triangles) || (d->n_spheres && !d->spheres) || (d->n_planes && !d->planes) ||
+        (d->n_materials && !d->materials) || (d->n_lights && !d->lights))
+        return fail(MRT_E_INVALID, "null array with a non-zero count");
+    if (d->n_triangles >= (1u << 30) || d->n_spheres >= (1u << 30) || d->n_planes >= (1u << 30) ||
+        d->n_lights >= (1u << 30))
+        return fail(MRT_E_INVALID, "more than 2^30 primitives in one category");
+    for (uint32_t i = 0; i < d->n_triangles; ++i)
+        if (d->triangles[i].material_id >= static_cast<int32_t>(d->n_materials))
+            return fail(MRT_E_INVALID, "triangle material index out of range");
+    for (uint32_t i = 0; i < d->n_spheres; ++i)
+        if (d->spheres[i].material_id >= static_cast<int32_t>(d->n_materials))
+            return fail(MRT_E_INVALID, "sphere material index out of range");
+    for (uint32_t i = 0; i < d->n_planes; ++i)
+        if (d->planes[i].material_id >= static_cast<int32_t>(d->n_materials))
+            return fail(MRT_E_INVALID, "plane material index out of range");
+    const uint64_t estimate = static_cast<uint64_t>(d->n_triangles) * (48 + 80) + d->n_spheres * 20ull +
+                              d->n_planes * 32ull + (d->n_materials + d->n_lights) * 64ull + d->n_lights * 160ull;
+    if (ctx->budget && estimate > ctx->budget)
+        return fail(MRT_E_BUDGET, "scene needs " + std::to_string(estimate) + " bytes, budget " +
+                                      std::to_string(ctx->budget));
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto s = std::make_unique<mrt_scene>();
+    s->ctx = ctx;
+    s->tris.assign(d->triangles, d->triangles + d->n_triangles);
+    s->sphs.assign(d->spheres, d->spheres + d->n_spheres);
+    s->plas.assign(d->planes, d->planes + d->n_planes);
+    s->mats.assign(d->materials, d->materials + d->n_materials);
+    s->lights.assign(d->lights, d->lights + d->n_lights);
+    s->n_scene_mats = d->n_materials;
+    for (const mrt_light& l : s->lights) s->mats.push_back(l.radiance);
+    MRT_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    int rc;
+    {
+        std::vector<float4> geo(3 * s->tris.size()), shade(5 * s->tris.size());
+        for (size_t i = 0; i < s->tris.size(); ++i) {
+            pack_geo(s->tris[i], &geo[3 * i]);
+            pack_shade(s->tris[i], &shade[5 * i]);
+        }
+        if ((rc = s->tri_geo.upload(geo.data(), geo.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        if ((rc = s->tri_shade.upload(shade.data(), shade.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        MRT_HIP(hipStreamSynchronize(st));
+    }
+    {
+        std::vector<float4> sp(s->sphs.size());
+        std::vector<int32_t> sm(s->sphs.size());
+        for (size_t i = 0; i < s->sphs.size(); ++i) {
+            sp[i] = make_float4(s->sphs[i].center[0], s->sphs[i].center[1], s->sphs[i].center[2],
+                                s->sphs[i].sq_radius);
+            sm[i] = s->sphs[i].material_id;
+        }
+        std::vector<float4> pl(2 * s->plas.size());
+        for (size_t i = 0; i < s->plas.size(); ++i) {
+            float mid;
+            std::memcpy(&mid, &s->plas[i].material_id, 4);
+            pl[2 * i] = make_float4(s->plas[i].normal[0], s->plas[i].normal[1], s->plas[i].normal[2], mid);
+            pl[2 * i + 1] = make_float4(s->plas[i].point[0], s->plas[i].point[1], s->plas[i].point[2], 0.0f);
+        }
+        std::vector<float4> mt(4 * s->mats.size());
+        for (size_t i = 0; i < s->mats.size(); ++i) {
+            const mrt_material& m = s->mats[i];
+            mt[4 * i + 0] = make_float4(m.le[0], m.le[1], m.le[2], m.kd[0]);
+            mt[4 * i + 1] = make_float4(m.kd[1], m.kd[2], m.ks[0], m.ks[1]);
+            mt[4 * i + 2] = make_float4(m.ks[2], m.kt[0], m.kt[1], m.kt[2]);
+            mt[4 * i + 3] = make_float4(m.ior, 0.0f, 0.0f, 0.0f);
+            if (any_gt0(ld3(m.ks)) || any_gt0(ld3(m.kt))) s->dev.has_branches = 1u;
+        }
+        std::vector<float4> lg(3 * s->lights.size()), ls(5 * s->lights.size());
+        std::vector<LightInfo> li(s->lights.size());
+        uint32_t nArea = 0;
+        std::vector<Box> lboxes;
+        std::vector<uint32_t> lidx;
+        for (size_t i = 0; i < s->lights.size(); ++i) {
+            pack_geo(s->lights[i].triangle, &lg[3 * i]);
+            pack_shade(s->lights[i].triangle, &ls[5 * i]);
+            li[i].kind = s->lights[i].kind;
+            li[i].material = s->n_scene_mats + static_cast<uint32_t>(i);
+            std::memcpy(li[i].pos, s->lights[i].position, sizeof(li[i].pos));
+            li[i].sampler = s->lights[i].sampler;
+            if (s->lights[i].kind == MRT_LIGHT_AREA) {
+                ++nArea;
+                lboxes.push_back(triangle_box(s->lights[i].triangle));
+                lidx.push_back(static_cast<uint32_t>(i));
+            }
+        }
+        if ((rc = s->sph.upload(sp.data(), sp.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        if ((rc = s->sph_mat.upload(sm.data(), sm.size() * sizeof(int32_t), st)) != MRT_OK) return rc;
+        if ((rc = s->pla.upload(pl.data(), pl.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        if ((rc = s->mat.upload(mt.data(), mt.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        if ((rc = s->light_geo.upload(lg.data(), lg.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        if ((rc = s->light_shade.upload(ls.data(), ls.size() * sizeof(float4), st)) != MRT_OK) return rc;
+        if ((rc = s->light_info.upload(li.data(), li.size() * sizeof(LightInfo), st)) != MRT_OK) return rc;
+        s->dev.n_lights = static_cast<uint32_t>(s->lights.size());
+        s->dev.n_area_lights = nArea;
+        s->dev.lights_linear = nArea <= kLinearLights ? 1u : 0u;
+        if (!s->dev.lights_linear) {
+            BvhOut lb;
+            std::vector<uint32_t> perm;
+            build_light_bvh(lboxes, &lb, &perm);
+            for (uint32_t& p : perm) p = lidx[p];
+            if ((rc = upload_bvh(s.get(), 3, lb, st)) != MRT_OK) return rc;
+            if ((rc = s->perm[3].upload(perm.data(), perm.size() * sizeof(uint32_t), st)) != MRT_OK) return rc;
+            fill_accel_bvh(&s->dev.light_acc, lb, s->nodes[3], &s->perm[3], nArea);
+            s->depth_need = std::max(s->depth_need, lb.depth);
+        }
+        MRT_HIP(hipStreamSynchronize(st));
+    }
+    s->dev.tri_geo = s->tri_geo.as<float4>();
+    s->dev.tri_shade = s->tri_shade.as<float4>();
+    s->dev.sph = s->sph.as<float4>();
+    s->dev.sph_mat = s->sph_mat.as<int32_t>();
+    s->dev.pla = s->pla.as<float4>();
+    s->dev.mat = s->mat.as<float4>();
+    s->dev.light_geo = s->light_geo.as<float4>();
+    s->dev.light_shade = s->light_shade.as<float4>();
+    s->dev.light_info = s->light_info.as<LightInfo>();
+    s->generic = !s->sphs.empty() || !s->plas.empty();
+    s->dev.acc[MRT_CAT_PLANE].count = static_cast<uint32_t>(s->plas.size());
+    s->dev.acc[MRT_CAT_SPHERE].count = static_cast<uint32_t>(s->sphs.size());
+    s->dev.acc[MRT_CAT_TRIANGLE].count = static_cast<uint32_t>(s->tris.size());
+    for (int c = 0; c < 3; ++c) s->dev.acc[c].kind = MRT_NAIVE;
+    *out = s.release();
+    return MRT_OK;
+}
+
+MRT_API int mrt_scene_destroy(mrt_scene* scene) {
+    if (!scene) return MRT_OK;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+    delete scene;
+    return MRT_OK;
+}
+
+MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_params* params) {
+    if (!s) return fail(MRT_E_INVALID, "null scene");
+    if (accelerator != MRT_NAIVE && accelerator != MRT_REGULAR_GRID && accelerator != MRT_BVH)
+        return fail(MRT_E_INVALID, "unknown accelerator");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    const uint32_t leaf = params && params->leaf_size ? params->leaf_size : 2u;
+    if (leaf > 16) return fail(MRT_E_UNSUPPORTED, "leaf size above 16");
+    uint32_t lightDepth = s->dev.lights_linear ? 0u : s->dev.light_acc.depth;
+    s->depth_need = lightDepth;
+    const uint32_t counts[3] = {static_cast<uint32_t>(s->plas.size()), static_cast<uint32_t>(s->sphs.size()),
+                                static_cast<uint32_t>(s->tris.size())};
+    int rc;
+    for (int c = 0; c < 3; ++c) {
+        Accel& a = s->dev.acc[c];
+        const Accel keep_count{};
+        a = keep_count;
+        a.count = counts[c];
+        a.kind = accelerator;
+    }
+    if (accelerator == MRT_BVH) {
+        for (int c = 0; c < 3; ++c) {
+            std::vector<Box> boxes(counts[c]);
+            for (uint32_t i = 0; i < counts[c]; ++i)
+                boxes[i] = c == MRT_CAT_TRIANGLE ? triangle_box(s->tris[i])
+                           : c == MRT_CAT_SPHERE ? sphere_box(s->sphs[i])
+                                                 : plane_box(s->plas[i]);
+            BvhOut b;
+            build_bvh_reference(boxes, leaf, &b);
+            const uint64_t need = buffers_bytes(s) + b.pairs.size() * sizeof(BNode);
+            if (s->ctx->budget && need > s->ctx->budget)
+                return fail(MRT_E_BUDGET, "BVH exceeds the HBM budget");
+            if ((rc = upload_bvh(s, c, b, st)) != MRT_OK) return rc;
+            fill_accel_bvh(&s->dev.acc[c], b, s->nodes[c], nullptr, counts[c]);
+            s->depth_need = std::max(s->depth_need, b.depth);
+        }
+    } else if (accelerator == MRT_REGULAR_GRID) {
+        const f3 camMin = params ? ld3(params->camera_min) : f3{0, 0, 0};
+        const f3 camMax = params ? ld3(params->camera_max) : f3{0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+            auto box_of = [&](uint32_t i) {
+                return c == MRT_CAT_TRIANGLE ? triangle_box(s->tris[i])
+                       : c == MRT_CAT_SPHERE ? sphere_box(s->sphs[i])
+                                             : plane_box(s->plas[i]);
+            };
+            // Scene::getBounds over the primitives and the camera, widened by
+            // Epsilon (Shader.cpp:45-62).
+            f3 mn{kRayLengthMax, kRayLengthMax, kRayLengthMax}, mx{-kRayLengthMax, -kRayLengthMax, -kRayLengthMax};
+            for (uint32_t i = 0; i < counts[c]; ++i) {
+                const Box b = box_of(i);
+                mn = vmin(b.mn, mn);
+                mx = vmax(b.mx, mx);
+            }
+            mn = vmin(camMin, mn);
+            mx = vmax(camMax, mx);
+            const Box bounds{subs(mn, kEpsilon), adds(mx, kEpsilon)};
+            const int32_t gs = grid_size_for(counts[c]);
+            GridOut g;
+            auto hits = [&](uint32_t i, const Box& cell) {
+                return c == MRT_CAT_TRIANGLE ? triangle_intersect_box(s->tris[i], cell)
+                       : c == MRT_CAT_SPHERE ? sphere_intersect_box(s->sphs[i], cell)
+                                             : plane_intersect_box(s->plas[i], cell);
+            };
+            build_grid(counts[c], bounds, gs, box_of, hits, &g);
+            const uint64_t need = buffers_bytes(s) + (g.offsets.size() + g.refs.size()) * 4ull;
+            if (s->ctx->budget && need > s->ctx->budget)
+                return fail(MRT_E_BUDGET, "grid exceeds the HBM budget");
+            if ((rc = s->cell_off[c].upload(g.offsets.data(), g.offsets.size() * 4, st)) != MRT_OK) return rc;
+            if ((rc = s->cell_ref[c].upload(g.refs.data(), g.refs.size() * 4, st)) != MRT_OK) return rc;
+            Accel& a = s->dev.acc[c];
+            a.cell_off = s->cell_off[c].as<uint32_t>();
+            a.cell_ref = s->cell_ref[c].as<uint32_t>();
+            a.grid_size = g.grid_size;
+            a.grid_shift = g.grid_shift;
+            st3(a.ext_mn, g.ext.mn);
+            st3(a.sr, g.sr);
+            st3(a.cw, g.cw);
+        }
+    }
+    MRT_HIP(hipStreamSynchronize(st));
+    s->built = true;
+    return MRT_OK;
+}
+
+MRT_API int mrt_scene_footprint(const mrt_scene* s, uint64_t* bytes) {
+    if (!s || !bytes) return fail(MRT_E_INVALID, "null argument");
+    *bytes = buffers_bytes(s);
+    return MRT_OK;
+}
+
+MRT_API int mrt_trace_closest(mrt_scene* s, const mrt_ray* rays, uint32_t n, mrt_hit* hits, int mem) {
+    if (!s || (n && (!rays || !hits))) return fail(MRT_E_INVALID, "null argument");
+    if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
+    if (n == 0) return MRT_OK;
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    int rc;
+    const void* din;
+    if ((rc = stage(s->stage_in, rays, n * sizeof(mrt_ray), mem, st, &din)) != MRT_OK) return rc;
+    void* dout = hits;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = s->stage_out.alloc(n * sizeof(mrt_hit))) != MRT_OK) return rc;
+        dout = s->stage_out.p;
+    }
+    if ((rc = prepare_aux(s)) != MRT_OK) return rc;
+    const uint32_t blocks = std::min<uint32_t>(cap_blocks(s->ctx), (n + 255u) / 256u);
+    StackCfg sc;
+    if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
+    MRT_HIP(launch_trace_closest(s->dev, s->generic, static_cast<const mrt_ray*>(din), n,
+                                 static_cast<mrt_hit*>(dout), sc, blocks, false,
+                                 s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), st));
+    if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(hits, dout, n * sizeof(mrt_hit), hipMemcpyDeviceToHost, st));
+    return check_err(s);
+}
+
+MRT_API int mrt_trace_any(mrt_scene* s, const mrt_ray* rays, uint32_t n, uint8_t* occluded, int mem) {
+    if (!s || (n && (!rays || !occluded))) return fail(MRT_E_INVALID, "null argument");
+    if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
+    if (n == 0) return MRT_OK;
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    int rc;
+    const void* din;
+    if ((rc = stage(s->stage_in, rays, n * sizeof(mrt_ray), mem, st, &din)) != MRT_OK) return rc;
+    void* dout = occluded;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = s->stage_out.alloc(n)) != MRT_OK) return rc;
+        dout = s->stage_out.p;
+    }
+    if ((rc = prepare_aux(s)) != MRT_OK) return rc;
+    const uint32_t blocks = std::min<uint32_t>(cap_blocks(s->ctx), (n + 255u) / 256u);
+    StackCfg sc;
+    if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
+    MRT_HIP(launch_trace_any(s->dev, s->generic, static_cast<const mrt_ray*>(din), n, static_cast<uint8_t*>(dout),
+                             sc, blocks, s->err.as<uint32_t>(), st));
+    if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(occluded, dout, n, hipMemcpyDeviceToHost, st));
+    return check_err(s);
+}
+
+MRT_API int mrt_shade_rays(mrt_scene* s, const mrt_render_desc* d, const mrt_ray* rays, const int32_t* depths,
+                           uint32_t n, uint32_t sample, float* rgb, uint8_t* hit_light, int mem) {
+    if (!s || (n && (!rays || !rgb || !hit_light))) return fail(MRT_E_INVALID, "null argument");
+    if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
+    int rc;
+    if ((rc = check_desc(s, d)) != MRT_OK) return rc;
+    if (d->sampler_pixel.kind >= MRT_SAMPLER_HALTON_COMPAT)
+        return fail(MRT_E_UNSUPPORTED, "compat pixel samplers apply to mrt_render only");
+    if (n == 0) return MRT_OK;
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    const void *din, *ddep = nullptr;
+    if ((rc = stage(s->stage_in, rays, n * sizeof(mrt_ray), mem, st, &din)) != MRT_OK) return rc;
+    if (depths && (rc = stage(s->stage_aux, depths, n * sizeof(int32_t), mem, st, &ddep)) != MRT_OK) return rc;
+    float* drgb = rgb;
+    uint8_t* dflag = hit_light;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = s->stage_out.alloc(n * (3 * sizeof(float) + 1))) != MRT_OK) return rc;
+        drgb = s->stage_out.as<float>();
+        dflag = reinterpret_cast<uint8_t*>(drgb + 3ull * n);
+    }
+    if ((rc = prepare_aux(s)) != MRT_OK) return rc;
+    const RenderParams p = params_from(*d);
+    const uint32_t blocks = std::min<uint32_t>(cap_blocks(s->ctx), (n + 255u) / 256u);
+    StackCfg sc;
+    if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
+    MRT_HIP(launch_shade_rays(s->dev, s->generic, s->dev.has_branches != 0u, p, static_cast<const mrt_ray*>(din),
+                              static_cast<const int32_t*>(ddep), n, sample, drgb, dflag, sc, blocks,
+                              s->err.as<uint32_t>(), st));
+    if (mem != MRT_MEM_DEVICE) {
+        MRT_HIP(hipMemcpyAsync(rgb, drgb, n * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
+        MRT_HIP(hipMemcpyAsync(hit_light, dflag, n, hipMemcpyDeviceToHost, st));
+    }
+    return check_err(s);
+}
+
+MRT_API int mrt_camera_rays(mrt_ctx* ctx, const mrt_camera* cam, const float* uvdd, uint32_t n, mrt_ray* rays,
+                            int mem) {
+    if (!ctx || !cam || (n && (!uvdd || !rays))) return fail(MRT_E_INVALID, "null argument");
+    if (n == 0) return MRT_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    DevBuf in, out;
+    const void* din = uvdd;
+    void* dout = rays;
+    int rc;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = in.upload(uvdd, n * 4 * sizeof(float), ctx->stream)) != MRT_OK) return rc;
+        if ((rc = out.alloc(n * sizeof(mrt_ray))) != MRT_OK) return rc;
+        din = in.p;
+        dout = out.p;
+    }
+    MRT_HIP(launch_camera_rays(*cam, static_cast<const float*>(din), n, static_cast<mrt_ray*>(dout), ctx->stream));
+    if (mem != MRT_MEM_DEVICE)
+        MRT_HIP(hipMemcpyAsync(rays, dout, n * sizeof(mrt_ray), hipMemcpyDeviceToHost, ctx->stream));
+    MRT_HIP(hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
+MRT_API int mrt_intersect_pairs(mrt_ctx* ctx, int category, const void* prims, const mrt_ray* rays, uint32_t n,
+                                mrt_hit* closest, uint8_t* any, int mem) {
+    if (!ctx || (n && (!prims || !rays))) return fail(MRT_E_INVALID, "null argument");
+    if (category < MRT_CAT_PLANE || category > MRT_CAT_LIGHT) return fail(MRT_E_INVALID, "category");
+    if (n == 0) return MRT_OK;
+    const size_t psize = category == MRT_CAT_PLANE    ? sizeof(mrt_plane)
+                         : category == MRT_CAT_SPHERE ? sizeof(mrt_sphere)
+                                                      : sizeof(mrt_triangle);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    DevBuf bp, br, bc, ba;
+    const void* dp = prims;
+    const void* dr = rays;
+    mrt_hit* dc = closest;
+    uint8_t* da = any;
+    int rc;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = bp.upload(prims, n * psize, ctx->stream)) != MRT_OK) return rc;
+        if ((rc = br.upload(rays, n * sizeof(mrt_ray), ctx->stream)) != MRT_OK) return rc;
+        dp = bp.p;
+        dr = br.p;
+        if (closest) {
+            if ((rc = bc.alloc(n * sizeof(mrt_hit))) != MRT_OK) return rc;
+            dc = bc.as<mrt_hit>();
+        }
+        if (any) {
+            if ((rc = ba.alloc(n)) != MRT_OK) return rc;
+            da = ba.as<uint8_t>();
+        }
+    }
+    MRT_HIP(launch_intersect_pairs(category, dp, static_cast<const mrt_ray*>(dr), n, dc, da, ctx->stream));
+    if (mem != MRT_MEM_DEVICE) {
+        if (closest) MRT_HIP(hipMemcpyAsync(closest, dc, n * sizeof(mrt_hit), hipMemcpyDeviceToHost, ctx->stream));
+        if (any) MRT_HIP(hipMemcpyAsync(any, da, n, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    MRT_HIP(hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
+MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap, uint32_t stride_bytes, float* accum,
+                       int mem, mrt_render_stats* stats) {
+    if (!s || !bitmap) return fail(MRT_E_INVALID, "null argument");
+    if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
+    int rc;
+    if ((rc = check_desc(s, d)) != MRT_OK) return rc;
+    const uint32_t w = d->width, h = d->height;
+    // Renderer ctor tile sizes, Renderer.cpp:12-29.
+    if (w < 16 || h < 16) return fail(MRT_E_INVALID, "width and height must be at least 16");
+    if (w % 16u != 0u)
+        return fail(MRT_E_UNSUPPORTED, "width must be a multiple of 16 (the Linux front end rounds it, "
+                                       "GTK_Interface.cpp:22-30)");
+    const uint32_t bsx = w / 16u, bsy = h / 16u;
+    if ((w / bsx) * (h / bsy) != kBlocks)
+        return fail(MRT_E_UNSUPPORTED, "tile domain (w/bsx)*(h/bsy) must be 256");
+    const uint32_t realWidth = stride_bytes / 4u;
+    if (stride_bytes % 4u != 0u || realWidth < w || realWidth % w != 0u)
+        return fail(MRT_E_UNSUPPORTED, "stride must be a multiple of width * 4 bytes");
+    // Tile claim order from the camera's block table (Renderer.cpp:79-88):
+    // roundf(block * 256) must visit every tile once.
+    std::vector<uint32_t> rank(kBlocks, 0xFFFFFFFFu);
+    for (uint32_t t = 0; t < kBlocks; ++t) {
+        const float rb = std::roundf(d->tile_values[t] * static_cast<float>(kBlocks));
+        if (!(rb >= 0.0f && rb < 256.0f)) return fail(MRT_E_UNSUPPORTED, "tile value out of range");
+        const uint32_t r = static_cast<uint32_t>(rb);
+        if (rank[r] != 0xFFFFFFFFu) return fail(MRT_E_UNSUPPORTED, "tile values do not form a permutation");
+        rank[r] = t;
+    }
+    const bool compat = d->sampler_pixel.kind == MRT_SAMPLER_HALTON_COMPAT ||
+                        d->sampler_pixel.kind == MRT_SAMPLER_STRATIFIED_COMPAT;
+    if (compat && d->first_sample != 0) return fail(MRT_E_UNSUPPORTED, "compat samplers render whole frames only");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    RenderParams p = params_from(*d);
+    p.bsx = bsx;
+    p.bsy = bsy;
+    p.real_width = realWidth;
+    p.inv_w = 1.0f / static_cast<float>(w);
+    p.inv_h = 1.0f / static_cast<float>(h);
+    p.pix_w = 0.5f / static_cast<float>(w);
+    p.pix_h = 0.5f / static_cast<float>(h);
+    p.compat = compat ? 1u : 0u;
+    p.pixels_per_pass = kBlocks * bsx * bsy;
+    if ((rc = s->tile_rank.upload(rank.data(), rank.size() * sizeof(uint32_t), st)) != MRT_OK) return rc;
+    p.tile_rank = s->tile_rank.as<uint32_t>();
+    const size_t bmBytes = static_cast<size_t>(h) * realWidth * sizeof(uint32_t);
+    const size_t accBytes = static_cast<size_t>(w) * h * 3 * sizeof(float);
+    uint32_t* dbm = bitmap;
+    float* dacc = accum;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = s->stage_in.upload(bitmap, bmBytes, st)) != MRT_OK) return rc;
+        dbm = s->stage_in.as<uint32_t>();
+        if (accum) {
+            if ((rc = s->stage_out.upload(accum, accBytes, st)) != MRT_OK) return rc;
+            dacc = s->stage_out.as<float>();
+        }
+    }
+    if ((rc = prepare_aux(s)) != MRT_OK) return rc;
+    const uint32_t blocks = render_pass_blocks(p);
+    StackCfg sc;
+    if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
+    s->stop.store(false);
+    s->progress.store(d->first_sample);
+    hipEvent_t e0, e1;
+    MRT_HIP(hipEventCreate(&e0));
+    MRT_HIP(hipEventCreate(&e1));
+    MRT_HIP(hipEventRecord(e0, st));
+    bool stopped = false;
+    for (uint32_t sample = d->first_sample; sample < d->samples_pixel; ++sample) {
+        if (s->stop.load()) {
+            stopped = true;
+            break;
+        }
+        MRT_HIP(launch_render_pass(s->dev, s->generic, s->dev.has_branches != 0u, d->instrument != 0u, p, sample,
+                                   dbm, dacc, sc, s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), st));
+        MRT_HIP(hipStreamSynchronize(st));
+        s->progress.store(sample + 1);
+    }
+    MRT_HIP(hipEventRecord(e1, st));
+    MRT_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    MRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (mem != MRT_MEM_DEVICE) {
+        MRT_HIP(hipMemcpyAsync(bitmap, dbm, bmBytes, hipMemcpyDeviceToHost, st));
+        if (accum) MRT_HIP(hipMemcpyAsync(accum, dacc, accBytes, hipMemcpyDeviceToHost, st));
+    }
+    unsigned long long cnt[8] = {0};
+    MRT_HIP(hipMemcpyAsync(cnt, s->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    if ((rc = check_err(s)) != MRT_OK) return rc;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->rays_camera = cnt[0];
+        stats->rays_extension = cnt[1];
+        stats->rays_shadow = cnt[2];
+        stats->rays_specular = cnt[3];
+        stats->node_visits = cnt[4];
+        stats->prim_tests = cnt[5];
+        stats->samples_done = s->progress.load();
+        stats->covered_pixels = 256u * bsx * bsy;
+        stats->ms_render = ms;
+        stats->ms_kernel = ms;
+    }
+    if (stopped) return fail(MRT_E_STOPPED, "render stopped");
+    return MRT_OK;
+}
+
+MRT_API int mrt_render_stop(mrt_scene* s) {
+    if (!s) return fail(MRT_E_INVALID, "null scene");
+    s->stop.store(true);
+    return MRT_OK;
+}
+
+MRT_API int mrt_render_progress(const mrt_scene* s, uint32_t* samples_done) {
+    if (!s || !samples_done) return fail(MRT_E_INVALID, "null argument");
+    *samples_done = s->progress.load();
+    return MRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+// Host-side builders: the reference BVH builder, the conversion to 64-byte
+// child pairs, the light BVH, primitive boxes and box-overlap tests.
+#include "mrt_build.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+namespace mrt {
+
+namespace {
+
+// getSplitIndex_SAH, BVH.hpp:274-311.  right[i] holds the area of
+// boxes[i + L .. N-1] (the reference's indexing, one past the split point).
+uint32_t split_sah(const Box* bx, uint32_t N, uint32_t L, std::vector<float>& left, std::vector<float>& right) {
+    left.assign(N - L, 0.0f);
+    right.assign(N - L, 0.0f);
+    Box lb = bx[0];
+    left[0] = surface_area(lb);
+    for (uint32_t i = 1; i < N - L; ++i) {
+        lb = surrounding(lb, bx[i]);
+        left[i] = surface_area(lb);
+    }
+    Box rb = bx[N - 1];
+    right[N - L - 1] = surface_area(rb);
+    for (uint32_t i = N - 2; i > L; --i) {
+        rb = surrounding(rb, bx[i]);
+        right[i - L] = surface_area(rb);
+    }
+    uint32_t split = L;
+    float best = static_cast<float>(L) * left[L - 1] + static_cast<float>(N - L) * right[L - 1];
+    for (uint32_t i = L; i < N - L; ++i) {
+        const float sl = static_cast<float>(i + 1) * left[i];
+        const float sr = static_cast<float>(N - (i + 1)) * right[i];
+        const float sah = sl + sr;
+        if (sah < best) {
+            split = i + 1;
+            best = sah;
+        }
+    }
+    return split;
+}
+
+void to_pairs(BvhOut* out) {
+    const std::vector<RefNode>& ref = out->ref;
+    out->pairs.clear();
+    out->depth = 0;
+    out->root = ref[0].box;
+    if (ref[0].count > 0) {
+        out->root_ref = ref[0].offset;
+        out->root_count = ref[0].count;
+        return;
+    }
+    out->root_ref = 0;
+    out->root_count = 0;
+    // Interior reference node i becomes pair g[i], numbered in reference order.
+    std::vector<uint32_t> g(ref.size(), 0xFFFFFFFFu);
+    uint32_t k = 0;
+    for (size_t i = 0; i < ref.size(); ++i)
+        if (ref[i].count == 0) g[i] = k++;
+    out->pairs.resize(k);
+    for (size_t i = 0; i < ref.size(); ++i) {
+        if (ref[i].count != 0) continue;
+        const RefNode& L = ref[ref[i].offset];
+        const RefNode& R = ref[ref[i].offset + 1];
+        BNode& p = out->pairs[g[i]];
+        p.q0 = make_float4(L.box.mn.x, L.box.mn.y, L.box.mn.z, L.box.mx.x);
+        p.q1 = make_float4(L.box.mx.y, L.box.mx.z, R.box.mn.x, R.box.mn.y);
+        p.q2 = make_float4(R.box.mn.z, R.box.mx.x, R.box.mx.y, R.box.mx.z);
+        const uint32_t lref = L.count > 0 ? L.offset : g[ref[i].offset];
+        const uint32_t rref = R.count > 0 ? R.offset : g[ref[i].offset + 1];
+        p.q3 = make_uint4(lref, L.count, rref, R.count);
+    }
+    // Longest chain of interior pairs (bounds the traversal stack).
+    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 1u}};
+    while (!st.empty()) {
+        const auto [node, d] = st.back();
+        st.pop_back();
+        out->depth = std::max(out->depth, d);
+        const uint4 q = out->pairs[node].q3;
+        if (q.y == 0) st.push_back({q.x, d + 1});
+        if (q.w == 0) st.push_back({q.z, d + 1});
+    }
+}
+
+}  // namespace
+
+void build_bvh_reference(const std::vector<Box>& boxes, uint32_t leaf, BvhOut* out) {
+    out->ref.clear();
+    const uint32_t n = static_cast<uint32_t>(boxes.size());
+    if (n == 0) {
+        out->ref.push_back(RefNode{Box{f3{0, 0, 0}, f3{0, 0, 0}}, 0, 0});
+        out->pairs.clear();
+        out->root = out->ref[0].box;
+        out->root_ref = 0;
+        out->root_count = 0;
+        out->depth = 0;
+        return;
+    }
+    std::vector<RefNode>& nodes = out->ref;
+    nodes.assign(static_cast<size_t>(n) * 2 - 1, RefNode{Box{}, 0, 0});
+    std::vector<float> la, ra;
+    uint32_t id = 0, begin = 0, end = n, maxId = 0;
+    // Explicit stacks as in BVH.hpp:93-104; entry 0 is the sentinel.
+    std::vector<uint32_t> sId{0}, sBegin{0}, sEnd{0};
+    do {
+        RefNode& node = nodes[id];
+        node.box = boxes[begin];
+        for (uint32_t i = begin + 1; i < end; ++i) node.box = surrounding(boxes[i], node.box);
+        const uint32_t count = end - begin;
+        if (count <= leaf) {
+            node.offset = begin;
+            node.count = count;
+            id = sId.back();
+            sId.pop_back();
+            begin = sBegin.back();
+            sBegin.pop_back();
+            end = sEnd.back();
+            sEnd.pop_back();
+        } else {
+            const uint32_t left = maxId + 1, right = left + 1;
+            node.offset = left;
+            node.count = 0;
+            maxId = std::max(right, maxId);
+            const uint32_t split = count <= 2 * leaf ? leaf : split_sah(boxes.data() + begin, count, leaf, la, ra);
+            sId.push_back(right);
+            sBegin.push_back(begin + split);
+            sEnd.push_back(end);
+            id = left;
+            end = begin + split;
+        }
+    } while (!sId.empty());
+    nodes.resize(static_cast<size_t>(maxId) + 1);
+    to_pairs(out);
+}
+
+Box triangle_box(const mrt_triangle& t) {
+    const f3 A = ld3(t.a);
+    const f3 B = add(A, ld3(t.ab));
+    const f3 C = add(A, ld3(t.ac));
+    return Box{vmin(A, vmin(B, C)), vmax(A, vmax(B, C))};
+}
+
+Box sphere_box(const mrt_sphere& s) {
+    const float r = std::sqrt(s.sq_radius);
+    const f3 c = ld3(s.center);
+    return Box{subs(c, r), adds(c, r)};
+}
+
+Box plane_box(const mrt_plane& p) {
+    const f3 n = ld3(p.normal);
+    f3 right{0.0f, 0.0f, 0.0f};
+    if (n.x >= 1.0f) right = f3{0, 1, 1};
+    else if (n.y >= 1.0f) right = f3{1, 0, 1};
+    else if (n.z >= 1.0f) right = f3{1, 1, 0};
+    else if (n.x <= -1.0f) right = f3{0, 1, 1};
+    else if (n.y <= -1.0f) right = f3{1, 0, 1};
+    else if (n.z <= -1.0f) right = f3{1, 1, 0};
+    right = normalize(right);
+    const f3 pt = ld3(p.point);
+    return Box{add(pt, muls(right, -100.0f)), add(pt, muls(right, 100.0f))};
+}
+
+bool triangle_intersect_box(const mrt_triangle& t, const Box& box) {
+    const f3 A = ld3(t.a), AB = ld3(t.ab), AC = ld3(t.ac);
+    auto rayBox = [&](f3 orig, f3 vec) -> bool {
+        float tNear = std::numeric_limits<float>::min();
+        float tFar = std::numeric_limits<float>::max();
+        for (int a = 0; a < 3; ++a) {
+            const float o = comp(orig, a), v = comp(vec, a);
+            const float bmn = comp(box.mn, a), bmx = comp(box.mx, a);
+            if (std::fabs(v) < kEpsilon) {
+                if ((o < bmn) || ((o + v) > bmx)) return false;
+            } else {
+                float t1 = (bmn - o) / v;
+                float t2 = (bmx - o) / v;
+                if (t1 > t2) std::swap(t1, t2);
+                tNear = smax(t1, tNear);
+                tFar = smin(t2, tFar);
+                if ((tNear > tFar) || (tFar < 0.0f)) return false;
+            }
+        }
+        return true;
+    };
+    const f3 vec = sub(box.mx, box.mn);
+    const bool iAB = rayBox(A, AB);
+    const bool iAC = rayBox(A, AC);
+    const f3 B = add(A, AB);
+    const f3 C = add(A, AC);
+    const bool iBC = rayBox(B, sub(C, B));
+    // The box diagonal as a ray from min (Triangle.cpp:230, 236).
+    bool iRay = false;
+    {
+        const f3 d = vec;
+        const f3 perp = cross(d, AC);
+        const float det = dot(AB, perp);
+        if (!(std::fabs(det) < kEpsilon)) {
+            const float inv = 1.0f / det;
+            const f3 s = sub(box.mn, A);
+            const float u = inv * dot(s, perp);
+            if (!(u < 0.0f || u > 1.0f)) {
+                const f3 q = cross(s, AB);
+                const float v = inv * dot(d, q);
+                if (!(v < 0.0f || (u + v) > 1.0f)) {
+                    const float tt = inv * dot(AC, q);
+                    iRay = !(tt < kEpsilon || tt >= kRayLengthMax);
+                }
+            }
+        }
+    }
+    const f3 perp = cross(vec, AC);
+    const bool inside = std::fabs(dot(AB, perp)) < kEpsilon;
+    return iAB || iAC || iBC || iRay || inside;
+}
+
+bool sphere_intersect_box(const mrt_sphere& s, const Box& box) {
+    float dmin = 0.0f;
+    const f3 c = ld3(s.center);
+    for (int a = 0; a < 3; ++a) {
+        const float ca = comp(c, a), mn = comp(box.mn, a), mx = comp(box.mx, a);
+        if (ca < mn) dmin = dmin + (ca - mn) * (ca - mn);
+        else if (ca > mx) dmin = dmin + (ca - mx) * (ca - mx);
+    }
+    return dmin <= s.sq_radius;
+}
+
+// Plane::distance, Plane.cpp:106-121.
+static float plane_distance(const mrt_plane& p, f3 q) {
+    const f3 n = ld3(p.normal), pt = ld3(p.point);
+    const float d = (n.x * -pt.x + n.y * -pt.y) + n.z * -pt.z;
+    const float num = ((n.x * q.x + n.y * q.y) + n.z * q.z) + d;
+    const float den = std::sqrt((n.x * n.x + n.y * n.y) + n.z * n.z);
+    return num / den;
+}
+
+bool plane_intersect_box(const mrt_plane& p, const Box& box) {
+    const float dP = plane_distance(p, box.mx);
+    const float dN = plane_distance(p, box.mn);
+    return (dP <= 0.0f && dN >= 0.0f) || (dP >= 0.0f && dN <= 0.0f);
+}
+
+int32_t grid_size_for(uint32_t n) {
+    auto bits = [](uint32_t v) {
+        uint32_t c = 0;
+        while (v > 0) { ++c; v >>= 1; }
+        return c;
+    };
+    auto pow2 = [](uint32_t v) {
+        --v;
+        v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+        return v + 1;
+    };
+    const int32_t aux = n > 6 ? static_cast<int32_t>(pow2(bits(n))) : 2;
+    const int32_t aux2 = static_cast<int32_t>(pow2(bits(static_cast<uint32_t>(aux))) / 2);
+    return std::max(aux * aux2, aux);
+}
+
+void build_light_bvh(const std::vector<Box>& boxes, BvhOut* out, std::vector<uint32_t>* perm) {
+    const uint32_t n = static_cast<uint32_t>(boxes.size());
+    // Morton order of box centres, then the in-order builder with leaf size 2.
+    Box all = boxes.empty() ? Box{} : boxes[0];
+    for (const Box& b : boxes) all = surrounding(b, all);
+    const f3 ext = sub(all.mx, all.mn);
+    auto spread = [](uint32_t v) {
+        v &= 0x3FFu;
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    std::vector<uint32_t> code(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const f3 c = muls(add(boxes[i].mn, boxes[i].mx), 0.5f);
+        auto q = [](float v, float lo, float e) {
+            const float t = e > 0.0f ? (v - lo) / e : 0.0f;
+            return static_cast<uint32_t>(std::min(1023.0f, std::max(0.0f, t * 1023.0f)));
+        };
+        code[i] = (spread(q(c.x, all.mn.x, ext.x)) << 2) | (spread(q(c.y, all.mn.y, ext.y)) << 1) |
+                  spread(q(c.z, all.mn.z, ext.z));
+    }
+    perm->resize(n);
+    std::iota(perm->begin(), perm->end(), 0u);
+    std::stable_sort(perm->begin(), perm->end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+    std::vector<Box> sorted(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        Box b = boxes[(*perm)[i]];
+        // Widen by a relative margin so rounding in the slab test can never
+        // reject a point of the triangle.
+        const f3 e = sub(b.mx, b.mn);
+        const float m = 1e-4f * std::max(e.x, std::max(e.y, e.z)) + 1e-6f;
+        const f3 ma = f3{m + 1e-6f * std::fabs(b.mn.x), m + 1e-6f * std::fabs(b.mn.y), m + 1e-6f * std::fabs(b.mn.z)};
+        const f3 mb = f3{m + 1e-6f * std::fabs(b.mx.x), m + 1e-6f * std::fabs(b.mx.y), m + 1e-6f * std::fabs(b.mx.z)};
+        b.mn = sub(b.mn, ma);
+        b.mx = add(b.mx, mb);
+        sorted[i] = b;
+    }
+    build_bvh_reference(sorted, 2u, out);
+}
+
+}  // namespace mrt

@@ -1,0 +1,37 @@
+// Host-callable launchers of the gfx950 kernels (mrt_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mrt_scene.hpp"
+
+namespace mrt {
+
+struct RenderParams;
+
+// Global spill area of the traversal stacks: lane g of the launch owns
+// entries spill[g + k * spill_stride], k < cap_spill.
+struct StackCfg {
+    uint32_t* spill;
+    uint32_t spill_stride;
+    uint32_t cap_spill;
+};
+
+hipError_t launch_trace_closest(const SceneDev& s, bool generic, const mrt_ray* rays, uint32_t n, mrt_hit* hits,
+                                const StackCfg& sc, uint32_t cap_blocks, bool count, unsigned long long* counters,
+                                uint32_t* err, hipStream_t stream);
+hipError_t launch_trace_any(const SceneDev& s, bool generic, const mrt_ray* rays, uint32_t n, uint8_t* occ,
+                            const StackCfg& sc, uint32_t cap_blocks, uint32_t* err, hipStream_t stream);
+hipError_t launch_camera_rays(const mrt_camera& cam, const float* uvdd, uint32_t n, mrt_ray* out,
+                              hipStream_t stream);
+hipError_t launch_intersect_pairs(int cat, const void* prims, const mrt_ray* rays, uint32_t n, mrt_hit* closest,
+                                  uint8_t* any, hipStream_t stream);
+hipError_t launch_shade_rays(const SceneDev& s, bool generic, bool branches, const RenderParams& p,
+                             const mrt_ray* rays, const int32_t* depths, uint32_t n, uint32_t sample, float* rgb,
+                             uint8_t* flag, const StackCfg& sc, uint32_t cap_blocks, uint32_t* err,
+                             hipStream_t stream);
+uint32_t render_pass_blocks(const RenderParams& p);
+hipError_t launch_render_pass(const SceneDev& s, bool generic, bool branches, bool count, const RenderParams& p,
+                              uint32_t sample, uint32_t* bitmap, float* accum, const StackCfg& sc,
+                              unsigned long long* counters, uint32_t* err, hipStream_t stream);
+
+}  // namespace mrt
