@@ -37,8 +37,11 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_desc(mrt, abi, cam, spp=SPP, instrument=False):
-    return mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam,
+KERNELS = {"auto": 0, "megakernel": 1, "wavefront": 2}
+
+
+def make_desc(mrt, abi, cam, spp=SPP, instrument=False, kernel="auto"):
+    return mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam, kernel=KERNELS[kernel],
                            pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 5),
                            rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 6),
                            light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 7),
@@ -80,6 +83,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel", choices=sorted(KERNELS), default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -107,16 +111,22 @@ def main() -> int:
     torch.cuda.synchronize()
 
     # Instrumented frame (1 spp): node visits and primitive tests per ray.
-    _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True), bitmap=bitmap, accum=accum,
+    _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True, kernel=args.kernel), bitmap=bitmap,
+                         accum=accum,
                          mem=abi.MEM_DEVICE)
     rays1 = st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
     closest1 = st.rays_camera + st.rays_extension + st.rays_specular
     pairs_per_ray = st.node_visits / rays1
     prims_per_ray = st.prim_tests / rays1
+    shadow1 = max(1, st.rays_shadow)
     log(f"instrumented pass: {rays1} rays, {pairs_per_ray:.1f} node pairs and {prims_per_ray:.1f} "
-        f"primitive tests per ray")
+        f"primitive tests per ray; per closest-hit ray: "
+        f"{(st.node_visits - st.light_node_visits - st.shadow_node_visits) / closest1:.1f} pairs "
+        f"{(st.prim_tests - st.light_prim_tests - st.shadow_prim_tests) / closest1:.1f} tests, lights "
+        f"{st.light_node_visits / closest1:.1f} pairs {st.light_prim_tests / closest1:.1f} tests; per shadow ray: "
+        f"{st.shadow_node_visits / shadow1:.1f} pairs {st.shadow_prim_tests / shadow1:.1f} tests")
 
-    desc = make_desc(mrt, abi, cam)
+    desc = make_desc(mrt, abi, cam, kernel=args.kernel)
     for _ in range(args.warmup):
         scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
     torch.cuda.synchronize()
@@ -157,7 +167,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded 1M-triangle soup (10k emissive), deterministic per-pixel sampling",
-        "config": {"workload": "soup-1M 1920x1080 PathTracer 16spp depth6 RR3 Halton BVH megakernel",
+        "config": {"workload": f"soup-1M 1920x1080 PathTracer 16spp depth6 RR3 Halton BVH {args.kernel}",
                    "triangles": N_TRIS, "lights": N_LIGHTS, "width": W, "height": H, "spp": SPP,
                    "rays_per_step": rays // args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -201,6 +201,8 @@ typedef struct mrt_render_stats {
     uint64_t rays_specular;    /* reflection + transmission rays traced        */
     uint64_t node_visits;      /* interior node pairs fetched (instrumented)   */
     uint64_t prim_tests;       /* primitive tests (instrumented)               */
+    uint64_t light_node_visits, light_prim_tests;    /* of which: light BVH / scan  */
+    uint64_t shadow_node_visits, shadow_prim_tests;  /* of which: shadowTrace       */
     uint32_t samples_done;     /* completed sample passes                      */
     uint32_t covered_pixels;   /* pixels inside the 16x16 tile grid            */
     float ms_render;           /* device time of the render, milliseconds      */
@@ -290,6 +292,20 @@ MRT_API int mrt_camera_make(int kind, const float position[3], const float look_
 MRT_API int mrt_tile_values(uint32_t seed, float out[256]);
 /* Triangle::intersectBox (Triangle.cpp:154-246), used by the grid builder. */
 MRT_API int mrt_triangle_intersect_box(const mrt_triangle* tri, const float box_min[3], const float box_max[3]);
+/* getAABB of n primitives of one category (Triangle.cpp:145-152,
+ * Sphere.cpp:91-96, Plane.cpp:80-104): 6 floats (min xyz, max xyz) each. */
+MRT_API int mrt_primitive_aabbs(int category, const void* prims, uint32_t n, float* boxes6);
+/* BVHNode, app/MobileRT/Accelerators/BVH.hpp:21-25 (32 bytes). */
+typedef struct mrt_bvh_node {
+    float min[3], max[3];
+    uint32_t index_offset;
+    uint32_t number_primitives;
+} mrt_bvh_node;
+/* The reference BVH builder (BVH.hpp:64-155, 274-311) over primitive boxes in
+ * scene order; the device layout mrt_scene_build uploads is derived from this
+ * node array.  Writes up to `capacity` nodes and the node count. */
+MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, mrt_bvh_node* nodes,
+                          uint32_t capacity, uint32_t* count);
 
 /* ---- synthetic scenes ---------------------------------------------------- */
 /* Procedural Cornell box: 10 wall triangles + 24 box triangles, 2 area lights. */

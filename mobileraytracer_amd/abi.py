@@ -104,6 +104,8 @@ class RenderDesc(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("rays_camera", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
                 ("rays_specular", C.c_uint64), ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64),
+                ("light_node_visits", C.c_uint64), ("light_prim_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64),
                 ("samples_done", C.c_uint32), ("covered_pixels", C.c_uint32),
                 ("ms_render", C.c_float), ("ms_kernel", C.c_float)]
 
@@ -164,6 +166,8 @@ def _proto(lib):
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
         "mrt_tile_values": (C.c_int, [u32, P]),
         "mrt_triangle_intersect_box": (C.c_int, [P, P, P]),
+        "mrt_primitive_aabbs": (C.c_int, [C.c_int, P, u32, P]),
+        "mrt_bvh_build": (C.c_int, [P, u32, u32, P, u32, C.POINTER(u32)]),
         "mrt_scene_cornell": (C.c_int, [u32, P, P, P]),
         "mrt_scene_soup": (C.c_int, [u32, u32, u32, f32, f32, P, P, C.c_int]),
     }

@@ -13,6 +13,7 @@
 #include "mrt_build.hpp"
 #include "mrt_kernels.hpp"
 #include "mrt_path.hpp"
+#include "mrt_wavefront.hpp"
 
 namespace {
 
@@ -87,7 +88,7 @@ struct mrt_scene {
     uint32_t n_scene_mats = 0;
     DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
     DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3];
-    DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank;
+    DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wavefront;
     SceneDev dev{};
     bool built = false;
     bool generic = false;
@@ -199,9 +200,9 @@ int ensure_spill(mrt_scene* s, uint32_t lanes, StackCfg* sc) {
 
 int prepare_aux(mrt_scene* s) {
     int rc;
-    if ((rc = s->counters.alloc(8 * sizeof(unsigned long long))) != MRT_OK) return rc;
+    if ((rc = s->counters.alloc(16 * sizeof(unsigned long long))) != MRT_OK) return rc;
     if ((rc = s->err.alloc(sizeof(uint32_t))) != MRT_OK) return rc;
-    MRT_HIP(hipMemsetAsync(s->counters.p, 0, 8 * sizeof(unsigned long long), s->ctx->stream));
+    MRT_HIP(hipMemsetAsync(s->counters.p, 0, 16 * sizeof(unsigned long long), s->ctx->stream));
     MRT_HIP(hipMemsetAsync(s->err.p, 0, sizeof(uint32_t), s->ctx->stream));
     return MRT_OK;
 }
@@ -722,24 +723,69 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         }
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
-    const uint32_t blocks = render_pass_blocks(p);
+    // Kernel shape: the wavefront pipeline covers the PathTracer with one
+    // light sample and no reflection/transmission; everything else runs the
+    // megakernel.
+    const bool wfEligible = d->shader == MRT_SHADER_PATHTRACER && d->samples_light == 1 && !s->dev.has_branches;
+    bool useWavefront = false;
+    if (d->kernel == MRT_KERNEL_WAVEFRONT) {
+        if (!wfEligible)
+            return fail(MRT_E_UNSUPPORTED, "the wavefront kernels need the PathTracer, samples_light == 1 and no "
+                                           "reflective or transmissive materials");
+        useWavefront = true;
+    } else if (d->kernel == MRT_KERNEL_AUTO) {
+        useWavefront = wfEligible;
+    }
+    const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
+    const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
     StackCfg sc;
     if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
+    WfBuffers wb{};
+    if (useWavefront) {
+        const uint32_t slots = wavefront_slots(p);
+        if ((rc = s->wavefront.alloc(wavefront_bytes(slots))) != MRT_OK) return rc;
+        float4* base = s->wavefront.as<float4>();
+        wb.sA = base;
+        wb.sB = base + slots;
+        wb.sC = base + 2ull * slots;
+        wb.queue_a = base + 3ull * slots;
+        wb.queue_b = base + 5ull * slots;
+        wb.hits = base + 7ull * slots;
+        wb.shadow = base + 8ull * slots;
+        wb.counts = reinterpret_cast<uint32_t*>(base + 12ull * slots);
+    }
     s->stop.store(false);
     s->progress.store(d->first_sample);
-    hipEvent_t e0, e1;
+    // Events around the whole render and around every pass launch (the pass
+    // kernel is launched on this context's stream, so these events time it).
+    hipEvent_t e0, e1, k0, k1;
     MRT_HIP(hipEventCreate(&e0));
     MRT_HIP(hipEventCreate(&e1));
+    MRT_HIP(hipEventCreate(&k0));
+    MRT_HIP(hipEventCreate(&k1));
     MRT_HIP(hipEventRecord(e0, st));
     bool stopped = false;
+    float kernel_ms = 0.0f;
     for (uint32_t sample = d->first_sample; sample < d->samples_pixel; ++sample) {
         if (s->stop.load()) {
             stopped = true;
             break;
         }
-        MRT_HIP(launch_render_pass(s->dev, s->generic, s->dev.has_branches != 0u, d->instrument != 0u, p, sample,
-                                   dbm, dacc, sc, s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), st));
-        MRT_HIP(hipStreamSynchronize(st));
+        MRT_HIP(hipEventRecord(k0, st));
+        if (useWavefront) {
+            MRT_HIP(launch_wavefront_pass(s->dev, s->generic, d->instrument != 0u, p, sample, dbm, dacc, wb,
+                                          traceBlocks, sc, s->counters.as<unsigned long long>(),
+                                          s->err.as<uint32_t>(), st));
+        } else {
+            MRT_HIP(launch_render_pass(s->dev, s->generic, s->dev.has_branches != 0u, d->instrument != 0u, p,
+                                       sample, dbm, dacc, sc, s->counters.as<unsigned long long>(),
+                                       s->err.as<uint32_t>(), st));
+        }
+        MRT_HIP(hipEventRecord(k1, st));
+        MRT_HIP(hipEventSynchronize(k1));
+        float kms = 0.0f;
+        MRT_HIP(hipEventElapsedTime(&kms, k0, k1));
+        kernel_ms += kms;
         s->progress.store(sample + 1);
     }
     MRT_HIP(hipEventRecord(e1, st));
@@ -748,11 +794,13 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     MRT_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(k0);
+    (void)hipEventDestroy(k1);
     if (mem != MRT_MEM_DEVICE) {
         MRT_HIP(hipMemcpyAsync(bitmap, dbm, bmBytes, hipMemcpyDeviceToHost, st));
         if (accum) MRT_HIP(hipMemcpyAsync(accum, dacc, accBytes, hipMemcpyDeviceToHost, st));
     }
-    unsigned long long cnt[8] = {0};
+    unsigned long long cnt[16] = {0};
     MRT_HIP(hipMemcpyAsync(cnt, s->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
     if ((rc = check_err(s)) != MRT_OK) return rc;
     if (stats) {
@@ -761,12 +809,16 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->rays_extension = cnt[1];
         stats->rays_shadow = cnt[2];
         stats->rays_specular = cnt[3];
-        stats->node_visits = cnt[4];
-        stats->prim_tests = cnt[5];
+        stats->node_visits = cnt[4] + cnt[5] + cnt[6];
+        stats->prim_tests = cnt[7] + cnt[8] + cnt[9];
+        stats->light_node_visits = cnt[5];
+        stats->light_prim_tests = cnt[8];
+        stats->shadow_node_visits = cnt[6];
+        stats->shadow_prim_tests = cnt[9];
         stats->samples_done = s->progress.load();
         stats->covered_pixels = 256u * bsx * bsy;
         stats->ms_render = ms;
-        stats->ms_kernel = ms;
+        stats->ms_kernel = kernel_ms;
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;

@@ -1,4 +1,5 @@
-// gfx950 kernels of the ray-scene path and their host launchers.
+// gfx950 kernels of the ray-scene path (traversal entry points and the
+// megakernel renderer) and their host launchers.
 //
 // Launch shape: 256-thread workgroups (4 wave64s).  Each lane traces its own
 // ray; the per-lane traversal stack keeps its first kStackLds entries in LDS
@@ -7,49 +8,9 @@
 // Render passes map one wave to an 8x8 pixel block and renumber workgroups so
 // that each of the 8 XCDs (which receive workgroups round-robin) works on a
 // contiguous band of the image, keeping neighbouring rays in one L2.
-#include "mrt_kernels.hpp"
-
-#include "mrt_path.hpp"
+#include "mrt_device_util.hpp"
 
 namespace mrt {
-
-constexpr uint32_t kBlock = 256;
-constexpr uint32_t kStackLds = 16;
-
-__device__ __forceinline__ Stack make_stack(uint32_t* lds, const StackCfg& sc, uint32_t gtid) {
-    Stack st;
-    st.lds = lds + threadIdx.x;
-    st.lds_stride = kBlock;
-    st.spill = sc.spill ? sc.spill + gtid : nullptr;
-    st.spill_stride = sc.spill_stride;
-    st.cap_lds = kStackLds;
-    st.cap_total = kStackLds + (sc.spill ? sc.cap_spill : 0u);
-    st.sp = 0;
-    st.overflow = 0;
-    return st;
-}
-
-// Sum of a lane value over the wave, returned on every lane.
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ void flush_counts(const RayCounts& rc, unsigned long long* counters, uint32_t overflow,
-                                             uint32_t* err) {
-    const uint32_t c0 = wave_sum(rc.camera), c1 = wave_sum(rc.extension), c2 = wave_sum(rc.shadow),
-                   c3 = wave_sum(rc.specular), c4 = wave_sum(rc.ts.pairs), c5 = wave_sum(rc.ts.prims),
-                   of = wave_sum(overflow);
-    if ((threadIdx.x & 63u) == 0u) {
-        if (c0) atomicAdd(counters + 0, static_cast<unsigned long long>(c0));
-        if (c1) atomicAdd(counters + 1, static_cast<unsigned long long>(c1));
-        if (c2) atomicAdd(counters + 2, static_cast<unsigned long long>(c2));
-        if (c3) atomicAdd(counters + 3, static_cast<unsigned long long>(c3));
-        if (c4) atomicAdd(counters + 4, static_cast<unsigned long long>(c4));
-        if (c5) atomicAdd(counters + 5, static_cast<unsigned long long>(c5));
-        if (of) atomicOr(err, 1u);
-    }
-}
 
 // ------------------------------------------------------------- trace --
 template <bool GENERIC, bool COUNT>
@@ -68,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneDev s, const mrt_
         reinterpret_cast<float4*>(hits)[i] =
             make_float4(h.id == MRT_ID_NONE ? 0.0f : h.t, __uint_as_float(h.id), h.u, h.v);
     }
-    if (COUNT) flush_counts(rc, counters, st.overflow, err);
+    if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
     else if (st.overflow) atomicOr(err, 1u);
 }
 
@@ -173,26 +134,6 @@ __global__ __launch_bounds__(kBlock) void k_shade_rays(SceneDev s, RenderParams 
 }
 
 // ------------------------------------------------------------- render --
-// incrementalAvg, app/MobileRT/Utils.cpp:48-71.
-__device__ __forceinline__ uint32_t incremental_avg(f3 c, uint32_t avg, uint32_t n) {
-    const uint32_t lr = avg & 0xFFu, lg = (avg >> 8u) & 0xFFu, lb = (avg >> 16u) & 0xFFu;
-    const uint32_t sr = static_cast<uint32_t>(c.x * 255.0f);
-    const uint32_t sg = static_cast<uint32_t>(c.y * 255.0f);
-    const uint32_t sb = static_cast<uint32_t>(c.z * 255.0f);
-    const uint32_t cr = ((n - 1u) * lr + sr) / n;
-    const uint32_t cg = ((n - 1u) * lg + sg) / n;
-    const uint32_t cb = ((n - 1u) * lb + sb) / n;
-    return 0xFF000000u | (min(cb, 255u) << 16u) | (min(cg, 255u) << 8u) | min(cr, 255u);
-}
-
-// Workgroup renumbering: XCD k (workgroups b with b % 8 == k) gets a
-// contiguous range of logical workgroups.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
-    const uint32_t xcd = b & 7u, idx = b >> 3;
-    const uint32_t full = nb >> 3, rem = nb & 7u;
-    return xcd < rem ? xcd * (full + 1u) + idx : rem * (full + 1u) + (xcd - rem) * full + idx;
-}
-
 // One sample pass of Renderer::renderScene (Renderer.cpp:67-116) over the
 // covered region: pixel sampler draws, Camera::generateRay, Shader::rayTrace,
 // incrementalAvg into the bitmap and the FP32 radiance into accum.
@@ -202,41 +143,16 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(SceneDev s, RenderParams
                                                         StackCfg sc, unsigned long long* counters, uint32_t* err) {
     __shared__ uint32_t lds[kStackLds * kBlock];
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = logical * (kBlock / 64u) + (threadIdx.x >> 6);
-    const uint32_t covW = 16u * p.bsx, covH = 16u * p.bsy;
-    const uint32_t tilesX = (covW + 7u) >> 3;
-    const uint32_t x = (wave % tilesX) * 8u + (lane & 7u);
-    const uint32_t y = (wave / tilesX) * 8u + (lane >> 3);
+    uint32_t x, y;
+    slot_pixel(p, logical * kBlock + threadIdx.x, &x, &y);
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     Stack st = make_stack(lds, sc, gtid);
     RayCounts rc{};
     uint32_t pendOverflow = 0;
-    if (x < covW && y < covH) {
+    if (x < 16u * p.bsx && y < 16u * p.bsy) {
         const uint32_t pid = y * p.width + x;
-        float r1, r2;
-        if (p.compat) {
-            // HaltonSeq/Stratified::getSample(0) with the frame-wide counter in
-            // single-thread claim order (HaltonSeq.cpp:14-22, Stratified.cpp:14-23).
-            const uint32_t tx = x / p.bsx, ty = y / p.bsy;
-            const uint32_t rank = p.tile_rank[ty * 16u + tx];
-            const uint32_t ord = sample * p.pixels_per_pass + rank * p.bsx * p.bsy + (y - ty * p.bsy) * p.bsx +
-                                 (x - tx * p.bsx);
-            const uint32_t c0 = 2u * ord, c1 = c0 + 1u;
-            const uint32_t D = p.s_pixel.domain_size;
-            const bool halton_kind = p.s_pixel.kind == MRT_SAMPLER_HALTON_COMPAT;
-            r1 = c0 < D ? (halton_kind ? halton(c0, 2u) : static_cast<float>(c0) / static_cast<float>(D)) : 1.0f;
-            r2 = c1 < D ? (halton_kind ? halton(c1, 2u) : static_cast<float>(c1) / static_cast<float>(D)) : 1.0f;
-        } else {
-            r1 = draw(p.s_pixel, pid, sample, draw_dim(kSlotJitterU, 0u, 1u, 0u), p.spp);
-            r2 = draw(p.s_pixel, pid, sample, draw_dim(kSlotJitterV, 0u, 1u, 0u), p.spp);
-        }
-        const float u = static_cast<float>(x) * p.inv_w;
-        const float v = static_cast<float>(y) * p.inv_h;
-        const float du = ((r1 - 0.5f) * 2.0f) * p.pix_w;
-        const float dv = ((r2 - 0.5f) * 2.0f) * p.pix_h;
         f3 o, d;
-        camera_ray(p.cam, u, v, du, dv, &o, &d);
+        primary_ray(p, x, y, pid, sample, &o, &d);
         bool hl = false;
         const f3 c = trace_path<GENERIC, BRANCHES, COUNT>(s, p, o, d, MRT_ID_NONE, 1, pid, sample, &hl, st, rc, 0,
                                                           &pendOverflow);
@@ -248,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(SceneDev s, RenderParams
             accum[3u * pid + 2u] += c.z;
         }
     }
-    flush_counts(rc, counters, st.overflow | (pendOverflow << 1), err);
+    flush_counts<COUNT>(rc, counters, st.overflow | (pendOverflow << 1), err);
 }
 
 // ------------------------------------------------------------- launchers --

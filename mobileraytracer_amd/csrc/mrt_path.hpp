@@ -158,41 +158,77 @@ __device__ __forceinline__ f3 cosine_hemisphere(const RenderParams& p, f3 n, con
 // The direct-light loop of Whitted.cpp:53-82 / PathTracer.cpp:71-103 with
 // Shader::getLight (Shader.cpp:225-236) and Light::getPosition
 // (AreaLight.cpp:22-31, PointLight.cpp:17-19).
+// One light sample i of the direct-light loop: chooses the light, samples a
+// point on it and returns true with the shadow-ray direction, distance and
+// Le * cos when the light is above the surface (cos > 0).
+__device__ __forceinline__ bool light_sample(const SceneDev& s, const RenderParams& p, const Surf& sf, int32_t depth,
+                                             const DrawCtx& c, uint32_t i, f3* dir, float* dist, f3* lc) {
+    const float rl = draw_at(p, p.s_light, c, kSlotLight, i, depth);
+    const uint32_t li = static_cast<uint32_t>(floorf((rl * static_cast<float>(s.n_lights)) * 0.99999f));
+    const LightInfo info = s.light_info[li];
+    f3 lp;
+    if (info.kind == MRT_LIGHT_AREA) {
+        float R = draw_at(p, info.sampler, c, kSlotLightR, i, depth);
+        float S = draw_at(p, info.sampler, c, kSlotLightS, i, depth);
+        if (R + S >= 1.0f) {
+            R = 1.0f - R;
+            S = 1.0f - S;
+        }
+        const TriG g = load_tri(s.light_geo, li);
+        lp = add(add(g.a, muls(g.ab, R)), muls(g.ac, S));
+    } else {
+        lp = f3{info.pos[0], info.pos[1], info.pos[2]};
+    }
+    f3 toLight = sub(lp, sf.point);
+    *dist = length(toLight);
+    toLight = normalize(toLight);
+    const float cosNL = dot(sf.normal, toLight);
+    if (!(cosNL > 0.0f)) return false;
+    *dir = toLight;
+    *lc = muls(load_mat(s.mat, info.material).le, cosNL);
+    return true;
+}
+
 template <bool GENERIC, bool COUNT>
 __device__ f3 direct_light(const SceneDev& s, const RenderParams& p, const Surf& sf, int32_t depth,
                            const DrawCtx& c, Stack& st, RayCounts& rc) {
     f3 L = f3{0.0f, 0.0f, 0.0f};
     for (uint32_t i = 0; i < p.samples_light; ++i) {
-        const float rl = draw_at(p, p.s_light, c, kSlotLight, i, depth);
-        const uint32_t li = static_cast<uint32_t>(floorf((rl * static_cast<float>(s.n_lights)) * 0.99999f));
-        const LightInfo info = s.light_info[li];
-        f3 lp;
-        if (info.kind == MRT_LIGHT_AREA) {
-            float R = draw_at(p, info.sampler, c, kSlotLightR, i, depth);
-            float S = draw_at(p, info.sampler, c, kSlotLightS, i, depth);
-            if (R + S >= 1.0f) {
-                R = 1.0f - R;
-                S = 1.0f - S;
-            }
-            const TriG g = load_tri(s.light_geo, li);
-            lp = add(add(g.a, muls(g.ab, R)), muls(g.ac, S));
-        } else {
-            lp = f3{info.pos[0], info.pos[1], info.pos[2]};
-        }
-        f3 toLight = sub(lp, sf.point);
-        const float dist = length(toLight);
-        toLight = normalize(toLight);
-        const float cosNL = dot(sf.normal, toLight);
-        if (cosNL > 0.0f) {
+        f3 dir, lc;
+        float dist;
+        if (light_sample(s, p, sf, depth, c, i, &dir, &dist, &lc)) {
             ++rc.shadow;
-            const RayT sr = make_ray(sf.point, toLight, sf.excl);
-            if (!any_hit<GENERIC, COUNT>(s, sr, dist, st, rc.ts)) {
-                const MatV m = load_mat(s.mat, info.material);
-                L = add(L, muls(m.le, cosNL));
-            }
+            const RayT sr = make_ray(sf.point, dir, sf.excl);
+            if (!any_hit<GENERIC, COUNT>(s, sr, dist, st, rc.ts)) L = add(L, lc);
         }
     }
     return L;
+}
+
+// Material of a hit (PathTracer.cpp:33-52): false with *le set when the hit
+// is an emitter.
+struct Shading {
+    f3 kd, ks, kt, le;
+    float ior;
+    bool emitter;
+};
+__device__ __forceinline__ Shading shading_of(const SceneDev& s, const Surf& sf) {
+    Shading m;
+    m.ks = m.kt = m.le = f3{0.0f, 0.0f, 0.0f};
+    m.ior = 1.0f;
+    m.emitter = false;
+    if (sf.mat >= 0) {
+        const MatV mv = load_mat(s.mat, static_cast<uint32_t>(sf.mat));
+        m.le = mv.le;
+        m.emitter = any_gt0(mv.le);
+        m.kd = mv.kd;
+        m.ks = mv.ks;
+        m.kt = mv.kt;
+        m.ior = mv.ior;
+    } else {
+        m.kd = sf.color;
+    }
+    return m;
 }
 
 // A pending reflection / transmission chain.
@@ -236,22 +272,14 @@ __device__ f3 trace_path(const SceneDev& s, const RenderParams& p, f3 o, f3 d, u
             else ++rc.specular;
             if (h.id == MRT_ID_NONE) break;
             const Surf sf = surface(s, r, h);
-            f3 kD, kS = f3{0.0f, 0.0f, 0.0f}, kT = f3{0.0f, 0.0f, 0.0f};
-            float ior = 1.0f;
-            if (sf.mat >= 0) {
-                const MatV m = load_mat(s.mat, static_cast<uint32_t>(sf.mat));
-                if (any_gt0(m.le)) {
-                    accAll = add(accAll, mul(T, m.le));
-                    ends = true;
-                    break;
-                }
-                kD = m.kd;
-                kS = m.ks;
-                kT = m.kt;
-                ior = m.ior;
-            } else {
-                kD = sf.color;
+            const Shading m = shading_of(s, sf);
+            if (m.emitter) {
+                accAll = add(accAll, mul(T, m.le));
+                ends = true;
+                break;
             }
+            const f3 kD = m.kd, kS = m.ks, kT = m.kt;
+            const float ior = m.ior;
             const bool diffuse = any_gt0(kD);
             f3 Ld = f3{0.0f, 0.0f, 0.0f};
             if (diffuse && s.n_lights > 0u) {

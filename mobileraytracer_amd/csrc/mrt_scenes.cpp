@@ -161,6 +161,42 @@ MRT_API int mrt_triangle_intersect_box(const mrt_triangle* tri, const float box_
     return triangle_intersect_box(*tri, Box{ld3(box_min), ld3(box_max)}) ? 1 : 0;
 }
 
+MRT_API int mrt_primitive_aabbs(int category, const void* prims, uint32_t n, float* boxes6) {
+    if ((n && (!prims || !boxes6)) || category < MRT_CAT_PLANE || category > MRT_CAT_LIGHT) return MRT_E_INVALID;
+    for (uint32_t i = 0; i < n; ++i) {
+        Box b;
+        if (category == MRT_CAT_TRIANGLE || category == MRT_CAT_LIGHT)
+            b = triangle_box(static_cast<const mrt_triangle*>(prims)[i]);
+        else if (category == MRT_CAT_SPHERE)
+            b = sphere_box(static_cast<const mrt_sphere*>(prims)[i]);
+        else
+            b = plane_box(static_cast<const mrt_plane*>(prims)[i]);
+        st3(boxes6 + 6 * static_cast<size_t>(i), b.mn);
+        st3(boxes6 + 6 * static_cast<size_t>(i) + 3, b.mx);
+    }
+    return MRT_OK;
+}
+
+MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, mrt_bvh_node* nodes, uint32_t capacity,
+                          uint32_t* count) {
+    if ((n && !boxes6) || !count || leaf_size == 0) return MRT_E_INVALID;
+    std::vector<Box> boxes(n);
+    for (uint32_t i = 0; i < n; ++i)
+        boxes[i] = Box{ld3(boxes6 + 6 * static_cast<size_t>(i)), ld3(boxes6 + 6 * static_cast<size_t>(i) + 3)};
+    BvhOut out;
+    build_bvh_reference(boxes, leaf_size, &out);
+    *count = static_cast<uint32_t>(out.ref.size());
+    if (nodes) {
+        for (uint32_t i = 0; i < *count && i < capacity; ++i) {
+            st3(nodes[i].min, out.ref[i].box.mn);
+            st3(nodes[i].max, out.ref[i].box.mx);
+            nodes[i].index_offset = out.ref[i].offset;
+            nodes[i].number_primitives = out.ref[i].count;
+        }
+    }
+    return MRT_OK;
+}
+
 // Procedural Cornell box: walls [-1,1]^3 without the front (10 triangles,
 // wound inwards), two axis-aligned boxes (24 triangles, wound outwards), one
 // 0.5 x 0.5 area-light quad at y = 0.99 split in two AreaLights with Le 0.9

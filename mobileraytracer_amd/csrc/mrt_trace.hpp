@@ -41,10 +41,11 @@ MRT_HD RayT make_ray(f3 o, f3 d, uint32_t excl) {
     return r;
 }
 
-// Traversal statistics (instrumented launches only).
+// Traversal statistics (instrumented launches only), per query kind:
+// 0 closest hit over planes/spheres/triangles, 1 lights, 2 any hit.
 struct TraceStats {
-    uint32_t pairs;
-    uint32_t prims;
+    uint32_t pairs[3];
+    uint32_t prims[3];
 };
 
 // Per-lane traversal stack: the first kLds entries live in LDS (stride =
@@ -224,7 +225,7 @@ __device__ void bvh_closest(const SceneDev& s, const Accel& a, const RayT& r, Hi
     if (a.root_count > 0u) {
         for (uint32_t k = 0; k < a.root_count; ++k) {
             const uint32_t slot = a.root_ref + k;
-            if (COUNT) ++ts.prims;
+            if (COUNT) ++ts.prims[CAT == MRT_CAT_LIGHT ? 1 : 0];
             prim_closest<CAT, false>(s, a.perm ? a.perm[slot] : slot, r, h);
         }
         return;
@@ -237,7 +238,7 @@ __device__ void bvh_closest(const SceneDev& s, const Accel& a, const RayT& r, Hi
         const float4 q1 = nd->q1;
         const float4 q2 = nd->q2;
         const uint4 q3 = nd->q3;
-        if (COUNT) ++ts.pairs;
+        if (COUNT) ++ts.pairs[CAT == MRT_CAT_LIGHT ? 1 : 0];
         float eL, eR;
         const float lim = cull_limit(h.t);
         bool hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, r.o, r.inv, &eL) && eL <= lim;
@@ -245,7 +246,7 @@ __device__ void bvh_closest(const SceneDev& s, const Accel& a, const RayT& r, Hi
         if (hL && q3.y > 0u) {
             for (uint32_t k = 0; k < q3.y; ++k) {
                 const uint32_t slot = q3.x + k;
-                if (COUNT) ++ts.prims;
+                if (COUNT) ++ts.prims[CAT == MRT_CAT_LIGHT ? 1 : 0];
                 prim_closest<CAT, false>(s, a.perm ? a.perm[slot] : slot, r, h);
             }
             hL = false;
@@ -253,7 +254,7 @@ __device__ void bvh_closest(const SceneDev& s, const Accel& a, const RayT& r, Hi
         if (hR && q3.w > 0u) {
             for (uint32_t k = 0; k < q3.w; ++k) {
                 const uint32_t slot = q3.z + k;
-                if (COUNT) ++ts.prims;
+                if (COUNT) ++ts.prims[CAT == MRT_CAT_LIGHT ? 1 : 0];
                 prim_closest<CAT, false>(s, a.perm ? a.perm[slot] : slot, r, h);
             }
             hR = false;
@@ -281,7 +282,7 @@ __device__ bool bvh_any(const SceneDev& s, const Accel& a, const RayT& r, float 
         return false;
     if (a.root_count > 0u) {
         for (uint32_t k = 0; k < a.root_count; ++k) {
-            if (COUNT) ++ts.prims;
+            if (COUNT) ++ts.prims[2];
             if (prim_any<CAT>(s, a.root_ref + k, r, dist)) return true;
         }
         return false;
@@ -295,20 +296,20 @@ __device__ bool bvh_any(const SceneDev& s, const Accel& a, const RayT& r, float 
         const float4 q1 = nd->q1;
         const float4 q2 = nd->q2;
         const uint4 q3 = nd->q3;
-        if (COUNT) ++ts.pairs;
+        if (COUNT) ++ts.pairs[2];
         float eL, eR;
         bool hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, r.o, r.inv, &eL) && eL <= lim;
         bool hR = slab(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, r.o, r.inv, &eR) && eR <= lim;
         if (hL && q3.y > 0u) {
             for (uint32_t k = 0; k < q3.y; ++k) {
-                if (COUNT) ++ts.prims;
+                if (COUNT) ++ts.prims[2];
                 if (prim_any<CAT>(s, q3.x + k, r, dist)) return true;
             }
             hL = false;
         }
         if (hR && q3.w > 0u) {
             for (uint32_t k = 0; k < q3.w; ++k) {
-                if (COUNT) ++ts.prims;
+                if (COUNT) ++ts.prims[2];
                 if (prim_any<CAT>(s, q3.z + k, r, dist)) return true;
             }
             hR = false;
@@ -405,18 +406,18 @@ __device__ void grid_closest(const SceneDev& s, const Accel& a, const RayT& r, H
         if (!phase2) {
             bool hit = false;
             for (uint32_t k = b; k < e; ++k) {
-                if (COUNT) ++ts.prims;
+                if (COUNT) ++ts.prims[0];
                 if (prim_closest<CAT, true>(s, a.cell_ref[k], r, h)) { hit = true; break; }
             }
             if (hit) { phase2 = true; continue; }
-            if (COUNT) ++ts.pairs;
+            if (COUNT) ++ts.pairs[0];
             if (!dda_step(g, false, 0.0f)) return;
         } else {
             for (uint32_t k = b; k < e; ++k) {
-                if (COUNT) ++ts.prims;
+                if (COUNT) ++ts.prims[0];
                 prim_closest<CAT, true>(s, a.cell_ref[k], r, h);
             }
-            if (COUNT) ++ts.pairs;
+            if (COUNT) ++ts.pairs[0];
             if (!dda_step(g, true, h.t)) return;
         }
     }
@@ -431,10 +432,10 @@ __device__ bool grid_any(const SceneDev& s, const Accel& a, const RayT& r, float
         const uint32_t c = dda_cell(a, g);
         const uint32_t b = a.cell_off[c], e = a.cell_off[c + 1u];
         for (uint32_t k = b; k < e; ++k) {
-            if (COUNT) ++ts.prims;
+            if (COUNT) ++ts.prims[2];
             if (prim_any<CAT>(s, a.cell_ref[k], r, dist)) return true;
         }
-        if (COUNT) ++ts.pairs;
+        if (COUNT) ++ts.pairs[2];
         if (!dda_step(g, false, 0.0f)) return false;
     }
 }
@@ -450,7 +451,7 @@ __device__ __forceinline__ void accel_closest(const SceneDev& s, const Accel& a,
         grid_closest<CAT, COUNT>(s, a, r, h, ts);
     } else {
         for (uint32_t i = 0; i < a.count; ++i) {
-            if (COUNT) ++ts.prims;
+            if (COUNT) ++ts.prims[0];
             prim_closest<CAT, true>(s, i, r, h);
         }
     }
@@ -462,7 +463,7 @@ __device__ __forceinline__ bool accel_any(const SceneDev& s, const Accel& a, con
     if (a.kind == MRT_BVH) return bvh_any<CAT, COUNT>(s, a, r, dist, st, ts);
     if (a.kind == MRT_REGULAR_GRID) return grid_any<CAT, COUNT>(s, a, r, dist, ts);
     for (uint32_t i = 0; i < a.count; ++i) {
-        if (COUNT) ++ts.prims;
+        if (COUNT) ++ts.prims[2];
         if (prim_any<CAT>(s, i, r, dist)) return true;
     }
     return false;
@@ -476,7 +477,7 @@ __device__ __forceinline__ void lights_closest(const SceneDev& s, const RayT& r,
     if (s.lights_linear) {
         for (uint32_t i = 0; i < s.n_lights; ++i) {
             if (s.light_info[i].kind != MRT_LIGHT_AREA) continue;
-            if (COUNT) ++ts.prims;
+            if (COUNT) ++ts.prims[1];
             prim_closest<MRT_CAT_LIGHT, true>(s, i, r, h);
         }
     } else {

@@ -1,0 +1,149 @@
+"""The C ABI library on the CPU: it loads, exports every entry point that
+include/mobilert_gpu.h declares, its struct layouts match the reference
+classes, and its host-side construction helpers (no device work) match the
+oracle bit for bit.  No compute entry point is called here."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mobileraytracer_amd as mrt
+import oracle
+from helpers import product_triangle, vec
+from mobileraytracer_amd import abi
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols() -> list[str]:
+    text = (ROOT / "include" / "mobilert_gpu.h").read_text()
+    return sorted(set(re.findall(r"MRT_API\s+[\w\s\*]+?\b(mrt_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.lib()
+    names = declared_symbols()
+    assert len(names) >= 26
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(abi.EXPORTS), "ctypes prototypes out of sync with the header"
+
+
+def test_struct_layouts():
+    sizes = oracle.struct_sizes()
+    ours = [C.sizeof(abi.Triangle), C.sizeof(abi.Sphere), C.sizeof(abi.Plane), C.sizeof(abi.Material),
+            C.sizeof(abi.Light), C.sizeof(abi.Ray), C.sizeof(abi.Hit), C.sizeof(abi.Camera),
+            C.sizeof(abi.RenderDesc), C.sizeof(abi.RenderStats), C.sizeof(abi.Sampler), C.sizeof(abi.SceneDesc)]
+    assert ours == sizes
+    # Reference class sizes (x86-64, 12-byte vec3): Triangle 112, Sphere 20,
+    # Plane 28, Material 52 (app/MobileRT/Shapes/*.hpp, Material.hpp).
+    assert sizes[:4] == [112, 20, 28, 52]
+    assert abi.TRIANGLE_DTYPE.itemsize == 112 and abi.RAY_DTYPE.itemsize == 32 and abi.HIT_DTYPE.itemsize == 16
+
+
+def test_no_device_here_fails_cleanly():
+    n = C.c_int(-1)
+    assert abi.lib().mrt_device_count(C.byref(n)) == 0
+    if n.value > 0:
+        pytest.skip("a GPU is visible")
+    h = C.c_void_p()
+    assert abi.lib().mrt_ctx_create(0, C.byref(h)) == abi.MRT_E_NODEVICE
+
+
+def test_triangle_make_matches_oracle():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        a, b, c = (rng.uniform(-5, 5, 3).astype(np.float32) for _ in range(3))
+        p = product_triangle(a, b, c, 3)
+        o = np.zeros(1, abi.TRIANGLE_DTYPE)
+        oracle.lib().oracle_triangle_make(a.ctypes.data, b.ctypes.data, c.ctypes.data, 3, None, None, o.ctypes.data)
+        assert p.tobytes() == o.tobytes()
+
+
+def test_camera_make_matches_oracle():
+    for kind, pos, look, up, a, b in [(abi.CAMERA_PERSPECTIVE, (0, 0, -3.4), (0, 0, 1), (0, 1, 0), 45.0, 45.0),
+                                      (abi.CAMERA_PERSPECTIVE, (0.5, 0.5, -1), (0.5, 0.5, 0.5), (0, 1, 0), 80.0, 45.0),
+                                      (abi.CAMERA_PERSPECTIVE, (1, 2, 3), (-1, 0.5, 7), (0.1, 1, 0), 60.0, 40.0),
+                                      (abi.CAMERA_ORTHOGRAPHIC, (0, 1, -10), (0, 1, 7), (0, 1, 0), 10.0, 10.0)]:
+        pc = mrt.make_camera(kind, pos, look, up, a, b)
+        oc = abi.Camera()
+        P, Lk, U = (np.array(v, np.float32) for v in (pos, look, up))
+        oracle.lib().oracle_camera_make(kind, P.ctypes.data, Lk.ctypes.data, U.ctypes.data, a, b, C.byref(oc))
+        assert bytes(pc) == bytes(oc)
+
+
+def test_intersect_box_matches_oracle():
+    rng = np.random.default_rng(1)
+    hits = 0
+    for _ in range(3000):
+        a, b, c = (rng.uniform(-1, 1, 3).astype(np.float32) for _ in range(3))
+        t = product_triangle(a, b, c)
+        lo = rng.uniform(-1.2, 1.0, 3).astype(np.float32)
+        hi = (lo + rng.uniform(0, 0.8, 3)).astype(np.float32)
+        got = abi.lib().mrt_triangle_intersect_box(t.ctypes.data, lo.ctypes.data, hi.ctypes.data)
+        exp = oracle.lib().oracle_triangle_intersect_box(t.ctypes.data, lo.ctypes.data, hi.ctypes.data)
+        assert got == exp
+        hits += got
+    assert 300 < hits < 2900
+
+
+def test_bvh_builder_matches_oracle():
+    data = mrt.soup_scene(8, 5000, 50)
+    n = len(data.triangles)
+    boxes = np.zeros((n, 6), np.float32)
+    abi.check(abi.lib().mrt_primitive_aabbs(abi.CAT_TRIANGLE, data.triangles.ctypes.data, n, boxes.ctypes.data),
+              "mrt_primitive_aabbs")
+    for i in range(0, n, 97):
+        ob = np.zeros(6, np.float32)
+        oracle.lib().oracle_triangle_aabb(data.triangles[i:i + 1].ctypes.data, ob.ctypes.data)
+        assert boxes[i].tobytes() == ob.tobytes()
+    count = C.c_uint32()
+    abi.check(abi.lib().mrt_bvh_build(boxes.ctypes.data, n, 2, None, 0, C.byref(count)), "mrt_bvh_build")
+    nodes = np.zeros(count.value, oracle.BVH_NODE_DTYPE)
+    abi.check(abi.lib().mrt_bvh_build(boxes.ctypes.data, n, 2, nodes.ctypes.data, count.value, C.byref(count)),
+              "mrt_bvh_build")
+    from helpers import oracle_scene
+    sc = oracle_scene(data, abi.BVH)
+    ref = sc.bvh_nodes(abi.CAT_TRIANGLE)
+    assert nodes.tobytes() == ref.tobytes()
+
+
+def test_tile_values():
+    v0 = mrt.tile_values(0)
+    assert [float(oracle.lib().oracle_halton(i, 2)) for i in range(256)] == v0.tolist()
+    v = mrt.tile_values(9)
+    assert sorted(v.tolist()) == sorted(v0.tolist()) and v.tolist() != v0.tolist()
+    assert sorted(np.rint(v * 256).astype(int).tolist()) == list(range(256))  # a tile permutation
+
+
+def test_cornell_generator():
+    d = mrt.cornell_scene(1)
+    assert len(d.triangles) == 34 and len(d.materials) == 7 and len(d.lights) == 2
+    # Walls face the inside of [-1,1]^3, box faces face outwards.
+    tris = d.triangles
+    centroid = tris["a"] + (tris["ab"] + tris["ac"]) / 3
+    n = tris["na"]
+    for i in range(10):
+        assert np.dot(n[i], -centroid[i]) > 0
+    assert np.all((d.materials["kd"] >= 0.1) & (d.materials["kd"] <= 0.9))
+    assert np.all(d.lights["kind"] == abi.LIGHT_AREA) if "kind" in (d.lights.dtype.names or ()) else True
+
+
+def test_soup_generator():
+    d = mrt.soup_scene(2, 20000, 200)
+    t = d.triangles
+    assert len(t) == 19800 and len(d.lights) == 200
+    c = t["a"] + (t["ab"] + t["ac"]) / 3
+    assert c.min() > -0.01 and c.max() < 1.01
+    assert np.all(np.abs(t["ab"]) <= 0.0101) and np.all(np.abs(t["ac"]) <= 0.0101)
+    assert np.all((t["ca"] >= 0.2) & (t["ca"] <= 0.8)) and np.all(t["material_id"] == -1)
+    # Morton order keeps neighbours close: mean centroid step far below the
+    # mean distance of random pairs.
+    step = np.linalg.norm(np.diff(c, axis=0), axis=1).mean()
+    assert step < 0.1
+    d2 = mrt.soup_scene(2, 20000, 200)
+    assert d2.triangles.tobytes() == t.tobytes()

@@ -91,13 +91,14 @@ def test_mirror_scene_branches(gpu_ctx, shader):
     assert np.max(np.abs(bg.view(np.uint8).astype(int) - bo.view(np.uint8).astype(int))) <= 1
 
 
-@pytest.mark.parametrize("accel", [abi.BVH, abi.REGULAR_GRID])
-def test_pathtracer_soup(gpu_ctx, accel):
+@pytest.mark.parametrize("accel,kernel", [(abi.BVH, abi.KERNEL_MEGAKERNEL), (abi.REGULAR_GRID, abi.KERNEL_MEGAKERNEL),
+                                          (abi.BVH, abi.KERNEL_WAVEFRONT), (abi.REGULAR_GRID, abi.KERNEL_WAVEFRONT)])
+def test_pathtracer_soup(gpu_ctx, accel, kernel):
     """Config-2 shape at reduced size: PathTracer, depth 6, RR from depth 3."""
     data = mrt.soup_scene(4, 60000, 600)
     cam = soup_camera(128, 72)
     desc = mrt.render_desc(abi.SHADER_PATHTRACER, 128, 80, 4, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
-                           rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6)
+                           rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6, kernel=kernel)
     (bg, ag, st), (bo, ao, cnt) = render_both(gpu_ctx, data, desc, accel, threads=16)
     print(f"pt soup accel={accel}: {st.rays_camera} cam {st.rays_extension} ext {st.rays_shadow} shadow", flush=True)
     assert (st.rays_camera, st.rays_extension, st.rays_shadow) == tuple(int(c) for c in cnt[:3])
@@ -190,3 +191,38 @@ def test_render_deterministic(gpu_ctx):
     b1, _ = gpu.render(desc, accum=a1)
     b2, _ = gpu.render(desc, accum=a2)
     assert np.array_equal(b1, b2) and np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("generic", [False, True])
+def test_wavefront_equals_megakernel(gpu_ctx, generic):
+    """Both kernel shapes evaluate every path with the same operations in the
+    same order: bitmaps, FP32 radiance and ray counts are identical."""
+    data = mrt.soup_scene(9, 80000, 800)
+    if generic:
+        # Planes and spheres without reflection/transmission.
+        data.planes = np.zeros(1, abi.PLANE_DTYPE)
+        data.planes[0]["point"] = (0, -0.01, 0)
+        data.planes[0]["normal"] = (0, 1, 0)
+        data.planes[0]["material_id"] = 0
+        data.spheres = np.zeros(2, abi.SPHERE_DTYPE)
+        data.spheres["center"] = [(0.3, 0.4, 0.5), (0.7, 0.6, 0.4)]
+        data.spheres["sq_radius"] = np.float32(0.1) * np.float32(0.1)
+        data.spheres["material_id"] = 0
+        data.materials = np.zeros(1, abi.MATERIAL_DTYPE)
+        data.materials[0]["kd"] = (0.5, 0.6, 0.7)
+        data.materials[0]["ior"] = 1.0
+    cam = soup_camera(256, 144)
+    res = []
+    for kernel in (abi.KERNEL_MEGAKERNEL, abi.KERNEL_WAVEFRONT):
+        desc = mrt.render_desc(abi.SHADER_PATHTRACER, 256, 144, 3, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
+                               rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6, kernel=kernel)
+        gpu = mrt.Scene(gpu_ctx, data)
+        gpu.build(abi.BVH)
+        acc = np.zeros((144, 256, 3), np.float32)
+        bm, st = gpu.render(desc, accum=acc)
+        res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow, st.rays_specular)))
+    (b0, a0, c0), (b1, a1, c1) = res
+    print(f"megakernel vs wavefront rays {c0} {c1}", flush=True)
+    assert c0 == c1
+    assert np.array_equal(b0, b1)
+    assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))

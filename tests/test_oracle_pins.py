@@ -1,0 +1,342 @@
+"""Pins of the oracle beyond the reference's own known-answer tests.
+
+Each check compares the oracle with something the mathematics fixes
+(closed forms, brute force, independent double-precision formulas,
+invariants), chosen so that a plausible restatement mistake fails:
+  * haltonSequence: exact dyadics for base 2, the rational value for base 3;
+  * fastArcTan: its published error bound against atan;
+  * incrementalAvg: integer closed forms;
+  * grid sizing: the values the reference's sizing gives for 34 and 990,000;
+  * Triangle / Sphere intersection: independent float64 formulas;
+  * BVH and RegularGrid closest hit: brute force over all primitives;
+  * camera: the basis and the centre ray of the Cornell camera;
+  * Whitted and PathTracer: scenes whose radiance has a closed form,
+    including the zeroed LiD when the diffuse child hits an emitter
+    (PathTracer.cpp:129-131);
+  * the FP64 and FP32 oracles agree where they should.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import mobileraytracer_amd as mrt
+import oracle
+from helpers import (accel_params, cornell_camera, make_rays, oracle_scene, pixel_uvdd, product_triangle,
+                     random_rays)
+from mobileraytracer_amd import abi
+
+L = oracle.lib
+
+
+def test_halton_base2_exact():
+    for i in list(range(0, 4096)) + [2 ** 20 + 12345, 2 ** 23 - 1]:
+        exact = Fraction(0)
+        f = Fraction(1)
+        k = i
+        while k:
+            f /= 2
+            exact += f * (k % 2)
+            k //= 2
+        assert L().oracle_halton(i, 2) == float(exact)
+
+
+def test_halton_base3_rational():
+    for i in range(1, 3000):
+        exact = Fraction(0)
+        f = Fraction(1)
+        k = i
+        while k:
+            f /= 3
+            exact += f * (k % 3)
+            k //= 3
+        assert abs(L().oracle_halton(i, 3) - float(exact)) <= 4 * np.finfo(np.float32).eps
+
+
+def test_fast_arctan_error_bound():
+    xs = np.linspace(-1.0, 1.0, 20001).astype(np.float32)
+    got = np.array([L().oracle_fast_arctan(float(x)) for x in xs[::7]])
+    err = np.abs(got - np.arctan(xs[::7].astype(np.float64)))
+    assert err.max() < 0.0016  # published bound of this approximation (~0.0015 rad)
+    assert L().oracle_fast_arctan(0.0) == 0.0
+    assert L().oracle_fast_arctan(1.0) == float(np.float32(np.pi / 4))
+    for x in (0.1, 0.37, 0.9):
+        assert L().oracle_fast_arctan(-x) == -L().oracle_fast_arctan(x)
+
+
+def inc_avg(rgb, avg, n):
+    c = np.asarray(rgb, np.float32)
+    return L().oracle_incremental_avg(c.ctypes.data, avg, n)
+
+
+def test_incremental_avg_closed_forms():
+    # First sample: truncation of c * 255, alpha 0xFF, ABGR packing.
+    assert inc_avg([1.0, 0.5, 0.0], 0x12345678, 1) == 0xFF00_7FFF
+    # Values above 1 saturate at 255.
+    assert inc_avg([3.0, 2.0, 1.5], 0, 1) == 0xFFFF_FFFF
+    # A constant colour is a fixed point of the running mean.
+    avg = inc_avg([0.25, 0.5, 0.75], 0, 1)
+    for n in range(2, 40):
+        avg = inc_avg([0.25, 0.5, 0.75], avg, n)
+    assert avg == inc_avg([0.25, 0.5, 0.75], 0, 1)
+    # ((n - 1) * old + new) / n with truncating integer division.
+    assert inc_avg([0.0, 0.0, 0.0], 0xFF80_8080, 2) == 0xFF40_4040
+    assert inc_avg([1.0, 1.0, 1.0], 0xFF00_0000, 3) == 0xFF55_5555
+
+
+def test_grid_size_formula():
+    assert L().oracle_grid_size_for(34) == 16      # Cornell box
+    assert L().oracle_grid_size_for(990000) == 128  # 1M soup
+    for n in range(0, 7):
+        assert L().oracle_grid_size_for(n) == 2
+    assert L().oracle_grid_size_for(7) == 8
+
+
+def tri_f64(a, b, c, o, d):
+    """Moller-Trumbore in float64, written from the textbook form."""
+    e1, e2 = b - a, c - a
+    p = np.cross(d, e2)
+    det = np.dot(e1, p)
+    if abs(det) < 1e-12:
+        return None
+    s = o - a
+    u = np.dot(s, p) / det
+    q = np.cross(s, e1)
+    v = np.dot(d, q) / det
+    if u < 0 or v < 0 or u + v > 1:
+        return None
+    t = np.dot(e2, q) / det
+    return (t, u, v) if t > 1e-6 else None
+
+
+def test_triangle_intersection_vs_float64():
+    rng = np.random.default_rng(1)
+    agree = 0
+    total = 0
+    for _ in range(3000):
+        a, b, c = (rng.uniform(-1, 1, 3).astype(np.float32) for _ in range(3))
+        o = rng.uniform(-3, 3, 3).astype(np.float32)
+        target = ((a + b + c) / 3 + rng.normal(0, 0.4, 3)).astype(np.float32)
+        d = (target - o).astype(np.float32)
+        tri = np.zeros(1, abi.TRIANGLE_DTYPE)
+        L().oracle_triangle_make(a.ctypes.data, b.ctypes.data, c.ctypes.data, -1, None, None, tri.ctypes.data)
+        ray = make_rays(o, d)
+        hit = np.zeros(1, abi.HIT_DTYPE)
+        got = L().oracle_triangle_intersect(tri.ctypes.data, ray.ctypes.data, hit.ctypes.data, None)
+        ref = tri_f64(a.astype(np.float64), b.astype(np.float64), c.astype(np.float64), o.astype(np.float64),
+                      d.astype(np.float64))
+        total += 1
+        if ref is None:
+            agree += got == 0
+            continue
+        t, u, v = ref
+        near_edge = min(u, v, 1 - u - v) < 1e-4
+        if got:
+            assert abs(hit[0]["t"] - t) <= 1e-4 * max(1.0, abs(t))
+            assert abs(hit[0]["u"] - u) <= 1e-4 and abs(hit[0]["v"] - v) <= 1e-4
+            agree += 1
+        else:
+            assert near_edge
+    assert agree / total > 0.995
+
+
+def test_sphere_root_vs_float64():
+    rng = np.random.default_rng(2)
+    sph = np.zeros(1, abi.SPHERE_DTYPE)
+    sph[0]["center"] = (0.2, -0.1, 0.3)
+    r = np.float32(0.5)
+    sph[0]["sq_radius"] = r * r
+    sph[0]["material_id"] = 0
+    mats = np.zeros(1, abi.MATERIAL_DTYPE)
+    mats[0]["kd"] = (0.5, 0.5, 0.5)
+    data = mrt.SceneData(None, mats, None, sph, None)
+    sc = oracle_scene(data, abi.NAIVE)
+    o = rng.uniform(-3, 3, (2000, 3)).astype(np.float32)
+    d = (np.array([0.2, -0.1, 0.3], np.float32) + rng.normal(0, 0.3, (2000, 3)) - o).astype(np.float32)
+    rays = make_rays(o, d)
+    hits, _, _ = sc.trace_closest(rays, hit_dtype=abi.HIT_DTYPE)
+    c = np.array([0.2, -0.1, 0.3])
+    for ray, h in zip(rays, hits):
+        oo, dd = ray["org"].astype(np.float64), ray["dir"].astype(np.float64)
+        A = dd @ dd
+        B = 2 * (oo - c) @ dd
+        Cc = (oo - c) @ (oo - c) - float(r) ** 2
+        disc = B * B - 4 * A * Cc
+        if disc < 0:
+            assert h["id"] == abi.ID_NONE
+            continue
+        t = (-B - np.sqrt(disc)) / (2 * A)
+        if t < 1e-3:
+            continue  # origin inside or behind: the reference keeps the smaller root only
+        assert h["id"] == abi.make_id(abi.CAT_SPHERE, 0)
+        assert abs(h["t"] - t) <= 1e-4 * max(1.0, t)
+
+
+@pytest.mark.parametrize("accel", [abi.BVH, abi.REGULAR_GRID])
+def test_accelerators_match_brute_force(accel):
+    data = mrt.soup_scene(3, 3000, 30)
+    naive = oracle_scene(data, abi.NAIVE, (0.5, 0.5, -1), (0.5, 0.5, -1))
+    acc = oracle_scene(data, accel, (0.5, 0.5, -1), (0.5, 0.5, -1))
+    rays = random_rays(20000, 4, 0.0, 1.0)
+    h0, _, _ = naive.trace_closest(rays, hit_dtype=abi.HIT_DTYPE)
+    h1, _, _ = acc.trace_closest(rays, hit_dtype=abi.HIT_DTYPE)
+    same = (h0["id"] == h1["id"]) & ((h0["t"] == h1["t"]) | (h0["id"] == abi.ID_NONE))
+    if accel == abi.BVH:
+        assert same.all()
+    else:
+        # The grid walk visits cells in ray order and keeps the first of equal
+        # t; its cell lists come from the quirky intersectBox: rare differences.
+        assert same.mean() > 0.999
+    srays = random_rays(20000, 5, 0.0, 1.0, tmax=0.4)
+    a0 = naive.trace_any(srays)
+    a1 = acc.trace_any(srays)
+    assert (a0 == a1).mean() > (0.9999 if accel == abi.BVH else 0.999)
+
+
+def test_bvh_invariants():
+    data = mrt.soup_scene(6, 2000, 0)
+    sc = oracle_scene(data, abi.BVH)
+    nodes = sc.bvh_nodes(abi.CAT_TRIANGLE)
+    leaves = []
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        nd = nodes[i]
+        if nd["count"] > 0:
+            assert nd["count"] <= 2
+            leaves.append((int(nd["offset"]), int(nd["count"])))
+        else:
+            for ch in (int(nd["offset"]), int(nd["offset"]) + 1):
+                assert np.all(nodes[ch]["mn"] >= nd["mn"]) and np.all(nodes[ch]["mx"] <= nd["mx"])
+            stack += [int(nd["offset"]) + 1, int(nd["offset"])]
+    # Left-first DFS visits the leaves in ascending primitive order, covering
+    # every primitive once (the basis of the (t, index) tie rule).
+    pos = 0
+    for off, cnt in leaves:
+        assert off == pos
+        pos += cnt
+    assert pos == len(data.triangles)
+
+
+def test_camera_basis_and_centre_ray():
+    cam = abi.Camera()
+    p, l, u = (np.array(v, np.float32) for v in ((0, 0, -3.4), (0, 0, 1), (0, 1, 0)))
+    L().oracle_camera_make(abi.CAMERA_PERSPECTIVE, p.ctypes.data, l.ctypes.data, u.ctypes.data, 45.0, 45.0,
+                           C.byref(cam))
+    assert list(cam.direction) == [0, 0, 1] and list(cam.right) == [1, 0, 0] and list(cam.up) == [0, 1, 0]
+    assert cam.fov_h == float(np.float32(np.float32(45.0) * np.float32(np.pi)) / np.float32(180.0))
+    rays = oracle.camera_rays(cam, np.array([[0.5, 0.5, 0.0, 0.0]], np.float32), abi.RAY_DTYPE)
+    assert list(rays[0]["dir"]) == [0, 0, 1]
+
+
+def one_triangle_scene(kd, light_le, light_pos):
+    tris = product_triangle((-1, -1, 0), (1, -1, 0), (0, 1, 0), 0)
+    # normalize(cross(AC, AB)) of this winding points towards -z (the camera).
+    mats = np.zeros(1, abi.MATERIAL_DTYPE)
+    mats[0]["kd"] = kd
+    mats[0]["ior"] = 1.0
+    lights = np.zeros(1, abi.LIGHT_DTYPE)
+    Lt = abi.Light()
+    Lt.kind = abi.LIGHT_POINT
+    Lt.radiance.le[:] = list(light_le)
+    Lt.radiance.ior = 1.0
+    Lt.position[:] = list(light_pos)
+    C.memmove(lights.ctypes.data, C.addressof(Lt), C.sizeof(abi.Light))
+    return mrt.SceneData(tris, mats, lights)
+
+
+def test_whitted_closed_form():
+    kd = np.array([0.5, 0.25, 0.8])
+    le = np.array([2.0, 1.0, 0.5])
+    lp = np.array([0.0, 0.0, -2.0])
+    data = one_triangle_scene(kd, le, lp)
+    assert data.triangles[0]["na"][2] < 0
+    sc = oracle_scene(data, abi.BVH)
+    desc = mrt.render_desc(abi.SHADER_WHITTED, 16, 16, 1, cornell_camera(16, 16))
+    rays = make_rays([(0.1, 0.0, -1.0)], [(0.0, 0.0, 1.0)])
+    rgb, flag = sc.shade_rays(desc, rays)
+    n = np.array([0, 0, -1.0])
+    to_l = lp - np.array([0.1, 0.0, 0.0])
+    cos = n @ (to_l / np.linalg.norm(to_l))
+    expect = le * cos * kd + 0.1 * kd
+    np.testing.assert_allclose(rgb[0], expect, rtol=1e-6)
+    assert flag[0] == 0
+
+
+def test_pathtracer_closed_forms():
+    kd = np.array([0.5, 0.25, 0.8])
+    le = np.array([2.0, 1.0, 0.5])
+    lp = np.array([0.0, 0.0, -2.0])
+    # (1) depth_max 1: the indirect ray is never traced, rgb = Ld.
+    data = one_triangle_scene(kd, le, lp)
+    sc = oracle_scene(data, abi.BVH)
+    const = mrt.sampler(abi.SAMPLER_CONSTANT, 0.3)
+    desc = mrt.render_desc(abi.SHADER_PATHTRACER, 16, 16, 1, cornell_camera(16, 16), rr=const, light=const,
+                           hemi=const, depth_min=1, depth_max=1)
+    rays = make_rays([(0.1, 0.0, -1.0)], [(0.0, 0.0, 1.0)])
+    rgb, flag = sc.shade_rays(desc, rays)
+    to_l = lp - np.array([0.1, 0.0, 0.0])
+    cos = np.array([0, 0, -1.0]) @ (to_l / np.linalg.norm(to_l))
+    np.testing.assert_allclose(rgb[0], le * cos * kd, rtol=1e-6)
+    # (2) a huge emitter in front of the surface: the light sample is
+    # unoccluded (Ld > 0) and the diffuse child hits the emitter, so the
+    # reference zeroes LiD and returns only Ld (PathTracer.cpp:129-131).
+    tris = product_triangle((-1, -1, 0), (1, -1, 0), (0, 1, 0), 0)
+    mats = np.zeros(1, abi.MATERIAL_DTYPE)
+    mats[0]["kd"] = kd
+    mats[0]["ior"] = 1.0
+    lights = np.zeros(1, abi.LIGHT_DTYPE)
+    Lt = abi.Light()
+    Lt.kind = abi.LIGHT_AREA
+    Lt.radiance.le[:] = list(le)
+    Lt.radiance.ior = 1.0
+    t = product_triangle((-1000, -1000, -0.5), (1000, -1000, -0.5), (0, 1000, -0.5), -1)
+    C.memmove(C.addressof(Lt.triangle), t.ctypes.data, 112)
+    Lt.sampler = mrt.sampler(abi.SAMPLER_CONSTANT, 0.25)
+    C.memmove(lights.ctypes.data, C.addressof(Lt), C.sizeof(abi.Light))
+    data2 = mrt.SceneData(tris, mats, lights)
+    sc2 = oracle_scene(data2, abi.BVH)
+    desc2 = mrt.render_desc(abi.SHADER_PATHTRACER, 16, 16, 1, cornell_camera(16, 16), rr=const, light=const,
+                            hemi=const, depth_min=3, depth_max=6)
+    rays2 = make_rays([(0.1, 0.0, -0.25)], [(0.0, 0.0, 1.0)])
+    rgb2, flag2 = sc2.shade_rays(desc2, rays2)
+    # The sampled light point: R = S = 0.25, A + R*AB + S*AC.
+    A = np.array([-1000, -1000, -0.5]); AB = np.array([2000, 0, 0.0]); AC = np.array([1000, 2000, 0.0])
+    lpt = A + 0.25 * AB + 0.25 * AC
+    to_l = lpt - np.array([0.1, 0.0, 0.0])
+    cos2 = np.array([0, 0, -1.0]) @ (to_l / np.linalg.norm(to_l))
+    np.testing.assert_allclose(rgb2[0], le * cos2 * kd, rtol=1e-5)
+    assert flag2[0] == 1  # the child chain ended on the emitter
+
+
+def test_fp64_oracle_agrees_with_fp32_on_whitted():
+    data = mrt.cornell_scene(1)
+    cam = cornell_camera(64, 64)
+    desc = mrt.render_desc(abi.SHADER_WHITTED, 64, 64, 1, cam)
+    sc = oracle_scene(data, abi.BVH, *mrt.camera_aabb(cam))
+    a32 = np.zeros((64, 64, 3))
+    a64 = np.zeros((64, 64, 3))
+    sc.render(desc, accum=a32, threads=4)
+    sc.render(desc, accum=a64, threads=4, use_double=True)
+    # Away from shadow edges and wall seams (where a rounding flips a visibility
+    # decision) the two precisions agree to FP32 rounding.
+    far = np.abs(a32 - a64).max(axis=2) > 1e-5
+    assert far.mean() < 0.01
+
+
+def test_sampler_stratification():
+    h = abi.Sampler(abi.SAMPLER_HALTON, 0.0, 7, 0)
+    s = abi.Sampler(abi.SAMPLER_STRATIFIED, 0.0, 7, 0)
+    r = abi.Sampler(abi.SAMPLER_RANDOM, 0.0, 7, 0)
+    for pixel in (0, 17, 12345):
+        for dim in (0, 1, 6, 7):
+            hv = np.array([L().oracle_draw(C.byref(h), pixel, k, dim, 16) for k in range(16)])
+            sv = np.array([L().oracle_draw(C.byref(s), pixel, k, dim, 16) for k in range(16)])
+            rv = np.array([L().oracle_draw(C.byref(r), pixel, k, dim, 16) for k in range(16)])
+            if dim % 2 == 0:
+                assert sorted(np.floor(hv * 16).astype(int)) == list(range(16))
+            assert sorted(np.floor(sv * 16).astype(int)) == list(range(16))
+            assert np.all((rv >= 0) & (rv < 1)) and len(set(rv.tolist())) == 16
