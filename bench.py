@@ -40,8 +40,8 @@ def log(msg: str) -> None:
 KERNELS = {"auto": 0, "megakernel": 1, "wavefront": 2}
 
 
-def make_desc(mrt, abi, cam, spp=SPP, instrument=False, kernel="auto"):
-    return mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam, kernel=KERNELS[kernel],
+def make_desc(mrt, abi, cam, spp=SPP, instrument=False, kernel="auto", ray_sort=0, persistent=0):
+    return mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam, kernel=KERNELS[kernel], ray_sort=ray_sort, persistent=persistent,
                            pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 5),
                            rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 6),
                            light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 7),
@@ -84,6 +84,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel", choices=sorted(KERNELS), default="auto")
+    ap.add_argument("--sort", action="store_true", help="wavefront: reorder extension rays by origin cell")
+    ap.add_argument("--no-persistent", action="store_true", help="wavefront: one ray per lane, no refill")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -111,7 +113,10 @@ def main() -> int:
     torch.cuda.synchronize()
 
     # Instrumented frame (1 spp): node visits and primitive tests per ray.
-    _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True, kernel=args.kernel), bitmap=bitmap,
+    sort = 1 if args.sort else 2
+    pers = 2 if args.no_persistent else 0
+    _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True, kernel=args.kernel, ray_sort=sort, persistent=pers),
+                         bitmap=bitmap,
                          accum=accum,
                          mem=abi.MEM_DEVICE)
     rays1 = st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
@@ -126,7 +131,7 @@ def main() -> int:
         f"{st.light_node_visits / closest1:.1f} pairs {st.light_prim_tests / closest1:.1f} tests; per shadow ray: "
         f"{st.shadow_node_visits / shadow1:.1f} pairs {st.shadow_prim_tests / shadow1:.1f} tests")
 
-    desc = make_desc(mrt, abi, cam, kernel=args.kernel)
+    desc = make_desc(mrt, abi, cam, kernel=args.kernel, ray_sort=sort, persistent=pers)
     for _ in range(args.warmup):
         scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
     torch.cuda.synchronize()

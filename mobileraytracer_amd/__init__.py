@@ -47,7 +47,7 @@ def tile_values(seed: int = 0) -> np.ndarray:
 def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Camera, *, samples_light: int = 1,
                 depth_min: int = 4, depth_max: int = 8, pixel=None, rr=None, light=None, hemi=None,
                 tile_seed: int = 0, kernel: int = abi.KERNEL_MEGAKERNEL, first_sample: int = 0,
-                instrument: bool = False) -> abi.RenderDesc:
+                instrument: bool = False, ray_sort: int = 0, persistent: int = 0) -> abi.RenderDesc:
     d = abi.RenderDesc()
     d.shader = shader
     d.width, d.height, d.samples_pixel, d.samples_light = width, height, spp, samples_light
@@ -62,6 +62,8 @@ def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Came
     d.kernel = kernel
     d.first_sample = first_sample
     d.instrument = 1 if instrument else 0
+    d.ray_sort = ray_sort
+    d.persistent = persistent
     return d
 
 

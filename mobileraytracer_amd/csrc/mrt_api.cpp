@@ -737,13 +737,16 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         useWavefront = wfEligible;
     }
     const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
+    const bool persistent = useWavefront && d->persistent != 2u && !s->generic &&
+                            s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH;
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
     StackCfg sc;
     if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
     WfBuffers wb{};
     if (useWavefront) {
         const uint32_t slots = wavefront_slots(p);
-        if ((rc = s->wavefront.alloc(wavefront_bytes(slots))) != MRT_OK) return rc;
+        const size_t scanBytes = wavefront_scan_bytes();
+        if ((rc = s->wavefront.alloc(wavefront_bytes(slots) + scanBytes + 256)) != MRT_OK) return rc;
         float4* base = s->wavefront.as<float4>();
         wb.sA = base;
         wb.sB = base + slots;
@@ -752,7 +755,28 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         wb.queue_b = base + 5ull * slots;
         wb.hits = base + 7ull * slots;
         wb.shadow = base + 8ull * slots;
-        wb.counts = reinterpret_cast<uint32_t*>(base + 12ull * slots);
+        uint32_t* u = reinterpret_cast<uint32_t*>(base + 12ull * slots);
+        wb.counts = u;
+        wb.keys = u + 64;
+        wb.hist = wb.keys + slots;
+        wb.offs = wb.hist + kSortBins;
+        const uintptr_t t = (reinterpret_cast<uintptr_t>(wb.offs + kSortBins) + 255u) & ~static_cast<uintptr_t>(255u);
+        wb.scan_temp = reinterpret_cast<void*>(t);
+        wb.scan_bytes = scanBytes;
+        wb.sort = d->ray_sort == 1u ? 1u : 0u;
+        // Key box: the triangle BVH's root box (the unit cube without triangles).
+        f3 mn{0.0f, 0.0f, 0.0f}, mx{1.0f, 1.0f, 1.0f};
+        const Accel& ta = s->dev.acc[MRT_CAT_TRIANGLE];
+        if (ta.kind == MRT_BVH && ta.count > 0) {
+            mn = ld3(ta.root_mn);
+            mx = ld3(ta.root_mx);
+        }
+        for (int k = 0; k < 3; ++k) {
+            const float e = comp(mx, k) - comp(mn, k);
+            wb.box_mn[k] = comp(mn, k);
+            wb.box_inv[k] = e > 0.0f ? 32.0f / e : 0.0f;
+        }
+        MRT_HIP(hipMemsetAsync(wb.hist, 0, kSortBins * sizeof(uint32_t), st));
     }
     s->stop.store(false);
     s->progress.store(d->first_sample);
@@ -773,8 +797,8 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         }
         MRT_HIP(hipEventRecord(k0, st));
         if (useWavefront) {
-            MRT_HIP(launch_wavefront_pass(s->dev, s->generic, d->instrument != 0u, p, sample, dbm, dacc, wb,
-                                          traceBlocks, sc, s->counters.as<unsigned long long>(),
+            MRT_HIP(launch_wavefront_pass(s->dev, s->generic, persistent, d->instrument != 0u, p, sample, dbm, dacc,
+                                          wb, traceBlocks, sc, s->counters.as<unsigned long long>(),
                                           s->err.as<uint32_t>(), st));
         } else {
             MRT_HIP(launch_render_pass(s->dev, s->generic, s->dev.has_branches != 0u, d->instrument != 0u, p,

@@ -503,6 +503,191 @@ __device__ __forceinline__ HitT closest_hit(const SceneDev& s, const RayT& r, fl
     return h;
 }
 
+// ---------------------------------------------------- stepwise traversal --
+// The same BVH walk cut into single steps (one 64-byte node pair, or the root
+// test, per call) so that persistent kernels can refill a lane with a new
+// ray the moment its ray finishes instead of idling until the slowest ray of
+// the wave is done.  Covers scenes whose closest hit is the triangle BVH plus
+// the lights (no planes or spheres); the visiting rule, the culling and the
+// (t, id) key are those of bvh_closest / bvh_any above.
+constexpr uint32_t kRootPending = 0xFFFFFFFFu;
+enum : uint32_t { kPhaseTri = 0, kPhaseLightBvh = 1, kPhaseLightScan = 2, kPhaseDone = 3 };
+
+struct Trav {
+    RayT r;
+    HitT h;
+    uint32_t node;
+    uint32_t phase;
+};
+
+__device__ __forceinline__ uint32_t phase_after_triangles(const SceneDev& s) {
+    if (s.n_area_lights == 0u) return kPhaseDone;
+    return s.lights_linear ? kPhaseLightScan : kPhaseLightBvh;
+}
+
+__device__ __forceinline__ void trav_begin(const SceneDev& s, Trav& t, Stack& st, float tmax) {
+    t.h.t = tmax;
+    t.h.id = MRT_ID_NONE;
+    t.h.u = 0.0f;
+    t.h.v = 0.0f;
+    t.node = kRootPending;
+    t.phase = s.acc[MRT_CAT_TRIANGLE].count > 0u ? kPhaseTri : phase_after_triangles(s);
+    st.sp = 0;
+}
+
+// Leaf primitives [ref, ref + count) of the triangle (phase 0) or light
+// (phase 1) BVH.
+template <bool COUNT>
+__device__ __forceinline__ void step_leaf(const SceneDev& s, Trav& t, uint32_t ref, uint32_t count, bool light,
+                                          TraceStats& ts) {
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t slot = ref + k;
+        if (COUNT) ++ts.prims[light ? 1 : 0];
+        if (light) prim_closest<MRT_CAT_LIGHT, false>(s, s.light_acc.perm[slot], t.r, t.h);
+        else prim_closest<MRT_CAT_TRIANGLE, false>(s, slot, t.r, t.h);
+    }
+}
+
+// One closest-hit step; returns true when the ray is finished.
+template <bool COUNT>
+__device__ bool step_closest(const SceneDev& s, Trav& t, Stack& st, TraceStats& ts) {
+    if (t.phase == kPhaseLightScan) {
+        for (uint32_t i = 0; i < s.n_lights; ++i) {
+            if (s.light_info[i].kind != MRT_LIGHT_AREA) continue;
+            if (COUNT) ++ts.prims[1];
+            prim_closest<MRT_CAT_LIGHT, true>(s, i, t.r, t.h);
+        }
+        t.phase = kPhaseDone;
+        return true;
+    }
+    if (t.phase == kPhaseDone) return true;
+    const bool light = t.phase == kPhaseLightBvh;
+    const Accel& a = light ? s.light_acc : s.acc[MRT_CAT_TRIANGLE];
+    bool advance = false;
+    if (t.node == kRootPending) {
+        float e;
+        if (!slab(f3{a.root_mn[0], a.root_mn[1], a.root_mn[2]}, f3{a.root_mx[0], a.root_mx[1], a.root_mx[2]}, t.r.o,
+                  t.r.inv, &e)) {
+            advance = true;
+        } else if (a.root_count > 0u) {
+            step_leaf<COUNT>(s, t, a.root_ref, a.root_count, light, ts);
+            advance = true;
+        } else {
+            t.node = 0;
+            st.sp = 0;
+        }
+    } else {
+        const BNode* nd = a.nodes + t.node;
+        const float4 q0 = nd->q0;
+        const float4 q1 = nd->q1;
+        const float4 q2 = nd->q2;
+        const uint4 q3 = nd->q3;
+        if (COUNT) ++ts.pairs[light ? 1 : 0];
+        float eL, eR;
+        const float lim = cull_limit(t.h.t);
+        bool hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, t.r.o, t.r.inv, &eL) && eL <= lim;
+        bool hR = slab(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, t.r.o, t.r.inv, &eR) && eR <= lim;
+        if (hL && q3.y > 0u) {
+            step_leaf<COUNT>(s, t, q3.x, q3.y, light, ts);
+            hL = false;
+        }
+        if (hR && q3.w > 0u) {
+            step_leaf<COUNT>(s, t, q3.z, q3.w, light, ts);
+            hR = false;
+        }
+        if (hL && hR) {
+            const bool leftFirst = eL <= eR;
+            t.node = leftFirst ? q3.x : q3.z;
+            st.push(leftFirst ? q3.z : q3.x);
+        } else if (hL) {
+            t.node = q3.x;
+        } else if (hR) {
+            t.node = q3.z;
+        } else if (st.sp == 0u) {
+            advance = true;
+        } else {
+            t.node = st.pop();
+        }
+    }
+    if (advance) {
+        t.phase = light ? kPhaseDone : phase_after_triangles(s);
+        t.node = kRootPending;
+        st.sp = 0;
+    }
+    return t.phase == kPhaseDone;
+}
+
+// One any-hit step over the triangle BVH; *occluded set when a blocker is
+// found.  Returns true when the ray is finished.
+template <bool COUNT>
+__device__ bool step_any(const SceneDev& s, Trav& t, float dist, Stack& st, TraceStats& ts, bool* occluded) {
+    const Accel& a = s.acc[MRT_CAT_TRIANGLE];
+    if (a.count == 0u) return true;
+    if (t.node == kRootPending) {
+        float e;
+        if (!slab(f3{a.root_mn[0], a.root_mn[1], a.root_mn[2]}, f3{a.root_mx[0], a.root_mx[1], a.root_mx[2]}, t.r.o,
+                  t.r.inv, &e))
+            return true;
+        if (a.root_count > 0u) {
+            for (uint32_t k = 0; k < a.root_count; ++k) {
+                if (COUNT) ++ts.prims[2];
+                if (prim_any<MRT_CAT_TRIANGLE>(s, a.root_ref + k, t.r, dist)) {
+                    *occluded = true;
+                    return true;
+                }
+            }
+            return true;
+        }
+        t.node = 0;
+        st.sp = 0;
+        return false;
+    }
+    const BNode* nd = a.nodes + t.node;
+    const float4 q0 = nd->q0;
+    const float4 q1 = nd->q1;
+    const float4 q2 = nd->q2;
+    const uint4 q3 = nd->q3;
+    if (COUNT) ++ts.pairs[2];
+    const float lim = cull_limit(dist);
+    float eL, eR;
+    bool hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, t.r.o, t.r.inv, &eL) && eL <= lim;
+    bool hR = slab(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, t.r.o, t.r.inv, &eR) && eR <= lim;
+    if (hL && q3.y > 0u) {
+        for (uint32_t k = 0; k < q3.y; ++k) {
+            if (COUNT) ++ts.prims[2];
+            if (prim_any<MRT_CAT_TRIANGLE>(s, q3.x + k, t.r, dist)) {
+                *occluded = true;
+                return true;
+            }
+        }
+        hL = false;
+    }
+    if (hR && q3.w > 0u) {
+        for (uint32_t k = 0; k < q3.w; ++k) {
+            if (COUNT) ++ts.prims[2];
+            if (prim_any<MRT_CAT_TRIANGLE>(s, q3.z + k, t.r, dist)) {
+                *occluded = true;
+                return true;
+            }
+        }
+        hR = false;
+    }
+    if (hL && hR) {
+        const bool leftFirst = eL <= eR;
+        t.node = leftFirst ? q3.x : q3.z;
+        st.push(leftFirst ? q3.z : q3.x);
+    } else if (hL) {
+        t.node = q3.x;
+    } else if (hR) {
+        t.node = q3.z;
+    } else if (st.sp == 0u) {
+        return true;
+    } else {
+        t.node = st.pop();
+    }
+    return false;
+}
+
 template <bool GENERIC, bool COUNT>
 __device__ __forceinline__ bool any_hit(const SceneDev& s, const RayT& r, float dist, Stack& st, TraceStats& ts) {
     if (GENERIC) {

@@ -213,16 +213,20 @@ def test_wavefront_equals_megakernel(gpu_ctx, generic):
         data.materials[0]["ior"] = 1.0
     cam = soup_camera(256, 144)
     res = []
-    for kernel in (abi.KERNEL_MEGAKERNEL, abi.KERNEL_WAVEFRONT):
+    for kernel, sort, pers in ((abi.KERNEL_MEGAKERNEL, 0, 0), (abi.KERNEL_WAVEFRONT, 2, 2),
+                               (abi.KERNEL_WAVEFRONT, 1, 2), (abi.KERNEL_WAVEFRONT, 2, 1),
+                               (abi.KERNEL_WAVEFRONT, 1, 1)):
         desc = mrt.render_desc(abi.SHADER_PATHTRACER, 256, 144, 3, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
-                               rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6, kernel=kernel)
+                               rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6, kernel=kernel,
+                               ray_sort=sort, persistent=pers)
         gpu = mrt.Scene(gpu_ctx, data)
         gpu.build(abi.BVH)
         acc = np.zeros((144, 256, 3), np.float32)
         bm, st = gpu.render(desc, accum=acc)
         res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow, st.rays_specular)))
-    (b0, a0, c0), (b1, a1, c1) = res
-    print(f"megakernel vs wavefront rays {c0} {c1}", flush=True)
-    assert c0 == c1
-    assert np.array_equal(b0, b1)
-    assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+    b0, a0, c0 = res[0]
+    for b1, a1, c1 in res[1:]:
+        print(f"megakernel vs wavefront rays {c0} {c1}", flush=True)
+        assert c0 == c1
+        assert np.array_equal(b0, b1)
+        assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
