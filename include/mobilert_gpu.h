@@ -206,10 +206,14 @@ typedef struct mrt_render_stats {
     uint64_t prim_tests;       /* primitive tests (instrumented)               */
     uint64_t light_node_visits, light_prim_tests;    /* of which: light BVH / scan  */
     uint64_t shadow_node_visits, shadow_prim_tests;  /* of which: shadowTrace       */
-    uint32_t samples_done;     /* completed sample passes                      */
+    uint32_t samples_done;     /* completed samples (every covered pixel)      */
     uint32_t covered_pixels;   /* pixels inside the 16x16 tile grid            */
     float ms_render;           /* device time of the render, milliseconds      */
-    float ms_kernel;           /* summed duration of the dominant kernel       */
+    float ms_kernel;           /* device time of the sample loop (megakernel:
+                                  summed pass launches; wavefront: the whole
+                                  iteration loop)                              */
+    float ms_trace;            /* wavefront: summed trace launches             */
+    uint32_t trace_launches;   /* wavefront: trace launches (iterations)       */
 } mrt_render_stats;
 
 /* Accelerator build parameters (Shader::initializeAccelerators inputs). */
@@ -278,7 +282,9 @@ MRT_API int mrt_intersect_pairs(mrt_ctx* ctx, int category, const void* prims, c
 MRT_API int mrt_render(mrt_scene* scene, const mrt_render_desc* desc, uint32_t* bitmap,
                        uint32_t stride_bytes, float* accum, int mem, mrt_render_stats* stats);
 /* Renderer::stopRender (Renderer.cpp:61-65): callable from any thread; the
- * running mrt_render returns MRT_E_STOPPED after the pass in flight. */
+ * running mrt_render starts no further sample (megakernel: pass; wavefront:
+ * per-pixel sample), lets the ones in flight finish and returns
+ * MRT_E_STOPPED unless every sample was already done. */
 MRT_API int mrt_render_stop(mrt_scene* scene);
 /* Renderer::getSample (Renderer.cpp:118-120). */
 MRT_API int mrt_render_progress(const mrt_scene* scene, uint32_t* samples_done);

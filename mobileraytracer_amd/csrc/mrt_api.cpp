@@ -87,9 +87,21 @@ struct mrt_scene {
     std::vector<mrt_light> lights;
     uint32_t n_scene_mats = 0;
     DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
-    DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3];
+    DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes;
     DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wavefront;
+    // Host copy of the light tree (pairs relative to its own numbering) for
+    // the combined walk tree; light slots follow the triangles in tri_geo.
+    BvhOut light_bvh;
+    uint32_t n_light_slots = 0;
+    // Wavefront loop: pinned snapshots of the device counters (two, read
+    // one batch behind the launches) and event pairs around trace launches.
+    uint32_t* counts_host = nullptr;
+    std::vector<hipEvent_t> trace_events;
     SceneDev dev{};
+    ~mrt_scene() {
+        if (counts_host) (void)hipHostFree(counts_host);
+        for (hipEvent_t e : trace_events) (void)hipEventDestroy(e);
+    }
     bool built = false;
     bool generic = false;
     uint32_t depth_need = 0;
@@ -107,10 +119,12 @@ uint64_t buffers_bytes(const mrt_scene* s) {
     return b;
 }
 
-void pack_geo(const mrt_triangle& t, float4* g) {
+void pack_geo(const mrt_triangle& t, uint32_t id, float4* g) {
+    float idf;
+    std::memcpy(&idf, &id, 4);
     g[0] = make_float4(t.a[0], t.a[1], t.a[2], t.ab[0]);
     g[1] = make_float4(t.ab[1], t.ab[2], t.ac[0], t.ac[1]);
-    g[2] = make_float4(t.ac[2], 0.0f, 0.0f, 0.0f);
+    g[2] = make_float4(t.ac[2], idf, 0.0f, 0.0f);
 }
 void pack_shade(const mrt_triangle& t, float4* s) {
     float mid;
@@ -137,6 +151,84 @@ void fill_accel_bvh(Accel* a, const BvhOut& b, const DevBuf& nodes, const DevBuf
 int upload_bvh(mrt_scene* s, int slot, const BvhOut& b, hipStream_t st) {
     if (b.pairs.empty()) return MRT_OK;
     return s->nodes[slot].upload(b.pairs.data(), b.pairs.size() * sizeof(BNode), st);
+}
+
+// Child `side` (0 left, 1 right) of a walk-tree pair.
+void set_child(BNode* p, int side, const Box& b, uint32_t ref, uint32_t word) {
+    if (side == 0) {
+        p->q0 = make_float4(b.mn.x, b.mn.y, b.mn.z, b.mx.x);
+        p->q1.x = b.mx.y;
+        p->q1.y = b.mx.z;
+        p->q3.x = ref;
+        p->q3.y = word;
+    } else {
+        p->q1.z = b.mn.x;
+        p->q1.w = b.mn.y;
+        p->q2 = make_float4(b.mn.z, b.mx.x, b.mx.y, b.mx.z);
+        p->q3.z = ref;
+        p->q3.w = word;
+    }
+}
+
+uint32_t leaf_word(uint32_t slot, uint32_t count) { return (count << kSlotBits) | slot; }
+
+// Copies the pairs of one tree into the walk tree: interior references move
+// by `pair_base`, leaves become (count << 24 | slot_base + first).
+void append_pairs(std::vector<BNode>* w, const BvhOut& t, uint32_t pair_base, uint32_t slot_base) {
+    for (const BNode& p : t.pairs) {
+        BNode q = p;
+        q.q3.x = p.q3.y ? slot_base + p.q3.x : pair_base + p.q3.x;
+        q.q3.y = p.q3.y ? leaf_word(slot_base + p.q3.x, p.q3.y) : 0u;
+        q.q3.z = p.q3.w ? slot_base + p.q3.z : pair_base + p.q3.z;
+        q.q3.w = p.q3.w ? leaf_word(slot_base + p.q3.z, p.q3.w) : 0u;
+        w->push_back(q);
+    }
+}
+
+// The root child of a tree: its root box, as a leaf or as its first pair.
+void set_root_child(BNode* p, int side, const BvhOut& t, uint32_t pair_base, uint32_t slot_base) {
+    if (t.root_count > 0u) set_child(p, side, t.root, slot_base + t.root_ref, leaf_word(slot_base + t.root_ref, t.root_count));
+    else set_child(p, side, t.root, pair_base, 0u);
+}
+
+// Builds the combined walk tree of the persistent kernels (layout in
+// mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 24/8-bit leaf
+// word get none; they run the non-persistent kernels.
+int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
+    s->dev.walk_nodes = nullptr;
+    const uint64_t slots = s->tris.size() + static_cast<uint64_t>(s->n_light_slots);
+    if (slots >= (1ull << kSlotBits)) return MRT_OK;
+    uint32_t maxLeaf = std::max(tri.root_count, s->n_light_slots <= kLinearLights ? s->n_light_slots : 0u);
+    for (const BNode& p : tri.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
+    for (const BNode& p : s->light_bvh.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
+    if (maxLeaf > 255u) return MRT_OK;
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    const float inf = std::numeric_limits<float>::infinity();
+    const Box empty{f3{nan, nan, nan}, f3{nan, nan, nan}};
+    const uint32_t nTri = static_cast<uint32_t>(s->tris.size());
+    const uint32_t triBase = kWalkFirstPair;
+    const uint32_t lightBase = triBase + static_cast<uint32_t>(tri.pairs.size());
+    std::vector<BNode> w(kWalkFirstPair);
+    std::memset(w.data(), 0, w.size() * sizeof(BNode));
+    for (uint32_t r = 0; r < kWalkFirstPair; ++r) {
+        if (nTri > 0u) set_root_child(&w[r], 0, tri, triBase, 0u);
+        else set_child(&w[r], 0, empty, 0u, 0u);
+        set_child(&w[r], 1, empty, 0u, 0u);
+    }
+    if (s->n_light_slots > 0u) {
+        if (s->dev.lights_linear) set_child(&w[kWalkClosestRoot], 1, Box{f3{-inf, -inf, -inf}, f3{inf, inf, inf}}, nTri,
+                                            leaf_word(nTri, s->n_light_slots));
+        else set_root_child(&w[kWalkClosestRoot], 1, s->light_bvh, lightBase, nTri);
+    }
+    append_pairs(&w, tri, triBase, 0u);
+    append_pairs(&w, s->light_bvh, lightBase, nTri);
+    int rc;
+    if ((rc = s->walk_nodes.upload(w.data(), w.size() * sizeof(BNode), st)) != MRT_OK) return rc;
+    s->dev.walk_nodes = s->walk_nodes.as<BNode>();
+    s->dev.walk_light_pair = lightBase;
+    s->dev.walk_light_slot = nTri;
+    s->depth_need = std::max(s->depth_need, std::max(tri.depth, s->light_bvh.depth) + 1u);
+    return MRT_OK;
 }
 
 // Render parameters common to mrt_render and mrt_shade_rays.
@@ -231,6 +323,95 @@ int stage(DevBuf& buf, const void* src, size_t bytes, int mem, hipStream_t st, c
 
 uint32_t cap_blocks(const mrt_ctx* c) { return (c->num_cu ? c->num_cu : 256u) * 16u; }
 
+// The wavefront frame: iterations in batches of kBatch; after each batch the
+// counters are copied to pinned memory, and the host reads the previous
+// batch's copy, so the GPU never waits for the host.  The frame is done when
+// the extension, shadow and ended queues a batch left behind are all empty
+// (the batch queued meanwhile then finds nothing to do).  A stop request
+// lets running paths finish without starting new samples.
+int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p, const WfBuffers& wb,
+                  bool persistent, uint32_t trace_blocks, uint32_t persist_blocks, const StackCfg& sc, uint32_t* dbm,
+                  float* dacc, bool* stopped, float* kernel_ms, float* trace_ms, uint32_t* launches) {
+    constexpr uint32_t kBatch = 4;
+    const hipStream_t st = s->ctx->stream;
+    if (!s->counts_host) MRT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->counts_host), 2 * kCntWords * 4));
+    hipEvent_t b0, b1, chk[2];
+    MRT_HIP(hipEventCreate(&b0));
+    MRT_HIP(hipEventCreate(&b1));
+    MRT_HIP(hipEventCreate(&chk[0]));
+    MRT_HIP(hipEventCreate(&chk[1]));
+    struct Guard {
+        hipEvent_t* e;
+        ~Guard() {
+            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+        }
+    };
+    hipEvent_t all[4] = {b0, b1, chk[0], chk[1]};
+    Guard guard{all};
+    MRT_HIP(hipMemsetAsync(wb.counts, 0, kCntWords * sizeof(uint32_t), st));
+    MRT_HIP(hipEventRecord(b0, st));
+    WfLoop L{};
+    MRT_HIP(launch_wavefront_start(p, d->first_sample, wb, &L, st));
+    const uint64_t covered = 256ull * p.bsx * p.bsy;
+    const uint32_t samples = d->samples_pixel - d->first_sample;
+    // A sample takes at most depth_max + 1 iterations (one per vertex, one to
+    // finish); more than that means the queues never drained.
+    const uint64_t maxIters = static_cast<uint64_t>(samples) * (static_cast<uint64_t>(d->ray_depth_max) + 2u) +
+                              4u * kBatch;
+    uint32_t pend[2][3] = {};
+    uint32_t checks = 0;
+    bool stopReq = false;
+    size_t ev = 0;
+    while (true) {
+        if (!stopReq && s->stop.load()) stopReq = true;
+        for (uint32_t k = 0; k < kBatch; ++k) {
+            while (ev + 2 > s->trace_events.size()) {
+                hipEvent_t e;
+                MRT_HIP(hipEventCreate(&e));
+                s->trace_events.push_back(e);
+            }
+            MRT_HIP(launch_wavefront_iteration(s->dev, s->generic, persistent, d->instrument != 0u, p,
+                                               d->samples_pixel, stopReq, dbm, dacc, wb, &L, trace_blocks,
+                                               persist_blocks, sc, s->counters.as<unsigned long long>(),
+                                               s->err.as<uint32_t>(), s->trace_events[ev], s->trace_events[ev + 1],
+                                               st));
+            ev += 2;
+        }
+        const uint32_t slot = checks & 1u;
+        MRT_HIP(hipMemcpyAsync(s->counts_host + slot * kCntWords, wb.counts, kCntWords * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, st));
+        MRT_HIP(hipEventRecord(chk[slot], st));
+        std::copy(L.pending, L.pending + 3, pend[slot]);
+        if (checks > 0) {
+            const uint32_t prev = slot ^ 1u;
+            MRT_HIP(hipEventSynchronize(chk[prev]));
+            const uint32_t* c = s->counts_host + prev * kCntWords;
+            s->progress.store(d->first_sample + static_cast<uint32_t>(c[kCntDone] / covered));
+            if (c[pend[prev][0]] == 0u && c[pend[prev][1]] == 0u && c[pend[prev][2]] == 0u) break;
+        }
+        ++checks;
+        if (L.iter > maxIters) return fail(MRT_E_HIP, "wavefront queues did not drain");
+    }
+    MRT_HIP(hipEventRecord(b1, st));
+    MRT_HIP(hipEventSynchronize(b1));
+    uint32_t done = 0;
+    MRT_HIP(hipMemcpyAsync(&done, wb.counts + kCntDone, sizeof(done), hipMemcpyDeviceToHost, st));
+    MRT_HIP(hipStreamSynchronize(st));
+    const uint32_t progress = d->first_sample + static_cast<uint32_t>(done / covered);
+    s->progress.store(progress);
+    *stopped = stopReq && progress < d->samples_pixel;
+    MRT_HIP(hipEventElapsedTime(kernel_ms, b0, b1));
+    float total = 0.0f;
+    for (size_t i = 0; i < ev; i += 2) {
+        float t = 0.0f;
+        MRT_HIP(hipEventElapsedTime(&t, s->trace_events[i], s->trace_events[i + 1]));
+        total += t;
+    }
+    *trace_ms = total;
+    *launches = static_cast<uint32_t>(ev / 2);
+    return MRT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -323,13 +504,14 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
     MRT_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
     int rc;
+    // Scene triangles, then (below) one slot per area light in light-tree order.
+    std::vector<float4> geo(3 * s->tris.size());
     {
-        std::vector<float4> geo(3 * s->tris.size()), shade(5 * s->tris.size());
+        std::vector<float4> shade(5 * s->tris.size());
         for (size_t i = 0; i < s->tris.size(); ++i) {
-            pack_geo(s->tris[i], &geo[3 * i]);
+            pack_geo(s->tris[i], MRT_MAKE_ID(MRT_CAT_TRIANGLE, static_cast<uint32_t>(i)), &geo[3 * i]);
             pack_shade(s->tris[i], &shade[5 * i]);
         }
-        if ((rc = s->tri_geo.upload(geo.data(), geo.size() * sizeof(float4), st)) != MRT_OK) return rc;
         if ((rc = s->tri_shade.upload(shade.data(), shade.size() * sizeof(float4), st)) != MRT_OK) return rc;
         MRT_HIP(hipStreamSynchronize(st));
     }
@@ -363,7 +545,7 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
         std::vector<Box> lboxes;
         std::vector<uint32_t> lidx;
         for (size_t i = 0; i < s->lights.size(); ++i) {
-            pack_geo(s->lights[i].triangle, &lg[3 * i]);
+            pack_geo(s->lights[i].triangle, MRT_MAKE_ID(MRT_CAT_LIGHT, static_cast<uint32_t>(i)), &lg[3 * i]);
             pack_shade(s->lights[i].triangle, &ls[5 * i]);
             li[i].kind = s->lights[i].kind;
             li[i].material = s->n_scene_mats + static_cast<uint32_t>(i);
@@ -385,8 +567,11 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
         s->dev.n_lights = static_cast<uint32_t>(s->lights.size());
         s->dev.n_area_lights = nArea;
         s->dev.lights_linear = nArea <= kLinearLights ? 1u : 0u;
+        // Light slots of the walk tree: light-tree leaf order, or index order
+        // for the linear scan.
+        std::vector<uint32_t> slotLight = lidx;
         if (!s->dev.lights_linear) {
-            BvhOut lb;
+            BvhOut& lb = s->light_bvh;
             std::vector<uint32_t> perm;
             build_light_bvh(lboxes, &lb, &perm);
             for (uint32_t& p : perm) p = lidx[p];
@@ -394,7 +579,14 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
             if ((rc = s->perm[3].upload(perm.data(), perm.size() * sizeof(uint32_t), st)) != MRT_OK) return rc;
             fill_accel_bvh(&s->dev.light_acc, lb, s->nodes[3], &s->perm[3], nArea);
             s->depth_need = std::max(s->depth_need, lb.depth);
+            slotLight = perm;
         }
+        s->n_light_slots = static_cast<uint32_t>(slotLight.size());
+        const size_t base = geo.size();
+        geo.resize(base + 3 * slotLight.size());
+        for (size_t k = 0; k < slotLight.size(); ++k)
+            std::copy(&lg[3 * slotLight[k]], &lg[3 * slotLight[k]] + 3, &geo[base + 3 * k]);
+        if ((rc = s->tri_geo.upload(geo.data(), geo.size() * sizeof(float4), st)) != MRT_OK) return rc;
         MRT_HIP(hipStreamSynchronize(st));
     }
     s->dev.tri_geo = s->tri_geo.as<float4>();
@@ -444,6 +636,7 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
         a.count = counts[c];
         a.kind = accelerator;
     }
+    s->dev.walk_nodes = nullptr;
     if (accelerator == MRT_BVH) {
         for (int c = 0; c < 3; ++c) {
             std::vector<Box> boxes(counts[c]);
@@ -453,12 +646,14 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
                                                  : plane_box(s->plas[i]);
             BvhOut b;
             build_bvh_reference(boxes, leaf, &b);
-            const uint64_t need = buffers_bytes(s) + b.pairs.size() * sizeof(BNode);
+            const uint64_t need = buffers_bytes(s) + b.pairs.size() * sizeof(BNode) *
+                                                         (c == MRT_CAT_TRIANGLE ? 2u : 1u);
             if (s->ctx->budget && need > s->ctx->budget)
                 return fail(MRT_E_BUDGET, "BVH exceeds the HBM budget");
             if ((rc = upload_bvh(s, c, b, st)) != MRT_OK) return rc;
             fill_accel_bvh(&s->dev.acc[c], b, s->nodes[c], nullptr, counts[c]);
             s->depth_need = std::max(s->depth_need, b.depth);
+            if (c == MRT_CAT_TRIANGLE && (rc = build_walk_tree(s, b, st)) != MRT_OK) return rc;
         }
     } else if (accelerator == MRT_REGULAR_GRID) {
         const f3 camMin = params ? ld3(params->camera_min) : f3{0, 0, 0};
@@ -724,21 +919,27 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
     // Kernel shape: the wavefront pipeline covers the PathTracer with one
-    // light sample and no reflection/transmission; everything else runs the
+    // light sample, no reflection/transmission, RayDepthMax >= 1 and at most
+    // 65535 samples (16-bit per-slot sample index); everything else runs the
     // megakernel.
-    const bool wfEligible = d->shader == MRT_SHADER_PATHTRACER && d->samples_light == 1 && !s->dev.has_branches;
+    const bool wfEligible = d->shader == MRT_SHADER_PATHTRACER && d->samples_light == 1 && !s->dev.has_branches &&
+                            d->ray_depth_max >= 1 && d->samples_pixel <= 65535u &&
+                            static_cast<uint64_t>(w) * h * d->samples_pixel < (1ull << 32);
     bool useWavefront = false;
     if (d->kernel == MRT_KERNEL_WAVEFRONT) {
         if (!wfEligible)
-            return fail(MRT_E_UNSUPPORTED, "the wavefront kernels need the PathTracer, samples_light == 1 and no "
-                                           "reflective or transmissive materials");
+            return fail(MRT_E_UNSUPPORTED, "the wavefront kernels need the PathTracer, samples_light == 1, no "
+                                           "reflective or transmissive materials, ray_depth_max >= 1 and at most "
+                                           "65535 samples per pixel");
         useWavefront = true;
     } else if (d->kernel == MRT_KERNEL_AUTO) {
         useWavefront = wfEligible;
     }
     const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
     const bool persistent = useWavefront && d->persistent != 2u && !s->generic &&
-                            s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH;
+                            s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH && s->dev.walk_nodes != nullptr;
+    const uint32_t persistBlocks =
+        persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks) : 0u;
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
     StackCfg sc;
     if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
@@ -757,7 +958,9 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         wb.shadow = base + 8ull * slots;
         uint32_t* u = reinterpret_cast<uint32_t*>(base + 12ull * slots);
         wb.counts = u;
-        wb.keys = u + 64;
+        wb.ended[0] = u + 64;
+        wb.ended[1] = wb.ended[0] + slots;
+        wb.keys = wb.ended[1] + slots;
         wb.hist = wb.keys + slots;
         wb.offs = wb.hist + kSortBins;
         const uintptr_t t = (reinterpret_cast<uintptr_t>(wb.offs + kSortBins) + 255u) & ~static_cast<uintptr_t>(255u);
@@ -789,22 +992,21 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     MRT_HIP(hipEventCreate(&k1));
     MRT_HIP(hipEventRecord(e0, st));
     bool stopped = false;
-    float kernel_ms = 0.0f;
-    for (uint32_t sample = d->first_sample; sample < d->samples_pixel; ++sample) {
+    float kernel_ms = 0.0f, trace_ms = 0.0f;
+    uint32_t traceLaunches = 0;
+    if (useWavefront && d->first_sample < d->samples_pixel) {
+        if ((rc = run_wavefront(s, d, p, wb, persistent, traceBlocks, persistBlocks, sc, dbm, dacc, &stopped,
+                                &kernel_ms, &trace_ms, &traceLaunches)) != MRT_OK)
+            return rc;
+    }
+    for (uint32_t sample = d->first_sample; !useWavefront && sample < d->samples_pixel; ++sample) {
         if (s->stop.load()) {
             stopped = true;
             break;
         }
         MRT_HIP(hipEventRecord(k0, st));
-        if (useWavefront) {
-            MRT_HIP(launch_wavefront_pass(s->dev, s->generic, persistent, d->instrument != 0u, p, sample, dbm, dacc,
-                                          wb, traceBlocks, sc, s->counters.as<unsigned long long>(),
-                                          s->err.as<uint32_t>(), st));
-        } else {
-            MRT_HIP(launch_render_pass(s->dev, s->generic, s->dev.has_branches != 0u, d->instrument != 0u, p,
-                                       sample, dbm, dacc, sc, s->counters.as<unsigned long long>(),
-                                       s->err.as<uint32_t>(), st));
-        }
+        MRT_HIP(launch_render_pass(s->dev, s->generic, s->dev.has_branches != 0u, d->instrument != 0u, p, sample, dbm,
+                                   dacc, sc, s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), st));
         MRT_HIP(hipEventRecord(k1, st));
         MRT_HIP(hipEventSynchronize(k1));
         float kms = 0.0f;
@@ -843,6 +1045,8 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->covered_pixels = 256u * bsx * bsy;
         stats->ms_render = ms;
         stats->ms_kernel = kernel_ms;
+        stats->ms_trace = trace_ms;
+        stats->trace_launches = traceLaunches;
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;

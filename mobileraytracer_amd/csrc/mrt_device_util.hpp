@@ -11,7 +11,7 @@ constexpr uint32_t kStackLds = 16;  // traversal-stack entries per lane in LDS
 
 __device__ __forceinline__ Stack make_stack(uint32_t* lds, const StackCfg& sc, uint32_t gtid) {
     Stack st;
-    st.lds = lds + threadIdx.x;
+    st.lds = (LdsWord*)(lds + threadIdx.x);  // address-space cast: needs the C form
     st.lds_stride = kBlock;
     st.spill = sc.spill ? sc.spill + gtid : nullptr;
     st.spill_stride = sc.spill_stride;

@@ -1,10 +1,12 @@
 // Device-side scene layout (HBM) shared by the kernels and the host uploader.
 //
 // Triangles are split into a hot and a cold array:
-//   tri_geo   3 x float4 per triangle  A.xyz AB.x | AB.yz AC.xy | AC.z 0 0 0
-//             the 36 bytes Triangle::intersect reads (Triangle.cpp:57-103),
-//             padded to 48 so a leaf's two triangles are three aligned 16-byte
-//             loads each;
+//   tri_geo   3 x float4 per triangle  A.xyz AB.x | AB.yz AC.xy | AC.z id 0 0
+//             the 36 bytes Triangle::intersect reads (Triangle.cpp:57-103)
+//             plus the primitive id, padded to 48 so a triangle is three
+//             aligned 16-byte loads; after the scene triangles come the area
+//             light triangles in the order the light tree's leaves use
+//             (slots of the combined walk tree below);
 //   tri_shade 5 x float4 per triangle  nA nB nC cA cB cC materialId, read
 //             once per closest hit for shading.
 // BVH nodes are 64-byte child pairs: the two child boxes of one reference
@@ -13,6 +15,12 @@
 //   q0 = L.min.xyz L.max.x, q1 = L.max.yz R.min.xy, q2 = R.min.z R.max.xyz,
 //   q3 = (Lref, Lcount, Rref, Rcount); count > 0 marks a leaf whose
 //   primitives are [ref, ref + count), count == 0 an interior pair index.
+// The combined walk tree (persistent kernels) uses the same pairs with one
+// change: a leaf child's count word holds (count << 24) | first slot, the
+// slot indexing tri_geo, so queuing a leaf is one word.  Pair 0 holds the
+// triangle root box and the light root box, pair 1 the triangle root box and
+// an all-NaN box that no ray enters; triangle pairs start at pair 2, light
+// pairs at walk_light_pair.
 #pragma once
 #include "mrt_math.hpp"
 #include "../../include/mobilert_gpu.h"
@@ -24,6 +32,11 @@ struct alignas(16) BNode {
     uint4 q3;
 };
 static_assert(sizeof(BNode) == 64, "BVH node pair must be 64 bytes");
+
+// Walk tree: root pairs and the leaf word (count << kSlotBits | slot).
+constexpr uint32_t kWalkClosestRoot = 0u, kWalkAnyRoot = 1u, kWalkFirstPair = 2u;
+constexpr uint32_t kSlotBits = 24u;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 
 // One accelerator (Shader::{naive,regularGrid,bvh}{Planes,Spheres,Triangles}_).
 struct Accel {
@@ -66,6 +79,9 @@ struct SceneDev {
     uint32_t has_branches;   // any material with Ks or Kt > 0
     Accel acc[3];            // planes, spheres, triangles (Shader.cpp:144-148 order)
     Accel light_acc;         // conservative BVH over area lights (lights_linear == 0)
+    const BNode* walk_nodes; // combined walk tree (NULL: not built)
+    uint32_t walk_light_pair;  // first light pair of the walk tree
+    uint32_t walk_light_slot;  // first light slot of tri_geo (= triangle count)
 };
 
 // Material fields.

@@ -48,11 +48,15 @@ struct TraceStats {
     uint32_t prims[3];
 };
 
+// A word of LDS; the explicit address space keeps stack accesses on ds_*
+// instructions instead of flat ones once the pointer sits in a struct.
+typedef __attribute__((address_space(3))) uint32_t LdsWord;
+
 // Per-lane traversal stack: the first kLds entries live in LDS (stride =
 // block size, so lanes hit distinct banks), deeper entries in a global spill
 // area sized on the host from the tree depth.
 struct Stack {
-    uint32_t* lds;
+    LdsWord* lds;
     uint32_t* spill;
     uint32_t lds_stride;
     uint32_t spill_stride;
@@ -503,189 +507,175 @@ __device__ __forceinline__ HitT closest_hit(const SceneDev& s, const RayT& r, fl
     return h;
 }
 
-// ---------------------------------------------------- stepwise traversal --
-// The same BVH walk cut into single steps (one 64-byte node pair, or the root
-// test, per call) so that persistent kernels can refill a lane with a new
-// ray the moment its ray finishes instead of idling until the slowest ray of
-// the wave is done.  Covers scenes whose closest hit is the triangle BVH plus
-// the lights (no planes or spheres); the visiting rule, the culling and the
-// (t, id) key are those of bvh_closest / bvh_any above.
-constexpr uint32_t kRootPending = 0xFFFFFFFFu;
-enum : uint32_t { kPhaseTri = 0, kPhaseLightBvh = 1, kPhaseLightScan = 2, kPhaseDone = 3 };
+// ----------------------------------------------------------- combined walk --
+// Persistent kernels walk one tree that joins the triangle BVH and the light
+// BVH under a virtual root pair (layout in mrt_scene.hpp), one unit of work
+// per lane and call: a node pair visit or a single triangle test.  Leaves a
+// visit reaches are queued (at most three ranges per lane) and tested later,
+// so a wave runs visits and tests as separate, mostly convergent phases and
+// a lane can take a new ray the moment its ray is done.  The slab predicate on
+// the reference's own boxes (the virtual root tests the triangle root box as
+// BVH<T>::trace does, BVH.hpp:164-166), the culling margin and the (t, id)
+// key are those of bvh_closest / bvh_any, so the answer does not depend on
+// the visiting order.  Area lights of a scene with at most 16 of them sit in
+// one leaf under an infinite box: Scene::traceLights tests every one of them
+// (Scene.cpp:31-37), and under the (t, id) key a scan in any order equals the
+// reference's ascending scan with t < length_.
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
-struct Trav {
+struct Walk {
     RayT r;
     HitT h;
-    uint32_t node;
-    uint32_t phase;
+    uint32_t node;        // next pair to visit, kNoNode when none is left
+    uint32_t p0, p1, p2;  // queued leaves (count << 24 | slot), 0 = empty
 };
 
-__device__ __forceinline__ uint32_t phase_after_triangles(const SceneDev& s) {
-    if (s.n_area_lights == 0u) return kPhaseDone;
-    return s.lights_linear ? kPhaseLightScan : kPhaseLightBvh;
+__device__ __forceinline__ bool inv_finite(const RayT& r) {
+    return fabsf(r.inv.x) < INFINITY && fabsf(r.inv.y) < INFINITY && fabsf(r.inv.z) < INFINITY;
 }
 
-__device__ __forceinline__ void trav_begin(const SceneDev& s, Trav& t, Stack& st, float tmax) {
-    t.h.t = tmax;
-    t.h.id = MRT_ID_NONE;
-    t.h.u = 0.0f;
-    t.h.v = 0.0f;
-    t.node = kRootPending;
-    t.phase = s.acc[MRT_CAT_TRIANGLE].count > 0u ? kPhaseTri : phase_after_triangles(s);
+__device__ __forceinline__ void walk_begin(Walk& w, uint32_t root, float tmax, Stack& st) {
+    w.h.t = tmax;
+    w.h.id = MRT_ID_NONE;
+    w.h.u = 0.0f;
+    w.h.v = 0.0f;
+    w.node = root;
+    w.p0 = w.p1 = w.p2 = 0u;
     st.sp = 0;
 }
 
-// Leaf primitives [ref, ref + count) of the triangle (phase 0) or light
-// (phase 1) BVH.
-template <bool COUNT>
-__device__ __forceinline__ void step_leaf(const SceneDev& s, Trav& t, uint32_t ref, uint32_t count, bool light,
-                                          TraceStats& ts) {
-    for (uint32_t k = 0; k < count; ++k) {
-        const uint32_t slot = ref + k;
-        if (COUNT) ++ts.prims[light ? 1 : 0];
-        if (light) prim_closest<MRT_CAT_LIGHT, false>(s, s.light_acc.perm[slot], t.r, t.h);
-        else prim_closest<MRT_CAT_TRIANGLE, false>(s, slot, t.r, t.h);
-    }
+__device__ __forceinline__ bool walk_done(const Walk& w) { return w.node == kNoNode && w.p0 == 0u; }
+
+// Appends `leaf` to the queue when `take` holds (branch free).
+__device__ __forceinline__ void walk_queue(Walk& w, bool take, uint32_t leaf) {
+    const bool e0 = w.p0 == 0u, e1 = w.p1 == 0u;
+    w.p2 = (take && !e0 && !e1) ? leaf : w.p2;
+    w.p1 = (take && !e0 && e1) ? leaf : w.p1;
+    w.p0 = (take && e0) ? leaf : w.p0;
 }
 
-// One closest-hit step; returns true when the ray is finished.
-template <bool COUNT>
-__device__ bool step_closest(const SceneDev& s, Trav& t, Stack& st, TraceStats& ts) {
-    if (t.phase == kPhaseLightScan) {
-        for (uint32_t i = 0; i < s.n_lights; ++i) {
-            if (s.light_info[i].kind != MRT_LIGHT_AREA) continue;
-            if (COUNT) ++ts.prims[1];
-            prim_closest<MRT_CAT_LIGHT, true>(s, i, t.r, t.h);
-        }
-        t.phase = kPhaseDone;
-        return true;
-    }
-    if (t.phase == kPhaseDone) return true;
-    const bool light = t.phase == kPhaseLightBvh;
-    const Accel& a = light ? s.light_acc : s.acc[MRT_CAT_TRIANGLE];
-    bool advance = false;
-    if (t.node == kRootPending) {
-        float e;
-        if (!slab(f3{a.root_mn[0], a.root_mn[1], a.root_mn[2]}, f3{a.root_mx[0], a.root_mx[1], a.root_mx[2]}, t.r.o,
-                  t.r.inv, &e)) {
-            advance = true;
-        } else if (a.root_count > 0u) {
-            step_leaf<COUNT>(s, t, a.root_ref, a.root_count, light, ts);
-            advance = true;
-        } else {
-            t.node = 0;
-            st.sp = 0;
-        }
-    } else {
-        const BNode* nd = a.nodes + t.node;
-        const float4 q0 = nd->q0;
-        const float4 q1 = nd->q1;
-        const float4 q2 = nd->q2;
-        const uint4 q3 = nd->q3;
-        if (COUNT) ++ts.pairs[light ? 1 : 0];
-        float eL, eR;
-        const float lim = cull_limit(t.h.t);
-        bool hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, t.r.o, t.r.inv, &eL) && eL <= lim;
-        bool hR = slab(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, t.r.o, t.r.inv, &eR) && eR <= lim;
-        if (hL && q3.y > 0u) {
-            step_leaf<COUNT>(s, t, q3.x, q3.y, light, ts);
-            hL = false;
-        }
-        if (hR && q3.w > 0u) {
-            step_leaf<COUNT>(s, t, q3.z, q3.w, light, ts);
-            hR = false;
-        }
-        if (hL && hR) {
-            const bool leftFirst = eL <= eR;
-            t.node = leftFirst ? q3.x : q3.z;
-            st.push(leftFirst ? q3.z : q3.x);
-        } else if (hL) {
-            t.node = q3.x;
-        } else if (hR) {
-            t.node = q3.z;
-        } else if (st.sp == 0u) {
-            advance = true;
-        } else {
-            t.node = st.pop();
-        }
-    }
-    if (advance) {
-        t.phase = light ? kPhaseDone : phase_after_triangles(s);
-        t.node = kRootPending;
-        st.sp = 0;
-    }
-    return t.phase == kPhaseDone;
+// Drops the first queued triangle.
+__device__ __forceinline__ void walk_pop_prim(Walk& w) {
+    const bool more = (w.p0 >> kSlotBits) > 1u;
+    const uint32_t next = w.p0 - kSlotMask;  // slot + 1, count - 1
+    w.p0 = more ? next : w.p1;
+    w.p1 = more ? w.p1 : w.p2;
+    w.p2 = more ? w.p2 : 0u;
 }
 
-// One any-hit step over the triangle BVH; *occluded set when a blocker is
-// found.  Returns true when the ray is finished.
-template <bool COUNT>
-__device__ bool step_any(const SceneDev& s, Trav& t, float dist, Stack& st, TraceStats& ts, bool* occluded) {
-    const Accel& a = s.acc[MRT_CAT_TRIANGLE];
-    if (a.count == 0u) return true;
-    if (t.node == kRootPending) {
-        float e;
-        if (!slab(f3{a.root_mn[0], a.root_mn[1], a.root_mn[2]}, f3{a.root_mx[0], a.root_mx[1], a.root_mx[2]}, t.r.o,
-                  t.r.inv, &e))
-            return true;
-        if (a.root_count > 0u) {
-            for (uint32_t k = 0; k < a.root_count; ++k) {
-                if (COUNT) ++ts.prims[2];
-                if (prim_any<MRT_CAT_TRIANGLE>(s, a.root_ref + k, t.r, dist)) {
-                    *occluded = true;
-                    return true;
-                }
-            }
-            return true;
-        }
-        t.node = 0;
-        st.sp = 0;
-        return false;
-    }
-    const BNode* nd = a.nodes + t.node;
+// Slab test for rays whose inverse direction is finite in every axis: then no
+// (bound - origin) * inv product is NaN (boxes are finite, or infinite with a
+// finite origin, or all-NaN), so std::min / std::max agree with IEEE min /
+// max and this returns what slab() returns, with fewer instructions.  An
+// all-NaN box yields tmax = NaN and is rejected by both.
+MRT_HD bool slab_finite(const f3& mn, const f3& mx, const f3& o, const f3& inv, float* entry) {
+    const float x1 = (mn.x - o.x) * inv.x, x2 = (mx.x - o.x) * inv.x;
+    const float y1 = (mn.y - o.y) * inv.y, y2 = (mx.y - o.y) * inv.y;
+    const float z1 = (mn.z - o.z) * inv.z, z2 = (mx.z - o.z) * inv.z;
+    const float tmin = fmaxf(fmaxf(fminf(x1, x2), fminf(y1, y2)), fmaxf(fminf(z1, z2), 0.0f));
+    const float tmax = fminf(fminf(fmaxf(x1, x2), fmaxf(y1, y2)), fmaxf(z1, z2));
+    *entry = tmin;
+    return tmax >= tmin;
+}
+
+// One pair visit: queues reached leaves, moves to the nearer reached interior
+// child (pushing the other) or pops the stack.
+template <bool FINITE, bool COUNT>
+__device__ __forceinline__ void walk_visit(const SceneDev& s, Walk& w, float lim, Stack& st, TraceStats& ts,
+                                           int kind) {
+    const BNode* nd = s.walk_nodes + w.node;
     const float4 q0 = nd->q0;
     const float4 q1 = nd->q1;
     const float4 q2 = nd->q2;
     const uint4 q3 = nd->q3;
-    if (COUNT) ++ts.pairs[2];
-    const float lim = cull_limit(dist);
+    if (COUNT && w.node >= kWalkFirstPair) ++ts.pairs[kind == 2 ? 2 : (w.node >= s.walk_light_pair ? 1 : 0)];
     float eL, eR;
-    bool hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, t.r.o, t.r.inv, &eL) && eL <= lim;
-    bool hR = slab(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, t.r.o, t.r.inv, &eR) && eR <= lim;
-    if (hL && q3.y > 0u) {
-        for (uint32_t k = 0; k < q3.y; ++k) {
-            if (COUNT) ++ts.prims[2];
-            if (prim_any<MRT_CAT_TRIANGLE>(s, q3.x + k, t.r, dist)) {
-                *occluded = true;
-                return true;
-            }
-        }
-        hL = false;
-    }
-    if (hR && q3.w > 0u) {
-        for (uint32_t k = 0; k < q3.w; ++k) {
-            if (COUNT) ++ts.prims[2];
-            if (prim_any<MRT_CAT_TRIANGLE>(s, q3.z + k, t.r, dist)) {
-                *occluded = true;
-                return true;
-            }
-        }
-        hR = false;
-    }
-    if (hL && hR) {
-        const bool leftFirst = eL <= eR;
-        t.node = leftFirst ? q3.x : q3.z;
-        st.push(leftFirst ? q3.z : q3.x);
-    } else if (hL) {
-        t.node = q3.x;
-    } else if (hR) {
-        t.node = q3.z;
-    } else if (st.sp == 0u) {
-        return true;
+    bool hL, hR;
+    if (FINITE) {
+        hL = slab_finite(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, w.r.o, w.r.inv, &eL);
+        hR = slab_finite(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, w.r.o, w.r.inv, &eR);
     } else {
-        t.node = st.pop();
+        hL = slab(f3{q0.x, q0.y, q0.z}, f3{q0.w, q1.x, q1.y}, w.r.o, w.r.inv, &eL);
+        hR = slab(f3{q1.z, q1.w, q2.x}, f3{q2.y, q2.z, q2.w}, w.r.o, w.r.inv, &eR);
     }
-    return false;
+    hL = hL && eL <= lim;
+    hR = hR && eR <= lim;
+    const bool leafL = q3.y != 0u, leafR = q3.w != 0u;
+    walk_queue(w, hL && leafL, q3.y);
+    walk_queue(w, hR && leafR, q3.w);
+    const bool inL = hL && !leafL, inR = hR && !leafR;
+    const bool leftFirst = eL <= eR;
+    uint32_t next = (inL && (!inR || leftFirst)) ? q3.x : q3.z;
+    if (inL && inR) {
+        st.push(leftFirst ? q3.z : q3.x);
+    } else if (!inL && !inR) {
+        next = st.sp == 0u ? kNoNode : st.pop();
+    }
+    w.node = next;
+}
+
+__device__ __forceinline__ TriG walk_tri(const SceneDev& s, uint32_t slot, uint32_t* id) {
+    const float4* g = s.tri_geo + 3u * slot;
+    const float4 g0 = g[0], g1 = g[1], g2 = g[2];
+    *id = __float_as_uint(g2.y);
+    TriG t;
+    t.a = f3{g0.x, g0.y, g0.z};
+    t.ab = f3{g0.w, g1.x, g1.y};
+    t.ac = f3{g1.z, g1.w, g2.x};
+    return t;
+}
+
+// tri_test without early exits (the same operations; lanes of a wave rarely
+// agree on an exit, so the branches would only cost mask bookkeeping).
+__device__ __forceinline__ bool tri_test_flat(const TriG& g, const RayT& r, float* tt, float* uu, float* vv) {
+    const f3 perp = cross(r.d, g.ac);
+    const float det = dot(g.ab, perp);
+    const float inv = 1.0f / det;
+    const f3 s = sub(r.o, g.a);
+    const float u = inv * dot(s, perp);
+    const f3 q = cross(s, g.ab);
+    const float v = inv * dot(r.d, q);
+    const float t = inv * dot(g.ac, q);
+    *tt = t;
+    *uu = u;
+    *vv = v;
+    return !(fabsf(det) < kEpsilon) && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || (u + v) > 1.0f) &&
+           !(t < kEpsilon);
+}
+
+// Closest-hit test of the first queued triangle.
+template <bool COUNT>
+__device__ __forceinline__ void walk_test_closest(const SceneDev& s, Walk& w, TraceStats& ts) {
+    const uint32_t slot = w.p0 & kSlotMask;
+    if (COUNT) ++ts.prims[slot >= s.walk_light_slot ? 1 : 0];
+    uint32_t id;
+    const TriG g = walk_tri(s, slot, &id);
+    float t, u, v;
+    const bool ok = tri_test_flat(g, w.r, &t, &u, &v) && id != w.r.excl && key_better(t, id, w.h);
+    w.h.t = ok ? t : w.h.t;
+    w.h.id = ok ? id : w.h.id;
+    w.h.u = ok ? u : w.h.u;
+    w.h.v = ok ? v : w.h.v;
+    walk_pop_prim(w);
+}
+
+// Any-hit test of the first queued triangle (Triangle.cpp:105-136); a
+// blocker ends the walk.  Returns true when the ray is occluded.
+template <bool COUNT>
+__device__ __forceinline__ bool walk_test_any(const SceneDev& s, Walk& w, float dist, TraceStats& ts) {
+    const uint32_t slot = w.p0 & kSlotMask;
+    if (COUNT) ++ts.prims[2];
+    uint32_t id;
+    const TriG g = walk_tri(s, slot, &id);
+    float t, u, v;
+    const bool hitAny = tri_test_flat(g, w.r, &t, &u, &v);
+    const bool blocked = hitAny && id != w.r.excl && t > kEpsilon && t < dist;
+    walk_pop_prim(w);
+    if (blocked) {
+        w.node = kNoNode;
+        w.p0 = w.p1 = w.p2 = 0u;
+    }
+    return blocked;
 }
 
 template <bool GENERIC, bool COUNT>

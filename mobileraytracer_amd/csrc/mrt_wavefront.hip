@@ -1,17 +1,29 @@
-// Wavefront path tracer: one sample pass runs as
-//   generate -> (extend -> shade -> shadow) x RayDepthMax -> finish
-// kernels joined by queues in HBM.  shade() appends extension and shadow rays
-// to compacted queues (one atomic per wave: ballot + popcount + mbcnt), so
-// every traversal launch runs full waves of live rays, and each kernel keeps
-// its own register budget instead of the megakernel's union of all of them.
+// Wavefront path tracer with path regeneration.  Every covered pixel owns
+// one path slot and runs its samples one after the other; the frame is a
+// loop of iterations
+//   trace -> regen -> [reorder] -> shade
+// joined by compacted queues in HBM (one atomic per wave: ballot + popcount +
+// mbcnt):
+//   trace  traces the extension queue (closest hit) and the shadow queue (any
+//          hit, applying visible light samples to their paths);
+//   regen  finishes the paths that ended in the previous shade (their last
+//          shadow ray was just traced), averages the sample into the pixel
+//          (incrementalAvg, in sample order per pixel) and queues the camera
+//          ray of the pixel's next sample;
+//   shade  runs one PathTracer vertex per extension hit and queues the next
+//          extension ray, the light sample's shadow ray, or the path's end.
+// Slots start new samples as soon as their previous sample ends, so every
+// iteration traces a full queue until the last samples drain, instead of
+// paying each bounce's slowest rays once per sample pass.
 //
-// Per-path state (48 bytes per covered pixel):
-//   sA = (T.xyz, flags)   flags: bit 0 found, bit 1 ends, bits 8..15 depth
+// Per-slot state (48 bytes):
+//   sA = (T.xyz, flags)   flags: bit 0 found, bit 1 ends, bits 8..15 depth,
+//                         bits 16..31 sample index
 //   sB = (acc_all.xyz, pixel id)
 //   sC = (acc_prefix.xyz, 0)
 // with the chain semantics of mrt_path.hpp (PathTracer.cpp:25-164): the
-// shadow kernel adds T_d * Ld_d and records the first vertex with
-// any(Ld > 0); finish() picks acc_prefix when the chain ended on an emitter.
+// shadow result adds T_d * Ld_d and records the first vertex with
+// any(Ld > 0); regen picks acc_prefix when the chain ended on an emitter.
 // Valid for the PathTracer with one light sample per vertex and no
 // reflection/transmission materials (the host checks; other cases use the
 // megakernel).
@@ -23,9 +35,12 @@
 namespace mrt {
 
 constexpr uint32_t kFound = 1u, kEnds = 2u;
+constexpr uint32_t kDeadEntry = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t flags_depth(uint32_t f) { return (f >> 8) & 0xFFu; }
+__device__ __forceinline__ uint32_t flags_sample(uint32_t f) { return f >> 16; }
 __device__ __forceinline__ uint32_t with_depth(uint32_t f, uint32_t d) { return (f & ~0xFF00u) | (d << 8); }
+__device__ __forceinline__ uint32_t new_flags(uint32_t sample) { return (sample << 16) | (1u << 8); }
 
 __device__ __forceinline__ uint32_t spread5(uint32_t v) {
     v &= 31u;
@@ -73,46 +88,81 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
     return base + rank;
 }
 
-// ------------------------------------------------------------- generate --
-__global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t sample, uint32_t slots,
-                                                        WfBuffers b) {
-    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t slot = logical * kBlock + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        b.counts[0] = slots;
-        b.counts[3] = 0u;
-        b.counts[4] = 0u;
+// Writes extension-queue entry `qe`, and with `keyed` its sort key when
+// reordering (entries that skip the reorder, the first camera rays, have
+// none: their histogram counts would shift every scattered position).
+__device__ __forceinline__ void put_ray(const WfBuffers& b, float4* queue, uint32_t qe, f3 o, f3 d, uint32_t excl,
+                                        uint32_t slot, bool keyed = true) {
+    queue[2u * qe] = make_float4(o.x, o.y, o.z, d.x);
+    queue[2u * qe + 1u] = make_float4(d.y, d.z, __uint_as_float(excl), __uint_as_float(slot));
+    if (keyed && b.sort) {
+        const uint32_t key = ray_key(b, o, d);
+        b.keys[qe] = key;
+        atomicAdd(b.hist + key, 1u);
     }
-    if (slot >= slots) return;
-    uint32_t x, y;
-    slot_pixel(p, slot, &x, &y);
-    float4* q = b.queue_a + 2u * slot;
-    if (x >= 16u * p.bsx || y >= 16u * p.bsy) {
-        q[1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
-        return;
-    }
-    const uint32_t pid = y * p.width + x;
-    f3 o, d;
-    primary_ray(p, x, y, pid, sample, &o, &d);
-    q[0] = make_float4(o.x, o.y, o.z, d.x);
-    q[1] = make_float4(d.y, d.z, __uint_as_float(MRT_ID_NONE), __uint_as_float(slot));
-    b.sA[slot] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(with_depth(0u, 1u)));
+}
+
+// Path state of a new sample of slot `slot` (pixel `pid`).
+__device__ __forceinline__ void reset_path(const WfBuffers& b, uint32_t slot, uint32_t pid, uint32_t sample) {
+    b.sA[slot] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(new_flags(sample)));
     b.sB[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(pid));
     b.sC[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// ------------------------------------------------------------- extend --
+// ------------------------------------------------------------- generate --
+// Camera rays of the first sample of every covered slot.
+__global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t sample, uint32_t slots,
+                                                        WfBuffers b, float4* queue, uint32_t* count) {
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t slot = logical * kBlock + threadIdx.x;
+    bool live = false;
+    uint32_t x = 0, y = 0;
+    if (slot < slots) {
+        slot_pixel(p, slot, &x, &y);
+        live = x < 16u * p.bsx && y < 16u * p.bsy;
+    }
+    f3 o{}, d{};
+    if (live) {
+        const uint32_t pid = y * p.width + x;
+        primary_ray(p, x, y, pid, sample, &o, &d);
+        reset_path(b, slot, pid, sample);
+    }
+    const uint32_t qe = wave_append(count, live);
+    if (live) put_ray(b, queue, qe, o, d, MRT_ID_NONE, slot, false);
+}
+
+// ------------------------------------------------------------- shadow --
+// Applies a visible light sample to its path: acc_all += T_d * Ld_d, and the
+// first vertex with any(Ld > 0) snapshots acc_prefix.
+__device__ __forceinline__ void apply_light(const WfBuffers& b, const float4* e) {
+    const float4 l = e[2], tv = e[3];
+    const uint32_t slot = __float_as_uint(l.w);
+    const f3 Ld = f3{l.x, l.y, l.z};
+    const float4 sbv = b.sB[slot];
+    const f3 acc = add(f3{sbv.x, sbv.y, sbv.z}, mul(f3{tv.x, tv.y, tv.z}, Ld));
+    b.sB[slot] = make_float4(acc.x, acc.y, acc.z, sbv.w);
+    const float4 sa = b.sA[slot];
+    const uint32_t flags = __float_as_uint(sa.w);
+    if (!(flags & kFound) && any_gt0(Ld)) {
+        b.sA[slot] = make_float4(sa.x, sa.y, sa.z, __uint_as_float(flags | kFound));
+        b.sC[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    }
+}
+
+// --------------------------------------------------- trace (chunked) --
+// Closest hit of every extension-queue entry, a static range per workgroup
+// (scenes with planes or spheres, or a grid / naive accelerator).  Clears the
+// counters that regen and shade fill next.
 template <bool GENERIC, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_wf_extend(SceneDev s, const float4* __restrict__ queue,
                                                       const uint32_t* __restrict__ count, float4* __restrict__ hits,
-                                                      uint32_t* clear_a, uint32_t* clear_b, StackCfg sc,
+                                                      uint32_t* clear_q, uint32_t* clear_e, StackCfg sc,
                                                       unsigned long long* counters, uint32_t* err) {
     __shared__ uint32_t lds[kStackLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    // The queues shade() fills next were drained by the previous launches.
     if (gtid == 0) {
-        *clear_a = 0u;
-        *clear_b = 0u;
+        *clear_q = 0u;
+        *clear_e = 0u;
     }
     Stack st = make_stack(lds, sc, gtid);
     RayCounts rc{};
@@ -120,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(SceneDev s, const float4* 
     block_range(*count, &begin, &end);
     for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
         const float4 a = queue[2u * i], c = queue[2u * i + 1u];
-        if (__float_as_uint(c.w) == 0xFFFFFFFFu) continue;
+        if (__float_as_uint(c.w) == kDeadEntry) continue;
         const RayT r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.z));
         const HitT h = closest_hit<GENERIC, COUNT>(s, r, kRayLengthMax, st, rc.ts);
         hits[i] = make_float4(h.t, __uint_as_float(h.id), h.u, h.v);
@@ -129,69 +179,227 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(SceneDev s, const float4* 
     else if (st.overflow) atomicOr(err, 1u);
 }
 
-// Persistent closest-hit over the queue for BVH scenes of triangles and
-// lights: a lane takes the next queue entry as soon as its ray is done
-// (refills are batched per wave: at least 8 idle lanes, or all of them), so
-// the wave never waits for its slowest ray.
+// Shader::shadowTrace of every queued light sample, a static range per
+// workgroup.
+template <bool GENERIC, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_wf_shadow(SceneDev s, const uint32_t* __restrict__ count, WfBuffers b,
+                                                      StackCfg sc, unsigned long long* counters, uint32_t* err) {
+    __shared__ uint32_t lds[kStackLds * kBlock];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st = make_stack(lds, sc, gtid);
+    RayCounts rc{};
+    uint32_t begin, end;
+    block_range(*count, &begin, &end);
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        const float4* e = b.shadow + 4u * i;
+        const float4 a = e[0], c = e[1];
+        ++rc.shadow;
+        const RayT r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.w));
+        if (any_hit<GENERIC, COUNT>(s, r, c.z, st, rc.ts)) continue;
+        apply_light(b, e);
+    }
+    if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
+    else flush_counts<false>(rc, counters, st.overflow, err);
+}
+
+// ------------------------------------------------- trace (persistent) --
+// Refill rule: a wave fetches new queue entries when at least 8 lanes are
+// idle, or when no lane has a ray left.
+__device__ __forceinline__ bool wave_wants_refill(bool need, bool has) {
+    const unsigned long long needMask = __ballot(need);
+    return needMask != 0ull && (__popcll(needMask) >= 8 || __ballot(has) == 0ull);
+}
+
+// Test of the first queued triangle for a closest-hit lane (the (t, id) key)
+// or a shadow lane (Triangle::intersect(Ray, dist), Triangle.cpp:105-136, with
+// dist held in h.t); a blocker ends the shadow lane's walk.  Returns true when
+// a shadow lane is occluded.
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_wf_extend_p(SceneDev s, const float4* __restrict__ queue,
-                                                        const uint32_t* __restrict__ count, float4* __restrict__ hits,
-                                                        uint32_t* clear_a, uint32_t* clear_b, uint32_t* clear_c,
-                                                        uint32_t* fetch, StackCfg sc, unsigned long long* counters,
-                                                        uint32_t* err) {
+__device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, bool shadow, TraceStats& ts) {
+    const uint32_t slot = w.p0 & kSlotMask;
+    if (COUNT) ++ts.prims[shadow ? 2 : (slot >= s.walk_light_slot ? 1 : 0)];
+    uint32_t id;
+    const TriG g = walk_tri(s, slot, &id);
+    float t, u, v;
+    const bool hit = tri_test_flat(g, w.r, &t, &u, &v) && id != w.r.excl;
+    const bool better = !shadow && hit && key_better(t, id, w.h);
+    const bool blocked = shadow && hit && t > kEpsilon && t < w.h.t;
+    w.h.t = better ? t : w.h.t;
+    w.h.id = better ? id : w.h.id;
+    w.h.u = better ? u : w.h.u;
+    w.h.v = better ? v : w.h.v;
+    walk_pop_prim(w);
+    if (blocked) {
+        w.node = kNoNode;
+        w.p0 = w.p1 = w.p2 = 0u;
+    }
+    return blocked;
+}
+
+// One unit of walk work for the wave: node visits when at least as many
+// lanes can visit as can test, else one triangle test per lane with queued
+// leaves.  A lane visits only with at most one queued leaf, so its three
+// queue entries never overflow.  Returns true when a shadow lane found a
+// blocker.
+template <bool COUNT>
+__device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, bool shadow, bool finite, Stack& st,
+                                          TraceStats& ts) {
+    const bool canVisit = has && w.node != kNoNode && w.p1 == 0u;
+    const bool canTest = has && w.p0 != 0u;
+    const uint32_t nv = static_cast<uint32_t>(__popcll(__ballot(canVisit)));
+    const uint32_t nt = static_cast<uint32_t>(__popcll(__ballot(canTest)));
+    if (nv >= nt) {
+        const float lim = cull_limit(w.h.t);
+        const int kind = shadow ? 2 : 0;
+        if (__ballot(canVisit && !finite) == 0ull) {
+            if (canVisit) walk_visit<true, COUNT>(s, w, lim, st, ts, kind);
+        } else if (canVisit) {
+            walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
+        }
+        return false;
+    }
+    return canTest && walk_test<COUNT>(s, w, shadow, ts);
+}
+
+// Persistent trace of the extension queue followed by the shadow queue, for
+// BVH scenes of triangles and lights: a lane takes the next entry the moment
+// its ray is done, so a wave never waits for its slowest ray.  Clears the
+// counters that regen and shade fill next.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4* __restrict__ queue,
+                                                       const uint32_t* __restrict__ count_q,
+                                                       const uint32_t* __restrict__ count_s, float4* __restrict__ hits,
+                                                       WfBuffers b, uint32_t* clear_q, uint32_t* clear_e,
+                                                       uint32_t* fetch, StackCfg sc, unsigned long long* counters,
+                                                       uint32_t* err) {
     __shared__ uint32_t lds[kStackLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     if (gtid == 0) {
-        *clear_a = 0u;
-        *clear_b = 0u;
-        *clear_c = 0u;
+        *clear_q = 0u;
+        *clear_e = 0u;
     }
     Stack st = make_stack(lds, sc, gtid);
     RayCounts rc{};
-    const uint32_t n = *count;
-    Trav t;
+    const uint32_t nq = *count_q;
+    const uint32_t n = nq + *count_s;
+    Walk w;
+    w.node = kNoNode;
+    w.p0 = w.p1 = w.p2 = 0u;
+    w.h.t = 0.0f;
     uint32_t idx = 0;
-    bool has = false, exhausted = false;
+    bool has = false, exhausted = false, finite = true, shadow = false;
     while (true) {
         const bool need = !has && !exhausted;
-        const unsigned long long needMask = __ballot(need);
-        if (needMask != 0ull && (__popcll(needMask) >= 8 || __ballot(has) == 0ull)) {
-            const uint32_t slot = wave_append(fetch, need);
+        if (wave_wants_refill(need, has)) {
+            const uint32_t e = wave_append(fetch, need);
             if (need) {
-                if (slot < n) {
-                    const float4 a = queue[2u * slot], c = queue[2u * slot + 1u];
-                    if (__float_as_uint(c.w) != 0xFFFFFFFFu) {
-                        idx = slot;
-                        t.r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.z));
-                        trav_begin(s, t, st, kRayLengthMax);
+                if (e < nq) {
+                    const float4 a = queue[2u * e], c = queue[2u * e + 1u];
+                    if (__float_as_uint(c.w) != kDeadEntry) {
+                        idx = e;
+                        shadow = false;
+                        w.r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.z));
+                        walk_begin(w, kWalkClosestRoot, kRayLengthMax, st);
+                        finite = inv_finite(w.r);
                         has = true;
                     }
+                } else if (e < n) {
+                    idx = e - nq;
+                    const float4* sh = b.shadow + 4u * idx;
+                    const float4 a = sh[0], c = sh[1];
+                    shadow = true;
+                    w.r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.w));
+                    walk_begin(w, kWalkAnyRoot, c.z, st);
+                    finite = inv_finite(w.r);
+                    ++rc.shadow;
+                    has = true;
                 } else {
                     exhausted = true;
                 }
             }
         }
         if (__ballot(has || !exhausted) == 0ull) break;
-        if (has && step_closest<COUNT>(s, t, st, rc.ts)) {
-            hits[idx] = make_float4(t.h.t, __uint_as_float(t.h.id), t.h.u, t.h.v);
+        const bool occluded = walk_work<COUNT>(s, w, has, shadow, finite, st, rc.ts);
+        if (has && walk_done(w)) {
+            if (!shadow) hits[idx] = make_float4(w.h.t, __uint_as_float(w.h.id), w.h.u, w.h.v);
+            else if (!occluded) apply_light(b, b.shadow + 4u * idx);
             has = false;
         }
     }
     if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
-    else if (st.overflow) atomicOr(err, 1u);
+    else flush_counts<false>(rc, counters, st.overflow, err);
+}
+
+// ------------------------------------------------------------- regen --
+// Finishes the paths that ended in the previous shade (PathTracer.cpp:144-164
+// result selection, then incrementalAvg of Renderer.cpp:101-104 with the
+// slot's own sample count) and starts each slot's next sample.  Clears the
+// shadow counter and the trace fetch cursor for the next iteration.
+__global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t spp, uint32_t stop,
+                                                     const uint32_t* __restrict__ ended,
+                                                     const uint32_t* __restrict__ count_e, WfBuffers b,
+                                                     float4* __restrict__ queue_out, uint32_t* count_out,
+                                                     uint32_t* clear_s, uint32_t* clear_fetch, uint32_t* done,
+                                                     uint32_t* __restrict__ bitmap, float* __restrict__ accum) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *clear_s = 0u;
+        *clear_fetch = 0u;
+    }
+    const uint32_t n = *count_e;
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t rounds = (n + stride - 1u) / stride;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t finished = 0;
+    for (uint32_t k = 0; k < rounds; ++k) {
+        const uint32_t i = gtid + k * stride;
+        bool again = false;
+        uint32_t slot = 0;
+        f3 o{}, d{};
+        if (i < n) {
+            slot = ended[i];
+            const float4 sa = b.sA[slot], sbv = b.sB[slot];
+            const uint32_t flags = __float_as_uint(sa.w);
+            f3 c = f3{sbv.x, sbv.y, sbv.z};
+            if ((flags & kEnds) && (flags & kFound)) {
+                const float4 scv = b.sC[slot];
+                c = f3{scv.x, scv.y, scv.z};
+            }
+            const uint32_t pid = __float_as_uint(sbv.w);
+            const uint32_t sample = flags_sample(flags);
+            uint32_t x, y;
+            slot_pixel(p, slot, &x, &y);
+            const uint32_t bi = y * p.real_width + x;
+            bitmap[bi] = incremental_avg(c, bitmap[bi], sample + 1u);
+            if (accum) {
+                accum[3u * pid + 0u] += c.x;
+                accum[3u * pid + 1u] += c.y;
+                accum[3u * pid + 2u] += c.z;
+            }
+            ++finished;
+            if (!stop && sample + 1u < spp) {
+                primary_ray(p, x, y, pid, sample + 1u, &o, &d);
+                reset_path(b, slot, pid, sample + 1u);
+                again = true;
+            }
+        }
+        const uint32_t qe = wave_append(count_out, again);
+        if (again) put_ray(b, queue_out, qe, o, d, MRT_ID_NONE, slot);
+    }
+    const uint32_t total = wave_sum(finished);
+    if ((threadIdx.x & 63u) == 0u && total) atomicAdd(done, total);
 }
 
 // ------------------------------------------------------------- shade --
-// One PathTracer vertex per queue entry (PathTracer.cpp:25-133).
-__global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p, uint32_t sample,
-                                                     const float4* __restrict__ queue_in,
+// One PathTracer vertex per extension-queue entry (PathTracer.cpp:25-133):
+// queues the next extension ray, the light sample's shadow ray, and the
+// slot when its path ends here.
+__global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p, const float4* __restrict__ queue_in,
                                                      const uint32_t* __restrict__ count_in,
                                                      const float4* __restrict__ hits, WfBuffers b,
                                                      float4* __restrict__ queue_out, uint32_t* count_out,
-                                                     uint32_t* count_shadow, uint32_t* clear_fetch,
-                                                     unsigned long long* counters) {
+                                                     uint32_t* count_shadow, uint32_t* __restrict__ ended,
+                                                     uint32_t* count_ended, unsigned long long* counters) {
     RayCounts rc{};
-    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_fetch = 0u;
     const uint32_t n = *count_in;
     const uint32_t stride = gridDim.x * kBlock;
     // Every lane of a wave runs the same number of iterations so that the
@@ -200,14 +408,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p,
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     for (uint32_t k = 0; k < rounds; ++k) {
         const uint32_t i = gtid + k * stride;
-        bool wantExt = false, wantShadow = false;
+        bool wantExt = false, wantShadow = false, wantEnd = false;
         f3 nextO{}, nextD{}, shD{}, Ld{}, Tv{};
-        uint32_t excl = MRT_ID_NONE, slot = 0xFFFFFFFFu;
+        uint32_t excl = MRT_ID_NONE, slot = kDeadEntry;
         float dist = 0.0f;
         if (i < n) {
             const float4 a = queue_in[2u * i], c = queue_in[2u * i + 1u];
             slot = __float_as_uint(c.w);
-            if (slot != 0xFFFFFFFFu) {
+            if (slot != kDeadEntry) {
                 const RayT r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.z));
                 const float4 hv = hits[i];
                 HitT h;
@@ -218,6 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p,
                 float4 sa = b.sA[slot];
                 uint32_t flags = __float_as_uint(sa.w);
                 const uint32_t depth = flags_depth(flags);
+                const uint32_t sample = flags_sample(flags);
                 if (depth == 1u) ++rc.camera;
                 else ++rc.extension;
                 if (h.id != MRT_ID_NONE) {
@@ -255,18 +464,11 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p,
                         }
                     }
                 }
+                wantEnd = !wantExt;
             }
         }
         const uint32_t qe = wave_append(count_out, wantExt);
-        if (wantExt) {
-            queue_out[2u * qe] = make_float4(nextO.x, nextO.y, nextO.z, nextD.x);
-            queue_out[2u * qe + 1u] = make_float4(nextD.y, nextD.z, __uint_as_float(excl), __uint_as_float(slot));
-            if (b.sort) {
-                const uint32_t key = ray_key(b, nextO, nextD);
-                b.keys[qe] = key;
-                atomicAdd(b.hist + key, 1u);
-            }
-        }
+        if (wantExt) put_ray(b, queue_out, qe, nextO, nextD, excl, slot);
         const uint32_t qs = wave_append(count_shadow, wantShadow);
         if (wantShadow) {
             float4* e = b.shadow + 4u * qs;
@@ -275,114 +477,16 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p,
             e[2] = make_float4(Ld.x, Ld.y, Ld.z, __uint_as_float(slot));
             e[3] = make_float4(Tv.x, Tv.y, Tv.z, 0.0f);
         }
+        const uint32_t qd = wave_append(count_ended, wantEnd);
+        if (wantEnd) ended[qd] = slot;
     }
     flush_counts<false>(rc, counters, 0u, nullptr);
 }
 
-// ------------------------------------------------------------- shadow --
-// Shader::shadowTrace of each queued light sample; a visible sample adds
-// T_d * Ld_d to acc_all and may mark the first vertex with any(Ld > 0).
-template <bool GENERIC, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_wf_shadow(SceneDev s, const uint32_t* __restrict__ count, WfBuffers b,
-                                                      StackCfg sc, unsigned long long* counters, uint32_t* err) {
-    __shared__ uint32_t lds[kStackLds * kBlock];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st = make_stack(lds, sc, gtid);
-    RayCounts rc{};
-    uint32_t begin, end;
-    block_range(*count, &begin, &end);
-    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
-        const float4* e = b.shadow + 4u * i;
-        const float4 a = e[0], c = e[1];
-        ++rc.shadow;
-        const RayT r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.w));
-        if (any_hit<GENERIC, COUNT>(s, r, c.z, st, rc.ts)) continue;
-        const float4 l = e[2], tv = e[3];
-        const uint32_t slot = __float_as_uint(l.w);
-        const f3 Ld = f3{l.x, l.y, l.z};
-        const float4 sbv = b.sB[slot];
-        const f3 acc = add(f3{sbv.x, sbv.y, sbv.z}, mul(f3{tv.x, tv.y, tv.z}, Ld));
-        b.sB[slot] = make_float4(acc.x, acc.y, acc.z, sbv.w);
-        const float4 sa = b.sA[slot];
-        const uint32_t flags = __float_as_uint(sa.w);
-        if (!(flags & kFound) && any_gt0(Ld)) {
-            b.sA[slot] = make_float4(sa.x, sa.y, sa.z, __uint_as_float(flags | kFound));
-            b.sC[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
-        }
-    }
-    if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
-    else flush_counts<false>(rc, counters, st.overflow, err);
-}
-
-// Applies a visible light sample to its path (see k_wf_shadow).
-__device__ __forceinline__ void apply_light(const WfBuffers& b, const float4* e) {
-    const float4 l = e[2], tv = e[3];
-    const uint32_t slot = __float_as_uint(l.w);
-    const f3 Ld = f3{l.x, l.y, l.z};
-    const float4 sbv = b.sB[slot];
-    const f3 acc = add(f3{sbv.x, sbv.y, sbv.z}, mul(f3{tv.x, tv.y, tv.z}, Ld));
-    b.sB[slot] = make_float4(acc.x, acc.y, acc.z, sbv.w);
-    const float4 sa = b.sA[slot];
-    const uint32_t flags = __float_as_uint(sa.w);
-    if (!(flags & kFound) && any_gt0(Ld)) {
-        b.sA[slot] = make_float4(sa.x, sa.y, sa.z, __uint_as_float(flags | kFound));
-        b.sC[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
-    }
-}
-
-// Persistent shadow rays for BVH scenes of triangles and lights (refill as
-// in k_wf_extend_p).
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_wf_shadow_p(SceneDev s, const uint32_t* __restrict__ count, WfBuffers b,
-                                                        uint32_t* fetch, StackCfg sc, unsigned long long* counters,
-                                                        uint32_t* err) {
-    __shared__ uint32_t lds[kStackLds * kBlock];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st = make_stack(lds, sc, gtid);
-    RayCounts rc{};
-    const uint32_t n = *count;
-    Trav t;
-    uint32_t idx = 0;
-    float dist = 0.0f;
-    bool has = false, exhausted = false;
-    while (true) {
-        const bool need = !has && !exhausted;
-        const unsigned long long needMask = __ballot(need);
-        if (needMask != 0ull && (__popcll(needMask) >= 8 || __ballot(has) == 0ull)) {
-            const uint32_t slot = wave_append(fetch, need);
-            if (need) {
-                if (slot < n) {
-                    const float4* e = b.shadow + 4u * slot;
-                    const float4 a = e[0], c = e[1];
-                    idx = slot;
-                    t.r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.w));
-                    dist = c.z;
-                    t.node = kRootPending;
-                    st.sp = 0;
-                    ++rc.shadow;
-                    has = true;
-                } else {
-                    exhausted = true;
-                }
-            }
-        }
-        if (__ballot(has || !exhausted) == 0ull) break;
-        if (has) {
-            bool occluded = false;
-            if (step_any<COUNT>(s, t, dist, st, rc.ts, &occluded)) {
-                if (!occluded) apply_light(b, b.shadow + 4u * idx);
-                has = false;
-            }
-        }
-    }
-    if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
-    else flush_counts<false>(rc, counters, st.overflow, err);
-}
-
 // ------------------------------------------------------------- reorder --
-// Counting-sort scatter of shade()'s output into the traced queue, after an
-// exclusive scan turned the histogram into offsets; clears the histogram for
-// the next bounce.  The order inside a bin varies from run to run; no path's
+// Counting-sort scatter of the next extension queue, after an exclusive scan
+// turned the histogram into offsets; clears the histogram for the next
+// iteration.  The order inside a bin varies from run to run; no path's
 // result depends on its queue position.
 __global__ __launch_bounds__(kBlock) void k_wf_scatter(const uint32_t* __restrict__ count, WfBuffers b) {
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -394,31 +498,6 @@ __global__ __launch_bounds__(kBlock) void k_wf_scatter(const uint32_t* __restric
         b.queue_a[2u * pos + 1u] = b.queue_b[2u * i + 1u];
     }
     for (uint32_t k = gtid; k < kSortBins; k += stride) b.hist[k] = 0u;
-}
-
-// ------------------------------------------------------------- finish --
-__global__ __launch_bounds__(kBlock) void k_wf_finish(RenderParams p, uint32_t sample, uint32_t slots, WfBuffers b,
-                                                      uint32_t* __restrict__ bitmap, float* __restrict__ accum) {
-    const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= slots) return;
-    uint32_t x, y;
-    slot_pixel(p, slot, &x, &y);
-    if (x >= 16u * p.bsx || y >= 16u * p.bsy) return;
-    const float4 sa = b.sA[slot], sbv = b.sB[slot];
-    const uint32_t flags = __float_as_uint(sa.w);
-    f3 c = f3{sbv.x, sbv.y, sbv.z};
-    if ((flags & kEnds) && (flags & kFound)) {
-        const float4 scv = b.sC[slot];
-        c = f3{scv.x, scv.y, scv.z};
-    }
-    const uint32_t pid = __float_as_uint(sbv.w);
-    const uint32_t bi = y * p.real_width + x;
-    bitmap[bi] = incremental_avg(c, bitmap[bi], sample + 1u);
-    if (accum) {
-        accum[3u * pid + 0u] += c.x;
-        accum[3u * pid + 1u] += c.y;
-        accum[3u * pid + 2u] += c.z;
-    }
 }
 
 // ------------------------------------------------------------- launcher --
@@ -434,66 +513,95 @@ uint32_t wavefront_slots(const RenderParams& p) {
     return ((covW + 7u) / 8u) * ((covH + 7u) / 8u) * 64u;
 }
 
-hipError_t launch_wavefront_pass(const SceneDev& s, bool generic, bool persistent, bool count, const RenderParams& p,
-                                 uint32_t sample, uint32_t* bitmap, float* accum, const WfBuffers& b,
-                                 uint32_t trace_blocks, const StackCfg& sc, unsigned long long* counters,
-                                 uint32_t* err, hipStream_t stream) {
-    const uint32_t slots = wavefront_slots(p);
-    const uint32_t gblocks = (slots + kBlock - 1u) / kBlock;
-    // counts[0] = entries of queue_a (set by generate), counts[1] = of
-    // queue_b, counts[2] = shadow entries; extend clears the two shade fills.
-    // counts[3] / counts[4] are the fetch cursors of the persistent extend /
-    // shadow kernels: generate clears both, shade() clears extend's for the
-    // next bounce, extend clears shadow's.
-    k_wf_generate<<<gblocks, kBlock, 0, stream>>>(p, sample, slots, b);
-    float4* qin = b.queue_a;
-    float4* qout = b.queue_b;
-    uint32_t* cin = b.counts;
-    uint32_t* cout = b.counts + 1;
-    uint32_t* csh = b.counts + 2;
-    uint32_t* fext = b.counts + 3;
-    uint32_t* fsh = b.counts + 4;
-    for (int32_t depth = 1; depth <= p.depth_max; ++depth) {
-        if (persistent) {
-            if (count) k_wf_extend_p<true><<<trace_blocks, kBlock, 0, stream>>>(s, qin, cin, b.hits, cout, csh, fsh, fext, sc, counters, err);
-            else k_wf_extend_p<false><<<trace_blocks, kBlock, 0, stream>>>(s, qin, cin, b.hits, cout, csh, fsh, fext, sc, counters, err);
-        } else {
-#define MRT_EXTEND(G, C) \
-    k_wf_extend<G, C><<<trace_blocks, kBlock, 0, stream>>>(s, qin, cin, b.hits, cout, csh, sc, counters, err)
-            if (generic) {
-                if (count) MRT_EXTEND(true, true); else MRT_EXTEND(true, false);
-            } else {
-                if (count) MRT_EXTEND(false, true); else MRT_EXTEND(false, false);
-            }
-#undef MRT_EXTEND
+uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap) {
+    static int perCu = 0;  // same for every device of this (single-arch) build
+    if (perCu == 0) {
+        int a = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace_p<false>, kBlock, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return cap;
         }
-        k_wf_shade<<<trace_blocks, kBlock, 0, stream>>>(s, p, sample, qin, cin, b.hits, b, qout, cout, csh, fext,
-                                                        counters);
-        if (persistent) {
-            if (count) k_wf_shadow_p<true><<<trace_blocks, kBlock, 0, stream>>>(s, csh, b, fsh, sc, counters, err);
-            else k_wf_shadow_p<false><<<trace_blocks, kBlock, 0, stream>>>(s, csh, b, fsh, sc, counters, err);
-        } else if (generic) {
-            if (count) k_wf_shadow<true, true><<<trace_blocks, kBlock, 0, stream>>>(s, csh, b, sc, counters, err);
-            else k_wf_shadow<true, false><<<trace_blocks, kBlock, 0, stream>>>(s, csh, b, sc, counters, err);
-        } else {
-            if (count) k_wf_shadow<false, true><<<trace_blocks, kBlock, 0, stream>>>(s, csh, b, sc, counters, err);
-            else k_wf_shadow<false, false><<<trace_blocks, kBlock, 0, stream>>>(s, csh, b, sc, counters, err);
-        }
-        if (b.sort && depth < p.depth_max) {
-            size_t bytes = b.scan_bytes;
-            hipError_t e = hipcub::DeviceScan::ExclusiveSum(b.scan_temp, bytes, b.hist, b.offs, kSortBins, stream);
-            if (e != hipSuccess) return e;
-            k_wf_scatter<<<trace_blocks, kBlock, 0, stream>>>(cout, b);
-        } else {
-            float4* tq = qin;
-            qin = qout;
-            qout = tq;
-        }
-        uint32_t* tc = cin;
-        cin = cout;
-        cout = tc;
+        perCu = std::max(1, a);
     }
-    k_wf_finish<<<gblocks, kBlock, 0, stream>>>(p, sample, slots, b, bitmap, accum);
+    return std::min(cap, static_cast<uint32_t>(perCu) * num_cu);
+}
+
+hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,
+                                  hipStream_t stream) {
+    const uint32_t slots = wavefront_slots(p);
+    L->qin = b.queue_a;
+    L->qout = b.queue_b;
+    L->cin = kCntQueue0;
+    L->cout = kCntQueue1;
+    L->iter = 0;
+    k_wf_generate<<<(slots + kBlock - 1u) / kBlock, kBlock, 0, stream>>>(p, first_sample, slots, b, L->qin,
+                                                                          b.counts + L->cin);
+    return hipGetLastError();
+}
+
+hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool persistent, bool count,
+                                      const RenderParams& p, uint32_t spp, bool stop, uint32_t* bitmap, float* accum,
+                                      const WfBuffers& b, WfLoop* L, uint32_t trace_blocks, uint32_t persist_blocks,
+                                      const StackCfg& sc, unsigned long long* counters, uint32_t* err,
+                                      hipEvent_t trace_begin, hipEvent_t trace_end, hipStream_t stream) {
+    uint32_t* c = b.counts;
+    const uint32_t ePrev = (L->iter & 1u) ? kCntEnded0 : kCntEnded1;  // written by the previous shade
+    const uint32_t eCur = (L->iter & 1u) ? kCntEnded1 : kCntEnded0;   // written by this iteration's shade
+    uint32_t* endedPrev = b.ended[ePrev - kCntEnded0];
+    uint32_t* endedCur = b.ended[eCur - kCntEnded0];
+    if (trace_begin) {
+        const hipError_t e = hipEventRecord(trace_begin, stream);
+        if (e != hipSuccess) return e;
+    }
+    if (persistent) {
+        if (count)
+            k_wf_trace_p<true><<<persist_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, c + kCntShadow, b.hits, b,
+                                                                      c + L->cout, c + eCur, c + kCntFetch, sc,
+                                                                      counters, err);
+        else
+            k_wf_trace_p<false><<<persist_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, c + kCntShadow, b.hits,
+                                                                       b, c + L->cout, c + eCur, c + kCntFetch, sc,
+                                                                       counters, err);
+    } else {
+#define MRT_EXTEND(G, C)                                                                                            \
+    k_wf_extend<G, C><<<trace_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, b.hits, c + L->cout, c + eCur, sc, \
+                                                           counters, err)
+#define MRT_SHADOW(G, C) k_wf_shadow<G, C><<<trace_blocks, kBlock, 0, stream>>>(s, c + kCntShadow, b, sc, counters, err)
+        if (generic) {
+            if (count) { MRT_EXTEND(true, true); MRT_SHADOW(true, true); }
+            else { MRT_EXTEND(true, false); MRT_SHADOW(true, false); }
+        } else {
+            if (count) { MRT_EXTEND(false, true); MRT_SHADOW(false, true); }
+            else { MRT_EXTEND(false, false); MRT_SHADOW(false, false); }
+        }
+#undef MRT_EXTEND
+#undef MRT_SHADOW
+    }
+    if (trace_end) {
+        const hipError_t e = hipEventRecord(trace_end, stream);
+        if (e != hipSuccess) return e;
+    }
+    // With reordering, regen and shade fill queue_b, which the scatter sorts
+    // into queue_a; without it the two queues alternate.
+    float4* fill = b.sort ? b.queue_b : L->qout;
+    k_wf_regen<<<trace_blocks, kBlock, 0, stream>>>(p, spp, stop ? 1u : 0u, endedPrev, c + ePrev, b, fill,
+                                                     c + L->cout, c + kCntShadow, c + kCntFetch, c + kCntDone, bitmap,
+                                                     accum);
+    k_wf_shade<<<trace_blocks, kBlock, 0, stream>>>(s, p, L->qin, c + L->cin, b.hits, b, fill, c + L->cout,
+                                                    c + kCntShadow, endedCur, c + eCur, counters);
+    if (b.sort) {
+        size_t bytes = b.scan_bytes;
+        const hipError_t e = hipcub::DeviceScan::ExclusiveSum(b.scan_temp, bytes, b.hist, b.offs, kSortBins, stream);
+        if (e != hipSuccess) return e;
+        k_wf_scatter<<<trace_blocks, kBlock, 0, stream>>>(c + L->cout, b);
+    } else {
+        std::swap(L->qin, L->qout);
+    }
+    std::swap(L->cin, L->cout);
+    L->pending[0] = L->cin;
+    L->pending[1] = kCntShadow;
+    L->pending[2] = eCur;
+    ++L->iter;
     return hipGetLastError();
 }
 
