@@ -84,7 +84,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel", choices=sorted(KERNELS), default="auto")
-    ap.add_argument("--sort", action="store_true", help="wavefront: reorder extension rays by origin cell")
+    ap.add_argument("--sort", choices=["default", "on", "off"], default="default",
+                    help="wavefront: reorder extension rays by origin cell (default: off)")
     ap.add_argument("--no-persistent", action="store_true", help="wavefront: one ray per lane, no refill")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -113,7 +114,7 @@ def main() -> int:
     torch.cuda.synchronize()
 
     # Instrumented frame (1 spp): node visits and primitive tests per ray.
-    sort = 1 if args.sort else 2
+    sort = {"default": 0, "on": 1, "off": 2}[args.sort]
     pers = 2 if args.no_persistent else 0
     _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True, kernel=args.kernel, ray_sort=sort, persistent=pers),
                          bitmap=bitmap,

@@ -191,7 +191,8 @@ typedef struct mrt_render_desc {
     int32_t kernel;            /* enum mrt_kernel                              */
     uint32_t first_sample;     /* first sample pass (progressive resume)       */
     uint32_t instrument;       /* 1: also count node visits / primitive tests  */
-    uint32_t ray_sort;         /* wavefront: 0 default (off), 1 on, 2 off      */
+    uint32_t ray_sort;         /* wavefront: 0 default (off), 1 on, 2 off:
+                                  extension rays in origin-cell order          */
     uint32_t persistent;       /* wavefront: 0 default (on), 1 on, 2 off: lanes
                                   refill from the queue (BVH, triangles+lights) */
     uint32_t reserved[3];

@@ -261,6 +261,46 @@ __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, 
     return canTest && walk_test<COUNT>(s, w, shadow, ts);
 }
 
+// Wave-private run of trace-queue entries [next, end): a wave reserves
+// kFetchChunk entries with one atomic and hands them to its lanes as they go
+// idle.  Same-address atomics serialise device-wide (about 10 ns each), so one
+// atomic per refill of a few lanes would cap the whole trace at ~100 M
+// refills per second.
+constexpr uint32_t kFetchChunk = 128;
+struct FetchRun {
+    uint32_t next, end;  // equal in every lane of the wave
+    bool drained;        // a run reached the queue's end: reserve no more
+};
+
+// Next queue entry for each lane that wants one (n entries in all);
+// kDeadEntry once the queue is used up.
+__device__ __forceinline__ uint32_t fetch_entry(uint32_t* cursor, uint32_t n, bool want, FetchRun& run) {
+    const unsigned long long mask = __ballot(want);
+    const uint32_t total = static_cast<uint32_t>(__popcll(mask));
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    const uint32_t avail = run.end - run.next;
+    uint32_t e = run.next + rank;
+    if (total <= avail) {
+        run.next += total;
+    } else if (!run.drained) {
+        // The run's rest goes to the first `avail` lanes, a new run to the
+        // others (total <= 64 <= kFetchChunk).
+        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask))) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(cursor, kFetchChunk);
+        base = __shfl(base, static_cast<int>(leader), 64);
+        if (rank >= avail) e = base + (rank - avail);
+        run.next = base + (total - avail);
+        run.end = base + kFetchChunk;
+        run.drained = base + kFetchChunk >= n;
+    } else {
+        if (rank >= avail) e = kDeadEntry;
+        run.next = run.end;
+    }
+    return e < n ? e : kDeadEntry;
+}
+
 // Persistent trace of the extension queue followed by the shadow queue, for
 // BVH scenes of triangles and lights: a lane takes the next entry the moment
 // its ray is done, so a wave never waits for its slowest ray.  Clears the
@@ -288,10 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
     w.h.t = 0.0f;
     uint32_t idx = 0;
     bool has = false, exhausted = false, finite = true, shadow = false;
+    FetchRun run{0u, 0u, false};
     while (true) {
         const bool need = !has && !exhausted;
         if (wave_wants_refill(need, has)) {
-            const uint32_t e = wave_append(fetch, need);
+            const uint32_t e = fetch_entry(fetch, n, need, run);
             if (need) {
                 if (e < nq) {
                     const float4 a = queue[2u * e], c = queue[2u * e + 1u];
@@ -334,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
 // Finishes the paths that ended in the previous shade (PathTracer.cpp:144-164
 // result selection, then incrementalAvg of Renderer.cpp:101-104 with the
 // slot's own sample count) and starts each slot's next sample.  Clears the
-// shadow counter and the trace fetch cursor for the next iteration.
+// shadow counter and the trace fetch cursors for the next iteration.
 __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t spp, uint32_t stop,
                                                      const uint32_t* __restrict__ ended,
                                                      const uint32_t* __restrict__ count_e, WfBuffers b,

@@ -19,11 +19,11 @@ enum : uint32_t {
     kCntQueue0 = 0,  // extension-queue entries (two queues, alternating)
     kCntQueue1 = 1,
     kCntShadow = 2,  // shadow-queue entries
-    kCntFetch = 3,   // fetch cursor of the persistent trace
-    kCntEnded0 = 4,  // ended-slot lists (alternating)
-    kCntEnded1 = 5,
-    kCntDone = 6,    // finished samples (progress)
-    kCntWords = 8
+    kCntEnded0 = 3,  // ended-slot lists (alternating)
+    kCntEnded1 = 4,
+    kCntDone = 5,    // finished samples (progress)
+    kCntFetch = 8,   // fetch cursor of the persistent trace (own 32-byte sector)
+    kCntWords = 16
 };
 
 struct WfBuffers {
