@@ -87,7 +87,7 @@ struct mrt_scene {
     std::vector<mrt_light> lights;
     uint32_t n_scene_mats = 0;
     DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
-    DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes;
+    DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes, wide_nodes;
     DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wavefront;
     // Host copy of the light tree (pairs relative to its own numbering) for
     // the combined walk tree; light slots follow the triangles in tri_geo.
@@ -191,6 +191,110 @@ void set_root_child(BNode* p, int side, const BvhOut& t, uint32_t pair_base, uin
     else set_child(p, side, t.root, pair_base, 0u);
 }
 
+bool is_nan_box(const BNode& p, int side) {
+    return side == 0 ? std::isnan(p.q0.x) : std::isnan(p.q1.z);
+}
+
+// One child slot of a wide node.
+struct WideSlot {
+    Box box;
+    uint32_t ref, word;
+};
+
+WideSlot pair_child(const BNode& p, int side) {
+    if (side == 0) return {Box{f3{p.q0.x, p.q0.y, p.q0.z}, f3{p.q0.w, p.q1.x, p.q1.y}}, p.q3.x, p.q3.y};
+    return {Box{f3{p.q1.z, p.q1.w, p.q2.x}, f3{p.q2.y, p.q2.z, p.q2.w}}, p.q3.z, p.q3.w};
+}
+
+// Child slots of the wide node made from pair `pi`: each leaf or empty child
+// as it is, each interior child replaced by its two children.
+int wide_slots(const std::vector<BNode>& w, uint32_t pi, WideSlot* out) {
+    int n = 0;
+    for (int side = 0; side < 2; ++side) {
+        const WideSlot c = pair_child(w[pi], side);
+        if (c.word != 0u || is_nan_box(w[pi], side)) {
+            out[n++] = c;
+        } else {
+            out[n++] = pair_child(w[c.ref], 0);
+            out[n++] = pair_child(w[c.ref], 1);
+        }
+    }
+    return n;
+}
+
+// Builds the wide walk tree from the pair walk tree `w` (layout in
+// mrt_scene.hpp).  For a ray whose inverse direction is finite in every axis
+// the reference slab test is monotone under box inclusion (the products
+// (bound - o) * inv round monotonically and a parent box is the exact union
+// of its children), so a grandchild that passes implies its parent passes:
+// the wide walk reaches exactly the leaves the pair walk reaches.
+int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, uint32_t light_pair, hipStream_t st) {
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    // Pairs that become wide nodes, in depth-first order from the roots.
+    std::vector<uint32_t> order, depth(w.size(), 0);
+    std::vector<char> seen(w.size(), 0);
+    std::vector<uint32_t> todo{kWalkAnyRoot, kWalkClosestRoot};
+    seen[kWalkClosestRoot] = seen[kWalkAnyRoot] = 1;
+    depth[kWalkClosestRoot] = depth[kWalkAnyRoot] = 1;
+    uint32_t maxDepth = 1;
+    WideSlot sl[4];
+    while (!todo.empty()) {
+        const uint32_t pi = todo.back();
+        todo.pop_back();
+        order.push_back(pi);
+        const int n = wide_slots(w, pi, sl);
+        for (int k = n - 1; k >= 0; --k) {
+            if (sl[k].word != 0u || std::isnan(sl[k].box.mn.x) || seen[sl[k].ref]) continue;
+            seen[sl[k].ref] = 1;
+            depth[sl[k].ref] = depth[pi] + 1u;
+            maxDepth = std::max(maxDepth, depth[pi] + 1u);
+            todo.push_back(sl[k].ref);
+        }
+    }
+    // Numbering: the two roots, then triangle pairs, then light pairs.
+    std::vector<uint32_t> index(w.size(), kNone);
+    index[kWalkClosestRoot] = kWalkClosestRoot;
+    index[kWalkAnyRoot] = kWalkAnyRoot;
+    uint32_t next = kWalkFirstPair;
+    for (uint32_t pi : order)
+        if (index[pi] == kNone && pi < light_pair) index[pi] = next++;
+    const uint32_t lightNode = next;
+    for (uint32_t pi : order)
+        if (index[pi] == kNone) index[pi] = next++;
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    std::vector<float4> out(8ull * next);
+    for (uint32_t pi : order) {
+        float4* q = &out[8ull * index[pi]];
+        const int n = wide_slots(w, pi, sl);
+        float v[6][4];
+        uint32_t r[8];
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < n;
+            const Box b = used ? sl[k].box : Box{f3{nan, nan, nan}, f3{nan, nan, nan}};
+            v[0][k] = b.mn.x;
+            v[1][k] = b.mn.y;
+            v[2][k] = b.mn.z;
+            v[3][k] = b.mx.x;
+            v[4][k] = b.mx.y;
+            v[5][k] = b.mx.z;
+            const bool interior = used && sl[k].word == 0u && !std::isnan(b.mn.x);
+            r[2 * k] = interior ? index[sl[k].ref] : (used ? sl[k].ref : 0u);
+            r[2 * k + 1] = used ? sl[k].word : 0u;
+        }
+        for (int j = 0; j < 6; ++j) q[j] = make_float4(v[j][0], v[j][1], v[j][2], v[j][3]);
+        std::memcpy(&q[6], r, 16);
+        std::memcpy(&q[7], r + 4, 16);
+    }
+    int rc;
+    if ((rc = s->wide_nodes.upload(out.data(), out.size() * sizeof(float4), st)) != MRT_OK) return rc;
+    MRT_HIP(hipStreamSynchronize(st));  // `out` is released on return
+    s->dev.wide_nodes = s->wide_nodes.as<float4>();
+    s->dev.wide_light_node = lightNode;
+    // A wide visit pushes at most three children.
+    s->depth_need = std::max(s->depth_need, 3u * maxDepth + 1u);
+    return MRT_OK;
+}
+
 // Builds the combined walk tree of the persistent kernels (layout in
 // mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 24/8-bit leaf
 // word get none; they run the non-persistent kernels.
@@ -228,6 +332,8 @@ int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
     s->dev.walk_light_pair = lightBase;
     s->dev.walk_light_slot = nTri;
     s->depth_need = std::max(s->depth_need, std::max(tri.depth, s->light_bvh.depth) + 1u);
+    if ((rc = build_wide_tree(s, w, lightBase, st)) != MRT_OK) return rc;
+    MRT_HIP(hipStreamSynchronize(st));
     return MRT_OK;
 }
 
@@ -637,6 +743,7 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
         a.kind = accelerator;
     }
     s->dev.walk_nodes = nullptr;
+    s->dev.wide_nodes = nullptr;
     if (accelerator == MRT_BVH) {
         for (int c = 0; c < 3; ++c) {
             std::vector<Box> boxes(counts[c]);
@@ -937,7 +1044,8 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     }
     const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
     const bool persistent = useWavefront && d->persistent != 2u && !s->generic &&
-                            s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH && s->dev.walk_nodes != nullptr;
+                            s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH && s->dev.walk_nodes != nullptr &&
+                            s->dev.wide_nodes != nullptr;
     const uint32_t persistBlocks =
         persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks) : 0u;
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);

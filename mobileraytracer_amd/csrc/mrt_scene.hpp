@@ -34,6 +34,13 @@ struct alignas(16) BNode {
 static_assert(sizeof(BNode) == 64, "BVH node pair must be 64 bytes");
 
 // Walk tree: root pairs and the leaf word (count << kSlotBits | slot).
+// The wide walk tree replaces every interior child of a pair by that child's
+// own two children: 4 child slots per 128-byte node, structure of arrays
+//   w[0..2] = min.x, min.y, min.z of slots 0..3, w[3..5] = max.x/y/z,
+//   w[6] = (ref0, word0, ref1, word1), w[7] = (ref2, word2, ref3, word3),
+// word != 0 a leaf word, word == 0 with a finite box an interior slot whose
+// ref is a wide node, unused slots all-NaN.  Wide nodes 0 and 1 are the two
+// roots; nodes from light pairs follow those from triangle pairs.
 constexpr uint32_t kWalkClosestRoot = 0u, kWalkAnyRoot = 1u, kWalkFirstPair = 2u;
 constexpr uint32_t kSlotBits = 24u;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
@@ -82,6 +89,8 @@ struct SceneDev {
     const BNode* walk_nodes; // combined walk tree (NULL: not built)
     uint32_t walk_light_pair;  // first light pair of the walk tree
     uint32_t walk_light_slot;  // first light slot of tri_geo (= triangle count)
+    const float4* wide_nodes;  // wide walk tree, 8 float4 per node
+    uint32_t wide_light_node;  // first wide node from a light pair
 };
 
 // Material fields.

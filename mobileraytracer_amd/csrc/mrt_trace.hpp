@@ -523,15 +523,25 @@ __device__ __forceinline__ HitT closest_hit(const SceneDev& s, const RayT& r, fl
 // reference's ascending scan with t < length_.
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
+// Per-lane walk state.  Leaf words are (count << 24 | slot), 0 = none: `cur`
+// is the leaf whose triangles are being tested, l0..l3 the leaves the last
+// visit reached (by child slot, zeros allowed).  A lane visits again only
+// once l0..l3 are empty, and `cur` is refilled from them as it runs out, so
+// cur == 0 means no triangle is pending.
 struct Walk {
     RayT r;
     HitT h;
-    uint32_t node;        // next pair to visit, kNoNode when none is left
-    uint32_t p0, p1, p2;  // queued leaves (count << 24 | slot), 0 = empty
+    uint32_t node;  // next node to visit, kNoNode when none is left
+    uint32_t cur, l0, l1, l2, l3;
 };
 
 __device__ __forceinline__ bool inv_finite(const RayT& r) {
     return fabsf(r.inv.x) < INFINITY && fabsf(r.inv.y) < INFINITY && fabsf(r.inv.z) < INFINITY;
+}
+
+__device__ __forceinline__ void walk_clear(Walk& w) {
+    w.node = kNoNode;
+    w.cur = w.l0 = w.l1 = w.l2 = w.l3 = 0u;
 }
 
 __device__ __forceinline__ void walk_begin(Walk& w, uint32_t root, float tmax, Stack& st) {
@@ -539,28 +549,35 @@ __device__ __forceinline__ void walk_begin(Walk& w, uint32_t root, float tmax, S
     w.h.id = MRT_ID_NONE;
     w.h.u = 0.0f;
     w.h.v = 0.0f;
+    walk_clear(w);
     w.node = root;
-    w.p0 = w.p1 = w.p2 = 0u;
     st.sp = 0;
 }
 
-__device__ __forceinline__ bool walk_done(const Walk& w) { return w.node == kNoNode && w.p0 == 0u; }
-
-// Appends `leaf` to the queue when `take` holds (branch free).
-__device__ __forceinline__ void walk_queue(Walk& w, bool take, uint32_t leaf) {
-    const bool e0 = w.p0 == 0u, e1 = w.p1 == 0u;
-    w.p2 = (take && !e0 && !e1) ? leaf : w.p2;
-    w.p1 = (take && !e0 && e1) ? leaf : w.p1;
-    w.p0 = (take && e0) ? leaf : w.p0;
+__device__ __forceinline__ bool walk_done(const Walk& w) { return w.node == kNoNode && w.cur == 0u; }
+__device__ __forceinline__ bool walk_can_visit(const Walk& w) {
+    return w.node != kNoNode && (w.l0 | w.l1 | w.l2 | w.l3) == 0u;
 }
 
-// Drops the first queued triangle.
+// cur <- the first reached leaf when cur is empty (branch free).
+__device__ __forceinline__ void walk_pull(Walk& w) {
+    const bool need = w.cur == 0u;
+    const bool t0 = need && w.l0 != 0u;
+    const bool t1 = need && !t0 && w.l1 != 0u;
+    const bool t2 = need && !t0 && !t1 && w.l2 != 0u;
+    const bool t3 = need && !t0 && !t1 && !t2 && w.l3 != 0u;
+    w.cur = t0 ? w.l0 : (t1 ? w.l1 : (t2 ? w.l2 : (t3 ? w.l3 : w.cur)));
+    w.l0 = t0 ? 0u : w.l0;
+    w.l1 = t1 ? 0u : w.l1;
+    w.l2 = t2 ? 0u : w.l2;
+    w.l3 = t3 ? 0u : w.l3;
+}
+
+// Drops the triangle just tested.
 __device__ __forceinline__ void walk_pop_prim(Walk& w) {
-    const bool more = (w.p0 >> kSlotBits) > 1u;
-    const uint32_t next = w.p0 - kSlotMask;  // slot + 1, count - 1
-    w.p0 = more ? next : w.p1;
-    w.p1 = more ? w.p1 : w.p2;
-    w.p2 = more ? w.p2 : 0u;
+    const bool more = (w.cur >> kSlotBits) > 1u;
+    w.cur = more ? w.cur - kSlotMask : 0u;  // slot + 1, count - 1
+    walk_pull(w);
 }
 
 // Slab test for rays whose inverse direction is finite in every axis: then no
@@ -601,8 +618,8 @@ __device__ __forceinline__ void walk_visit(const SceneDev& s, Walk& w, float lim
     hL = hL && eL <= lim;
     hR = hR && eR <= lim;
     const bool leafL = q3.y != 0u, leafR = q3.w != 0u;
-    walk_queue(w, hL && leafL, q3.y);
-    walk_queue(w, hR && leafR, q3.w);
+    w.l0 = (hL && leafL) ? q3.y : 0u;
+    w.l1 = (hR && leafR) ? q3.w : 0u;
     const bool inL = hL && !leafL, inR = hR && !leafR;
     const bool leftFirst = eL <= eR;
     uint32_t next = (inL && (!inR || leftFirst)) ? q3.x : q3.z;
@@ -612,6 +629,59 @@ __device__ __forceinline__ void walk_visit(const SceneDev& s, Walk& w, float lim
         next = st.sp == 0u ? kNoNode : st.pop();
     }
     w.node = next;
+    walk_pull(w);
+}
+
+// Orders (key, ref) pairs so that ka <= kb.
+__device__ __forceinline__ void cswap(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    const uint32_t r = sw ? rb : ra;
+    kb = sw ? ka : kb;
+    rb = sw ? ra : rb;
+    ka = k;
+    ra = r;
+}
+
+// One wide-node visit (rays with a finite inverse direction, wide tree):
+// reached leaves go to l0..l3, reached interior slots are ordered by entry
+// distance; the nearest is visited next and the others are pushed, farthest
+// first.
+template <bool COUNT>
+__device__ __forceinline__ void walk_visit_wide(const SceneDev& s, Walk& w, float lim, Stack& st, TraceStats& ts,
+                                                int kind) {
+    const float4* nd = s.wide_nodes + 8u * w.node;
+    const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5];
+    const float4 ra = nd[6], rb = nd[7];
+    if (COUNT) ++ts.pairs[kind == 2 ? 2 : (w.node >= s.wide_light_node ? 1 : 0)];
+    float e0, e1, e2, e3;
+    const bool h0 = slab_finite(f3{mnx.x, mny.x, mnz.x}, f3{mxx.x, mxy.x, mxz.x}, w.r.o, w.r.inv, &e0) && e0 <= lim;
+    const bool h1 = slab_finite(f3{mnx.y, mny.y, mnz.y}, f3{mxx.y, mxy.y, mxz.y}, w.r.o, w.r.inv, &e1) && e1 <= lim;
+    const bool h2 = slab_finite(f3{mnx.z, mny.z, mnz.z}, f3{mxx.z, mxy.z, mxz.z}, w.r.o, w.r.inv, &e2) && e2 <= lim;
+    const bool h3 = slab_finite(f3{mnx.w, mny.w, mnz.w}, f3{mxx.w, mxy.w, mxz.w}, w.r.o, w.r.inv, &e3) && e3 <= lim;
+    const uint32_t wd0 = __float_as_uint(ra.y), wd1 = __float_as_uint(ra.w);
+    const uint32_t wd2 = __float_as_uint(rb.y), wd3 = __float_as_uint(rb.w);
+    w.l0 = (h0 && wd0 != 0u) ? wd0 : 0u;
+    w.l1 = (h1 && wd1 != 0u) ? wd1 : 0u;
+    w.l2 = (h2 && wd2 != 0u) ? wd2 : 0u;
+    w.l3 = (h3 && wd3 != 0u) ? wd3 : 0u;
+    // Interior slots keyed by entry; +inf marks "not reached" (a reached
+    // slot's entry is at most lim, which is finite).
+    float k0 = (h0 && wd0 == 0u) ? e0 : INFINITY, k1 = (h1 && wd1 == 0u) ? e1 : INFINITY;
+    float k2 = (h2 && wd2 == 0u) ? e2 : INFINITY, k3 = (h3 && wd3 == 0u) ? e3 : INFINITY;
+    uint32_t n0 = __float_as_uint(ra.x), n1 = __float_as_uint(ra.z);
+    uint32_t n2 = __float_as_uint(rb.x), n3 = __float_as_uint(rb.z);
+    cswap(k0, n0, k1, n1);
+    cswap(k2, n2, k3, n3);
+    cswap(k0, n0, k2, n2);
+    cswap(k1, n1, k3, n3);
+    cswap(k1, n1, k2, n2);
+    if (k3 < INFINITY) st.push(n3);
+    if (k2 < INFINITY) st.push(n2);
+    if (k1 < INFINITY) st.push(n1);
+    if (k0 < INFINITY) w.node = n0;
+    else w.node = st.sp == 0u ? kNoNode : st.pop();
+    walk_pull(w);
 }
 
 __device__ __forceinline__ TriG walk_tri(const SceneDev& s, uint32_t slot, uint32_t* id) {
@@ -641,41 +711,6 @@ __device__ __forceinline__ bool tri_test_flat(const TriG& g, const RayT& r, floa
     *vv = v;
     return !(fabsf(det) < kEpsilon) && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || (u + v) > 1.0f) &&
            !(t < kEpsilon);
-}
-
-// Closest-hit test of the first queued triangle.
-template <bool COUNT>
-__device__ __forceinline__ void walk_test_closest(const SceneDev& s, Walk& w, TraceStats& ts) {
-    const uint32_t slot = w.p0 & kSlotMask;
-    if (COUNT) ++ts.prims[slot >= s.walk_light_slot ? 1 : 0];
-    uint32_t id;
-    const TriG g = walk_tri(s, slot, &id);
-    float t, u, v;
-    const bool ok = tri_test_flat(g, w.r, &t, &u, &v) && id != w.r.excl && key_better(t, id, w.h);
-    w.h.t = ok ? t : w.h.t;
-    w.h.id = ok ? id : w.h.id;
-    w.h.u = ok ? u : w.h.u;
-    w.h.v = ok ? v : w.h.v;
-    walk_pop_prim(w);
-}
-
-// Any-hit test of the first queued triangle (Triangle.cpp:105-136); a
-// blocker ends the walk.  Returns true when the ray is occluded.
-template <bool COUNT>
-__device__ __forceinline__ bool walk_test_any(const SceneDev& s, Walk& w, float dist, TraceStats& ts) {
-    const uint32_t slot = w.p0 & kSlotMask;
-    if (COUNT) ++ts.prims[2];
-    uint32_t id;
-    const TriG g = walk_tri(s, slot, &id);
-    float t, u, v;
-    const bool hitAny = tri_test_flat(g, w.r, &t, &u, &v);
-    const bool blocked = hitAny && id != w.r.excl && t > kEpsilon && t < dist;
-    walk_pop_prim(w);
-    if (blocked) {
-        w.node = kNoNode;
-        w.p0 = w.p1 = w.p2 = 0u;
-    }
-    return blocked;
 }
 
 template <bool GENERIC, bool COUNT>

@@ -216,7 +216,7 @@ __device__ __forceinline__ bool wave_wants_refill(bool need, bool has) {
 // a shadow lane is occluded.
 template <bool COUNT>
 __device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, bool shadow, TraceStats& ts) {
-    const uint32_t slot = w.p0 & kSlotMask;
+    const uint32_t slot = w.cur & kSlotMask;
     if (COUNT) ++ts.prims[shadow ? 2 : (slot >= s.walk_light_slot ? 1 : 0)];
     uint32_t id;
     const TriG g = walk_tri(s, slot, &id);
@@ -229,33 +229,28 @@ __device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, bool shado
     w.h.u = better ? u : w.h.u;
     w.h.v = better ? v : w.h.v;
     walk_pop_prim(w);
-    if (blocked) {
-        w.node = kNoNode;
-        w.p0 = w.p1 = w.p2 = 0u;
-    }
+    if (blocked) walk_clear(w);
     return blocked;
 }
 
 // One unit of walk work for the wave: node visits when at least as many
-// lanes can visit as can test, else one triangle test per lane with queued
-// leaves.  A lane visits only with at most one queued leaf, so its three
-// queue entries never overflow.  Returns true when a shadow lane found a
-// blocker.
+// lanes can visit as can test, else one triangle test per lane with a
+// pending leaf.  Lanes with a finite inverse direction walk the wide tree,
+// the others the pair tree with the reference slab test.  Returns true when
+// a shadow lane found a blocker.
 template <bool COUNT>
 __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, bool shadow, bool finite, Stack& st,
                                           TraceStats& ts) {
-    const bool canVisit = has && w.node != kNoNode && w.p1 == 0u;
-    const bool canTest = has && w.p0 != 0u;
+    const bool canVisit = has && walk_can_visit(w);
+    const bool canTest = has && w.cur != 0u;
     const uint32_t nv = static_cast<uint32_t>(__popcll(__ballot(canVisit)));
     const uint32_t nt = static_cast<uint32_t>(__popcll(__ballot(canTest)));
     if (nv >= nt) {
         const float lim = cull_limit(w.h.t);
         const int kind = shadow ? 2 : 0;
-        if (__ballot(canVisit && !finite) == 0ull) {
-            if (canVisit) walk_visit<true, COUNT>(s, w, lim, st, ts, kind);
-        } else if (canVisit) {
+        if (canVisit && finite) walk_visit_wide<COUNT>(s, w, lim, st, ts, kind);
+        if (__ballot(canVisit && !finite) != 0ull && canVisit && !finite)
             walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
-        }
         return false;
     }
     return canTest && walk_test<COUNT>(s, w, shadow, ts);
@@ -323,8 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
     const uint32_t nq = *count_q;
     const uint32_t n = nq + *count_s;
     Walk w;
-    w.node = kNoNode;
-    w.p0 = w.p1 = w.p2 = 0u;
+    walk_clear(w);
     w.h.t = 0.0f;
     uint32_t idx = 0;
     bool has = false, exhausted = false, finite = true, shadow = false;
