@@ -49,6 +49,37 @@ def make_desc(mrt, abi, cam, spp=SPP, instrument=False, kernel="auto", ray_sort=
                            depth_min=2, depth_max=6, instrument=instrument)
 
 
+COPY_BYTES = 1 << 30
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel`: the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the counter
+    reads for a plain copy of known size, see --copy-calibration); None when
+    no summary exists for it."""
+    p = ROOT / "profiles" / f"{kernel}_soup1m_pmc.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(float(line.split("=")[1]))
+    return None
+
+
+def copy_calibration() -> int:
+    """A plain device copy of COPY_BYTES (reads and writes COPY_BYTES each):
+    the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {COPY_BYTES} bytes")
+    return 0
+
+
 def soup_camera(mrt, abi):
     ratio = np.float32(W) / np.float32(H)
     return mrt.make_camera(abi.CAMERA_PERSPECTIVE, (0.5, 0.5, -1.0), (0.5, 0.5, 0.5), (0, 1, 0),
@@ -87,7 +118,11 @@ def main() -> int:
     ap.add_argument("--sort", choices=["default", "on", "off"], default="default",
                     help="wavefront: reorder extension rays by origin cell (default: off)")
     ap.add_argument("--no-persistent", action="store_true", help="wavefront: one ray per lane, no refill")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the frame would shard by tiles or samples, see DESIGN.md)", file=sys.stderr)
@@ -140,25 +175,44 @@ def main() -> int:
     rays = 0
     closest = 0
     kernel_ms = 0.0
+    trace_ms = 0.0
+    launches = 0
     for _ in range(args.steps):
         _, st = scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
         rays += st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
         closest += st.rays_camera + st.rays_extension + st.rays_specular
         kernel_ms += st.ms_kernel
+        trace_ms += st.ms_trace
+        launches += st.trace_launches
         log(f"step: {st.ms_render:.1f} ms, {st.rays_camera} camera / {st.rays_extension} extension / "
-            f"{st.rays_shadow} shadow rays")
+            f"{st.rays_shadow} shadow rays; trace {st.ms_trace:.1f} ms in {st.trace_launches} launches")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     mrays = rays / elapsed / 1e6
 
-    # Roofline of the render-pass kernel: algorithmic bytes per ray (ray in 32,
-    # hit out 16, 64 per node pair, 36 per primitive test, 256 of shading per
-    # closest-hit ray) x rays per launch / average launch duration.
-    launches = args.steps * SPP
-    bytes_per_ray = 48 + 64 * pairs_per_ray + 36 * prims_per_ray
-    bytes_per_launch = (rays / launches) * bytes_per_ray + (closest / launches) * 256
-    avg_launch_s = (kernel_ms / 1000.0) / launches
+    # Roofline of the dominant kernel.  Wavefront: the trace kernel, timed
+    # with HIP events around each of its launches on the render stream;
+    # algorithmic bytes per ray = node bytes per visit (128 per wide node of
+    # the persistent walk, 64 per node pair otherwise) + 48 per triangle test
+    # (three float4 of tri_geo) + 48 of ray record in and result out.
+    # Megakernel: the render-pass kernel, one launch per sample pass, plus 256
+    # bytes of shading per closest-hit ray.
+    wavefront = launches > 0
+    persistent = wavefront and not args.no_persistent
+    node_bytes = 128 if persistent else 64
+    bytes_per_ray = 48 + node_bytes * pairs_per_ray + 48 * prims_per_ray
+    if wavefront:
+        kernel_name = "k_wf_trace_p" if persistent else "k_wf_extend + k_wf_shadow"
+        bytes_total = rays * bytes_per_ray
+        avg_launch_s = trace_ms / 1000.0 / launches
+    else:
+        kernel_name = "k_render_pass"
+        launches = args.steps * SPP
+        bytes_total = rays * bytes_per_ray + closest * 256
+        avg_launch_s = kernel_ms / 1000.0 / launches
+    bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = measured_traffic(kernel_name) if persistent else None
 
     out = {
         "metric": "Mrays/s (1M-tri path trace, 16 spp) vs CPU; p50/p99 render-request latency",
@@ -177,8 +231,11 @@ def main() -> int:
                    "triangles": N_TRIS, "lights": N_LIGHTS, "width": W, "height": H, "spp": SPP,
                    "rays_per_step": rays // args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_ray": round(bytes_per_ray, 1)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kernel_name, "launches": launches,
+                     "avg_launch_ms": round(avg_launch_s * 1000.0, 4),
+                     "bytes_per_launch": round(bytes_per_launch), "bytes_per_ray": round(bytes_per_ray, 1),
+                     "node_visits_per_ray": round(pairs_per_ray, 2), "tests_per_ray": round(prims_per_ray, 2)},
     }
     if not args.no_cpu_baseline:
         t0 = time.time()
