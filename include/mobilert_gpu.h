@@ -273,6 +273,11 @@ MRT_API int mrt_camera_rays(mrt_ctx* ctx, const mrt_camera* camera, const float*
  * Triangle::intersect (both forms) and Plane::intersect, Sphere::intersect). */
 MRT_API int mrt_intersect_pairs(mrt_ctx* ctx, int category, const void* prims, const mrt_ray* rays,
                                 uint32_t n, mrt_hit* closest, uint8_t* any, int mem);
+/* ::MobileRT::intersect(AABB, Ray) (AABB.cpp:15-35): box i (min xyz, max xyz)
+ * against ray i; hit[i] = 1 when the slab test passes, entry[i] (optional)
+ * = max(tmin, 0). */
+MRT_API int mrt_box_rays(mrt_ctx* ctx, const float* boxes6, const mrt_ray* rays, uint32_t n, uint8_t* hit,
+                         float* entry, int mem);
 
 /* ---- render ------------------------------------------------------------- */
 /* Renderer::renderFrame (Renderer.cpp:31-59, 67-116): renders desc->samples_pixel
@@ -302,6 +307,10 @@ MRT_API int mrt_camera_make(int kind, const float position[3], const float look_
 MRT_API int mrt_tile_values(uint32_t seed, float out[256]);
 /* Triangle::intersectBox (Triangle.cpp:154-246), used by the grid builder. */
 MRT_API int mrt_triangle_intersect_box(const mrt_triangle* tri, const float box_min[3], const float box_max[3]);
+/* intersectBox of one primitive of any category: Triangle.cpp:154-246,
+ * Sphere.cpp:102-123, Plane.cpp:123-155.  1 overlap, 0 none, < 0 error. */
+MRT_API int mrt_primitive_intersect_box(int category, const void* prim, const float box_min[3],
+                                        const float box_max[3]);
 /* getAABB of n primitives of one category (Triangle.cpp:145-152,
  * Sphere.cpp:91-96, Plane.cpp:80-104): 6 floats (min xyz, max xyz) each. */
 MRT_API int mrt_primitive_aabbs(int category, const void* prims, uint32_t n, float* boxes6);

@@ -160,6 +160,7 @@ def _proto(lib):
         "mrt_shade_rays": (C.c_int, [P, C.POINTER(RenderDesc), P, P, u32, u32, P, P, C.c_int]),
         "mrt_camera_rays": (C.c_int, [P, C.POINTER(Camera), P, u32, P, C.c_int]),
         "mrt_intersect_pairs": (C.c_int, [P, C.c_int, P, P, u32, P, P, C.c_int]),
+        "mrt_box_rays": (C.c_int, [P, P, P, u32, P, P, C.c_int]),
         "mrt_render": (C.c_int, [P, C.POINTER(RenderDesc), P, u32, P, C.c_int, C.POINTER(RenderStats)]),
         "mrt_render_stop": (C.c_int, [P]),
         "mrt_render_progress": (C.c_int, [P, C.POINTER(u32)]),
@@ -167,6 +168,7 @@ def _proto(lib):
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
         "mrt_tile_values": (C.c_int, [u32, P]),
         "mrt_triangle_intersect_box": (C.c_int, [P, P, P]),
+        "mrt_primitive_intersect_box": (C.c_int, [C.c_int, P, P, P]),
         "mrt_primitive_aabbs": (C.c_int, [C.c_int, P, u32, P]),
         "mrt_bvh_build": (C.c_int, [P, u32, u32, P, u32, C.POINTER(u32)]),
         "mrt_scene_cornell": (C.c_int, [u32, P, P, P]),

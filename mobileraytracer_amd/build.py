@@ -30,10 +30,12 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
 
 GPU_DEVICE_SOURCES = ["mrt_kernels.hip", "mrt_wavefront.hip"]
 GPU_HOST_SOURCES = ["mrt_api.cpp", "mrt_build.cpp", "mrt_scenes.cpp"]
-HOST_API_SOURCES = [p for p in ["host/mobilert_api.cpp"] if (CSRC / p).exists()]
+HOST_API_SOURCES = ["host/mobilert_api.cpp", "host/c_wrapper.cpp"]
 
 LIB_GPU = HERE / "libmrt_gpu.so"
 LIB_HOST = HERE / "libmobilert.so"
+API_CHECK_SRC = ROOT / "tests" / "cpp" / "api_check.cpp"
+API_CHECK_BIN = ROOT / "tests" / "cpp" / "api_check"
 
 
 def _hipcc() -> str:
@@ -82,7 +84,11 @@ def build(verbose: bool = False) -> list[Path]:
         return [LIB_GPU]
     _run(["g++", "-shared", "-fPIC", *map(str, objs_host), "-o", str(LIB_HOST), f"-L{HERE}",
           "-lmrt_gpu", "-Wl,-rpath,$ORIGIN", "-pthread"], verbose)
-    return [LIB_GPU, LIB_HOST]
+    # The C++ API's test program (tests/cpp), built here so that no test
+    # compiles anything.
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(API_CHECK_SRC), "-o",
+          str(API_CHECK_BIN), f"-L{HERE}", "-lmobilert", "-lmrt_gpu", "-Wl,-rpath,$ORIGIN/../../mobileraytracer_amd", "-pthread"], verbose)
+    return [LIB_GPU, LIB_HOST, API_CHECK_BIN]
 
 
 if __name__ == "__main__":

@@ -966,6 +966,39 @@ MRT_API int mrt_intersect_pairs(mrt_ctx* ctx, int category, const void* prims, c
     return MRT_OK;
 }
 
+MRT_API int mrt_box_rays(mrt_ctx* ctx, const float* boxes6, const mrt_ray* rays, uint32_t n, uint8_t* hit,
+                         float* entry, int mem) {
+    if (!ctx || (n && (!boxes6 || !rays || !hit))) return fail(MRT_E_INVALID, "null argument");
+    if (n == 0) return MRT_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    DevBuf bb, br, bh, be;
+    const void* db = boxes6;
+    const void* dr = rays;
+    uint8_t* dh = hit;
+    float* de = entry;
+    int rc;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = bb.upload(boxes6, n * 6ull * sizeof(float), ctx->stream)) != MRT_OK) return rc;
+        if ((rc = br.upload(rays, n * sizeof(mrt_ray), ctx->stream)) != MRT_OK) return rc;
+        if ((rc = bh.alloc(n)) != MRT_OK) return rc;
+        db = bb.p;
+        dr = br.p;
+        dh = bh.as<uint8_t>();
+        if (entry) {
+            if ((rc = be.alloc(n * sizeof(float))) != MRT_OK) return rc;
+            de = be.as<float>();
+        }
+    }
+    MRT_HIP(launch_box_rays(static_cast<const float*>(db), static_cast<const mrt_ray*>(dr), n, dh, de, ctx->stream));
+    if (mem != MRT_MEM_DEVICE) {
+        MRT_HIP(hipMemcpyAsync(hit, dh, n, hipMemcpyDeviceToHost, ctx->stream));
+        if (entry) MRT_HIP(hipMemcpyAsync(entry, de, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    MRT_HIP(hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
 MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap, uint32_t stride_bytes, float* accum,
                        int mem, mrt_render_stats* stats) {
     if (!s || !bitmap) return fail(MRT_E_INVALID, "null argument");

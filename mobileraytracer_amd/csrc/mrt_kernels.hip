@@ -208,6 +208,27 @@ hipError_t launch_intersect_pairs(int cat, const void* prims, const mrt_ray* ray
     return hipGetLastError();
 }
 
+// ::MobileRT::intersect(AABB, Ray) (AABB.cpp:15-35) of box i and ray i.
+__global__ __launch_bounds__(kBlock) void k_box_rays(const float* __restrict__ boxes6,
+                                                     const mrt_ray* __restrict__ rays, uint32_t n,
+                                                     uint8_t* __restrict__ hit, float* __restrict__ entry) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const mrt_ray ry = rays[i];
+    const RayT r = make_ray(ld3(ry.org), ld3(ry.dir), ry.exclude);
+    const float* b = boxes6 + 6u * i;
+    float e = 0.0f;
+    const bool h = slab(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]}, r.o, r.inv, &e);
+    hit[i] = h ? 1u : 0u;
+    if (entry) entry[i] = e;
+}
+
+hipError_t launch_box_rays(const float* boxes6, const mrt_ray* rays, uint32_t n, uint8_t* hit, float* entry,
+                           hipStream_t stream) {
+    k_box_rays<<<grid_for(n, 0), kBlock, 0, stream>>>(boxes6, rays, n, hit, entry);
+    return hipGetLastError();
+}
+
 hipError_t launch_shade_rays(const SceneDev& s, bool generic, bool branches, const RenderParams& p,
                              const mrt_ray* rays, const int32_t* depths, uint32_t n, uint32_t sample, float* rgb,
                              uint8_t* flag, const StackCfg& sc, uint32_t cap_blocks, uint32_t* err,

@@ -25,6 +25,8 @@ hipError_t launch_camera_rays(const mrt_camera& cam, const float* uvdd, uint32_t
                               hipStream_t stream);
 hipError_t launch_intersect_pairs(int cat, const void* prims, const mrt_ray* rays, uint32_t n, mrt_hit* closest,
                                   uint8_t* any, hipStream_t stream);
+hipError_t launch_box_rays(const float* boxes6, const mrt_ray* rays, uint32_t n, uint8_t* hit, float* entry,
+                           hipStream_t stream);
 hipError_t launch_shade_rays(const SceneDev& s, bool generic, bool branches, const RenderParams& p,
                              const mrt_ray* rays, const int32_t* depths, uint32_t n, uint32_t sample, float* rgb,
                              uint8_t* flag, const StackCfg& sc, uint32_t cap_blocks, uint32_t* err,

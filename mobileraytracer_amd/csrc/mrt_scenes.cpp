@@ -161,6 +161,19 @@ MRT_API int mrt_triangle_intersect_box(const mrt_triangle* tri, const float box_
     return triangle_intersect_box(*tri, Box{ld3(box_min), ld3(box_max)}) ? 1 : 0;
 }
 
+MRT_API int mrt_primitive_intersect_box(int category, const void* prim, const float box_min[3],
+                                        const float box_max[3]) {
+    if (!prim || !box_min || !box_max) return MRT_E_INVALID;
+    const Box box{ld3(box_min), ld3(box_max)};
+    switch (category) {
+        case MRT_CAT_TRIANGLE:
+        case MRT_CAT_LIGHT: return triangle_intersect_box(*static_cast<const mrt_triangle*>(prim), box) ? 1 : 0;
+        case MRT_CAT_SPHERE: return sphere_intersect_box(*static_cast<const mrt_sphere*>(prim), box) ? 1 : 0;
+        case MRT_CAT_PLANE: return plane_intersect_box(*static_cast<const mrt_plane*>(prim), box) ? 1 : 0;
+        default: return MRT_E_INVALID;
+    }
+}
+
 MRT_API int mrt_primitive_aabbs(int category, const void* prims, uint32_t n, float* boxes6) {
     if ((n && (!prims || !boxes6)) || category < MRT_CAT_PLANE || category > MRT_CAT_LIGHT) return MRT_E_INVALID;
     for (uint32_t i = 0; i < n; ++i) {
