@@ -125,7 +125,7 @@ def main() -> int:
         return copy_calibration()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the frame would shard by tiles or samples, see DESIGN.md)", file=sys.stderr)
+              "(a frame would shard by tiles or by sample index; only one GPU's share is built)", file=sys.stderr)
         return 2
 
     import torch

@@ -116,7 +116,7 @@ uint64_t buffers_bytes(const mrt_scene* s) {
                  s->mat.bytes + s->light_geo.bytes + s->light_shade.bytes + s->light_info.bytes;
     for (int i = 0; i < 4; ++i) b += s->nodes[i].bytes + s->perm[i].bytes;
     for (int i = 0; i < 3; ++i) b += s->cell_off[i].bytes + s->cell_ref[i].bytes;
-    return b;
+    return b + s->walk_nodes.bytes + s->wide_nodes.bytes;
 }
 
 void pack_geo(const mrt_triangle& t, uint32_t id, float4* g) {
