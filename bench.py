@@ -166,6 +166,10 @@ def main() -> int:
         f"{(st.prim_tests - st.light_prim_tests - st.shadow_prim_tests) / closest1:.1f} tests, lights "
         f"{st.light_node_visits / closest1:.1f} pairs {st.light_prim_tests / closest1:.1f} tests; per shadow ray: "
         f"{st.shadow_node_visits / shadow1:.1f} pairs {st.shadow_prim_tests / shadow1:.1f} tests")
+    if st.trace_lane_slots:
+        log(f"persistent trace: {st.trace_lane_iters / rays1:.1f} lane iterations per ray, lanes holding a ray "
+            f"{100.0 * st.trace_lane_iters / st.trace_lane_slots:.1f}% of lane iterations, visits+tests per lane "
+            f"iteration {(st.node_visits + st.prim_tests) / max(1, st.trace_lane_iters):.2f}")
 
     desc = make_desc(mrt, abi, cam, kernel=args.kernel, ray_sort=sort, persistent=pers)
     for _ in range(args.warmup):

@@ -215,6 +215,9 @@ typedef struct mrt_render_stats {
                                   iteration loop)                              */
     float ms_trace;            /* wavefront: summed trace launches             */
     uint32_t trace_launches;   /* wavefront: trace launches (iterations)       */
+    uint64_t trace_lane_iters; /* persistent trace, instrumented: iterations in
+                                  which a lane held a ray ...                  */
+    uint64_t trace_lane_slots; /* ... and all lane iterations (SIMD use)       */
 } mrt_render_stats;
 
 /* Accelerator build parameters (Shader::initializeAccelerators inputs). */

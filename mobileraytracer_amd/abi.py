@@ -108,7 +108,8 @@ class RenderStats(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_prim_tests", C.c_uint64),
                 ("samples_done", C.c_uint32), ("covered_pixels", C.c_uint32),
                 ("ms_render", C.c_float), ("ms_kernel", C.c_float), ("ms_trace", C.c_float),
-                ("trace_launches", C.c_uint32)]
+                ("trace_launches", C.c_uint32), ("trace_lane_iters", C.c_uint64),
+                ("trace_lane_slots", C.c_uint64)]
 
 
 class AccelParams(C.Structure):

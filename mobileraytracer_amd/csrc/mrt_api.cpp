@@ -73,6 +73,7 @@ constexpr uint32_t kLinearLights = 16; // up to this many area lights are scanne
 struct mrt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[kWfMaxGroups - 1] = {};  // wavefront slot groups beyond the first
     uint32_t num_cu = 0;
     uint64_t budget = 0;
     std::mutex mu;
@@ -88,7 +89,7 @@ struct mrt_scene {
     uint32_t n_scene_mats = 0;
     DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
     DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes, wide_nodes;
-    DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wavefront;
+    DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wf_state, wf_group[kWfMaxGroups];
     // Host copy of the light tree (pairs relative to its own numbering) for
     // the combined walk tree; light slots follow the triangles in tri_geo.
     BvhOut light_bvh;
@@ -429,81 +430,132 @@ int stage(DevBuf& buf, const void* src, size_t bytes, int mem, hipStream_t st, c
 
 uint32_t cap_blocks(const mrt_ctx* c) { return (c->num_cu ? c->num_cu : 256u) * 16u; }
 
-// The wavefront frame: iterations in batches of kBatch; after each batch the
-// counters are copied to pinned memory, and the host reads the previous
-// batch's copy, so the GPU never waits for the host.  The frame is done when
-// the extension, shadow and ended queues a batch left behind are all empty
-// (the batch queued meanwhile then finds nothing to do).  A stop request
-// lets running paths finish without starting new samples.
-int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p, const WfBuffers& wb,
-                  bool persistent, uint32_t trace_blocks, uint32_t persist_blocks, const StackCfg& sc, uint32_t* dbm,
-                  float* dacc, bool* stopped, float* kernel_ms, float* trace_ms, uint32_t* launches) {
+// Stream of slot group g: the context's stream, then its auxiliary streams
+// (created on first use, default priority).
+int group_stream(mrt_ctx* c, uint32_t g, hipStream_t* out) {
+    if (g == 0) {
+        *out = c->stream;
+        return MRT_OK;
+    }
+    hipStream_t& aux = c->aux[g - 1];
+    if (!aux) MRT_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    *out = aux;
+    return MRT_OK;
+}
+
+// The wavefront frame.  Each slot group runs its own iteration loop on its
+// own stream; the host issues the groups' iterations interleaved, in batches
+// of kBatch.  After each batch a group's counters are copied to pinned memory
+// and the host reads the previous batch's copy, so the GPU never waits for the
+// host.  A group is done when the extension, shadow and ended queues a batch
+// left behind are all empty (the batch queued meanwhile finds nothing to do).
+// A stop request lets running paths finish without starting new samples.
+int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p, std::vector<WfBuffers>& wbs,
+                  bool persistent, uint32_t trace_blocks, uint32_t persist_blocks, const std::vector<StackCfg>& scs,
+                  uint32_t* dbm, float* dacc, bool* stopped, float* kernel_ms, float* trace_ms, uint32_t* launches) {
     constexpr uint32_t kBatch = 4;
-    const hipStream_t st = s->ctx->stream;
-    if (!s->counts_host) MRT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->counts_host), 2 * kCntWords * 4));
-    hipEvent_t b0, b1, chk[2];
-    MRT_HIP(hipEventCreate(&b0));
-    MRT_HIP(hipEventCreate(&b1));
-    MRT_HIP(hipEventCreate(&chk[0]));
-    MRT_HIP(hipEventCreate(&chk[1]));
+    const uint32_t G = static_cast<uint32_t>(wbs.size());
+    hipStream_t st[kWfMaxGroups];
+    for (uint32_t g = 0; g < G; ++g) {
+        const int rc = group_stream(s->ctx, g, &st[g]);
+        if (rc != MRT_OK) return rc;
+    }
+    if (!s->counts_host)
+        MRT_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->counts_host), 2ull * kCntWords * 4 * kWfMaxGroups));
+    // Events: begin, end, per-group end, per-group checks (two each).
+    std::vector<hipEvent_t> evs(2 + 3 * G, nullptr);
     struct Guard {
-        hipEvent_t* e;
+        std::vector<hipEvent_t>& e;
         ~Guard() {
-            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
         }
-    };
-    hipEvent_t all[4] = {b0, b1, chk[0], chk[1]};
-    Guard guard{all};
-    MRT_HIP(hipMemsetAsync(wb.counts, 0, kCntWords * sizeof(uint32_t), st));
-    MRT_HIP(hipEventRecord(b0, st));
-    WfLoop L{};
-    MRT_HIP(launch_wavefront_start(p, d->first_sample, wb, &L, st));
+    } guard{evs};
+    for (hipEvent_t& e : evs) MRT_HIP(hipEventCreate(&e));
+    hipEvent_t b0 = evs[0], b1 = evs[1];
+    auto chk = [&](uint32_t g, uint32_t k) { return evs[2 + 3 * g + k]; };
+    auto gend = [&](uint32_t g) { return evs[2 + 3 * g + 2]; };
+    // Everything queued before the frame on the context stream (uploads,
+    // counter clears) precedes every group.
+    MRT_HIP(hipEventRecord(b0, st[0]));
+    std::vector<WfLoop> L(G);
+    for (uint32_t g = 0; g < G; ++g) {
+        if (g > 0) MRT_HIP(hipStreamWaitEvent(st[g], b0, 0));
+        MRT_HIP(hipMemsetAsync(wbs[g].counts, 0, kCntWords * sizeof(uint32_t), st[g]));
+        MRT_HIP(launch_wavefront_start(p, d->first_sample, wbs[g], &L[g], st[g]));
+    }
     const uint64_t covered = 256ull * p.bsx * p.bsy;
     const uint32_t samples = d->samples_pixel - d->first_sample;
     // A sample takes at most depth_max + 1 iterations (one per vertex, one to
     // finish); more than that means the queues never drained.
     const uint64_t maxIters = static_cast<uint64_t>(samples) * (static_cast<uint64_t>(d->ray_depth_max) + 2u) +
                               4u * kBatch;
-    uint32_t pend[2][3] = {};
+    uint32_t pend[kWfMaxGroups][2][3] = {};
+    uint64_t doneCount[kWfMaxGroups] = {};
+    bool done[kWfMaxGroups] = {};
     uint32_t checks = 0;
     bool stopReq = false;
     size_t ev = 0;
     while (true) {
         if (!stopReq && s->stop.load()) stopReq = true;
         for (uint32_t k = 0; k < kBatch; ++k) {
-            while (ev + 2 > s->trace_events.size()) {
-                hipEvent_t e;
-                MRT_HIP(hipEventCreate(&e));
-                s->trace_events.push_back(e);
+            for (uint32_t g = 0; g < G; ++g) {
+                if (done[g]) continue;
+                while (ev + 2 > s->trace_events.size()) {
+                    hipEvent_t e;
+                    MRT_HIP(hipEventCreate(&e));
+                    s->trace_events.push_back(e);
+                }
+                MRT_HIP(launch_wavefront_iteration(s->dev, s->generic, persistent, d->instrument != 0u, p,
+                                                   d->samples_pixel, stopReq, dbm, dacc, wbs[g], &L[g], trace_blocks,
+                                                   persist_blocks, scs[g], s->counters.as<unsigned long long>(),
+                                                   s->err.as<uint32_t>(), s->trace_events[ev],
+                                                   s->trace_events[ev + 1], st[g]));
+                ev += 2;
             }
-            MRT_HIP(launch_wavefront_iteration(s->dev, s->generic, persistent, d->instrument != 0u, p,
-                                               d->samples_pixel, stopReq, dbm, dacc, wb, &L, trace_blocks,
-                                               persist_blocks, sc, s->counters.as<unsigned long long>(),
-                                               s->err.as<uint32_t>(), s->trace_events[ev], s->trace_events[ev + 1],
-                                               st));
-            ev += 2;
         }
         const uint32_t slot = checks & 1u;
-        MRT_HIP(hipMemcpyAsync(s->counts_host + slot * kCntWords, wb.counts, kCntWords * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, st));
-        MRT_HIP(hipEventRecord(chk[slot], st));
-        std::copy(L.pending, L.pending + 3, pend[slot]);
+        for (uint32_t g = 0; g < G; ++g) {
+            if (done[g]) continue;
+            MRT_HIP(hipMemcpyAsync(s->counts_host + (2 * g + slot) * kCntWords, wbs[g].counts,
+                                   kCntWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st[g]));
+            MRT_HIP(hipEventRecord(chk(g, slot), st[g]));
+            std::copy(L[g].pending, L[g].pending + 3, pend[g][slot]);
+        }
         if (checks > 0) {
             const uint32_t prev = slot ^ 1u;
-            MRT_HIP(hipEventSynchronize(chk[prev]));
-            const uint32_t* c = s->counts_host + prev * kCntWords;
-            s->progress.store(d->first_sample + static_cast<uint32_t>(c[kCntDone] / covered));
-            if (c[pend[prev][0]] == 0u && c[pend[prev][1]] == 0u && c[pend[prev][2]] == 0u) break;
+            bool all = true;
+            for (uint32_t g = 0; g < G; ++g) {
+                if (done[g]) continue;
+                MRT_HIP(hipEventSynchronize(chk(g, prev)));
+                const uint32_t* c = s->counts_host + (2 * g + prev) * kCntWords;
+                doneCount[g] = c[kCntDone];
+                done[g] = c[pend[g][prev][0]] == 0u && c[pend[g][prev][1]] == 0u && c[pend[g][prev][2]] == 0u;
+                all = all && done[g];
+            }
+            uint64_t finished = 0;
+            for (uint32_t g = 0; g < G; ++g) finished += doneCount[g];
+            s->progress.store(d->first_sample + static_cast<uint32_t>(finished / covered));
+            if (all) break;
         }
         ++checks;
-        if (L.iter > maxIters) return fail(MRT_E_HIP, "wavefront queues did not drain");
+        for (uint32_t g = 0; g < G; ++g)
+            if (L[g].iter > maxIters) return fail(MRT_E_HIP, "wavefront queues did not drain");
     }
-    MRT_HIP(hipEventRecord(b1, st));
+    for (uint32_t g = 1; g < G; ++g) {
+        MRT_HIP(hipEventRecord(gend(g), st[g]));
+        MRT_HIP(hipStreamWaitEvent(st[0], gend(g), 0));
+    }
+    MRT_HIP(hipEventRecord(b1, st[0]));
     MRT_HIP(hipEventSynchronize(b1));
-    uint32_t done = 0;
-    MRT_HIP(hipMemcpyAsync(&done, wb.counts + kCntDone, sizeof(done), hipMemcpyDeviceToHost, st));
-    MRT_HIP(hipStreamSynchronize(st));
-    const uint32_t progress = d->first_sample + static_cast<uint32_t>(done / covered);
+    uint64_t finished = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        uint32_t c = 0;
+        MRT_HIP(hipMemcpyAsync(&c, wbs[g].counts + kCntDone, sizeof(c), hipMemcpyDeviceToHost, st[0]));
+        MRT_HIP(hipStreamSynchronize(st[0]));
+        finished += c;
+    }
+    const uint32_t progress = d->first_sample + static_cast<uint32_t>(finished / covered);
     s->progress.store(progress);
     *stopped = stopReq && progress < d->samples_pixel;
     MRT_HIP(hipEventElapsedTime(kernel_ms, b0, b1));
@@ -557,6 +609,11 @@ MRT_API int mrt_ctx_destroy(mrt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
+    for (hipStream_t a : ctx->aux) {
+        if (!a) continue;
+        (void)hipStreamSynchronize(a);
+        (void)hipStreamDestroy(a);
+    }
     delete ctx;
     return MRT_OK;
 }
@@ -1082,32 +1139,22 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     const uint32_t persistBlocks =
         persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks) : 0u;
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
+    // Wavefront slot groups run concurrently, each with its own spill region.
+    const uint32_t slots = useWavefront ? wavefront_slots(p) : 0u;
+    const uint32_t groups = useWavefront ? wavefront_groups(slots) : 1u;
     StackCfg sc;
-    if ((rc = ensure_spill(s, blocks * 256u, &sc)) != MRT_OK) return rc;
-    WfBuffers wb{};
+    if ((rc = ensure_spill(s, blocks * 256u * groups, &sc)) != MRT_OK) return rc;
+    std::vector<StackCfg> scs(groups, sc);
+    for (uint32_t g = 0; g < groups; ++g) {
+        scs[g].spill_stride = blocks * 256u;
+        if (sc.spill) scs[g].spill = sc.spill + static_cast<size_t>(g) * sc.cap_spill * blocks * 256u;
+    }
+    sc = scs[0];
+    std::vector<WfBuffers> wbs;
     if (useWavefront) {
-        const uint32_t slots = wavefront_slots(p);
         const size_t scanBytes = wavefront_scan_bytes();
-        if ((rc = s->wavefront.alloc(wavefront_bytes(slots) + scanBytes + 256)) != MRT_OK) return rc;
-        float4* base = s->wavefront.as<float4>();
-        wb.sA = base;
-        wb.sB = base + slots;
-        wb.sC = base + 2ull * slots;
-        wb.queue_a = base + 3ull * slots;
-        wb.queue_b = base + 5ull * slots;
-        wb.hits = base + 7ull * slots;
-        wb.shadow = base + 8ull * slots;
-        uint32_t* u = reinterpret_cast<uint32_t*>(base + 12ull * slots);
-        wb.counts = u;
-        wb.ended[0] = u + 64;
-        wb.ended[1] = wb.ended[0] + slots;
-        wb.keys = wb.ended[1] + slots;
-        wb.hist = wb.keys + slots;
-        wb.offs = wb.hist + kSortBins;
-        const uintptr_t t = (reinterpret_cast<uintptr_t>(wb.offs + kSortBins) + 255u) & ~static_cast<uintptr_t>(255u);
-        wb.scan_temp = reinterpret_cast<void*>(t);
-        wb.scan_bytes = scanBytes;
-        wb.sort = d->ray_sort == 1u ? 1u : 0u;
+        if ((rc = s->wf_state.alloc(wavefront_state_bytes(slots))) != MRT_OK) return rc;
+        float4* state = s->wf_state.as<float4>();
         // Key box: the triangle BVH's root box (the unit cube without triangles).
         f3 mn{0.0f, 0.0f, 0.0f}, mx{1.0f, 1.0f, 1.0f};
         const Accel& ta = s->dev.acc[MRT_CAT_TRIANGLE];
@@ -1115,12 +1162,43 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             mn = ld3(ta.root_mn);
             mx = ld3(ta.root_mx);
         }
-        for (int k = 0; k < 3; ++k) {
-            const float e = comp(mx, k) - comp(mn, k);
-            wb.box_mn[k] = comp(mn, k);
-            wb.box_inv[k] = e > 0.0f ? 32.0f / e : 0.0f;
+        // Groups own whole waves of slots (64 slots, one 8x8 pixel block each).
+        const uint32_t waves = slots / 64u;
+        for (uint32_t g = 0; g < groups; ++g) {
+            WfBuffers wb{};
+            const uint32_t w0 = waves * g / groups, w1 = waves * (g + 1) / groups;
+            wb.slot_base = 64u * w0;
+            wb.slot_count = 64u * (w1 - w0);
+            const uint32_t n = wb.slot_count;
+            if ((rc = s->wf_group[g].alloc(wavefront_group_bytes(n) + scanBytes + 256)) != MRT_OK) return rc;
+            float4* base = s->wf_group[g].as<float4>();
+            wb.sA = state;
+            wb.sB = state + slots;
+            wb.sC = state + 2ull * slots;
+            wb.queue_a = base;
+            wb.queue_b = base + 2ull * n;
+            wb.hits = base + 4ull * n;
+            wb.shadow = base + 5ull * n;
+            uint32_t* u = reinterpret_cast<uint32_t*>(base + 9ull * n);
+            wb.counts = u;
+            wb.ended[0] = u + 64;
+            wb.ended[1] = wb.ended[0] + n;
+            wb.keys = wb.ended[1] + n;
+            wb.hist = wb.keys + n;
+            wb.offs = wb.hist + kSortBins;
+            const uintptr_t t =
+                (reinterpret_cast<uintptr_t>(wb.offs + kSortBins) + 255u) & ~static_cast<uintptr_t>(255u);
+            wb.scan_temp = reinterpret_cast<void*>(t);
+            wb.scan_bytes = scanBytes;
+            wb.sort = d->ray_sort == 1u ? 1u : 0u;
+            for (int k = 0; k < 3; ++k) {
+                const float e = comp(mx, k) - comp(mn, k);
+                wb.box_mn[k] = comp(mn, k);
+                wb.box_inv[k] = e > 0.0f ? 32.0f / e : 0.0f;
+            }
+            MRT_HIP(hipMemsetAsync(wb.hist, 0, kSortBins * sizeof(uint32_t), st));
+            wbs.push_back(wb);
         }
-        MRT_HIP(hipMemsetAsync(wb.hist, 0, kSortBins * sizeof(uint32_t), st));
     }
     s->stop.store(false);
     s->progress.store(d->first_sample);
@@ -1136,7 +1214,7 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     float kernel_ms = 0.0f, trace_ms = 0.0f;
     uint32_t traceLaunches = 0;
     if (useWavefront && d->first_sample < d->samples_pixel) {
-        if ((rc = run_wavefront(s, d, p, wb, persistent, traceBlocks, persistBlocks, sc, dbm, dacc, &stopped,
+        if ((rc = run_wavefront(s, d, p, wbs, persistent, traceBlocks, persistBlocks, scs, dbm, dacc, &stopped,
                                 &kernel_ms, &trace_ms, &traceLaunches)) != MRT_OK)
             return rc;
     }
@@ -1188,6 +1266,8 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->ms_kernel = kernel_ms;
         stats->ms_trace = trace_ms;
         stats->trace_launches = traceLaunches;
+        stats->trace_lane_iters = cnt[10];
+        stats->trace_lane_slots = cnt[11];
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;

@@ -29,19 +29,22 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // Counter slots: 0 camera, 1 extension, 2 shadow, 3 specular rays; 4..6 node
-// pairs (closest, lights, any hit); 7..9 primitive tests (same order).
+// pairs (closest, lights, any hit); 7..9 primitive tests (same order); 10/11
+// lane iterations holding a ray / all lane iterations of the persistent trace.
 template <bool COUNT>
 __device__ __forceinline__ void flush_counts(const RayCounts& rc, unsigned long long* counters, uint32_t overflow,
                                              uint32_t* err) {
-    uint32_t v[10] = {rc.camera, rc.extension, rc.shadow, rc.specular, 0, 0, 0, 0, 0, 0};
+    uint32_t v[12] = {rc.camera, rc.extension, rc.shadow, rc.specular, 0, 0, 0, 0, 0, 0, 0, 0};
     if (COUNT) {
         for (int k = 0; k < 3; ++k) {
             v[4 + k] = rc.ts.pairs[k];
             v[7 + k] = rc.ts.prims[k];
         }
+        v[10] = rc.ts.lane_iters;
+        v[11] = rc.ts.lane_slots;
     }
     const uint32_t of = wave_sum(overflow);
-    constexpr int kN = COUNT ? 10 : 4;
+    constexpr int kN = COUNT ? 12 : 4;
     uint32_t s[kN];
     for (int k = 0; k < kN; ++k) s[k] = wave_sum(v[k]);
     if ((threadIdx.x & 63u) == 0u) {

@@ -46,6 +46,8 @@ MRT_HD RayT make_ray(f3 o, f3 d, uint32_t excl) {
 struct TraceStats {
     uint32_t pairs[3];
     uint32_t prims[3];
+    uint32_t lane_iters;  // persistent trace: iterations a lane held a ray
+    uint32_t lane_slots;  // persistent trace: iterations x lanes
 };
 
 // A word of LDS; the explicit address space keeps stack accesses on ds_*

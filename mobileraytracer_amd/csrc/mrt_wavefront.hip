@@ -110,14 +110,15 @@ __device__ __forceinline__ void reset_path(const WfBuffers& b, uint32_t slot, ui
 }
 
 // ------------------------------------------------------------- generate --
-// Camera rays of the first sample of every covered slot.
-__global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t sample, uint32_t slots,
-                                                        WfBuffers b, float4* queue, uint32_t* count) {
+// Camera rays of the first sample of every covered slot of the group.
+__global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t sample, WfBuffers b, float4* queue,
+                                                        uint32_t* count) {
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t slot = logical * kBlock + threadIdx.x;
+    const uint32_t local = logical * kBlock + threadIdx.x;
+    const uint32_t slot = b.slot_base + local;
     bool live = false;
     uint32_t x = 0, y = 0;
-    if (slot < slots) {
+    if (local < b.slot_count) {
         slot_pixel(p, slot, &x, &y);
         live = x < 16u * p.bsx && y < 16u * p.bsy;
     }
@@ -243,17 +244,14 @@ __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, 
                                           TraceStats& ts) {
     const bool canVisit = has && walk_can_visit(w);
     const bool canTest = has && w.cur != 0u;
-    const uint32_t nv = static_cast<uint32_t>(__popcll(__ballot(canVisit)));
-    const uint32_t nt = static_cast<uint32_t>(__popcll(__ballot(canTest)));
-    if (nv >= nt) {
+    const bool occluded = canTest && walk_test<COUNT>(s, w, shadow, ts);
+    if (canVisit && !occluded) {
         const float lim = cull_limit(w.h.t);
         const int kind = shadow ? 2 : 0;
-        if (canVisit && finite) walk_visit_wide<COUNT>(s, w, lim, st, ts, kind);
-        if (__ballot(canVisit && !finite) != 0ull && canVisit && !finite)
-            walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
-        return false;
+        if (finite) walk_visit_wide<COUNT>(s, w, lim, st, ts, kind);
+        else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
     }
-    return canTest && walk_test<COUNT>(s, w, shadow, ts);
+    return occluded;
 }
 
 // Wave-private run of trace-queue entries [next, end): a wave reserves
@@ -354,6 +352,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
             }
         }
         if (__ballot(has || !exhausted) == 0ull) break;
+        if (COUNT) {
+            ++rc.ts.lane_slots;
+            if (has) ++rc.ts.lane_iters;
+        }
         const bool occluded = walk_work<COUNT>(s, w, has, shadow, finite, st, rc.ts);
         if (has && walk_done(w)) {
             if (!shadow) hits[idx] = make_float4(w.h.t, __uint_as_float(w.h.id), w.h.u, w.h.v);
@@ -563,14 +565,14 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap) {
 
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,
                                   hipStream_t stream) {
-    const uint32_t slots = wavefront_slots(p);
     L->qin = b.queue_a;
     L->qout = b.queue_b;
     L->cin = kCntQueue0;
     L->cout = kCntQueue1;
     L->iter = 0;
-    k_wf_generate<<<(slots + kBlock - 1u) / kBlock, kBlock, 0, stream>>>(p, first_sample, slots, b, L->qin,
-                                                                          b.counts + L->cin);
+    if (b.slot_count == 0u) return hipSuccess;
+    k_wf_generate<<<(b.slot_count + kBlock - 1u) / kBlock, kBlock, 0, stream>>>(p, first_sample, b, L->qin,
+                                                                                 b.counts + L->cin);
     return hipGetLastError();
 }
 

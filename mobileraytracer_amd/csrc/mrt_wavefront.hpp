@@ -26,7 +26,11 @@ enum : uint32_t {
     kCntWords = 16
 };
 
+// Buffers of one slot group.  The path state is indexed by the global slot;
+// the queues, lists and counters belong to the group (slots [slot_base,
+// slot_base + slot_count)), so groups run independently on their own streams.
 struct WfBuffers {
+    uint32_t slot_base, slot_count;
     float4* sA;        // per slot: T.xyz, flags
     float4* sB;        // per slot: acc_all.xyz, pixel id
     float4* sC;        // per slot: acc_prefix.xyz
@@ -60,11 +64,21 @@ uint32_t wavefront_slots(const RenderParams& p);
 // Scratch bytes of the device scan over kSortBins counters.
 size_t wavefront_scan_bytes();
 
-// Bytes of the wavefront buffers for `slots` slots (scan scratch excluded).
-inline size_t wavefront_bytes(uint32_t slots) {
-    return static_cast<size_t>(slots) * sizeof(float4) * (3 + 2 + 2 + 1 + 4) +
+// Bytes of the path state of `slots` slots (48 per slot).
+inline size_t wavefront_state_bytes(uint32_t slots) { return static_cast<size_t>(slots) * sizeof(float4) * 3; }
+
+// Bytes of one group's queues, lists and counters for `slots` slots (scan
+// scratch excluded).
+inline size_t wavefront_group_bytes(uint32_t slots) {
+    return static_cast<size_t>(slots) * sizeof(float4) * (2 + 2 + 1 + 4) +
            static_cast<size_t>(slots) * sizeof(uint32_t) * 3 + 2ull * kSortBins * sizeof(uint32_t) + 512;
 }
+
+// Slot groups of a frame: two pipelines on two streams, so that one group's
+// kernels fill the compute units while the other's trace drains its slowest
+// rays (a persistent trace launch ends only with its last ray).
+constexpr uint32_t kWfMaxGroups = 2;
+inline uint32_t wavefront_groups(uint32_t slots) { return slots >= (1u << 16) ? kWfMaxGroups : 1u; }
 
 // Workgroups of the persistent trace kernel: as many as can be resident at
 // once on `num_cu` compute units, at most `cap`.

@@ -193,10 +193,12 @@ def test_render_deterministic(gpu_ctx):
     assert np.array_equal(b1, b2) and np.array_equal(a1, a2)
 
 
-@pytest.mark.parametrize("generic", [False, True])
-def test_wavefront_equals_megakernel(gpu_ctx, generic):
+@pytest.mark.parametrize("generic,size", [(False, (256, 144)), (True, (256, 144)), (False, (512, 288))])
+def test_wavefront_equals_megakernel(gpu_ctx, generic, size):
     """Both kernel shapes evaluate every path with the same operations in the
-    same order: bitmaps, FP32 radiance and ray counts are identical."""
+    same order: bitmaps, FP32 radiance and ray counts are identical.  At
+    512x288 the wavefront frame runs as two slot groups on two streams."""
+    W, H = size
     data = mrt.soup_scene(9, 80000, 800)
     if generic:
         # Planes and spheres without reflection/transmission.
@@ -211,17 +213,17 @@ def test_wavefront_equals_megakernel(gpu_ctx, generic):
         data.materials = np.zeros(1, abi.MATERIAL_DTYPE)
         data.materials[0]["kd"] = (0.5, 0.6, 0.7)
         data.materials[0]["ior"] = 1.0
-    cam = soup_camera(256, 144)
+    cam = soup_camera(W, H)
     res = []
     for kernel, sort, pers in ((abi.KERNEL_MEGAKERNEL, 0, 0), (abi.KERNEL_WAVEFRONT, 2, 2),
                                (abi.KERNEL_WAVEFRONT, 1, 2), (abi.KERNEL_WAVEFRONT, 2, 1),
                                (abi.KERNEL_WAVEFRONT, 1, 1)):
-        desc = mrt.render_desc(abi.SHADER_PATHTRACER, 256, 144, 3, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
+        desc = mrt.render_desc(abi.SHADER_PATHTRACER, W, H, 3, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
                                rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6, kernel=kernel,
                                ray_sort=sort, persistent=pers)
         gpu = mrt.Scene(gpu_ctx, data)
         gpu.build(abi.BVH)
-        acc = np.zeros((144, 256, 3), np.float32)
+        acc = np.zeros((H, W, 3), np.float32)
         bm, st = gpu.render(desc, accum=acc)
         res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow, st.rays_specular)))
     b0, a0, c0 = res[0]
