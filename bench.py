@@ -180,16 +180,21 @@ def main() -> int:
     closest = 0
     kernel_ms = 0.0
     trace_ms = 0.0
+    busy_ms = 0.0
     launches = 0
+    groups = 0
     for _ in range(args.steps):
         _, st = scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
         rays += st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
         closest += st.rays_camera + st.rays_extension + st.rays_specular
         kernel_ms += st.ms_kernel
         trace_ms += st.ms_trace
+        busy_ms += st.ms_trace_busy
         launches += st.trace_launches
+        groups = st.trace_groups
         log(f"step: {st.ms_render:.1f} ms, {st.rays_camera} camera / {st.rays_extension} extension / "
-            f"{st.rays_shadow} shadow rays; trace {st.ms_trace:.1f} ms in {st.trace_launches} launches")
+            f"{st.rays_shadow} shadow rays; trace {st.ms_trace:.1f} ms in {st.trace_launches} launches "
+            f"({st.trace_groups} slot groups, some trace launch running {st.ms_trace_busy:.1f} ms)")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     mrays = rays / elapsed / 1e6
@@ -200,7 +205,10 @@ def main() -> int:
     # the persistent walk, 64 per node pair otherwise) + 48 per triangle test
     # (three float4 of tri_geo) + 48 of ray record in and result out.
     # Megakernel: the render-pass kernel, one launch per sample pass, plus 256
-    # bytes of shading per closest-hit ray.
+    # bytes of shading per closest-hit ray.  Slot groups run their trace
+    # launches concurrently on separate streams, so a launch's duration
+    # includes time shared with the other groups' launches; device_achieved
+    # divides all trace bytes by the time at least one trace launch ran.
     wavefront = launches > 0
     persistent = wavefront and not args.no_persistent
     node_bytes = 128 if persistent else 64
@@ -216,6 +224,7 @@ def main() -> int:
         avg_launch_s = kernel_ms / 1000.0 / launches
     bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    device_achieved = bytes_total / (busy_ms / 1000.0) / 1e9 if wavefront and busy_ms > 0 else achieved
     traffic = measured_traffic(kernel_name) if persistent else None
 
     out = {
@@ -238,6 +247,8 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel_name, "launches": launches,
                      "avg_launch_ms": round(avg_launch_s * 1000.0, 4),
+                     "concurrent_groups": groups, "device_achieved": round(device_achieved, 1),
+                     "device_frac": round(device_achieved / HBM_PEAK_GBS, 4),
                      "bytes_per_launch": round(bytes_per_launch), "bytes_per_ray": round(bytes_per_ray, 1),
                      "node_visits_per_ray": round(pairs_per_ray, 2), "tests_per_ray": round(prims_per_ray, 2)},
     }

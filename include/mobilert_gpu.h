@@ -218,6 +218,9 @@ typedef struct mrt_render_stats {
     uint64_t trace_lane_iters; /* persistent trace, instrumented: iterations in
                                   which a lane held a ray ...                  */
     uint64_t trace_lane_slots; /* ... and all lane iterations (SIMD use)       */
+    float ms_trace_busy;       /* wavefront: time at least one trace launch ran
+                                  (launches of slot groups overlap)            */
+    uint32_t trace_groups;     /* wavefront: slot groups run concurrently      */
 } mrt_render_stats;
 
 /* Accelerator build parameters (Shader::initializeAccelerators inputs). */

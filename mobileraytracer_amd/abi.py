@@ -109,7 +109,7 @@ class RenderStats(C.Structure):
                 ("samples_done", C.c_uint32), ("covered_pixels", C.c_uint32),
                 ("ms_render", C.c_float), ("ms_kernel", C.c_float), ("ms_trace", C.c_float),
                 ("trace_launches", C.c_uint32), ("trace_lane_iters", C.c_uint64),
-                ("trace_lane_slots", C.c_uint64)]
+                ("trace_lane_slots", C.c_uint64), ("ms_trace_busy", C.c_float), ("trace_groups", C.c_uint32)]
 
 
 class AccelParams(C.Structure):

@@ -452,7 +452,8 @@ int group_stream(mrt_ctx* c, uint32_t g, hipStream_t* out) {
 // A stop request lets running paths finish without starting new samples.
 int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p, std::vector<WfBuffers>& wbs,
                   bool persistent, uint32_t trace_blocks, uint32_t persist_blocks, const std::vector<StackCfg>& scs,
-                  uint32_t* dbm, float* dacc, bool* stopped, float* kernel_ms, float* trace_ms, uint32_t* launches) {
+                  uint32_t* dbm, float* dacc, bool* stopped, float* kernel_ms, float* trace_ms, float* busy_ms,
+                  uint32_t* launches) {
     constexpr uint32_t kBatch = 4;
     const uint32_t G = static_cast<uint32_t>(wbs.size());
     hipStream_t st[kWfMaxGroups];
@@ -559,13 +560,29 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
     s->progress.store(progress);
     *stopped = stopReq && progress < d->samples_pixel;
     MRT_HIP(hipEventElapsedTime(kernel_ms, b0, b1));
+    // Summed launch durations, and the union of the launch intervals.
     float total = 0.0f;
+    std::vector<std::pair<float, float>> spans;
     for (size_t i = 0; i < ev; i += 2) {
-        float t = 0.0f;
-        MRT_HIP(hipEventElapsedTime(&t, s->trace_events[i], s->trace_events[i + 1]));
-        total += t;
+        float a = 0.0f, b = 0.0f;
+        MRT_HIP(hipEventElapsedTime(&a, b0, s->trace_events[i]));
+        MRT_HIP(hipEventElapsedTime(&b, b0, s->trace_events[i + 1]));
+        total += b - a;
+        spans.emplace_back(a, b);
+    }
+    std::sort(spans.begin(), spans.end());
+    float busy = 0.0f, end = -1.0f;
+    for (const auto& sp : spans) {
+        if (sp.first > end) {
+            busy += sp.second - sp.first;
+            end = sp.second;
+        } else if (sp.second > end) {
+            busy += sp.second - end;
+            end = sp.second;
+        }
     }
     *trace_ms = total;
+    *busy_ms = busy;
     *launches = static_cast<uint32_t>(ev / 2);
     return MRT_OK;
 }
@@ -1211,11 +1228,11 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     MRT_HIP(hipEventCreate(&k1));
     MRT_HIP(hipEventRecord(e0, st));
     bool stopped = false;
-    float kernel_ms = 0.0f, trace_ms = 0.0f;
+    float kernel_ms = 0.0f, trace_ms = 0.0f, trace_busy = 0.0f;
     uint32_t traceLaunches = 0;
     if (useWavefront && d->first_sample < d->samples_pixel) {
         if ((rc = run_wavefront(s, d, p, wbs, persistent, traceBlocks, persistBlocks, scs, dbm, dacc, &stopped,
-                                &kernel_ms, &trace_ms, &traceLaunches)) != MRT_OK)
+                                &kernel_ms, &trace_ms, &trace_busy, &traceLaunches)) != MRT_OK)
             return rc;
     }
     for (uint32_t sample = d->first_sample; !useWavefront && sample < d->samples_pixel; ++sample) {
@@ -1268,6 +1285,8 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->trace_launches = traceLaunches;
         stats->trace_lane_iters = cnt[10];
         stats->trace_lane_slots = cnt[11];
+        stats->ms_trace_busy = trace_busy;
+        stats->trace_groups = useWavefront ? static_cast<uint32_t>(wbs.size()) : 0u;
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;
