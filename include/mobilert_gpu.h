@@ -43,7 +43,8 @@ enum mrt_status {
     MRT_E_UNSUPPORTED = -4, /* configuration not supported by the kernels    */
     MRT_E_BUDGET = -5,      /* admission refused: HBM budget exceeded        */
     MRT_E_STOPPED = -6,     /* render stopped by mrt_render_stop             */
-    MRT_E_NODEVICE = -7     /* no HIP device available                       */
+    MRT_E_NODEVICE = -7,    /* no HIP device available                       */
+    MRT_E_TIMEOUT = -8      /* mrt_server_wait: the job is not done yet       */
 };
 
 /* ---- enums that keep the reference's integer values -------------------- */
@@ -241,6 +242,7 @@ enum mrt_mem { MRT_MEM_HOST = 0, MRT_MEM_DEVICE = 1 };
 MRT_API const char* mrt_last_error(void);
 MRT_API int mrt_device_count(int* count);
 MRT_API int mrt_ctx_create(int device, mrt_ctx** out);
+/* Fails with MRT_E_INVALID while scenes or render servers of the context live. */
 MRT_API int mrt_ctx_destroy(mrt_ctx* ctx);
 /* HBM budget for scene admission, bytes (0 = unlimited). Mirrors the memory
  * admission of app/System_dependent/Android_JNI/DrawView.cpp:373-416. */
@@ -250,6 +252,7 @@ MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx);
 /* ---- scene -------------------------------------------------------------- */
 /* Copies the scene to the device (Shader(Scene&&, ...), Shader.cpp:18-25). */
 MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* desc, mrt_scene** out);
+/* Fails with MRT_E_INVALID while a render server holds the scene. */
 MRT_API int mrt_scene_destroy(mrt_scene* scene);
 /* Builds the accelerator of every category (Shader::initializeAccelerators,
  * Shader.cpp:31-122): BVH with the reference in-order SAH builder
@@ -300,6 +303,77 @@ MRT_API int mrt_render(mrt_scene* scene, const mrt_render_desc* desc, uint32_t* 
 MRT_API int mrt_render_stop(mrt_scene* scene);
 /* Renderer::getSample (Renderer.cpp:118-120). */
 MRT_API int mrt_render_progress(const mrt_scene* scene, uint32_t* samples_done);
+
+/* ---- render server (continuous batching) -------------------------------- */
+/* No counterpart in the reference, whose Renderer renders one frame at a time
+ * (Renderer.cpp:31-59); this is the serving loop around it.  A server keeps
+ * scenes resident (their walk trees in one device arena) and runs the render
+ * jobs submitted to it in shared wavefront pipelines: a job is admitted, in
+ * submission order, as soon as enough path slots and job memory are free; its
+ * paths then run in the same kernel launches as those of the jobs already
+ * running, and its bitmap / accumulation end up exactly as mrt_render leaves
+ * them for the same request.  Jobs need the wavefront pipeline (PathTracer,
+ * samples_light 1, no reflection/transmission, scenes of triangles and lights
+ * with a BVH).  All calls are thread-safe. */
+typedef struct mrt_server mrt_server;
+
+typedef struct mrt_server_desc {
+    uint32_t max_slots;        /* path slots (0: 6291456, three 1920x1080 frames) */
+    uint32_t max_jobs;         /* jobs admitted at once (0: 256)               */
+    uint32_t max_scenes;       /* resident scenes (0: 64)                      */
+    uint32_t pipelines;        /* concurrent pipelines, 1..3 (0: 3)            */
+    uint64_t job_memory;       /* device bytes for the outputs of admitted
+                                  jobs (host-memory jobs stage their bitmap and
+                                  accumulation there) (0: 1 GiB)               */
+    uint64_t scene_memory;     /* device bytes for the scene arena (0: no limit
+                                  beyond the context budget)                   */
+    uint32_t reserved[4];
+} mrt_server_desc;
+
+enum mrt_job_state { MRT_JOB_QUEUED = 0, MRT_JOB_RUNNING = 1, MRT_JOB_DONE = 2, MRT_JOB_FAILED = 3 };
+
+typedef struct mrt_job_info {
+    int32_t state;             /* enum mrt_job_state                           */
+    int32_t status;            /* MRT_OK, MRT_E_STOPPED after mrt_server_cancel,
+                                  or the error of a failed job                 */
+    double t_submit_us;        /* host steady clock, microseconds: submitted,  */
+    double t_admit_us;         /* admitted (0 while queued),                   */
+    double t_gpu_done_us;      /* last sample seen done on the device,         */
+    double t_done_us;          /* result in the caller's buffers               */
+} mrt_job_info;
+
+typedef struct mrt_serving_stats {
+    uint64_t jobs_submitted, jobs_done, jobs_failed;
+    uint64_t rays_camera, rays_extension, rays_shadow;  /* all jobs so far      */
+    uint32_t jobs_queued, jobs_running;
+    uint32_t slots_total, slots_used;
+    uint64_t scene_bytes;      /* scene arena in use                           */
+    uint64_t job_bytes_used;   /* job memory in use                            */
+    uint64_t batches;          /* iteration batches issued                     */
+} mrt_serving_stats;
+
+MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_server** out);
+/* Cancels queued jobs, lets running ones finish, then frees the server. */
+MRT_API int mrt_server_destroy(mrt_server* server);
+/* Makes a built scene resident (copies its walk trees into the arena).  The
+ * scene must outlive the server; mrt_scene_destroy refuses it meanwhile. */
+MRT_API int mrt_server_add_scene(mrt_server* server, mrt_scene* scene, uint32_t* scene_id);
+/* Queues a render of a resident scene: the arguments mean what they mean for
+ * mrt_render.  Host buffers are read at admission and written when the job
+ * is done; they must stay valid until then. */
+MRT_API int mrt_server_submit(mrt_server* server, uint32_t scene_id, const mrt_render_desc* desc,
+                              uint32_t* bitmap, uint32_t stride_bytes, float* accum, int mem,
+                              uint64_t* job_id);
+/* Waits until the job is done or failed (timeout_us < 0: no limit) and
+ * returns its status (MRT_E_TIMEOUT on time-out); the job is forgotten once a
+ * wait has returned its end. */
+MRT_API int mrt_server_wait(mrt_server* server, uint64_t job_id, int64_t timeout_us, mrt_job_info* info);
+MRT_API int mrt_server_poll(mrt_server* server, uint64_t job_id, mrt_job_info* info);
+/* Renderer::stopRender for one job: queued, it fails with MRT_E_STOPPED;
+ * running, its pixels finish the sample in flight, the samples done so far
+ * reach the caller's buffers and it fails with MRT_E_STOPPED. */
+MRT_API int mrt_server_cancel(mrt_server* server, uint64_t job_id);
+MRT_API int mrt_server_stats(mrt_server* server, mrt_serving_stats* stats);
 
 /* ---- host-side construction helpers (no device work) ------------------- */
 /* Triangle constructors (Triangle.cpp:12-55): normals/colors may be NULL. */

@@ -115,8 +115,9 @@ class Context:
         self.h = h
 
     def close(self):
-        if self.h:
-            lib().mrt_ctx_destroy(self.h)
+        # The library refuses while scenes or servers of the context live
+        # (the handle is kept, and freed by the last close that succeeds).
+        if self.h and lib().mrt_ctx_destroy(self.h) == abi.MRT_OK:
             self.h = None
 
     def __del__(self):
@@ -154,8 +155,8 @@ class Scene:
         self.h = h
 
     def close(self):
-        if self.h:
-            lib().mrt_scene_destroy(self.h)
+        # Refused while a render server holds the scene.
+        if self.h and lib().mrt_scene_destroy(self.h) == abi.MRT_OK:
             self.h = None
 
     def __del__(self):
@@ -219,3 +220,75 @@ class Scene:
         v = C.c_uint32()
         check(lib().mrt_render_progress(self.h, C.byref(v)), "mrt_render_progress")
         return v.value
+
+
+class RenderServer:
+    """Continuous-batching render server over resident scenes (mrt_server_*).
+
+    submit() queues a render and returns a job id; wait() blocks until its
+    bitmap (and accumulation) are in the caller's buffers."""
+
+    def __init__(self, ctx: Context, max_slots: int = 0, max_jobs: int = 0, max_scenes: int = 0,
+                 pipelines: int = 0, job_memory: int = 0, scene_memory: int = 0):
+        self.ctx = ctx
+        d = abi.ServerDesc()
+        d.max_slots, d.max_jobs, d.max_scenes, d.pipelines = max_slots, max_jobs, max_scenes, pipelines
+        d.job_memory, d.scene_memory = job_memory, scene_memory
+        h = C.c_void_p()
+        check(lib().mrt_server_create(ctx.h, C.byref(d), C.byref(h)), "mrt_server_create")
+        self.h = h
+        self.scenes = []
+        self._buffers = {}
+
+    def close(self):
+        if self.h:
+            lib().mrt_server_destroy(self.h)
+            self.h = None
+            self._buffers.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene: Scene) -> int:
+        v = C.c_uint32()
+        check(lib().mrt_server_add_scene(self.h, scene.h, C.byref(v)), "mrt_server_add_scene")
+        self.scenes.append(scene)
+        return v.value
+
+    def submit(self, scene_id: int, desc: abi.RenderDesc, bitmap=None, accum=None, stride_bytes: int | None = None,
+               mem: int = MEM_HOST) -> int:
+        if bitmap is None:
+            bitmap = np.zeros((desc.height, desc.width), np.uint32)
+        if stride_bytes is None:
+            stride_bytes = desc.width * 4
+        jid = C.c_uint64()
+        check(lib().mrt_server_submit(self.h, scene_id, C.byref(desc), ptr(bitmap), stride_bytes, ptr(accum), mem,
+                                      C.byref(jid)), "mrt_server_submit")
+        self._buffers[jid.value] = (bitmap, accum)
+        return jid.value
+
+    def wait(self, job_id: int, timeout_us: int = -1):
+        """Returns (bitmap, accum, JobInfo); raises MrtError on a failed job."""
+        info = abi.JobInfo()
+        rc = lib().mrt_server_wait(self.h, job_id, timeout_us, C.byref(info))
+        if rc == abi.MRT_E_TIMEOUT:
+            return None, None, info
+        bufs = self._buffers.pop(job_id, (None, None))
+        check(rc, "mrt_server_wait")
+        return bufs[0], bufs[1], info
+
+    def poll(self, job_id: int) -> abi.JobInfo:
+        info = abi.JobInfo()
+        check(lib().mrt_server_poll(self.h, job_id, C.byref(info)), "mrt_server_poll")
+        return info
+
+    def cancel(self, job_id: int) -> None:
+        check(lib().mrt_server_cancel(self.h, job_id), "mrt_server_cancel")
+
+    def stats(self) -> abi.ServingStats:
+        st = abi.ServingStats()
+        check(lib().mrt_server_stats(self.h, C.byref(st)), "mrt_server_stats")
+        return st

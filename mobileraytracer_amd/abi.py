@@ -21,6 +21,7 @@ MRT_E_UNSUPPORTED = -4
 MRT_E_BUDGET = -5
 MRT_E_STOPPED = -6
 MRT_E_NODEVICE = -7
+MRT_E_TIMEOUT = -8
 
 NAIVE, REGULAR_GRID, BVH = 0, 1, 2
 SHADER_NOSHADOWS, SHADER_WHITTED, SHADER_PATHTRACER, SHADER_DEPTHMAP, SHADER_DIFFUSE = 0, 1, 2, 3, 4
@@ -32,6 +33,7 @@ LIGHT_AREA, LIGHT_POINT = 0, 1
 CAMERA_PERSPECTIVE, CAMERA_ORTHOGRAPHIC = 0, 1
 KERNEL_AUTO, KERNEL_MEGAKERNEL, KERNEL_WAVEFRONT = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 CORNELL_TRIANGLES, CORNELL_MATERIALS, CORNELL_LIGHTS = 34, 7, 2
 
 
@@ -112,6 +114,25 @@ class RenderStats(C.Structure):
                 ("trace_lane_slots", C.c_uint64), ("ms_trace_busy", C.c_float), ("trace_groups", C.c_uint32)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("max_slots", C.c_uint32), ("max_jobs", C.c_uint32), ("max_scenes", C.c_uint32),
+                ("pipelines", C.c_uint32), ("job_memory", C.c_uint64), ("scene_memory", C.c_uint64),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("t_submit_us", C.c_double),
+                ("t_admit_us", C.c_double), ("t_gpu_done_us", C.c_double), ("t_done_us", C.c_double)]
+
+
+class ServingStats(C.Structure):
+    _fields_ = [("jobs_submitted", C.c_uint64), ("jobs_done", C.c_uint64), ("jobs_failed", C.c_uint64),
+                ("rays_camera", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("jobs_queued", C.c_uint32), ("jobs_running", C.c_uint32), ("slots_total", C.c_uint32),
+                ("slots_used", C.c_uint32), ("scene_bytes", C.c_uint64), ("job_bytes_used", C.c_uint64),
+                ("batches", C.c_uint64)]
+
+
 class AccelParams(C.Structure):
     _fields_ = [("camera_min", F3), ("camera_max", F3), ("leaf_size", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
@@ -165,6 +186,14 @@ def _proto(lib):
         "mrt_render": (C.c_int, [P, C.POINTER(RenderDesc), P, u32, P, C.c_int, C.POINTER(RenderStats)]),
         "mrt_render_stop": (C.c_int, [P]),
         "mrt_render_progress": (C.c_int, [P, C.POINTER(u32)]),
+        "mrt_server_create": (C.c_int, [P, C.POINTER(ServerDesc), C.POINTER(P)]),
+        "mrt_server_destroy": (C.c_int, [P]),
+        "mrt_server_add_scene": (C.c_int, [P, P, C.POINTER(u32)]),
+        "mrt_server_submit": (C.c_int, [P, u32, C.POINTER(RenderDesc), P, u32, P, C.c_int, C.POINTER(u64)]),
+        "mrt_server_wait": (C.c_int, [P, u64, C.c_int64, C.POINTER(JobInfo)]),
+        "mrt_server_poll": (C.c_int, [P, u64, C.POINTER(JobInfo)]),
+        "mrt_server_cancel": (C.c_int, [P, u64]),
+        "mrt_server_stats": (C.c_int, [P, C.POINTER(ServingStats)]),
         "mrt_triangle_make": (C.c_int, [P, P, P, i32, P, P, P]),
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
         "mrt_tile_values": (C.c_int, [u32, P]),

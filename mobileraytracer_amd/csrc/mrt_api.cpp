@@ -10,107 +10,15 @@
 #include <string>
 #include <vector>
 
-#include "mrt_build.hpp"
-#include "mrt_kernels.hpp"
-#include "mrt_path.hpp"
-#include "mrt_wavefront.hpp"
+#include "mrt_internal.hpp"
+
+namespace mrt {
+thread_local std::string g_error;
+}  // namespace mrt
 
 namespace {
 
 using namespace mrt;
-
-thread_local std::string g_error;
-
-int fail(int code, const std::string& msg) {
-    g_error = msg;
-    return code;
-}
-
-#define MRT_HIP(expr)                                                                                        \
-    do {                                                                                                     \
-        const hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess) return fail(MRT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
-// Device allocation owned by a scene or context.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    int alloc(size_t n) {
-        if (n <= bytes && p) return MRT_OK;
-        release();
-        if (n == 0) return MRT_OK;
-        if (hipMalloc(&p, n) != hipSuccess) {
-            p = nullptr;
-            return fail(MRT_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
-        }
-        bytes = n;
-        return MRT_OK;
-    }
-    int upload(const void* src, size_t n, hipStream_t s) {
-        const int rc = alloc(n);
-        if (rc != MRT_OK) return rc;
-        if (n) MRT_HIP(hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s));
-        return MRT_OK;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-constexpr uint32_t kStackLds = 16;     // entries per lane in LDS (mrt_kernels.hip)
-constexpr uint32_t kLinearLights = 16; // up to this many area lights are scanned linearly
-
-}  // namespace
-
-struct mrt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t aux[kWfMaxGroups - 1] = {};  // wavefront slot groups beyond the first
-    uint32_t num_cu = 0;
-    uint64_t budget = 0;
-    std::mutex mu;
-};
-
-struct mrt_scene {
-    mrt_ctx* ctx = nullptr;
-    std::vector<mrt_triangle> tris;
-    std::vector<mrt_sphere> sphs;
-    std::vector<mrt_plane> plas;
-    std::vector<mrt_material> mats;   // scene materials then one per light
-    std::vector<mrt_light> lights;
-    uint32_t n_scene_mats = 0;
-    DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
-    DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes, wide_nodes;
-    DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wf_state, wf_group[kWfMaxGroups];
-    // Host copy of the light tree (pairs relative to its own numbering) for
-    // the combined walk tree; light slots follow the triangles in tri_geo.
-    BvhOut light_bvh;
-    uint32_t n_light_slots = 0;
-    // Wavefront loop: pinned snapshots of the device counters (two, read
-    // one batch behind the launches) and event pairs around trace launches.
-    uint32_t* counts_host = nullptr;
-    std::vector<hipEvent_t> trace_events;
-    SceneDev dev{};
-    ~mrt_scene() {
-        if (counts_host) (void)hipHostFree(counts_host);
-        for (hipEvent_t e : trace_events) (void)hipEventDestroy(e);
-    }
-    bool built = false;
-    bool generic = false;
-    uint32_t depth_need = 0;
-    std::atomic<bool> stop{false};
-    std::atomic<uint32_t> progress{0};
-};
-
-namespace {
 
 uint64_t buffers_bytes(const mrt_scene* s) {
     uint64_t b = s->tri_geo.bytes + s->tri_shade.bytes + s->sph.bytes + s->sph_mat.bytes + s->pla.bytes +
@@ -174,7 +82,7 @@ void set_child(BNode* p, int side, const Box& b, uint32_t ref, uint32_t word) {
 uint32_t leaf_word(uint32_t slot, uint32_t count) { return (count << kSlotBits) | slot; }
 
 // Copies the pairs of one tree into the walk tree: interior references move
-// by `pair_base`, leaves become (count << 24 | slot_base + first).
+// by `pair_base`, leaves become (count << 27 | slot_base + first).
 void append_pairs(std::vector<BNode>* w, const BvhOut& t, uint32_t pair_base, uint32_t slot_base) {
     for (const BNode& p : t.pairs) {
         BNode q = p;
@@ -297,7 +205,7 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, uint32_t light_pa
 }
 
 // Builds the combined walk tree of the persistent kernels (layout in
-// mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 24/8-bit leaf
+// mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 27/5-bit leaf
 // word get none; they run the non-persistent kernels.
 int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
     s->dev.walk_nodes = nullptr;
@@ -306,7 +214,7 @@ int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
     uint32_t maxLeaf = std::max(tri.root_count, s->n_light_slots <= kLinearLights ? s->n_light_slots : 0u);
     for (const BNode& p : tri.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
     for (const BNode& p : s->light_bvh.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
-    if (maxLeaf > 255u) return MRT_OK;
+    if (maxLeaf >= (1u << (32u - kSlotBits))) return MRT_OK;
     const float nan = std::numeric_limits<float>::quiet_NaN();
     const float inf = std::numeric_limits<float>::infinity();
     const Box empty{f3{nan, nan, nan}, f3{nan, nan, nan}};
@@ -589,6 +497,61 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
 
 }  // namespace
 
+namespace mrt {
+
+int render_setup(const mrt_scene* s, const mrt_render_desc* d, uint32_t stride_bytes, RenderParams* out,
+                 std::vector<uint32_t>* rank_out) {
+    int rc;
+    if ((rc = check_desc(s, d)) != MRT_OK) return rc;
+    const uint32_t w = d->width, h = d->height;
+    // Renderer ctor tile sizes, Renderer.cpp:12-29.
+    if (w < 16 || h < 16) return fail(MRT_E_INVALID, "width and height must be at least 16");
+    if (w % 16u != 0u)
+        return fail(MRT_E_UNSUPPORTED, "width must be a multiple of 16 (the Linux front end rounds it, "
+                                       "GTK_Interface.cpp:22-30)");
+    const uint32_t bsx = w / 16u, bsy = h / 16u;
+    if ((w / bsx) * (h / bsy) != kBlocks)
+        return fail(MRT_E_UNSUPPORTED, "tile domain (w/bsx)*(h/bsy) must be 256");
+    const uint32_t realWidth = stride_bytes / 4u;
+    if (stride_bytes % 4u != 0u || realWidth < w || realWidth % w != 0u)
+        return fail(MRT_E_UNSUPPORTED, "stride must be a multiple of width * 4 bytes");
+    // Tile claim order from the camera's block table (Renderer.cpp:79-88):
+    // roundf(block * 256) must visit every tile once.
+    std::vector<uint32_t> rank(kBlocks, 0xFFFFFFFFu);
+    for (uint32_t t = 0; t < kBlocks; ++t) {
+        const float rb = std::roundf(d->tile_values[t] * static_cast<float>(kBlocks));
+        if (!(rb >= 0.0f && rb < 256.0f)) return fail(MRT_E_UNSUPPORTED, "tile value out of range");
+        const uint32_t r = static_cast<uint32_t>(rb);
+        if (rank[r] != 0xFFFFFFFFu) return fail(MRT_E_UNSUPPORTED, "tile values do not form a permutation");
+        rank[r] = t;
+    }
+    const bool compat = d->sampler_pixel.kind == MRT_SAMPLER_HALTON_COMPAT ||
+                        d->sampler_pixel.kind == MRT_SAMPLER_STRATIFIED_COMPAT;
+    if (compat && d->first_sample != 0) return fail(MRT_E_UNSUPPORTED, "compat samplers render whole frames only");
+    RenderParams p = params_from(*d);
+    p.bsx = bsx;
+    p.bsy = bsy;
+    p.real_width = realWidth;
+    p.inv_w = 1.0f / static_cast<float>(w);
+    p.inv_h = 1.0f / static_cast<float>(h);
+    p.pix_w = 0.5f / static_cast<float>(w);
+    p.pix_h = 0.5f / static_cast<float>(h);
+    p.compat = compat ? 1u : 0u;
+    p.pixels_per_pass = kBlocks * bsx * bsy;
+    p.tile_rank = nullptr;
+    *out = p;
+    *rank_out = std::move(rank);
+    return MRT_OK;
+}
+
+bool wavefront_eligible(const mrt_scene* s, const mrt_render_desc* d) {
+    return d->shader == MRT_SHADER_PATHTRACER && d->samples_light == 1 && !s->dev.has_branches &&
+           d->ray_depth_max >= 1 && d->samples_pixel <= 65535u &&
+           static_cast<uint64_t>(d->width) * d->height * d->samples_pixel < (1ull << 32);
+}
+
+}  // namespace mrt
+
 extern "C" {
 
 MRT_API const char* mrt_last_error(void) { return g_error.c_str(); }
@@ -623,6 +586,8 @@ MRT_API int mrt_ctx_create(int device, mrt_ctx** out) {
 
 MRT_API int mrt_ctx_destroy(mrt_ctx* ctx) {
     if (!ctx) return MRT_OK;
+    if (ctx->children.load() != 0)
+        return fail(MRT_E_INVALID, "context still has scenes or render servers: destroy them first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
@@ -682,6 +647,8 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
     s->n_scene_mats = d->n_materials;
     for (const mrt_light& l : s->lights) s->mats.push_back(l.radiance);
     MRT_HIP(hipSetDevice(ctx->device));
+    ++ctx->children;  // the scene's destructor gives it back
+    s->counted = true;
     const hipStream_t st = ctx->stream;
     int rc;
     // Scene triangles, then (below) one slot per area light in light-tree order.
@@ -789,6 +756,10 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
 
 MRT_API int mrt_scene_destroy(mrt_scene* scene) {
     if (!scene) return MRT_OK;
+    {
+        std::lock_guard<std::mutex> lock(scene->ctx->mu);
+        if (scene->servers > 0) return fail(MRT_E_INVALID, "scene is resident in a render server");
+    }
     (void)hipSetDevice(scene->ctx->device);
     (void)hipStreamSynchronize(scene->ctx->stream);
     delete scene;
@@ -1078,45 +1049,15 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     if (!s || !bitmap) return fail(MRT_E_INVALID, "null argument");
     if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
     int rc;
-    if ((rc = check_desc(s, d)) != MRT_OK) return rc;
+    RenderParams p;
+    std::vector<uint32_t> rank;
+    if ((rc = render_setup(s, d, stride_bytes, &p, &rank)) != MRT_OK) return rc;
     const uint32_t w = d->width, h = d->height;
-    // Renderer ctor tile sizes, Renderer.cpp:12-29.
-    if (w < 16 || h < 16) return fail(MRT_E_INVALID, "width and height must be at least 16");
-    if (w % 16u != 0u)
-        return fail(MRT_E_UNSUPPORTED, "width must be a multiple of 16 (the Linux front end rounds it, "
-                                       "GTK_Interface.cpp:22-30)");
-    const uint32_t bsx = w / 16u, bsy = h / 16u;
-    if ((w / bsx) * (h / bsy) != kBlocks)
-        return fail(MRT_E_UNSUPPORTED, "tile domain (w/bsx)*(h/bsy) must be 256");
-    const uint32_t realWidth = stride_bytes / 4u;
-    if (stride_bytes % 4u != 0u || realWidth < w || realWidth % w != 0u)
-        return fail(MRT_E_UNSUPPORTED, "stride must be a multiple of width * 4 bytes");
-    // Tile claim order from the camera's block table (Renderer.cpp:79-88):
-    // roundf(block * 256) must visit every tile once.
-    std::vector<uint32_t> rank(kBlocks, 0xFFFFFFFFu);
-    for (uint32_t t = 0; t < kBlocks; ++t) {
-        const float rb = std::roundf(d->tile_values[t] * static_cast<float>(kBlocks));
-        if (!(rb >= 0.0f && rb < 256.0f)) return fail(MRT_E_UNSUPPORTED, "tile value out of range");
-        const uint32_t r = static_cast<uint32_t>(rb);
-        if (rank[r] != 0xFFFFFFFFu) return fail(MRT_E_UNSUPPORTED, "tile values do not form a permutation");
-        rank[r] = t;
-    }
-    const bool compat = d->sampler_pixel.kind == MRT_SAMPLER_HALTON_COMPAT ||
-                        d->sampler_pixel.kind == MRT_SAMPLER_STRATIFIED_COMPAT;
-    if (compat && d->first_sample != 0) return fail(MRT_E_UNSUPPORTED, "compat samplers render whole frames only");
+    const uint32_t bsx = p.bsx, bsy = p.bsy;
+    const uint32_t realWidth = p.real_width;
     std::lock_guard<std::mutex> lock(s->ctx->mu);
     MRT_HIP(hipSetDevice(s->ctx->device));
     const hipStream_t st = s->ctx->stream;
-    RenderParams p = params_from(*d);
-    p.bsx = bsx;
-    p.bsy = bsy;
-    p.real_width = realWidth;
-    p.inv_w = 1.0f / static_cast<float>(w);
-    p.inv_h = 1.0f / static_cast<float>(h);
-    p.pix_w = 0.5f / static_cast<float>(w);
-    p.pix_h = 0.5f / static_cast<float>(h);
-    p.compat = compat ? 1u : 0u;
-    p.pixels_per_pass = kBlocks * bsx * bsy;
     if ((rc = s->tile_rank.upload(rank.data(), rank.size() * sizeof(uint32_t), st)) != MRT_OK) return rc;
     p.tile_rank = s->tile_rank.as<uint32_t>();
     const size_t bmBytes = static_cast<size_t>(h) * realWidth * sizeof(uint32_t);
@@ -1132,13 +1073,9 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         }
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
-    // Kernel shape: the wavefront pipeline covers the PathTracer with one
-    // light sample, no reflection/transmission, RayDepthMax >= 1 and at most
-    // 65535 samples (16-bit per-slot sample index); everything else runs the
+    // Kernel shape: the wavefront pipeline where it applies, else the
     // megakernel.
-    const bool wfEligible = d->shader == MRT_SHADER_PATHTRACER && d->samples_light == 1 && !s->dev.has_branches &&
-                            d->ray_depth_max >= 1 && d->samples_pixel <= 65535u &&
-                            static_cast<uint64_t>(w) * h * d->samples_pixel < (1ull << 32);
+    const bool wfEligible = wavefront_eligible(s, d);
     bool useWavefront = false;
     if (d->kernel == MRT_KERNEL_WAVEFRONT) {
         if (!wfEligible)

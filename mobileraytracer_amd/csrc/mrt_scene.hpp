@@ -16,8 +16,9 @@
 //   q3 = (Lref, Lcount, Rref, Rcount); count > 0 marks a leaf whose
 //   primitives are [ref, ref + count), count == 0 an interior pair index.
 // The combined walk tree (persistent kernels) uses the same pairs with one
-// change: a leaf child's count word holds (count << 24) | first slot, the
-// slot indexing tri_geo, so queuing a leaf is one word.  Pair 0 holds the
+// change: a leaf child's count word holds (count << 27) | first slot, the
+// slot indexing tri_geo, so queuing a leaf is one word (leaves hold at most
+// 16 primitives; 27 slot bits let one arena hold the trees of many scenes).  Pair 0 holds the
 // triangle root box and the light root box, pair 1 the triangle root box and
 // an all-NaN box that no ray enters; triangle pairs start at pair 2, light
 // pairs at walk_light_pair.
@@ -42,7 +43,7 @@ static_assert(sizeof(BNode) == 64, "BVH node pair must be 64 bytes");
 // ref is a wide node, unused slots all-NaN.  Wide nodes 0 and 1 are the two
 // roots; nodes from light pairs follow those from triangle pairs.
 constexpr uint32_t kWalkClosestRoot = 0u, kWalkAnyRoot = 1u, kWalkFirstPair = 2u;
-constexpr uint32_t kSlotBits = 24u;
+constexpr uint32_t kSlotBits = 27u;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 
 // One accelerator (Shader::{naive,regularGrid,bvh}{Planes,Spheres,Triangles}_).

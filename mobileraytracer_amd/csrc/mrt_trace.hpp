@@ -525,7 +525,7 @@ __device__ __forceinline__ HitT closest_hit(const SceneDev& s, const RayT& r, fl
 // reference's ascending scan with t < length_.
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
-// Per-lane walk state.  Leaf words are (count << 24 | slot), 0 = none: `cur`
+// Per-lane walk state.  Leaf words are (count << 27 | slot), 0 = none: `cur`
 // is the leaf whose triangles are being tested, l0..l3 the leaves the last
 // visit reached (by child slot, zeros allowed).  A lane visits again only
 // once l0..l3 are empty, and `cur` is refilled from them as it runs out, so

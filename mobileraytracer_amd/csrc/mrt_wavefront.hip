@@ -132,6 +132,52 @@ __global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t
     if (live) put_ray(b, queue, qe, o, d, MRT_ID_NONE, slot, false);
 }
 
+// Camera rays of the first sample of job `job` on pool waves [pool_wave,
+// pool_wave + n_waves), which become the job's waves [job_wave, ...); records
+// each wave's owner for the kernels that follow on this pipeline.
+__global__ __launch_bounds__(kBlock) void k_wf_admit(BatchDev bd, uint32_t job, uint32_t pool_wave,
+                                                     uint32_t job_wave, uint32_t n_waves, WfBuffers b,
+                                                     float4* queue, uint32_t* count) {
+    const uint32_t wl = blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    const JobDev& J = bd.jobs[job];
+    bool live = false;
+    uint32_t slot = 0;
+    f3 o{}, d{};
+    if (wl < n_waves) {
+        const uint32_t wv = pool_wave + wl;
+        if (lane == 0u) bd.waves[wv] = make_uint2(job, job_wave + wl);
+        slot = (wv << 6) | lane;
+        uint32_t x, y;
+        slot_pixel(J.p, ((job_wave + wl) << 6) | lane, &x, &y);
+        live = x < 16u * J.p.bsx && y < 16u * J.p.bsy;
+        if (live) {
+            const uint32_t pid = y * J.p.width + x;
+            primary_ray(J.p, x, y, pid, J.first_sample, &o, &d);
+            reset_path(b, slot, pid, J.first_sample);
+        }
+    }
+    const uint32_t qe = wave_append(count, live);
+    if (live) put_ray(b, queue, qe, o, d, MRT_ID_NONE, slot, false);
+}
+
+// Adds, per job, the lanes that finished their slot's last sample to
+// bd.finished (one atomic per job present in the wave, after a fence that
+// makes the lanes' bitmap and accumulation writes visible first).
+__device__ __forceinline__ void count_finished(const BatchDev& bd, uint32_t job, bool last) {
+    unsigned long long m = __ballot(last);
+    if (m == 0ull) return;
+    __threadfence();
+    const uint32_t lane = threadIdx.x & 63u;
+    while (m != 0ull) {
+        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(m))) - 1u;
+        const uint32_t lj = __shfl(job, static_cast<int>(leader), 64);
+        const unsigned long long same = __ballot(last && job == lj);
+        if (lane == leader) atomicAdd(bd.finished + lj, static_cast<uint32_t>(__popcll(same)));
+        m &= ~same;
+    }
+}
+
 // ------------------------------------------------------------- shadow --
 // Applies a visible light sample to its path: acc_all += T_d * Ld_d, and the
 // first vertex with any(Ld > 0) snapshots acc_prefix.
@@ -298,13 +344,15 @@ __device__ __forceinline__ uint32_t fetch_entry(uint32_t* cursor, uint32_t n, bo
 // BVH scenes of triangles and lights: a lane takes the next entry the moment
 // its ray is done, so a wave never waits for its slowest ray.  Clears the
 // counters that regen and shade fill next.
-template <bool COUNT>
+// MULTI: the rays belong to the jobs of a batch and start at their scenes'
+// roots in the walk arena `s`.
+template <bool COUNT, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4* __restrict__ queue,
                                                        const uint32_t* __restrict__ count_q,
                                                        const uint32_t* __restrict__ count_s, float4* __restrict__ hits,
                                                        WfBuffers b, uint32_t* clear_q, uint32_t* clear_e,
                                                        uint32_t* fetch, StackCfg sc, unsigned long long* counters,
-                                                       uint32_t* err) {
+                                                       uint32_t* err, BatchDev bd) {
     __shared__ uint32_t lds[kStackLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     if (gtid == 0) {
@@ -332,8 +380,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
                         idx = e;
                         shadow = false;
                         w.r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.z));
-                        walk_begin(w, kWalkClosestRoot, kRayLengthMax, st);
                         finite = inv_finite(w.r);
+                        uint32_t root = kWalkClosestRoot;
+                        if (MULTI) {
+                            const JobDev& J = bd.jobs[bd.waves[__float_as_uint(c.w) >> 6].x];
+                            root = finite ? J.wide_root : J.walk_root;
+                        }
+                        walk_begin(w, root, kRayLengthMax, st);
                         has = true;
                     }
                 } else if (e < n) {
@@ -342,8 +395,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
                     const float4 a = sh[0], c = sh[1];
                     shadow = true;
                     w.r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.w));
-                    walk_begin(w, kWalkAnyRoot, c.z, st);
                     finite = inv_finite(w.r);
+                    uint32_t root = kWalkAnyRoot;
+                    if (MULTI) {
+                        const JobDev& J = bd.jobs[__float_as_uint(sh[3].w)];
+                        root = (finite ? J.wide_root : J.walk_root) + 1u;
+                    }
+                    walk_begin(w, root, c.z, st);
                     ++rc.shadow;
                     has = true;
                 } else {
@@ -372,12 +430,15 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
 // result selection, then incrementalAvg of Renderer.cpp:101-104 with the
 // slot's own sample count) and starts each slot's next sample.  Clears the
 // shadow counter and the trace fetch cursors for the next iteration.
-__global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t spp, uint32_t stop,
+// MULTI: each slot's parameters, sample count, stop flag and outputs come
+// from its job, and finished jobs are counted in bd.finished.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p0, uint32_t spp0, uint32_t stop0,
                                                      const uint32_t* __restrict__ ended,
                                                      const uint32_t* __restrict__ count_e, WfBuffers b,
                                                      float4* __restrict__ queue_out, uint32_t* count_out,
                                                      uint32_t* clear_s, uint32_t* clear_fetch, uint32_t* done,
-                                                     uint32_t* __restrict__ bitmap, float* __restrict__ accum) {
+                                                     uint32_t* bitmap0, float* accum0, BatchDev bd) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *clear_s = 0u;
         *clear_fetch = 0u;
@@ -389,11 +450,23 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t sp
     uint32_t finished = 0;
     for (uint32_t k = 0; k < rounds; ++k) {
         const uint32_t i = gtid + k * stride;
-        bool again = false;
-        uint32_t slot = 0;
+        bool again = false, last = false;
+        uint32_t slot = 0, job = 0;
         f3 o{}, d{};
         if (i < n) {
             slot = ended[i];
+            uint32_t local = slot;
+            if (MULTI) {
+                const uint2 wv = bd.waves[slot >> 6];
+                job = wv.x;
+                local = (wv.y << 6) | (slot & 63u);
+            }
+            const JobDev* J = bd.jobs + job;  // read only when MULTI
+            const RenderParams& p = MULTI ? J->p : p0;
+            const uint32_t spp = MULTI ? J->spp : spp0;
+            const bool stop = MULTI ? J->stop != 0u : stop0 != 0u;
+            uint32_t* bitmap = MULTI ? J->bitmap : bitmap0;
+            float* accum = MULTI ? J->accum : accum0;
             const float4 sa = b.sA[slot], sbv = b.sB[slot];
             const uint32_t flags = __float_as_uint(sa.w);
             f3 c = f3{sbv.x, sbv.y, sbv.z};
@@ -404,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t sp
             const uint32_t pid = __float_as_uint(sbv.w);
             const uint32_t sample = flags_sample(flags);
             uint32_t x, y;
-            slot_pixel(p, slot, &x, &y);
+            slot_pixel(p, local, &x, &y);
             const uint32_t bi = y * p.real_width + x;
             bitmap[bi] = incremental_avg(c, bitmap[bi], sample + 1u);
             if (accum) {
@@ -417,10 +490,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t sp
                 primary_ray(p, x, y, pid, sample + 1u, &o, &d);
                 reset_path(b, slot, pid, sample + 1u);
                 again = true;
+            } else {
+                last = true;
             }
         }
         const uint32_t qe = wave_append(count_out, again);
         if (again) put_ray(b, queue_out, qe, o, d, MRT_ID_NONE, slot);
+        if (MULTI) count_finished(bd, job, last);
     }
     const uint32_t total = wave_sum(finished);
     if ((threadIdx.x & 63u) == 0u && total) atomicAdd(done, total);
@@ -430,12 +506,17 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p, uint32_t sp
 // One PathTracer vertex per extension-queue entry (PathTracer.cpp:25-133):
 // queues the next extension ray, the light sample's shadow ray, and the
 // slot when its path ends here.
-__global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p, const float4* __restrict__ queue_in,
+// MULTI: each entry's scene and parameters come from its slot's job; a
+// shadow entry records the job for the trace.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s0, RenderParams p0,
+                                                     const float4* __restrict__ queue_in,
                                                      const uint32_t* __restrict__ count_in,
                                                      const float4* __restrict__ hits, WfBuffers b,
                                                      float4* __restrict__ queue_out, uint32_t* count_out,
                                                      uint32_t* count_shadow, uint32_t* __restrict__ ended,
-                                                     uint32_t* count_ended, unsigned long long* counters) {
+                                                     uint32_t* count_ended, unsigned long long* counters,
+                                                     BatchDev bd) {
     RayCounts rc{};
     const uint32_t n = *count_in;
     const uint32_t stride = gridDim.x * kBlock;
@@ -447,12 +528,16 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p,
         const uint32_t i = gtid + k * stride;
         bool wantExt = false, wantShadow = false, wantEnd = false;
         f3 nextO{}, nextD{}, shD{}, Ld{}, Tv{};
-        uint32_t excl = MRT_ID_NONE, slot = kDeadEntry;
+        uint32_t excl = MRT_ID_NONE, slot = kDeadEntry, job = 0;
         float dist = 0.0f;
         if (i < n) {
             const float4 a = queue_in[2u * i], c = queue_in[2u * i + 1u];
             slot = __float_as_uint(c.w);
             if (slot != kDeadEntry) {
+                if (MULTI) job = bd.waves[slot >> 6].x;
+                const JobDev* J = bd.jobs + job;  // read only when MULTI
+                const RenderParams& p = MULTI ? J->p : p0;
+                const SceneDev& s = MULTI ? bd.scenes[J->scene] : s0;
                 const RayT r = make_ray(f3{a.x, a.y, a.z}, f3{a.w, c.x, c.y}, __float_as_uint(c.z));
                 const float4 hv = hits[i];
                 HitT h;
@@ -512,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s, RenderParams p,
             e[0] = make_float4(nextO.x, nextO.y, nextO.z, shD.x);
             e[1] = make_float4(shD.y, shD.z, dist, __uint_as_float(excl));
             e[2] = make_float4(Ld.x, Ld.y, Ld.z, __uint_as_float(slot));
-            e[3] = make_float4(Tv.x, Tv.y, Tv.z, 0.0f);
+            e[3] = make_float4(Tv.x, Tv.y, Tv.z, __uint_as_float(job));
         }
         const uint32_t qd = wave_append(count_ended, wantEnd);
         if (wantEnd) ended[qd] = slot;
@@ -554,7 +639,7 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap) {
     static int perCu = 0;  // same for every device of this (single-arch) build
     if (perCu == 0) {
         int a = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace_p<false>, kBlock, 0) != hipSuccess) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace_p<false, false>, kBlock, 0) != hipSuccess) {
             (void)hipGetLastError();
             return cap;
         }
@@ -576,11 +661,24 @@ hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, 
     return hipGetLastError();
 }
 
+hipError_t launch_wavefront_admit(const BatchDev& bd, uint32_t job, uint32_t pool_wave, uint32_t job_wave,
+                                  uint32_t n_waves, const WfBuffers& b, const WfLoop& L, hipStream_t stream) {
+    if (n_waves == 0u) return hipSuccess;
+    constexpr uint32_t wavesPerBlock = kBlock / 64u;
+    k_wf_admit<<<(n_waves + wavesPerBlock - 1u) / wavesPerBlock, kBlock, 0, stream>>>(
+        bd, job, pool_wave, job_wave, n_waves, b, L.qin, b.counts + L.cin);
+    return hipGetLastError();
+}
+
 hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool persistent, bool count,
                                       const RenderParams& p, uint32_t spp, bool stop, uint32_t* bitmap, float* accum,
                                       const WfBuffers& b, WfLoop* L, uint32_t trace_blocks, uint32_t persist_blocks,
                                       const StackCfg& sc, unsigned long long* counters, uint32_t* err,
-                                      hipEvent_t trace_begin, hipEvent_t trace_end, hipStream_t stream) {
+                                      hipEvent_t trace_begin, hipEvent_t trace_end, hipStream_t stream,
+                                      const BatchDev* batch) {
+    // A batch runs the persistent trace without instrumentation or reordering.
+    if (batch && (!persistent || count || b.sort)) return hipErrorInvalidValue;
+    const BatchDev bd = batch ? *batch : BatchDev{};
     uint32_t* c = b.counts;
     const uint32_t ePrev = (L->iter & 1u) ? kCntEnded0 : kCntEnded1;  // written by the previous shade
     const uint32_t eCur = (L->iter & 1u) ? kCntEnded1 : kCntEnded0;   // written by this iteration's shade
@@ -591,14 +689,18 @@ hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool pers
         if (e != hipSuccess) return e;
     }
     if (persistent) {
-        if (count)
-            k_wf_trace_p<true><<<persist_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, c + kCntShadow, b.hits, b,
-                                                                      c + L->cout, c + eCur, c + kCntFetch, sc,
-                                                                      counters, err);
+        if (batch)
+            k_wf_trace_p<false, true><<<persist_blocks, kBlock, 0, stream>>>(
+                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
+                err, bd);
+        else if (count)
+            k_wf_trace_p<true, false><<<persist_blocks, kBlock, 0, stream>>>(
+                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
+                err, bd);
         else
-            k_wf_trace_p<false><<<persist_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, c + kCntShadow, b.hits,
-                                                                       b, c + L->cout, c + eCur, c + kCntFetch, sc,
-                                                                       counters, err);
+            k_wf_trace_p<false, false><<<persist_blocks, kBlock, 0, stream>>>(
+                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
+                err, bd);
     } else {
 #define MRT_EXTEND(G, C)                                                                                            \
     k_wf_extend<G, C><<<trace_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, b.hits, c + L->cout, c + eCur, sc, \
@@ -621,11 +723,20 @@ hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool pers
     // With reordering, regen and shade fill queue_b, which the scatter sorts
     // into queue_a; without it the two queues alternate.
     float4* fill = b.sort ? b.queue_b : L->qout;
-    k_wf_regen<<<trace_blocks, kBlock, 0, stream>>>(p, spp, stop ? 1u : 0u, endedPrev, c + ePrev, b, fill,
-                                                     c + L->cout, c + kCntShadow, c + kCntFetch, c + kCntDone, bitmap,
-                                                     accum);
-    k_wf_shade<<<trace_blocks, kBlock, 0, stream>>>(s, p, L->qin, c + L->cin, b.hits, b, fill, c + L->cout,
-                                                    c + kCntShadow, endedCur, c + eCur, counters);
+    if (batch) {
+        k_wf_regen<true><<<trace_blocks, kBlock, 0, stream>>>(p, spp, stop ? 1u : 0u, endedPrev, c + ePrev, b, fill,
+                                                               c + L->cout, c + kCntShadow, c + kCntFetch,
+                                                               c + kCntDone, bitmap, accum, bd);
+        k_wf_shade<true><<<trace_blocks, kBlock, 0, stream>>>(s, p, L->qin, c + L->cin, b.hits, b, fill, c + L->cout,
+                                                              c + kCntShadow, endedCur, c + eCur, counters, bd);
+    } else {
+        k_wf_regen<false><<<trace_blocks, kBlock, 0, stream>>>(p, spp, stop ? 1u : 0u, endedPrev, c + ePrev, b, fill,
+                                                                c + L->cout, c + kCntShadow, c + kCntFetch,
+                                                                c + kCntDone, bitmap, accum, bd);
+        k_wf_shade<false><<<trace_blocks, kBlock, 0, stream>>>(s, p, L->qin, c + L->cin, b.hits, b, fill,
+                                                               c + L->cout, c + kCntShadow, endedCur, c + eCur,
+                                                               counters, bd);
+    }
     if (b.sort) {
         size_t bytes = b.scan_bytes;
         const hipError_t e = hipcub::DeviceScan::ExclusiveSum(b.scan_temp, bytes, b.hist, b.offs, kSortBins, stream);

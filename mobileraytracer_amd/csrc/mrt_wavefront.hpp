@@ -3,10 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include "mrt_kernels.hpp"
+#include "mrt_path.hpp"
 
 namespace mrt {
-
-struct RenderParams;
 
 // Ray reordering between bounces: extension rays are counting-sorted by a
 // key of 5-bit-per-axis Morton cell of the origin (inside the triangle BVH's
@@ -47,6 +46,29 @@ struct WfBuffers {
     uint32_t* counts;  // kCntWords counters, zero before launch_wavefront_start
     float box_mn[3], box_inv[3];  // key quantisation box
     uint32_t sort;     // 1: reorder extension rays
+};
+
+// A render job of a batch (the render server): the pool waves it owns carry
+// its slots, and its scene's walk trees sit in the batch's shared arena.
+struct JobDev {
+    RenderParams p;
+    uint32_t first_sample;  // samples [first_sample, spp) are rendered
+    uint32_t spp;
+    uint32_t stop;          // 1: start no new samples
+    uint32_t scene;         // index into BatchDev::scenes
+    uint32_t wide_root;     // arena wide node of the scene's closest-hit root (any-hit root: + 1)
+    uint32_t walk_root;     // arena pair of the scene's closest-hit root (any-hit root: + 1)
+    uint32_t* bitmap;       // RGBA8, p.real_width pixels per row
+    float* accum;           // optional FP32 radiance sums
+};
+
+// Device tables of a batch.  Every slot of pool wave w belongs to job
+// waves[w].x and is slot (waves[w].y * 64 + lane) of that job's frame.
+struct BatchDev {
+    const JobDev* jobs;
+    const SceneDev* scenes;
+    uint2* waves;
+    uint32_t* finished;  // per job: slots whose last sample is done
 };
 
 // Host side of the iteration loop.
@@ -91,15 +113,24 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap);
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,
                                   hipStream_t stream);
 
+// Queues the camera rays of the first sample of job `job` on pool waves
+// [pool_wave, pool_wave + n_waves), which become the job's waves
+// [job_wave, job_wave + n_waves), into the queue the next iteration traces.
+hipError_t launch_wavefront_admit(const BatchDev& bd, uint32_t job, uint32_t pool_wave, uint32_t job_wave,
+                                  uint32_t n_waves, const WfBuffers& b, const WfLoop& L, hipStream_t stream);
+
 // One trace -> regen -> [reorder] -> shade iteration.  persistent: trace with
 // the refilling kernel over the combined walk tree (BVH scenes of triangles
 // and lights), `persist_blocks` workgroups; stop: finish ended paths without
 // starting new samples.  trace_begin / trace_end (optional) bracket the trace
-// launches.
+// launches.  With `batch`, the slots belong to the jobs of a batch (the job
+// tables give each slot's parameters, scene and outputs; `s` holds the walk
+// arena); p, spp, stop, bitmap and accum are then unused.
 hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool persistent, bool count,
                                       const RenderParams& p, uint32_t spp, bool stop, uint32_t* bitmap, float* accum,
                                       const WfBuffers& b, WfLoop* L, uint32_t trace_blocks, uint32_t persist_blocks,
                                       const StackCfg& sc, unsigned long long* counters, uint32_t* err,
-                                      hipEvent_t trace_begin, hipEvent_t trace_end, hipStream_t stream);
+                                      hipEvent_t trace_begin, hipEvent_t trace_end, hipStream_t stream,
+                                      const BatchDev* batch = nullptr);
 
 }  // namespace mrt

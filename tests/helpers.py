@@ -135,3 +135,43 @@ def oracle_scene(data: mrt.SceneData, accel: int, cam_min=(0, 0, 0), cam_max=(0,
     sc._keep = (data, desc)
     sc.build(accel, accel_params(cam_min, cam_max, leaf))
     return sc
+
+
+class DeviceArray:
+    """Device memory from the HIP runtime the library itself loaded (no
+    torch: its wheel ships another HIP runtime, and two in one process do not
+    share devices)."""
+
+    _hip = None
+
+    def __init__(self, nbytes: int):
+        if DeviceArray._hip is None:
+            abi.lib()  # loads the runtime the library links
+            DeviceArray._hip = C.CDLL("libamdhip64.so.7")
+        p = C.c_void_p()
+        rc = DeviceArray._hip.hipMalloc(C.byref(p), C.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed: {rc}")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def upload(self, a: np.ndarray) -> None:
+        assert a.nbytes == self.nbytes
+        rc = DeviceArray._hip.hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data), C.c_size_t(a.nbytes), 1)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        assert a.nbytes == self.nbytes
+        rc = DeviceArray._hip.hipMemcpy(C.c_void_p(a.ctypes.data), C.c_void_p(self.ptr), C.c_size_t(a.nbytes), 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+        return a
+
+    def free(self) -> None:
+        if self.ptr:
+            DeviceArray._hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
