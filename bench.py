@@ -12,6 +12,14 @@ Mrays/s counts every traced camera, extension, reflection/transmission and
 shadow ray (rays the reference traces beyond the maximum depth, whose result
 it discards, are not traced and not counted).
 
+Serving leg (after the timed frames, outside their timing): a render server
+holding a resident pool of 16 seeded scenes (50k-8M triangles, soups and
+displaced-icosphere clusters, 1% emissive) receives a Poisson stream of
+requests (random camera pose, 256x256 to 1920x1080, 1-64 spp) for a fixed
+window; p50/p99 request latency and the window's Mrays/s go into the line's
+"serving" object.  --serve-sweep runs only the serving workload over a list of
+rates and prints one JSON line per rate.
+
 Prints one JSON line (see the repository's benchmark contract).
 """
 from __future__ import annotations
@@ -120,6 +128,14 @@ def main() -> int:
     ap.add_argument("--no-persistent", action="store_true", help="wavefront: one ray per lane, no refill")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=5.0, help="serving leg: requests per second")
+    ap.add_argument("--serve-duration", type=float, default=20.0, help="serving leg: arrival window, seconds")
+    ap.add_argument("--serve-scenes", type=int, default=16)
+    ap.add_argument("--serve-min-tris", type=int, default=50_000)
+    ap.add_argument("--serve-max-tris", type=int, default=8_000_000)
+    ap.add_argument("--serve-sweep", type=str, default="",
+                    help="comma-separated rates: run only the serving workload, one JSON line per rate")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -132,6 +148,18 @@ def main() -> int:
 
     import mobileraytracer_amd as mrt
     from mobileraytracer_amd import abi
+
+    if args.serve_sweep:
+        ctx = mrt.Context(0)
+        srv, pool = serving_setup(mrt, ctx, args)
+        for rate in [float(r) for r in args.serve_sweep.split(",")]:
+            # Bounded request count at high rates (the host buffers of queued
+            # requests are held until they are served).
+            dur = min(args.serve_duration, 600.0 / rate)
+            res = serving_run(mrt, srv, pool, rate, dur)
+            print(json.dumps({"metric": "render-request latency (serving sweep)", "serving": res}), flush=True)
+        srv.close()
+        return 0
 
     t0 = time.time()
     data = mrt.soup_scene(2, N_TRIS, N_LIGHTS, 5.0, 0.005, abi.SAMPLER_RANDOM)
@@ -257,8 +285,44 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt)
         log(f"cpu baseline took {time.time() - t0:.1f} s")
         out["vs_cpu"] = round(mrays / out["cpu_baseline"]["value"], 1)
+    if not args.no_serve:
+        scene.close()
+        del bitmap, accum
+        srv, pool = serving_setup(mrt, ctx, args)
+        out["serving"] = serving_run(mrt, srv, pool, args.serve_rate, args.serve_duration)
+        srv.close()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serving_setup(mrt, ctx, args):
+    """Render server with the resident pool (see the module docstring)."""
+    from mobileraytracer_amd import serving
+
+    t0 = time.time()
+    sizes = serving.pool_sizes(args.serve_scenes, args.serve_min_tris, args.serve_max_tris, seed=4)
+    pool = serving.build_pool(ctx, sizes, seed=400)
+    srv = mrt.RenderServer(ctx)
+    for p in pool:
+        p.server_id = srv.add_scene(p.scene)
+    st = srv.stats()
+    log(f"serving pool: {len(pool)} scenes, {sum(p.triangles for p in pool)} triangles "
+        f"({', '.join(f'{p.kind} {p.triangles}' for p in pool)}), arena {st.scene_bytes / 1e6:.0f} MB, "
+        f"built in {time.time() - t0:.1f} s")
+    return srv, pool
+
+
+def serving_run(mrt, srv, pool, rate, duration):
+    from mobileraytracer_amd import serving
+
+    t0 = time.time()
+    res = serving.run_stream(srv, pool, rate, duration, seed=4)
+    res["config"] = (f"{len(pool)} resident scenes {min(p.triangles for p in pool)}-"
+                     f"{max(p.triangles for p in pool)} triangles; Poisson arrivals; 256x256..1920x1080; "
+                     f"1-64 spp; PathTracer depth 6 RR 3; FP32 accumulation")
+    log(f"serving at {rate} req/s: {res['done']} of {res['requests']} requests in {time.time() - t0:.1f} s, "
+        f"p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms, {res['mrays_per_s']} Mrays/s")
+    return res
 
 
 if __name__ == "__main__":

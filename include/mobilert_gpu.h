@@ -417,6 +417,13 @@ MRT_API int mrt_scene_cornell(uint32_t seed, mrt_triangle* tris, mrt_material* m
  * (n - n_emissive entries) with vertex colours U(0.2,0.8) and material -1. */
 MRT_API int mrt_scene_soup(uint32_t seed, uint32_t n, uint32_t n_emissive, float le, float half_extent,
                            mrt_triangle* tris, mrt_light* lights, int light_sampler_kind);
+/* Cluster of n_spheres displaced icospheres (icosahedron subdivided subdiv
+ * times: 20 * 4^subdiv triangles each) in the unit cube, Morton ordered;
+ * n_emissive of the triangles become area lights (outputs as for
+ * mrt_scene_soup, total = n_spheres * 20 * 4^subdiv). */
+MRT_API int mrt_scene_icospheres(uint32_t seed, uint32_t n_spheres, uint32_t subdiv, float displace,
+                                 uint32_t n_emissive, float le, mrt_triangle* tris, mrt_light* lights,
+                                 int light_sampler_kind);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,17 @@ def soup_scene(seed: int, n: int, n_emissive: int, le: float = 5.0, half_extent:
     return SceneData(tris, None, lights)
 
 
+def icosphere_scene(seed: int, n_spheres: int, subdiv: int, n_emissive: int, displace: float = 0.3, le: float = 5.0,
+                    light_sampler: int = abi.SAMPLER_RANDOM) -> SceneData:
+    n = n_spheres * 20 * 4 ** subdiv
+    tris = np.zeros(n - n_emissive, abi.TRIANGLE_DTYPE)
+    lights = np.zeros(n_emissive, abi.LIGHT_DTYPE)
+    check(lib().mrt_scene_icospheres(seed, n_spheres, subdiv, displace, n_emissive, le, tris.ctypes.data,
+                                     lights.ctypes.data if n_emissive else None, light_sampler),
+          "mrt_scene_icospheres")
+    return SceneData(tris, None, lights)
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = C.c_void_p()

@@ -338,19 +338,6 @@ int stage(DevBuf& buf, const void* src, size_t bytes, int mem, hipStream_t st, c
 
 uint32_t cap_blocks(const mrt_ctx* c) { return (c->num_cu ? c->num_cu : 256u) * 16u; }
 
-// Stream of slot group g: the context's stream, then its auxiliary streams
-// (created on first use, default priority).
-int group_stream(mrt_ctx* c, uint32_t g, hipStream_t* out) {
-    if (g == 0) {
-        *out = c->stream;
-        return MRT_OK;
-    }
-    hipStream_t& aux = c->aux[g - 1];
-    if (!aux) MRT_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
-    *out = aux;
-    return MRT_OK;
-}
-
 // The wavefront frame.  Each slot group runs its own iteration loop on its
 // own stream; the host issues the groups' iterations interleaved, in batches
 // of kBatch.  After each batch a group's counters are copied to pinned memory
@@ -498,6 +485,19 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
 }  // namespace
 
 namespace mrt {
+
+// Stream of slot group g: the context's stream, then its auxiliary streams
+// (created on first use, default priority).
+int group_stream(mrt_ctx* c, uint32_t g, hipStream_t* out) {
+    if (g == 0) {
+        *out = c->stream;
+        return MRT_OK;
+    }
+    hipStream_t& aux = c->aux[g - 1];
+    if (!aux) MRT_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    *out = aux;
+    return MRT_OK;
+}
 
 int render_setup(const mrt_scene* s, const mrt_render_desc* d, uint32_t stride_bytes, RenderParams* out,
                  std::vector<uint32_t>* rank_out) {

@@ -74,6 +74,10 @@ int render_setup(const mrt_scene* s, const mrt_render_desc* d, uint32_t stride_b
 // samples (16-bit per-slot sample index) and fewer than 2^32 pixel samples.
 bool wavefront_eligible(const mrt_scene* s, const mrt_render_desc* d);
 
+// Stream of slot group g of a context: its stream, then its auxiliary streams
+// (created on first use, default priority).  Callers hold ctx->mu.
+int group_stream(mrt_ctx* c, uint32_t g, hipStream_t* out);
+
 }  // namespace mrt
 
 struct mrt_ctx {

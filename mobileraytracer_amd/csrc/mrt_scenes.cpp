@@ -289,4 +289,105 @@ MRT_API int mrt_scene_soup(uint32_t seed, uint32_t n, uint32_t n_emissive, float
     return MRT_OK;
 }
 
+// Cluster of displaced, subdivided icospheres in the unit cube.  Each sphere
+// is an icosahedron split `subdiv` times into four (20 * 4^subdiv
+// triangles) with its vertices on the unit sphere, then scaled by a radius
+// U(0.03, 0.12), displaced along the direction by displace * radius * a sum of
+// four seeded sinusoids of the direction (so vertices shared by neighbouring
+// triangles stay shared) and moved to a centre U(0.15, 0.85)^3; one albedo
+// U(0.2, 0.8)^3 per sphere.  All triangles are then ordered by the Morton code
+// of their centroid, as in mrt_scene_soup, and n_emissive of them (seeded
+// choice) become area lights.
+MRT_API int mrt_scene_icospheres(uint32_t seed, uint32_t n_spheres, uint32_t subdiv, float displace,
+                                 uint32_t n_emissive, float le, mrt_triangle* tris, mrt_light* lights,
+                                 int light_sampler_kind) {
+    if (subdiv > 10 || n_spheres == 0) return MRT_E_INVALID;
+    const uint64_t perSphere = 20ull << (2 * subdiv);
+    const uint64_t total = perSphere * n_spheres;
+    if (total >= (1ull << 30) || !tris || n_emissive > total || (n_emissive && !lights)) return MRT_E_INVALID;
+    const uint32_t n = static_cast<uint32_t>(total);
+    SplitMix rng{seed ^ 0x1c05ull};
+    // Icosahedron: 12 vertices (0, +-1, +-t) cyclic, 20 faces.
+    const float t = (1.0f + std::sqrt(5.0f)) * 0.5f;
+    const f3 iv[12] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+                       {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    const int faces[20][3] = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+                              {11, 10, 2}, {10, 7, 6}, {7, 1, 8},  {3, 9, 4},  {3, 4, 2},   {3, 2, 6}, {3, 6, 8},
+                              {3, 8, 9},  {4, 9, 5},  {2, 4, 11}, {6, 2, 10}, {8, 6, 7},   {9, 8, 1}};
+    std::vector<f3> unit;  // 3 directions per triangle of one sphere
+    unit.reserve(3 * perSphere);
+    for (const auto& f : faces)
+        for (int k = 0; k < 3; ++k) unit.push_back(normalize(iv[f[k]]));
+    for (uint32_t level = 0; level < subdiv; ++level) {
+        std::vector<f3> next;
+        next.reserve(unit.size() * 4);
+        for (size_t i = 0; i < unit.size(); i += 3) {
+            const f3 a = unit[i], b = unit[i + 1], c = unit[i + 2];
+            const f3 ab = normalize(add(a, b)), bc = normalize(add(b, c)), ca = normalize(add(c, a));
+            const f3 tri[4][3] = {{a, ab, ca}, {ab, b, bc}, {ca, bc, c}, {ab, bc, ca}};
+            for (const auto& q : tri)
+                for (int k = 0; k < 3; ++k) next.push_back(q[k]);
+        }
+        unit.swap(next);
+    }
+    struct Raw {
+        f3 v[3];
+        f3 albedo;
+        uint32_t code;
+    };
+    std::vector<Raw> raw(n);
+    uint32_t w = 0;
+    for (uint32_t sp = 0; sp < n_spheres; ++sp) {
+        const f3 centre{rng.range(0.15f, 0.85f), rng.range(0.15f, 0.85f), rng.range(0.15f, 0.85f)};
+        const float radius = rng.range(0.03f, 0.12f);
+        const f3 albedo{rng.range(0.2f, 0.8f), rng.range(0.2f, 0.8f), rng.range(0.2f, 0.8f)};
+        f3 wave[4];
+        float freq[4], phase[4];
+        for (int k = 0; k < 4; ++k) {
+            wave[k] = normalize(f3{rng.range(-1, 1), rng.range(-1, 1), rng.range(-1, 1) + 1e-3f});
+            freq[k] = rng.range(2.0f, 9.0f);
+            phase[k] = rng.range(0.0f, 6.2831853f);
+        }
+        auto place = [&](f3 u) {
+            float d = 0.0f;
+            for (int k = 0; k < 4; ++k) d += 0.25f * std::sin(freq[k] * dot(u, wave[k]) + phase[k]);
+            return add(centre, muls(u, radius * (1.0f + displace * d)));
+        };
+        for (size_t i = 0; i < unit.size(); i += 3, ++w) {
+            for (int k = 0; k < 3; ++k) raw[w].v[k] = place(unit[i + k]);
+            raw[w].albedo = albedo;
+            const f3 c = muls(add(add(raw[w].v[0], raw[w].v[1]), raw[w].v[2]), 1.0f / 3.0f);
+            raw[w].code = morton30(f3{std::min(std::max(c.x, 0.0f), 1.0f), std::min(std::max(c.y, 0.0f), 1.0f),
+                                      std::min(std::max(c.z, 0.0f), 1.0f)});
+        }
+    }
+    std::vector<uint8_t> emissive(n, 0);
+    {
+        std::vector<uint32_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0u);
+        for (uint32_t i = 0; i < n_emissive; ++i) {
+            const uint32_t j = i + static_cast<uint32_t>(rng.next() % (n - i));
+            std::swap(idx[i], idx[j]);
+            emissive[idx[i]] = 1;
+        }
+    }
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return raw[a].code < raw[b].code; });
+    uint32_t nt = 0, nl = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const Raw& r = raw[order[k]];
+        if (emissive[order[k]]) {
+            area_light(r.v[0], r.v[1], r.v[2], le, light_sampler_kind, seed * 2654435761u + nl, &lights[nl]);
+            ++nl;
+        } else {
+            const float col[9] = {r.albedo.x, r.albedo.y, r.albedo.z, r.albedo.x, r.albedo.y,
+                                  r.albedo.z, r.albedo.x, r.albedo.y, r.albedo.z};
+            make_triangle(r.v[0], r.v[1], r.v[2], -1, nullptr, col, &tris[nt]);
+            ++nt;
+        }
+    }
+    return MRT_OK;
+}
+
 }  // extern "C"

@@ -14,8 +14,15 @@
 //   * a worker thread issues the pipelines' iterations in batches, admits
 //     queued jobs between batches (in submission order, as waves, a job-table
 //     entry and job memory are free), and reads per-job finished-slot counters
-//     one batch behind; a job whose every covered pixel finished its last
-//     sample is copied out and its waves go to the next job.
+//     (one array per pipeline, copied on the pipeline's own stream) one batch
+//     behind; a job whose every covered pixel finished its last sample is
+//     copied out and its waves go to the next job.
+// The pipelines are the context's slot-group streams and never wait on one
+// another: a stream that waits on the others' events stalls every stream
+// that shares its hardware queue.  Job inputs go up by DMA from pinned
+// staging before the admit kernels are queued, and results come back once
+// the counters show the job done, which means the kernels that wrote them have
+// completed.
 // A job's pixels take the same draws and the same arithmetic as in mrt_render,
 // so its bitmap and accumulation are identical to rendering it alone.
 #include <algorithm>
@@ -134,6 +141,7 @@ struct Job {
     uint64_t mem_off = 0, mem_units = 0;
     uint64_t off_bm = 0, off_acc = 0;  // inside the job memory block
     std::vector<Run> runs;
+    uint32_t pipes = 0;               // bit g: the job has waves on pipeline g
     hipEvent_t done_ev = nullptr;
     bool cancel = false;
     double t_submit = 0, t_admit = 0, t_gpu = 0, t_done = 0;
@@ -150,11 +158,9 @@ struct mrt_server {
     mrt_ctx* ctx = nullptr;
     uint32_t G = 0, max_jobs = 0, max_scenes = 0, pool_waves = 0;
     uint64_t job_memory = 0, scene_memory = 0;
-    hipStream_t pipe[kWfMaxGroups] = {};
-    hipStream_t xfer = nullptr;
-    hipEvent_t batch_ev[kWfMaxGroups] = {};
-    hipEvent_t admit_ev = nullptr;
-    hipEvent_t chk[2] = {};
+    hipStream_t pipe[kWfMaxGroups] = {};  // the context's slot-group streams
+    hipStream_t xfer = nullptr;           // result copies
+    hipEvent_t chk[2][kWfMaxGroups] = {};
     // Device state.
     DevBuf state, group[kWfMaxGroups], jobs_dev, scenes_dev, waves_dev, finished_dev, counters, err, spill, job_mem;
     DevBuf arena_wide, arena_walk, arena_geo;
@@ -166,8 +172,9 @@ struct mrt_server {
     uint32_t trace_blocks = 0, persist_blocks = 0;
     // Pinned host memory: job table mirror, counter snapshots, job staging.
     JobDev* jobs_host = nullptr;
-    uint32_t* snap_host = nullptr;  // 2 x (max_jobs + 1): finished counters, then the error word
+    uint32_t* snap_host = nullptr;  // [2][G][max_jobs + 1]: finished counters, then the error word
     uint64_t snap_batch[2] = {};    // batch number of each snapshot
+    uint32_t snap_mask[2] = {};     // pipelines each snapshot covers
     uint8_t* staging = nullptr;
     // Host bookkeeping (guarded by mu).
     std::mutex mu;
@@ -193,13 +200,17 @@ bool pipe_active(const mrt_server* s, uint32_t g) {
     return s->waves_free[g].free_total() < s->waves_free[g].total();
 }
 
-BatchDev batch_dev(const mrt_server* s) {
+BatchDev batch_dev(const mrt_server* s, uint32_t g) {
     BatchDev bd;
     bd.jobs = s->jobs_dev.as<JobDev>();
     bd.scenes = s->scenes_dev.as<SceneDev>();
     bd.waves = s->waves_dev.as<uint2>();
-    bd.finished = s->finished_dev.as<uint32_t>();
+    bd.finished = s->finished_dev.as<uint32_t>() + static_cast<size_t>(g) * s->max_jobs;
     return bd;
+}
+
+uint32_t* snap_of(const mrt_server* s, uint32_t snap, uint32_t g) {
+    return s->snap_host + (static_cast<size_t>(snap) * s->G + g) * (s->max_jobs + 1u);
 }
 
 // The arena as a scene for the trace kernel: walk trees and triangle slots.
@@ -316,13 +327,12 @@ int admit(mrt_server* s) {
         admitted.push_back(j);
     }
     if (admitted.empty()) return MRT_OK;
-    const BatchDev bd = batch_dev(s);
     uint8_t* dmem = s->job_mem.as<uint8_t>();
     for (Job* j : admitted) {
         const uint64_t base = j->mem_off * kAlign;
         uint8_t* host = s->staging + base;
         std::memcpy(host, j->rank.data(), j->rank.size() * sizeof(uint32_t));
-        JobDev& jd = s->jobs_host[j->slot];
+        JobDev jd{};
         jd.p = j->p;
         jd.p.tile_rank = reinterpret_cast<const uint32_t*>(dmem + base);
         jd.first_sample = j->desc.first_sample;
@@ -343,22 +353,16 @@ int admit(mrt_server* s) {
                 jd.accum = reinterpret_cast<float*>(dmem + base + j->off_acc);
             }
         }
-        MRT_HIP(hipMemcpyAsync(s->jobs_dev.as<JobDev>() + j->slot, &jd, sizeof(JobDev), hipMemcpyHostToDevice,
-                               s->xfer));
-        MRT_HIP(hipMemsetAsync(s->finished_dev.as<uint32_t>() + j->slot, 0, sizeof(uint32_t), s->xfer));
+        s->jobs_host[j->slot] = jd;
+        // DMA from pinned staging, ordered before the admit kernels by the
+        // host; no pipeline stream waits for it.
         const size_t upBytes = j->mem == MRT_MEM_DEVICE ? j->rank.size() * sizeof(uint32_t) : j->mem_bytes;
-        MRT_HIP(hipMemcpyAsync(dmem + base, host, upBytes, hipMemcpyHostToDevice, s->xfer));
-    }
-    MRT_HIP(hipEventRecord(s->admit_ev, s->xfer));
-    bool waited[kWfMaxGroups] = {};
-    for (Job* j : admitted) {
+        MRT_HIP(hipMemcpy(dmem + base, host, upBytes, hipMemcpyHostToDevice));
         for (const Run& r : j->runs) {
-            if (!waited[r.pipe]) {
-                MRT_HIP(hipStreamWaitEvent(s->pipe[r.pipe], s->admit_ev, 0));
-                waited[r.pipe] = true;
-            }
-            MRT_HIP(launch_wavefront_admit(bd, j->slot, r.pool_wave, r.job_wave, r.n, s->wb[r.pipe], s->L[r.pipe],
-                                           s->pipe[r.pipe]));
+            const bool first = !(j->pipes & (1u << r.pipe));
+            j->pipes |= 1u << r.pipe;
+            MRT_HIP(launch_wavefront_admit(batch_dev(s, r.pipe), jd, j->slot, r.pool_wave, r.job_wave, r.n, first,
+                                           s->wb[r.pipe], s->L[r.pipe], s->pipe[r.pipe]));
         }
     }
     return MRT_OK;
@@ -368,15 +372,16 @@ int admit(mrt_server* s) {
 // snapshot of the finished counters.  Returns whether anything was issued.
 int issue_batch(mrt_server* s, uint32_t snap, bool* issued) {
     *issued = false;
-    const BatchDev bd = batch_dev(s);
     const SceneDev arena = arena_scene(s);
     RenderParams none{};
-    bool any = false;
-    for (uint32_t g = 0; g < s->G; ++g) any = any || pipe_active(s, g);
-    if (!any) return MRT_OK;
+    uint32_t mask = 0;
+    for (uint32_t g = 0; g < s->G; ++g)
+        if (pipe_active(s, g)) mask |= 1u << g;
+    if (!mask) return MRT_OK;
     for (uint32_t k = 0; k < kBatch; ++k) {
         for (uint32_t g = 0; g < s->G; ++g) {
-            if (!pipe_active(s, g)) continue;
+            if (!(mask & (1u << g))) continue;
+            const BatchDev bd = batch_dev(s, g);
             MRT_HIP(launch_wavefront_iteration(arena, false, true, false, none, 0u, false, nullptr, nullptr, s->wb[g],
                                                &s->L[g], s->trace_blocks, s->persist_blocks, s->sc[g],
                                                s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), nullptr,
@@ -384,15 +389,15 @@ int issue_batch(mrt_server* s, uint32_t snap, bool* issued) {
         }
     }
     for (uint32_t g = 0; g < s->G; ++g) {
-        if (!pipe_active(s, g)) continue;
-        MRT_HIP(hipEventRecord(s->batch_ev[g], s->pipe[g]));
-        MRT_HIP(hipStreamWaitEvent(s->xfer, s->batch_ev[g], 0));
+        if (!(mask & (1u << g))) continue;
+        uint32_t* dst = snap_of(s, snap, g);
+        MRT_HIP(hipMemcpyAsync(dst, s->finished_dev.as<uint32_t>() + static_cast<size_t>(g) * s->max_jobs,
+                               s->max_jobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->pipe[g]));
+        MRT_HIP(hipMemcpyAsync(dst + s->max_jobs, s->err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->pipe[g]));
+        MRT_HIP(hipEventRecord(s->chk[snap][g], s->pipe[g]));
     }
-    uint32_t* dst = s->snap_host + static_cast<size_t>(snap) * (s->max_jobs + 1u);
-    MRT_HIP(hipMemcpyAsync(dst, s->finished_dev.p, s->max_jobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->xfer));
-    MRT_HIP(hipMemcpyAsync(dst + s->max_jobs, s->err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->xfer));
-    MRT_HIP(hipEventRecord(s->chk[snap], s->xfer));
     s->snap_batch[snap] = s->batches;
+    s->snap_mask[snap] = mask;
     ++s->batches;
     *issued = true;
     return MRT_OK;
@@ -401,19 +406,27 @@ int issue_batch(mrt_server* s, uint32_t snap, bool* issued) {
 // Jobs whose every covered pixel finished in snapshot `snap`: free their
 // waves and queue the copy of their outputs.
 int process_snapshot(mrt_server* s, uint32_t snap) {
-    const uint32_t* fin = s->snap_host + static_cast<size_t>(snap) * (s->max_jobs + 1u);
-    if (fin[s->max_jobs] != 0u && !s->broken) {
-        s->broken = true;
-        s->broken_msg = "traversal stack overflow in the render server";
+    const uint32_t mask = s->snap_mask[snap];
+    for (uint32_t g = 0; g < s->G; ++g) {
+        if ((mask & (1u << g)) && snap_of(s, snap, g)[s->max_jobs] != 0u && !s->broken) {
+            s->broken = true;
+            s->broken_msg = "traversal stack overflow in the render server";
+        }
     }
     uint8_t* dmem = s->job_mem.as<uint8_t>();
     for (uint64_t id : s->running) {
         Job* j = s->jobs[id].get();
-        if (j->phase != kRunning || s->snap_batch[snap] < j->first_snap || fin[j->slot] < j->live) continue;
+        if (j->phase != kRunning || s->snap_batch[snap] < j->first_snap || (j->pipes & ~mask) != 0u) continue;
+        uint64_t fin = 0;
+        for (uint32_t g = 0; g < s->G; ++g)
+            if (j->pipes & (1u << g)) fin += snap_of(s, snap, g)[j->slot];
+        if (fin < j->live) continue;
         j->phase = kCopying;
         j->t_gpu = now_us();
         for (const Run& r : j->runs) s->waves_free[r.pipe].release(r.pool_wave, r.n);
         j->runs.clear();
+        // The snapshot was copied after the kernels that finished the job,
+        // and those have completed: the copy needs no stream dependency.
         if (j->mem != MRT_MEM_DEVICE) {
             const uint64_t base = j->mem_off * kAlign;
             MRT_HIP(hipMemcpyAsync(s->staging + base + j->off_bm, dmem + base + j->off_bm, j->bm_bytes,
@@ -506,9 +519,13 @@ void worker_main(mrt_server* s) {
         }
         if (pending) {
             const uint32_t prev = snap ^ 1u;
-            hipEvent_t e = s->chk[prev];
+            hipEvent_t ev[kWfMaxGroups];
+            uint32_t nev = 0;
+            for (uint32_t g = 0; g < s->G; ++g)
+                if (s->snap_mask[prev] & (1u << g)) ev[nev++] = s->chk[prev][g];
             lk.unlock();
-            const hipError_t q = hipEventSynchronize(e);
+            hipError_t q = hipSuccess;
+            for (uint32_t k = 0; k < nev && q == hipSuccess; ++k) q = hipEventSynchronize(ev[k]);
             lk.lock();
             if (q != hipSuccess) {
                 fail(MRT_E_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(q));
@@ -624,19 +641,14 @@ void destroy_server(mrt_server* s) {
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->ctx->device);
     for (uint32_t g = 0; g < s->G; ++g)
-        if (s->pipe[g]) {
-            (void)hipStreamSynchronize(s->pipe[g]);
-            (void)hipStreamDestroy(s->pipe[g]);
-        }
+        if (s->pipe[g]) (void)hipStreamSynchronize(s->pipe[g]);  // the context's streams
     if (s->xfer) {
         (void)hipStreamSynchronize(s->xfer);
         (void)hipStreamDestroy(s->xfer);
     }
-    for (hipEvent_t e : s->batch_ev)
-        if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : s->chk)
-        if (e) (void)hipEventDestroy(e);
-    if (s->admit_ev) (void)hipEventDestroy(s->admit_ev);
+    for (auto& row : s->chk)
+        for (hipEvent_t e : row)
+            if (e) (void)hipEventDestroy(e);
     for (auto& kv : s->jobs)
         if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
     if (s->jobs_host) (void)hipHostFree(s->jobs_host);
@@ -686,17 +698,18 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
         destroy_server(s);
         return rc;
     };
-    int rc;
-    for (uint32_t g = 0; g < G; ++g) {
-        if (hipStreamCreateWithFlags(&s->pipe[g], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s->batch_ev[g], hipEventDisableTiming) != hipSuccess)
-            return bail(fail(MRT_E_HIP, "stream or event creation failed"));
+    int rc = MRT_OK;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        for (uint32_t g = 0; g < G; ++g)
+            if ((rc = group_stream(ctx, g, &s->pipe[g])) != MRT_OK) return bail(rc);
     }
-    if (hipStreamCreateWithFlags(&s->xfer, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&s->admit_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s->chk[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s->chk[1], hipEventDisableTiming) != hipSuccess)
-        return bail(fail(MRT_E_HIP, "stream or event creation failed"));
+    for (uint32_t g = 0; g < G; ++g)
+        for (uint32_t k = 0; k < 2; ++k)
+            if (hipEventCreateWithFlags(&s->chk[k][g], hipEventDisableTiming) != hipSuccess)
+                return bail(fail(MRT_E_HIP, "event creation failed"));
+    if (hipStreamCreateWithFlags(&s->xfer, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(MRT_E_HIP, "stream creation failed"));
     if ((rc = s->state.alloc(wavefront_state_bytes(slots))) != MRT_OK) return bail(rc);
     float4* state = s->state.as<float4>();
     const uint32_t ncu = ctx->num_cu ? ctx->num_cu : 256u;
@@ -727,7 +740,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
         wb.scan_temp = nullptr;
         wb.scan_bytes = 0;
         wb.sort = 0u;
-        MRT_HIP(hipMemsetAsync(wb.counts, 0, kCntWords * sizeof(uint32_t), s->xfer));
+        MRT_HIP(hipMemsetAsync(wb.counts, 0, kCntWords * sizeof(uint32_t), s->pipe[g]));
         s->L[g].qin = wb.queue_a;
         s->L[g].qout = wb.queue_b;
         s->L[g].cin = kCntQueue0;
@@ -738,22 +751,22 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     if ((rc = s->jobs_dev.alloc(maxJobs * sizeof(JobDev))) != MRT_OK) return bail(rc);
     if ((rc = s->scenes_dev.alloc(maxScenes * sizeof(SceneDev))) != MRT_OK) return bail(rc);
     if ((rc = s->waves_dev.alloc(s->pool_waves * sizeof(uint2))) != MRT_OK) return bail(rc);
-    if ((rc = s->finished_dev.alloc(maxJobs * sizeof(uint32_t))) != MRT_OK) return bail(rc);
+    if ((rc = s->finished_dev.alloc(G * maxJobs * sizeof(uint32_t))) != MRT_OK) return bail(rc);
     if ((rc = s->counters.alloc(16 * sizeof(unsigned long long))) != MRT_OK) return bail(rc);
     if ((rc = s->err.alloc(sizeof(uint32_t))) != MRT_OK) return bail(rc);
     if ((rc = s->job_mem.alloc(jobMem)) != MRT_OK) return bail(rc);
-    MRT_HIP(hipMemsetAsync(s->counters.p, 0, 16 * sizeof(unsigned long long), s->xfer));
-    MRT_HIP(hipMemsetAsync(s->err.p, 0, sizeof(uint32_t), s->xfer));
-    MRT_HIP(hipMemsetAsync(s->finished_dev.p, 0, maxJobs * sizeof(uint32_t), s->xfer));
+    MRT_HIP(hipMemset(s->counters.p, 0, 16 * sizeof(unsigned long long)));
+    MRT_HIP(hipMemset(s->err.p, 0, sizeof(uint32_t)));
+    MRT_HIP(hipMemset(s->finished_dev.p, 0, G * maxJobs * sizeof(uint32_t)));
     if (hipHostMalloc(reinterpret_cast<void**>(&s->jobs_host), maxJobs * sizeof(JobDev)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&s->snap_host), 2ull * (maxJobs + 1u) * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&s->snap_host), 2ull * G * (maxJobs + 1u) * sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&s->staging), jobMem) != hipSuccess)
         return bail(fail(MRT_E_NOMEM, "pinned host allocation failed"));
     std::memset(s->jobs_host, 0, maxJobs * sizeof(JobDev));
     s->mem_free.reset(0, jobMem / kAlign);
     for (uint32_t i = 0; i < maxJobs; ++i) s->free_slots.push_back(maxJobs - 1u - i);
     if ((rc = ensure_spill(s, 0u)) != MRT_OK) return bail(rc);
-    MRT_HIP(hipStreamSynchronize(s->xfer));
+    for (uint32_t g = 0; g < G; ++g) MRT_HIP(hipStreamSynchronize(s->pipe[g]));
     s->worker = std::thread(worker_main, s);
     *out = s;
     return MRT_OK;
@@ -897,8 +910,11 @@ MRT_API int mrt_server_cancel(mrt_server* s, uint64_t job_id) {
         j->cancel = true;
         s->jobs_host[j->slot].stop = 1u;
         MRT_HIP(hipSetDevice(s->ctx->device));
-        MRT_HIP(hipMemcpyAsync(&s->jobs_dev.as<JobDev>()[j->slot].stop, &s->jobs_host[j->slot].stop,
-                               sizeof(uint32_t), hipMemcpyHostToDevice, s->xfer));
+        // On the job's pipelines, after their admit kernels wrote the entry.
+        for (uint32_t g = 0; g < s->G; ++g)
+            if (j->pipes & (1u << g))
+                MRT_HIP(hipMemcpyAsync(&s->jobs_dev.as<JobDev>()[j->slot].stop, &s->jobs_host[j->slot].stop,
+                                       sizeof(uint32_t), hipMemcpyHostToDevice, s->pipe[g]));
     }
     return MRT_OK;
 }

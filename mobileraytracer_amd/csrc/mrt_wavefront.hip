@@ -134,13 +134,21 @@ __global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t
 
 // Camera rays of the first sample of job `job` on pool waves [pool_wave,
 // pool_wave + n_waves), which become the job's waves [job_wave, ...); records
-// each wave's owner for the kernels that follow on this pipeline.
-__global__ __launch_bounds__(kBlock) void k_wf_admit(BatchDev bd, uint32_t job, uint32_t pool_wave,
-                                                     uint32_t job_wave, uint32_t n_waves, WfBuffers b,
-                                                     float4* queue, uint32_t* count) {
+// each wave's owner and the job's table entry (identical from every pipeline
+// the job runs on) for the kernels that follow on this pipeline.
+__global__ __launch_bounds__(kBlock) void k_wf_admit(BatchDev bd, JobDev J, uint32_t job, uint32_t pool_wave,
+                                                     uint32_t job_wave, uint32_t n_waves, uint32_t first,
+                                                     WfBuffers b, float4* queue, uint32_t* count) {
     const uint32_t wl = blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
-    const JobDev& J = bd.jobs[job];
+    if (blockIdx.x == 0u) {
+        constexpr uint32_t kWords = sizeof(JobDev) / 4u;
+        static_assert(sizeof(JobDev) % 4u == 0u, "job entry is copied as words");
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&J);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(const_cast<JobDev*>(bd.jobs) + job);
+        for (uint32_t k = threadIdx.x; k < kWords; k += kBlock) dst[k] = src[k];
+        if (first && threadIdx.x == 0u) bd.finished[job] = 0u;
+    }
     bool live = false;
     uint32_t slot = 0;
     f3 o{}, d{};
@@ -661,12 +669,13 @@ hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, 
     return hipGetLastError();
 }
 
-hipError_t launch_wavefront_admit(const BatchDev& bd, uint32_t job, uint32_t pool_wave, uint32_t job_wave,
-                                  uint32_t n_waves, const WfBuffers& b, const WfLoop& L, hipStream_t stream) {
+hipError_t launch_wavefront_admit(const BatchDev& bd, const JobDev& jd, uint32_t job, uint32_t pool_wave,
+                                  uint32_t job_wave, uint32_t n_waves, bool first, const WfBuffers& b,
+                                  const WfLoop& L, hipStream_t stream) {
     if (n_waves == 0u) return hipSuccess;
     constexpr uint32_t wavesPerBlock = kBlock / 64u;
     k_wf_admit<<<(n_waves + wavesPerBlock - 1u) / wavesPerBlock, kBlock, 0, stream>>>(
-        bd, job, pool_wave, job_wave, n_waves, b, L.qin, b.counts + L.cin);
+        bd, jd, job, pool_wave, job_wave, n_waves, first ? 1u : 0u, b, L.qin, b.counts + L.cin);
     return hipGetLastError();
 }
 

@@ -68,7 +68,7 @@ struct BatchDev {
     const JobDev* jobs;
     const SceneDev* scenes;
     uint2* waves;
-    uint32_t* finished;  // per job: slots whose last sample is done
+    uint32_t* finished;  // per job: slots of this pipeline whose last sample is done
 };
 
 // Host side of the iteration loop.
@@ -113,11 +113,14 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap);
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,
                                   hipStream_t stream);
 
-// Queues the camera rays of the first sample of job `job` on pool waves
-// [pool_wave, pool_wave + n_waves), which become the job's waves
-// [job_wave, job_wave + n_waves), into the queue the next iteration traces.
-hipError_t launch_wavefront_admit(const BatchDev& bd, uint32_t job, uint32_t pool_wave, uint32_t job_wave,
-                                  uint32_t n_waves, const WfBuffers& b, const WfLoop& L, hipStream_t stream);
+// Queues the camera rays of the first sample of job `job` (parameters `jd`,
+// also stored as the job's table entry) on pool waves [pool_wave, pool_wave +
+// n_waves), which become the job's waves [job_wave, job_wave + n_waves), into
+// the queue the next iteration traces; `first` (the job's first run on this
+// pipeline) also clears the pipeline's finished count of the job.
+hipError_t launch_wavefront_admit(const BatchDev& bd, const JobDev& jd, uint32_t job, uint32_t pool_wave,
+                                  uint32_t job_wave, uint32_t n_waves, bool first, const WfBuffers& b,
+                                  const WfLoop& L, hipStream_t stream);
 
 // One trace -> regen -> [reorder] -> shade iteration.  persistent: trace with
 // the refilling kernel over the combined walk tree (BVH scenes of triangles

@@ -162,6 +162,6 @@ def camera_rays(cam, uvdd: np.ndarray, ray_dtype) -> np.ndarray:
 
 
 def struct_sizes() -> list[int]:
-    out = np.zeros(12, np.uint32)
+    out = np.zeros(16, np.uint32)
     n = lib().oracle_struct_sizes(out.ctypes.data)
     return out[:n].tolist()

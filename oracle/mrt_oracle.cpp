@@ -889,7 +889,10 @@ int oracle_struct_sizes(uint32_t* out) {
     out[9] = sizeof(mrt_render_stats);
     out[10] = sizeof(mrt_sampler);
     out[11] = sizeof(mrt_scene_desc);
-    return 12;
+    out[12] = sizeof(mrt_server_desc);
+    out[13] = sizeof(mrt_job_info);
+    out[14] = sizeof(mrt_serving_stats);
+    return 15;
 }
 
 }  // extern "C"

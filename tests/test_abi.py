@@ -37,7 +37,8 @@ def test_struct_layouts():
     sizes = oracle.struct_sizes()
     ours = [C.sizeof(abi.Triangle), C.sizeof(abi.Sphere), C.sizeof(abi.Plane), C.sizeof(abi.Material),
             C.sizeof(abi.Light), C.sizeof(abi.Ray), C.sizeof(abi.Hit), C.sizeof(abi.Camera),
-            C.sizeof(abi.RenderDesc), C.sizeof(abi.RenderStats), C.sizeof(abi.Sampler), C.sizeof(abi.SceneDesc)]
+            C.sizeof(abi.RenderDesc), C.sizeof(abi.RenderStats), C.sizeof(abi.Sampler), C.sizeof(abi.SceneDesc),
+            C.sizeof(abi.ServerDesc), C.sizeof(abi.JobInfo), C.sizeof(abi.ServingStats)]
     assert ours == sizes
     # Reference class sizes (x86-64, 12-byte vec3): Triangle 112, Sphere 20,
     # Plane 28, Material 52 (app/MobileRT/Shapes/*.hpp, Material.hpp).
@@ -147,3 +148,18 @@ def test_soup_generator():
     assert step < 0.1
     d2 = mrt.soup_scene(2, 20000, 200)
     assert d2.triangles.tobytes() == t.tobytes()
+
+
+def test_icosphere_generator():
+    d = mrt.icosphere_scene(3, 5, 3, 64)
+    t = d.triangles
+    assert len(t) + len(d.lights) == 5 * 20 * 4 ** 3 and len(d.lights) == 64
+    c = t["a"] + (t["ab"] + t["ac"]) / 3
+    # Centres in [0.15, 0.85], radius <= 0.12 * (1 + displacement 0.3).
+    assert c.min() > 0.15 - 0.16 and c.max() < 0.85 + 0.16
+    # Triangles of a sphere are small against the unit cube.
+    assert np.all(np.abs(t["ab"]) < 0.2) and np.all(np.abs(t["ac"]) < 0.2)
+    assert np.all(t["material_id"] == -1) and np.all((t["ca"] >= 0.2) & (t["ca"] <= 0.8))
+    step = np.linalg.norm(np.diff(c, axis=0), axis=1).mean()
+    assert step < 0.1
+    assert mrt.icosphere_scene(3, 5, 3, 64).triangles.tobytes() == t.tobytes()
