@@ -151,3 +151,66 @@ def run_stream(srv: RenderServer, pool: list[PoolScene], rate: float, duration: 
             "window_s": round(window, 3), "achieved_rps": round(len(infos) / window, 2) if infos else 0.0,
             "mrays_per_s": round(rays / window / 1e6, 2) if infos else 0.0, "rays": int(rays),
             "pixel_samples": int(px_samples), "errors": errors[:3]}
+
+
+# ---------------------------------------------------------------- batched --
+BATCH_SPPS = [1, 4, 16, 64]
+
+
+def batch_requests(pool: list[PoolScene], seed: int) -> list:
+    """One request per pool scene: random camera pose, resolution of
+    RESOLUTIONS and spp of {1, 4, 16, 64}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(len(pool)):
+        d = request_desc(rng, seed * 104729 + 16 * i)
+        d.samples_pixel = BATCH_SPPS[rng.integers(len(BATCH_SPPS))]
+        out.append((i, d))
+    return out
+
+
+def run_batch(srv: RenderServer, pool: list[PoolScene], reqs: list) -> dict:
+    """Submits every request at once and waits for all: makespan and Mrays/s."""
+    st0 = srv.stats()
+    t0 = time.perf_counter()
+    ids = []
+    for si, desc in reqs:
+        bm = np.zeros((desc.height, desc.width), np.uint32)
+        acc = np.zeros((desc.height, desc.width, 3), np.float32)
+        ids.append(srv.submit(pool[si].server_id, desc, bitmap=bm, accum=acc))
+    lat = []
+    for jid in ids:
+        _, _, info = srv.wait(jid)
+        lat.append((info.t_done_us - info.t_submit_us) / 1000.0)
+    el = time.perf_counter() - t0
+    st1 = srv.stats()
+    rays = (st1.rays_camera + st1.rays_extension + st1.rays_shadow) - (
+        st0.rays_camera + st0.rays_extension + st0.rays_shadow)
+    return {"jobs": len(reqs), "makespan_s": round(el, 3), "rays": int(rays), "mrays_per_s": round(rays / el / 1e6, 2),
+            "p50_ms": round(pct(lat, 50), 2), "max_ms": round(max(lat), 2)}
+
+
+def tune_server(ctx, pool: list[PoolScene], reqs: list, candidates=None) -> tuple[dict, list]:
+    """Autotuner of the server shape (pipelines, slot-pool size) on a probe
+    batch: the requests at 1 spp.  These choices change speed only, never a
+    pixel.  Returns the best configuration and every probe's result."""
+    if candidates is None:
+        candidates = [dict(pipelines=p, max_slots=s) for p in (1, 2, 3)
+                      for s in (2_097_152, 6_291_456, 12_582_912)]
+    probe = []
+    for si, d in reqs:
+        q = abi.RenderDesc.from_buffer_copy(d)
+        q.samples_pixel = 1
+        probe.append((si, q))
+    results = []
+    for cand in candidates:
+        srv = RenderServer(ctx, job_memory=4 << 30, **cand)
+        try:
+            for p in pool:
+                p.server_id = srv.add_scene(p.scene)
+            r = run_batch(srv, pool, probe)
+        finally:
+            srv.close()
+        results.append(dict(cand, probe_mrays_per_s=r["mrays_per_s"], probe_makespan_s=r["makespan_s"]))
+    best = max(results, key=lambda r: r["probe_mrays_per_s"])
+    return {k: best[k] for k in candidates[0]}, results
