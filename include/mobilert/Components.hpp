@@ -23,6 +23,44 @@ public:
     explicit Whitted(::MobileRT::Scene&& scene, std::uint32_t samplesLight, Accelerator accelerator) noexcept;
 };
 
+/* NoShadows.hpp: the direct light of every light sample without shadow
+ * rays, plus 0.1 * Kd ambient (NoShadows.cpp:19-64). */
+class NoShadows final : public ::MobileRT::Shader {
+private:
+    bool shade(::MobileRT::vec3* rgb, const ::MobileRT::Intersection& intersection,
+               const ::MobileRT::Ray& ray) noexcept final;
+    int kind() const noexcept final { return MRT_SHADER_NOSHADOWS; }
+
+public:
+    explicit NoShadows(::MobileRT::Scene&& scene, std::uint32_t samplesLight, Accelerator accelerator) noexcept;
+};
+
+/* DepthMap.hpp: grey level of the hit distance against 2 |maxPoint - origin|
+ * (DepthMap.cpp:19-26). */
+class DepthMap final : public ::MobileRT::Shader {
+private:
+    ::MobileRT::vec3 maxPoint_{};
+    bool shade(::MobileRT::vec3* rgb, const ::MobileRT::Intersection& intersection,
+               const ::MobileRT::Ray& ray) noexcept final;
+    int kind() const noexcept final { return MRT_SHADER_DEPTHMAP; }
+    void describeShader(mrt_render_desc* desc) const noexcept final;
+
+public:
+    explicit DepthMap(::MobileRT::Scene&& scene, const ::MobileRT::vec3& maxPoint, Accelerator accelerator) noexcept;
+};
+
+/* DiffuseMaterial.hpp: the first non-zero of Kd, Ks, Kt, Le, or the
+ * primitive colour (DiffuseMaterial.cpp:16-38). */
+class DiffuseMaterial final : public ::MobileRT::Shader {
+private:
+    bool shade(::MobileRT::vec3* rgb, const ::MobileRT::Intersection& intersection,
+               const ::MobileRT::Ray& ray) noexcept final;
+    int kind() const noexcept final { return MRT_SHADER_DIFFUSE; }
+
+public:
+    explicit DiffuseMaterial(::MobileRT::Scene&& scene, Accelerator accelerator) noexcept;
+};
+
 /* PathTracer.hpp: one light sample estimate per vertex, cosine-weighted
  * continuation with Russian roulette beyond RayDepthMin
  * (PathTracer.cpp:25-164). */

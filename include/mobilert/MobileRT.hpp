@@ -431,6 +431,8 @@ protected:
     /* Device shader kind and its Russian-roulette sampler. */
     virtual int kind() const noexcept = 0;
     virtual mrt_sampler samplerRussianRoulette() const noexcept;
+    /* Shader-specific descriptor fields (DepthMap's maxPoint). */
+    virtual void describeShader(mrt_render_desc* /*desc*/) const noexcept {}
 
 public:
     void initializeAccelerators(Camera* camera) noexcept;

@@ -1,8 +1,9 @@
 /* The reference's outer C entry point (app/System_dependent/Linux/c_wrapper.h:
  * 10-15), served by the MI355X library (libmobilert.so).
  *
- * shader: 1 Whitted, 2 PathTracer (0 NoShadows, 3 DepthMap, 4 DiffuseMaterial
- *         are not implemented on the GPU and are refused).
+ * shader: 1 Whitted, 2 PathTracer, 3 DepthMap (maxPoint (1, 1, 1), as for the
+ *         reference's unit scenes), 4 DiffuseMaterial, anything else NoShadows
+ *         (c_wrapper.cpp:444-486).
  * scene:  0 the Cornell box of this build's generator (34 triangles, 2 area
  *         lights, camera at (0, 0, -3.4)), 1 the 1M-triangle soup (990k
  *         diffuse, 10k area lights); the reference's other demo scenes and OBJ

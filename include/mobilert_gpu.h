@@ -196,7 +196,7 @@ typedef struct mrt_render_desc {
                                   extension rays in origin-cell order          */
     uint32_t persistent;       /* wavefront: 0 default (on), 1 on, 2 off: lanes
                                   refill from the queue (BVH, triangles+lights) */
-    uint32_t reserved[3];
+    float depth_max_point[3];  /* DepthMap maxPoint_ (DepthMap.cpp:12-16)       */
 } mrt_render_desc;
 
 typedef struct mrt_render_stats {

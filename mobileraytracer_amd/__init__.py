@@ -47,7 +47,8 @@ def tile_values(seed: int = 0) -> np.ndarray:
 def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Camera, *, samples_light: int = 1,
                 depth_min: int = 4, depth_max: int = 8, pixel=None, rr=None, light=None, hemi=None,
                 tile_seed: int = 0, kernel: int = abi.KERNEL_MEGAKERNEL, first_sample: int = 0,
-                instrument: bool = False, ray_sort: int = 0, persistent: int = 0) -> abi.RenderDesc:
+                instrument: bool = False, ray_sort: int = 0, persistent: int = 0,
+                depth_max_point=(1.0, 1.0, 1.0)) -> abi.RenderDesc:
     d = abi.RenderDesc()
     d.shader = shader
     d.width, d.height, d.samples_pixel, d.samples_light = width, height, spp, samples_light
@@ -64,6 +65,7 @@ def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Came
     d.instrument = 1 if instrument else 0
     d.ray_sort = ray_sort
     d.persistent = persistent
+    d.depth_max_point[:] = [float(x) for x in depth_max_point]
     return d
 
 

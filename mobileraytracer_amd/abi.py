@@ -100,7 +100,7 @@ class RenderDesc(C.Structure):
                 ("sampler_hemi", Sampler),
                 ("tile_values", C.c_float * 256),
                 ("kernel", C.c_int32), ("first_sample", C.c_uint32), ("instrument", C.c_uint32),
-                ("ray_sort", C.c_uint32), ("persistent", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("ray_sort", C.c_uint32), ("persistent", C.c_uint32), ("depth_max_point", F3)]
 
 
 class RenderStats(C.Structure):

@@ -64,10 +64,6 @@ void work(uint32_t* bitmap, int32_t width, int32_t height, int32_t threads, int3
         std::fprintf(stderr, "RayTrace: OBJ scenes are not supported by this library\n");
         return;
     }
-    if (nShader != MRT_SHADER_WHITTED && nShader != MRT_SHADER_PATHTRACER) {
-        std::fprintf(stderr, "RayTrace: shader %d is not implemented on the GPU (1 Whitted, 2 PathTracer)\n", nShader);
-        return;
-    }
     if (accelerator < 0 || accelerator > 2 || width <= 0 || height <= 0 || samplesPixel <= 0 || samplesLight <= 0) {
         std::fprintf(stderr, "RayTrace: invalid arguments\n");
         return;
@@ -90,12 +86,21 @@ void work(uint32_t* bitmap, int32_t width, int32_t height, int32_t threads, int3
     else samplerPixel = std::make_unique<Components::Constant>(0.5f);
     const auto acc = static_cast<MobileRT::Shader::Accelerator>(accelerator);
     std::unique_ptr<MobileRT::Shader> shader;
+    // c_wrapper.cpp:444-486: 1 Whitted, 2 PathTracer, 3 DepthMap, 4
+    // DiffuseMaterial, anything else NoShadows; maxDist (1, 1, 1) for the
+    // unit-sized scenes (c_wrapper.cpp:78).
     if (nShader == MRT_SHADER_WHITTED)
         shader = std::make_unique<Components::Whitted>(std::move(scene), static_cast<uint32_t>(samplesLight), acc);
-    else
+    else if (nShader == MRT_SHADER_PATHTRACER)
         shader = std::make_unique<Components::PathTracer>(std::move(scene),
                                                           std::make_unique<Components::MersenneTwister>(),
                                                           static_cast<uint32_t>(samplesLight), acc);
+    else if (nShader == MRT_SHADER_DEPTHMAP)
+        shader = std::make_unique<Components::DepthMap>(std::move(scene), MobileRT::vec3{1.0f, 1.0f, 1.0f}, acc);
+    else if (nShader == MRT_SHADER_DIFFUSE)
+        shader = std::make_unique<Components::DiffuseMaterial>(std::move(scene), acc);
+    else
+        shader = std::make_unique<Components::NoShadows>(std::move(scene), static_cast<uint32_t>(samplesLight), acc);
     MobileRT::Renderer renderer{std::move(shader), std::move(camera), std::move(samplerPixel),
                                 static_cast<uint32_t>(width), static_cast<uint32_t>(height),
                                 static_cast<uint32_t>(samplesPixel)};

@@ -675,6 +675,7 @@ void Shader::describe(mrt_render_desc* d) const noexcept {
     d->sampler_rr = samplerRussianRoulette();
     d->sampler_light = samplerLight_;
     d->sampler_hemi = samplerHemi_;
+    describeShader(d);
 }
 
 // Shader.cpp:124-160 and the concrete shade: the ray's whole recursion on
@@ -815,6 +816,35 @@ Whitted::Whitted(MobileRT::Scene&& scene, std::uint32_t samplesLight, Accelerato
 // The device shades the intersection the ray produces: tracing the same ray
 // again yields this intersection, so shade() is rayTrace() of that ray.
 bool Whitted::shade(vec3* rgb, const MobileRT::Intersection&, const MobileRT::Ray& ray) noexcept {
+    return rayTrace(rgb, ray);
+}
+
+NoShadows::NoShadows(MobileRT::Scene&& scene, std::uint32_t samplesLight, Accelerator accelerator) noexcept
+    : Shader(std::move(scene), samplesLight, accelerator) {}
+
+bool NoShadows::shade(vec3* rgb, const MobileRT::Intersection&, const MobileRT::Ray& ray) noexcept {
+    return rayTrace(rgb, ray);
+}
+
+// DepthMap.cpp:12-16: no light samples.
+DepthMap::DepthMap(MobileRT::Scene&& scene, const vec3& maxPoint, Accelerator accelerator) noexcept
+    : Shader(std::move(scene), 0, accelerator), maxPoint_{maxPoint} {}
+
+bool DepthMap::shade(vec3* rgb, const MobileRT::Intersection&, const MobileRT::Ray& ray) noexcept {
+    return rayTrace(rgb, ray);
+}
+
+void DepthMap::describeShader(mrt_render_desc* d) const noexcept {
+    d->depth_max_point[0] = maxPoint_.x;
+    d->depth_max_point[1] = maxPoint_.y;
+    d->depth_max_point[2] = maxPoint_.z;
+}
+
+// DiffuseMaterial.cpp:11-14: no light samples.
+DiffuseMaterial::DiffuseMaterial(MobileRT::Scene&& scene, Accelerator accelerator) noexcept
+    : Shader(std::move(scene), 0, accelerator) {}
+
+bool DiffuseMaterial::shade(vec3* rgb, const MobileRT::Intersection&, const MobileRT::Ray& ray) noexcept {
     return rayTrace(rgb, ray);
 }
 

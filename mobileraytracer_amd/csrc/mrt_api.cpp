@@ -261,6 +261,7 @@ RenderParams params_from(const mrt_render_desc& d) {
     p.s_rr = d.sampler_rr;
     p.s_light = d.sampler_light;
     p.s_hemi = d.sampler_hemi;
+    for (int k = 0; k < 3; ++k) p.max_point[k] = d.depth_max_point[k];
     return p;
 }
 
@@ -274,8 +275,8 @@ int check_sampler(const mrt_sampler& s, bool pixel, const char* what) {
 
 int check_desc(const mrt_scene* s, const mrt_render_desc* d) {
     if (!d) return fail(MRT_E_INVALID, "null render descriptor");
-    if (d->shader != MRT_SHADER_WHITTED && d->shader != MRT_SHADER_PATHTRACER)
-        return fail(MRT_E_UNSUPPORTED, "shader " + std::to_string(d->shader) + " is not implemented on the GPU");
+    if (d->shader < MRT_SHADER_NOSHADOWS || d->shader > MRT_SHADER_DIFFUSE)
+        return fail(MRT_E_INVALID, "unknown shader " + std::to_string(d->shader));
     if (d->ray_depth_max < 0 || d->ray_depth_max > 15) return fail(MRT_E_UNSUPPORTED, "ray_depth_max must be 0..15");
     if (d->samples_light > 255) return fail(MRT_E_UNSUPPORTED, "samples_light must be <= 255");
     int rc;

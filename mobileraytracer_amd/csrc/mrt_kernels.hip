@@ -122,9 +122,12 @@ __global__ __launch_bounds__(kBlock) void k_shade_rays(SceneDev s, RenderParams 
     for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
         const mrt_ray ry = rays[i];
         bool hl = false;
-        const f3 c = trace_path<GENERIC, BRANCHES, false>(s, p, ld3(ry.org), ld3(ry.dir), ry.exclude,
-                                                          depths ? depths[i] : 1, i, sample, &hl, st, rc, 0,
-                                                          &pendOverflow);
+        const int32_t depth = depths ? depths[i] : 1;
+        const f3 c = simple_shader(p.shader)
+                         ? trace_simple<GENERIC, false>(s, p, ld3(ry.org), ld3(ry.dir), ry.exclude, depth, i, sample,
+                                                        &hl, st, rc, 0)
+                         : trace_path<GENERIC, BRANCHES, false>(s, p, ld3(ry.org), ld3(ry.dir), ry.exclude, depth, i,
+                                                                sample, &hl, st, rc, 0, &pendOverflow);
         rgb[3u * i + 0u] = c.x;
         rgb[3u * i + 1u] = c.y;
         rgb[3u * i + 2u] = c.z;
@@ -154,8 +157,10 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(SceneDev s, RenderParams
         f3 o, d;
         primary_ray(p, x, y, pid, sample, &o, &d);
         bool hl = false;
-        const f3 c = trace_path<GENERIC, BRANCHES, COUNT>(s, p, o, d, MRT_ID_NONE, 1, pid, sample, &hl, st, rc, 0,
-                                                          &pendOverflow);
+        const f3 c = simple_shader(p.shader)
+                         ? trace_simple<GENERIC, COUNT>(s, p, o, d, MRT_ID_NONE, 1, pid, sample, &hl, st, rc, 0)
+                         : trace_path<GENERIC, BRANCHES, COUNT>(s, p, o, d, MRT_ID_NONE, 1, pid, sample, &hl, st, rc,
+                                                                0, &pendOverflow);
         const uint32_t bi = y * p.real_width + x;
         bitmap[bi] = incremental_avg(c, bitmap[bi], sample + 1u);
         if (accum) {

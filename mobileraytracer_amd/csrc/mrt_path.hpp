@@ -40,6 +40,7 @@ struct RenderParams {
     const uint32_t* tile_rank;    // claim rank of tile (ty * 16 + tx)
     uint32_t compat;              // pixel sampler is a *_COMPAT kind
     uint32_t pixels_per_pass;     // 256 * bsx * bsy
+    float max_point[3];           // DepthMap maxPoint_
 };
 
 // Ray counters of one lane, reduced per wave at the end of a launch.
@@ -364,6 +365,61 @@ __device__ f3 trace_path(const SceneDev& s, const RenderParams& p, f3 o, f3 d, u
         kind = 2;
     }
     return pix;
+}
+
+__device__ __forceinline__ bool simple_shader(int32_t shader) {
+    return shader == MRT_SHADER_NOSHADOWS || shader == MRT_SHADER_DEPTHMAP || shader == MRT_SHADER_DIFFUSE;
+}
+
+// Shader::rayTrace (Shader.cpp:124-160) of the one-hit shaders: NoShadows
+// (NoShadows.cpp:19-64: emitters return Le; else the unoccluded direct light
+// Le * cos over the light samples, times kD over samplesLight, plus 0.1 kD),
+// DepthMap (DepthMap.cpp:19-26: grey max((maxDist - t) / maxDist, 0) with
+// maxDist = 2 |maxPoint - origin|) and DiffuseMaterial
+// (DiffuseMaterial.cpp:16-38: the first non-zero of kD, kS, kT, Le, or the
+// primitive colour without a material).  A miss leaves the pixel black.
+template <bool GENERIC, bool COUNT>
+__device__ f3 trace_simple(const SceneDev& s, const RenderParams& p, f3 o, f3 d, uint32_t excl, int32_t depth,
+                           uint32_t pixel, uint32_t sample, bool* flag, Stack& st, RayCounts& rc, int kind) {
+    const RayT r = make_ray(o, d, excl);
+    const HitT h = closest_hit<GENERIC, COUNT>(s, r, kRayLengthMax, st, rc.ts);
+    if (kind == 0) ++rc.camera;
+    else if (kind == 1) ++rc.extension;
+    else ++rc.specular;
+    *flag = false;
+    f3 rgb = f3{0.0f, 0.0f, 0.0f};
+    if (h.id == MRT_ID_NONE) return rgb;
+    const Surf sf = surface(s, r, h);
+    if (p.shader == MRT_SHADER_DEPTHMAP) {
+        const float maxDist = length(sub(f3{p.max_point[0], p.max_point[1], p.max_point[2]}, o)) * 2.0f;
+        const float v = fmaxf((maxDist - h.t) / maxDist, 0.0f);
+        return f3{v, v, v};
+    }
+    if (p.shader == MRT_SHADER_DIFFUSE) {
+        if (sf.mat < 0) return sf.color;
+        const MatV mv = load_mat(s.mat, static_cast<uint32_t>(sf.mat));
+        if (any_gt0(mv.kd)) return mv.kd;
+        if (any_gt0(mv.ks)) return mv.ks;
+        if (any_gt0(mv.kt)) return mv.kt;
+        if (any_gt0(mv.le)) return mv.le;
+        return rgb;
+    }
+    const Shading m = shading_of(s, sf);
+    if (m.emitter) {
+        *flag = true;
+        return m.le;
+    }
+    if (any_gt0(m.kd) && s.n_lights > 0u) {
+        const DrawCtx c{pixel, sample, 0u};
+        for (uint32_t i = 0; i < p.samples_light; ++i) {
+            f3 dir, lc;
+            float dist;
+            if (light_sample(s, p, sf, depth, c, i, &dir, &dist, &lc)) rgb = add(rgb, lc);
+        }
+        rgb = mul(rgb, m.kd);
+        rgb = divs(rgb, static_cast<float>(p.samples_light));
+    }
+    return add(rgb, muls(m.kd, 0.1f));
 }
 
 }  // namespace mrt

@@ -236,8 +236,69 @@ struct Shader {
             else counters->specular.fetch_add(1, std::memory_order_relaxed);
         }
         if (!hit) return false;
-        if (desc->shader == MRT_SHADER_WHITTED) return whitted(rgb, is, ray, c);
-        return path_tracer(rgb, is, ray, c);
+        // Shader::rayTrace resolves the material, Shader.cpp:151-155, then
+        // calls the shader's shade().
+        switch (desc->shader) {
+            case MRT_SHADER_WHITTED: return whitted(rgb, is, ray, c);
+            case MRT_SHADER_NOSHADOWS: return no_shadows(rgb, is, ray, c);
+            case MRT_SHADER_DEPTHMAP: return depth_map(rgb, is, ray);
+            case MRT_SHADER_DIFFUSE: return diffuse_material(rgb, is);
+            default: return path_tracer(rgb, is, ray, c);
+        }
+    }
+
+    // NoShadows::shade, app/Components/Shaders/NoShadows.cpp:19-64: the light
+    // loop of Whitted without the shadow ray, then the 0.1 * kD ambient.
+    bool no_shadows(V3<R>* rgb, const Isect<R>& is, const Ray<R>& ray, const DrawCtx& c) const {
+        V3<R> kD{};
+        if (is.material != nullptr) {
+            if (any_gt0(is.material->Le)) {
+                *rgb = is.material->Le;
+                return true;
+            }
+            kD = is.material->Kd;
+        } else {
+            kD = is.color;
+        }
+        if (any_gt0(kD)) {
+            if (!scene->lights.empty()) {
+                const uint32_t spl = desc->samples_light;
+                for (uint32_t j = 0; j < spl; ++j) {
+                    const Light<R>& light = scene->lights[choose_light(c, j, ray.depth)];
+                    const V3<R> lp = light_position(light, c, j, ray.depth);
+                    const V3<R> toLight = normalize(lp - is.point);
+                    const R cosNL = dot(is.normal, toLight);
+                    if (cosNL > 0) *rgb = *rgb + light.radiance.Le * cosNL;
+                }
+                *rgb = *rgb * kD;
+                *rgb = *rgb / static_cast<R>(spl);
+            }
+        }
+        *rgb = *rgb + kD * static_cast<R>(0.1f);
+        return false;
+    }
+
+    // DepthMap::shade, app/Components/Shaders/DepthMap.cpp:19-26.
+    bool depth_map(V3<R>* rgb, const Isect<R>& is, const Ray<R>& ray) const {
+        const V3<R> maxPoint{static_cast<R>(desc->depth_max_point[0]), static_cast<R>(desc->depth_max_point[1]),
+                             static_cast<R>(desc->depth_max_point[2])};
+        const R maxDist = length(maxPoint - ray.o) * static_cast<R>(2.0f);
+        const R depth = std::max((maxDist - is.length) / maxDist, static_cast<R>(0));
+        *rgb = V3<R>{depth, depth, depth};
+        return false;
+    }
+
+    // DiffuseMaterial::shade, app/Components/Shaders/DiffuseMaterial.cpp:16-38.
+    bool diffuse_material(V3<R>* rgb, const Isect<R>& is) const {
+        if (is.material != nullptr) {
+            if (any_gt0(is.material->Kd)) *rgb = is.material->Kd;
+            else if (any_gt0(is.material->Ks)) *rgb = is.material->Ks;
+            else if (any_gt0(is.material->Kt)) *rgb = is.material->Kt;
+            else if (any_gt0(is.material->Le)) *rgb = is.material->Le;
+        } else {
+            *rgb = is.color;
+        }
+        return false;
     }
 
     // Whitted::shade, app/Components/Shaders/Whitted.cpp:17-107.

@@ -4,10 +4,13 @@
 //   api_check render <out> <accel>      Cornell Whitted 64x64 through
 //                                       Renderer::renderFrame; bitmap to <out>
 //   api_check raytrace <out>            the same frame through RayTrace
+//   api_check shader <kind> <out> [rt]  the frame with NoShadows / DepthMap /
+//                                       DiffuseMaterial (rt: through RayTrace)
 //   api_check rays                      per-ray methods against the batch C ABI
 // Prints "ok" and exits 0 on success; prints the failed check and exits 1.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -143,6 +146,32 @@ static int render(const char* out, int accel, bool viaRayTrace) {
     return g_failed;
 }
 
+static int render_shader(int kind, const char* out, bool viaRayTrace) {
+    const uint32_t w = 64, h = 64;
+    std::vector<uint32_t> bm(w * h, 0u);
+    if (viaRayTrace) {
+        RayTrace(bm.data(), static_cast<int32_t>(w), static_cast<int32_t>(h), 4, kind, 0, 1, 1, 1, MRT_BVH, false,
+                 false, nullptr, nullptr);
+    } else {
+        std::unique_ptr<MobileRT::Shader> shader;
+        if (kind == MRT_SHADER_DEPTHMAP)
+            shader = std::make_unique<Components::DepthMap>(cornell(), vec3{1, 1, 1}, MobileRT::Shader::BVH);
+        else if (kind == MRT_SHADER_DIFFUSE)
+            shader = std::make_unique<Components::DiffuseMaterial>(cornell(), MobileRT::Shader::BVH);
+        else
+            shader = std::make_unique<Components::NoShadows>(cornell(), 1u, MobileRT::Shader::BVH);
+        MobileRT::Renderer r{std::move(shader), cornell_camera(), std::make_unique<Components::Constant>(0.5f), w, h, 1};
+        r.renderFrame(bm.data(), 8, w * 4u);
+        CHECK(r.lastStats().rays_camera == w * h);
+        CHECK(r.lastStats().rays_shadow == 0u);
+    }
+    FILE* f = std::fopen(out, "wb");
+    if (!f) return 1;
+    std::fwrite(bm.data(), 4, bm.size(), f);
+    std::fclose(f);
+    return g_failed;
+}
+
 // Per-ray methods against the batch C ABI on the same inputs.
 static int rays() {
     auto shader = std::make_unique<Components::Whitted>(cornell(), 1u, MobileRT::Shader::BVH);
@@ -201,6 +230,7 @@ int main(int argc, char** argv) {
     else if (mode == "render" && argc >= 4) rc = render(argv[2], std::atoi(argv[3]), false);
     else if (mode == "raytrace" && argc >= 3) rc = render(argv[2], MRT_BVH, true);
     else if (mode == "rays") rc = rays();
+    else if (mode == "shader" && argc >= 4) rc = render_shader(std::atoi(argv[2]), argv[3], argc >= 5);
     if (rc == 0) std::printf("ok\n");
     return rc == 0 ? 0 : 1;
 }

@@ -75,3 +75,27 @@ def test_cpp_raytrace_entry_matches_renderer(tmp_path):
 def test_cpp_per_ray_methods():
     r = run("rays")
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shader", [abi.SHADER_NOSHADOWS, abi.SHADER_DEPTHMAP, abi.SHADER_DIFFUSE])
+def test_cpp_one_hit_shaders(tmp_path, shader):
+    """Components::NoShadows / DepthMap / DiffuseMaterial through the C++
+    Renderer and through RayTrace (its shader ids 0, 3, 4) equal the oracle's
+    frame."""
+    a, b = tmp_path / "renderer.bin", tmp_path / "raytrace.bin"
+    r1 = run("shader", str(shader), str(a))
+    assert r1.returncode == 0, r1.stderr
+    r2 = run("shader", str(shader), str(b), "rt")
+    assert r2.returncode == 0, r2.stderr
+    bm = np.fromfile(a, np.uint32).reshape(64, 64)
+    assert np.array_equal(bm, np.fromfile(b, np.uint32).reshape(64, 64))
+    desc = cpp_desc()
+    desc.shader = shader
+    desc.depth_max_point[:] = [1.0, 1.0, 1.0]
+    if shader != abi.SHADER_NOSHADOWS:
+        desc.samples_light = 0
+    cam_min, cam_max = mrt.camera_aabb(desc.camera)
+    orc = oracle_scene(mrt.cornell_scene(1), abi.BVH, cam_min, cam_max)
+    bo, cnt, _ = orc.render(desc, threads=8)
+    assert np.array_equal(bm, bo)

@@ -232,3 +232,22 @@ def test_wavefront_equals_megakernel(gpu_ctx, generic, size):
         assert c0 == c1
         assert np.array_equal(b0, b1)
         assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+
+
+@pytest.mark.parametrize("shader", [abi.SHADER_NOSHADOWS, abi.SHADER_DEPTHMAP, abi.SHADER_DIFFUSE])
+@pytest.mark.parametrize("accel", [abi.BVH, abi.REGULAR_GRID, abi.NAIVE])
+def test_one_hit_shaders_exact(gpu_ctx, shader, accel):
+    """NoShadows, DepthMap and DiffuseMaterial (one closest hit, no
+    recursion): bitmap and FP32 radiance bit-exact against the oracle, on the
+    Cornell box (two area lights, light samples drawn per pixel) and on the
+    mirror scene (planes, spheres, reflective and transmissive materials,
+    which DiffuseMaterial reports as Ks / Kt)."""
+    for name, data, cam in (("cornell", mrt.cornell_scene(1), cornell_camera(128, 128)),
+                            ("mirror", mirror_scene(), cornell_camera(128, 128))):
+        desc = mrt.render_desc(shader, 128, 128, 2, cam, samples_light=2, pixel=rnd(3), light=rnd(4),
+                               depth_max_point=(1, 1, 1))
+        (bg, ag, st), (bo, ao, cnt) = render_both(gpu_ctx, data, desc, accel)
+        print(f"{name} shader {shader} accel {accel}: {st.rays_camera} camera rays", flush=True)
+        assert (st.rays_camera, st.rays_shadow) == (int(cnt[0]), int(cnt[2]))
+        assert np.array_equal(ag.view(np.uint32), ao.astype(np.float32).view(np.uint32)), f"{name}: radiance"
+        assert np.array_equal(bg, bo), f"{name}: bitmap"

@@ -13,7 +13,10 @@ invariants), chosen so that a plausible restatement mistake fails:
   * Whitted and PathTracer: scenes whose radiance has a closed form,
     including the zeroed LiD when the diffuse child hits an emitter
     (PathTracer.cpp:129-131);
-  * the FP64 and FP32 oracles agree where they should.
+  * the FP64 and FP32 oracles agree where they should;
+  * NoShadows equals Whitted where no light sample is occluded and exceeds it
+    where one is; DepthMap on planes at known distances; DiffuseMaterial's
+    Kd > Ks > Kt > Le priority and the vertex colour without a material.
 """
 from __future__ import annotations
 
@@ -340,3 +343,103 @@ def test_sampler_stratification():
                 assert sorted(np.floor(hv * 16).astype(int)) == list(range(16))
             assert sorted(np.floor(sv * 16).astype(int)) == list(range(16))
             assert np.all((rv >= 0) & (rv < 1)) and len(set(rv.tolist())) == 16
+
+
+def occluded_light_scene():
+    """A lit triangle at z = 0 and a small blocker triangle between part of
+    it and a point light at z = -2: some light samples are occluded."""
+    base = product_triangle((-1, -1, 0), (1, -1, 0), (0, 1, 0), 0)
+    blocker = product_triangle((-0.3, -0.3, -1), (0.3, -0.3, -1), (0, 0.3, -1), 0)
+    tris = np.concatenate([base, blocker])
+    mats = np.zeros(1, abi.MATERIAL_DTYPE)
+    mats[0]["kd"] = (0.5, 0.25, 0.8)
+    mats[0]["ior"] = 1.0
+    lights = np.zeros(1, abi.LIGHT_DTYPE)
+    Lt = abi.Light()
+    Lt.kind = abi.LIGHT_POINT
+    Lt.radiance.le[:] = [2.0, 1.0, 0.5]
+    Lt.radiance.ior = 1.0
+    Lt.position[:] = [0.0, 0.0, -2.0]
+    C.memmove(lights.ctypes.data, C.addressof(Lt), C.sizeof(abi.Light))
+    return mrt.SceneData(tris, mats, lights)
+
+
+def test_noshadows_is_whitted_without_occlusion():
+    """NoShadows (NoShadows.cpp:19-64) is Whitted's direct light without the
+    shadow ray: equal to Whitted wherever no light sample is blocked, never
+    darker where one is, and exactly Whitted's unoccluded value there."""
+    data = occluded_light_scene()
+    sc = oracle_scene(data, abi.BVH)
+    # Rays onto the base triangle at z = 0 from behind the blocker plane.
+    xs = np.linspace(-0.6, 0.6, 25, dtype=np.float32)
+    rays = make_rays([(float(x), -0.2, -0.5) for x in xs], [(0.0, 0.0, 1.0)])
+    w = mrt.render_desc(abi.SHADER_WHITTED, 16, 16, 1, cornell_camera(16, 16))
+    n = mrt.render_desc(abi.SHADER_NOSHADOWS, 16, 16, 1, cornell_camera(16, 16))
+    rgb_w, _ = sc.shade_rays(w, rays)
+    rgb_n, flag = sc.shade_rays(n, rays)
+    assert not flag.any()
+    kd = np.array([0.5, 0.25, 0.8], np.float32)
+    occluded = np.all(rgb_w == np.float32(0.1) * kd, axis=1)
+    assert occluded.any() and not occluded.all()
+    assert np.array_equal(rgb_n[~occluded], rgb_w[~occluded])
+    assert np.all(rgb_n[occluded] > rgb_w[occluded])
+    # Without the blocker Whitted gives NoShadows' values everywhere.
+    free = mrt.SceneData(data.triangles[:1], data.materials, data.lights)
+    rgb_free, _ = oracle_scene(free, abi.BVH).shade_rays(w, rays)
+    assert np.array_equal(rgb_free, rgb_n)
+
+
+def quads_scene(z_near, z_far, mats):
+    """Two camera-facing quads (z_near over x < 0, z_far over x > 0); the
+    near one with material 0, the far one material -1 with vertex colour
+    (0.3, 0.6, 0.9)."""
+    near = np.concatenate([product_triangle((-2, -1, z_near), (0, -1, z_near), (0, 1, z_near), 0),
+                           product_triangle((-2, -1, z_near), (0, 1, z_near), (-2, 1, z_near), 0)])
+    col = np.tile(np.array([0.3, 0.6, 0.9], np.float32), 3)
+    far = np.concatenate([product_triangle((0, -1, z_far), (2, -1, z_far), (2, 1, z_far), -1, colors=col),
+                          product_triangle((0, -1, z_far), (2, 1, z_far), (0, 1, z_far), -1, colors=col)])
+    return mrt.SceneData(np.concatenate([near, far]), mats, None)
+
+
+def test_depthmap_known_distances():
+    """DepthMap.cpp:19-26 on rays from the origin to planes at z = 1 and 2
+    with maxPoint (0, 0, 3): maxDist = 6, grey (6 - t) / 6; a ray that misses
+    stays black, and points beyond maxDist clamp to 0."""
+    mats = np.zeros(1, abi.MATERIAL_DTYPE)
+    mats[0]["kd"] = (0.5, 0.5, 0.5)
+    mats[0]["ior"] = 1.0
+    sc = oracle_scene(quads_scene(1.0, 2.0, mats), abi.BVH)
+    d = mrt.render_desc(abi.SHADER_DEPTHMAP, 16, 16, 1, cornell_camera(16, 16), depth_max_point=(0, 0, 3))
+    # Unit directions towards (-1, 0, 1) on the near plane, (1, 0, 2) on the
+    # far one, and away from both.
+    dirs = np.array([(-1, 0, 1), (1, 0, 2), (0, 0, -1)], np.float64)
+    rays = make_rays([(0, 0, 0)] * 3, dirs / np.linalg.norm(dirs, axis=1, keepdims=True))
+    rgb, flag = sc.shade_rays(d, rays)
+    t0 = np.float32(np.sqrt(2.0))
+    t1 = np.float32(np.sqrt(5.0))
+    np.testing.assert_allclose(rgb[0], (6 - t0) / 6, rtol=1e-6)
+    np.testing.assert_allclose(rgb[1], (6 - t1) / 6, rtol=1e-6)
+    assert np.all(rgb[2] == 0) and not flag.any()
+    # maxPoint at the origin's mirror: maxDist = 2 < t, clamped to 0.
+    d2 = mrt.render_desc(abi.SHADER_DEPTHMAP, 16, 16, 1, cornell_camera(16, 16), depth_max_point=(1, 0, 0))
+    rgb2, _ = sc.shade_rays(d2, rays[1:2])
+    assert np.all(rgb2 == 0)
+
+
+@pytest.mark.parametrize("field", ["kd", "ks", "kt", "le"])
+def test_diffuse_material_priority(field):
+    """DiffuseMaterial.cpp:16-38: the first non-zero of Kd, Ks, Kt, Le; the
+    vertex colour without a material."""
+    mats = np.zeros(1, abi.MATERIAL_DTYPE)
+    mats[0]["ior"] = 1.0
+    order = ["kd", "ks", "kt", "le"]
+    vals = {"kd": (0.1, 0.2, 0.3), "ks": (0.4, 0.5, 0.6), "kt": (0.7, 0.8, 0.9), "le": (1.5, 1.25, 1.0)}
+    for f in order[order.index(field):]:
+        mats[0][f] = vals[f]
+    sc = oracle_scene(quads_scene(1.0, 2.0, mats), abi.BVH)
+    d = mrt.render_desc(abi.SHADER_DIFFUSE, 16, 16, 1, cornell_camera(16, 16))
+    rays = make_rays([(0, 0, 0)] * 2, [(-0.5, 0, 0.5), (0.5, 0, 1.0)])
+    rgb, flag = sc.shade_rays(d, rays)
+    assert np.array_equal(rgb[0], np.array(vals[field], np.float32))
+    assert np.array_equal(rgb[1], np.array([0.3, 0.6, 0.9], np.float32))
+    assert not flag.any()
