@@ -405,6 +405,16 @@ typedef struct mrt_bvh_node {
  * node array.  Writes up to `capacity` nodes and the node count. */
 MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, mrt_bvh_node* nodes,
                           uint32_t capacity, uint32_t* count);
+/* The 4-wide tree the persistent trace walks above a set of leaves (box and
+ * non-zero leaf word each; scene builds pass the reference BVH's leaves), as
+ * built for rays with a finite inverse direction: 32 floats per node (min x,
+ * y, z and max x, y, z of its 4 slots as 4-vectors, then 4 (ref, word) pairs
+ * as raw bits; a leaf slot holds (leaf index, word), an interior slot (node,
+ * 0), an unused slot all-NaN boxes); node 0 holds the top slots; *depth is the
+ * number of node levels below node 0.  No counterpart in the reference: a
+ * check on the tree's structure. */
+MRT_API int mrt_wide_build(const float* boxes6, const uint32_t* words, uint32_t n, float* nodes, uint32_t capacity,
+                           uint32_t* count, uint32_t* depth);
 
 /* ---- synthetic scenes ---------------------------------------------------- */
 /* Procedural Cornell box: 10 wall triangles + 24 box triangles, 2 area lights. */

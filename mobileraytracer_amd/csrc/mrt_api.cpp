@@ -104,12 +104,6 @@ bool is_nan_box(const BNode& p, int side) {
     return side == 0 ? std::isnan(p.q0.x) : std::isnan(p.q1.z);
 }
 
-// One child slot of a wide node.
-struct WideSlot {
-    Box box;
-    uint32_t ref, word;
-};
-
 WideSlot pair_child(const BNode& p, int side) {
     if (side == 0) return {Box{f3{p.q0.x, p.q0.y, p.q0.z}, f3{p.q0.w, p.q1.x, p.q1.y}}, p.q3.x, p.q3.y};
     return {Box{f3{p.q1.z, p.q1.w, p.q2.x}, f3{p.q2.y, p.q2.z, p.q2.w}}, p.q3.z, p.q3.w};
@@ -131,69 +125,86 @@ int wide_slots(const std::vector<BNode>& w, uint32_t pi, WideSlot* out) {
     return n;
 }
 
-// Builds the wide walk tree from the pair walk tree `w` (layout in
-// mrt_scene.hpp).  For a ray whose inverse direction is finite in every axis
-// the reference slab test is monotone under box inclusion (the products
-// (bound - o) * inv round monotonically and a parent box is the exact union
-// of its children), so a grandchild that passes implies its parent passes:
-// the wide walk reaches exactly the leaves the pair walk reaches.
-int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, uint32_t light_pair, hipStream_t st) {
+// Builds the wide walk tree (layout in mrt_scene.hpp) for rays whose inverse
+// direction is finite in every axis.  For such rays the reference slab test
+// is monotone under box inclusion (the products (bound - o) * inv round
+// monotonically), so a leaf whose own box the ray passes also passes every box
+// that contains it: the leaves a ray reaches in the reference tree are exactly
+// those whose own boxes it passes, whatever tree is built above them, as long
+// as every interior box is the exact union of the leaf boxes below it.  The
+// triangle part is therefore a binned-SAH 4-wide tree over the reference
+// BVH's leaves (each kept with its own box); the light part collapses the
+// light tree's pairs (each interior child replaced by its two children).
+// Nodes: 0 and 1 the roots, then the triangle nodes, then the light nodes.
+int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri, hipStream_t st) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
-    // Pairs that become wide nodes, in depth-first order from the roots.
-    std::vector<uint32_t> order, depth(w.size(), 0);
-    std::vector<char> seen(w.size(), 0);
-    std::vector<uint32_t> todo{kWalkAnyRoot, kWalkClosestRoot};
-    seen[kWalkClosestRoot] = seen[kWalkAnyRoot] = 1;
-    depth[kWalkClosestRoot] = depth[kWalkAnyRoot] = 1;
-    uint32_t maxDepth = 1;
-    WideSlot sl[4];
-    while (!todo.empty()) {
-        const uint32_t pi = todo.back();
-        todo.pop_back();
-        order.push_back(pi);
-        const int n = wide_slots(w, pi, sl);
-        for (int k = n - 1; k >= 0; --k) {
-            if (sl[k].word != 0u || std::isnan(sl[k].box.mn.x) || seen[sl[k].ref]) continue;
-            seen[sl[k].ref] = 1;
-            depth[sl[k].ref] = depth[pi] + 1u;
-            maxDepth = std::max(maxDepth, depth[pi] + 1u);
-            todo.push_back(sl[k].ref);
+    // Light slots of root 0: the light root's children (interior root), the
+    // root itself (a leaf: the linear light scan), or none.
+    std::vector<WideSlot> lightTop;
+    if (!is_nan_box(w[kWalkClosestRoot], 1)) {
+        const WideSlot c = pair_child(w[kWalkClosestRoot], 1);
+        if (c.word != 0u) {
+            lightTop.push_back(c);
+        } else {
+            lightTop.push_back(pair_child(w[c.ref], 0));
+            lightTop.push_back(pair_child(w[c.ref], 1));
         }
     }
-    // Numbering: the two roots, then triangle pairs, then light pairs.
+    // Triangle units: the reference leaves.
+    std::vector<WideSlot> units;
+    if (!s->tris.empty()) {
+        if (tri.root_count > 0) {
+            units.push_back(WideSlot{tri.root, tri.root_ref, leaf_word(tri.root_ref, tri.root_count)});
+        } else {
+            for (const BNode& p : tri.pairs) {
+                for (int side = 0; side < 2; ++side) {
+                    const WideSlot c = pair_child(p, side);
+                    if (c.word != 0u) units.push_back(WideSlot{c.box, c.ref, leaf_word(c.ref, c.word)});
+                }
+            }
+        }
+    }
+    std::vector<float4> triNodes, out(16, make_float4(0, 0, 0, 0));
+    std::vector<WideSlot> triTop;
+    uint32_t triDepth = 0;
+    build_wide_sah(units, kWalkFirstPair, 4 - static_cast<int>(lightTop.size()), &triNodes, &triTop, &triDepth);
+    const uint32_t nTriNodes = static_cast<uint32_t>(triNodes.size() / 8);
+    const uint32_t lightNode = kWalkFirstPair + nTriNodes;
+    // Light nodes: pairs below the light top slots, depth first.
     std::vector<uint32_t> index(w.size(), kNone);
-    index[kWalkClosestRoot] = kWalkClosestRoot;
-    index[kWalkAnyRoot] = kWalkAnyRoot;
-    uint32_t next = kWalkFirstPair;
-    for (uint32_t pi : order)
-        if (index[pi] == kNone && pi < light_pair) index[pi] = next++;
-    const uint32_t lightNode = next;
-    for (uint32_t pi : order)
-        if (index[pi] == kNone) index[pi] = next++;
-    const float nan = std::numeric_limits<float>::quiet_NaN();
-    std::vector<float4> out(8ull * next);
-    for (uint32_t pi : order) {
-        float4* q = &out[8ull * index[pi]];
-        const int n = wide_slots(w, pi, sl);
-        float v[6][4];
-        uint32_t r[8];
-        for (int k = 0; k < 4; ++k) {
-            const bool used = k < n;
-            const Box b = used ? sl[k].box : Box{f3{nan, nan, nan}, f3{nan, nan, nan}};
-            v[0][k] = b.mn.x;
-            v[1][k] = b.mn.y;
-            v[2][k] = b.mn.z;
-            v[3][k] = b.mx.x;
-            v[4][k] = b.mx.y;
-            v[5][k] = b.mx.z;
-            const bool interior = used && sl[k].word == 0u && !std::isnan(b.mn.x);
-            r[2 * k] = interior ? index[sl[k].ref] : (used ? sl[k].ref : 0u);
-            r[2 * k + 1] = used ? sl[k].word : 0u;
+    uint32_t next = lightNode;
+    std::vector<std::pair<uint32_t, uint32_t>> todo;  // (pair, level)
+    auto map_slot = [&](WideSlot c, uint32_t level) {
+        if (c.word != 0u || std::isnan(c.box.mn.x)) return c;
+        if (index[c.ref] == kNone) {
+            index[c.ref] = next++;
+            todo.emplace_back(c.ref, level);
         }
-        for (int j = 0; j < 6; ++j) q[j] = make_float4(v[j][0], v[j][1], v[j][2], v[j][3]);
-        std::memcpy(&q[6], r, 16);
-        std::memcpy(&q[7], r + 4, 16);
+        c.ref = index[c.ref];
+        return c;
+    };
+    for (WideSlot& c : lightTop) c = map_slot(c, 2u);
+    uint32_t maxDepth = 1u + triDepth;
+    std::vector<float4> lightNodes;
+    while (!todo.empty()) {
+        const auto [pi, level] = todo.back();
+        todo.pop_back();
+        maxDepth = std::max(maxDepth, level);
+        WideSlot sl[4];
+        const int n = wide_slots(w, pi, sl);
+        for (int k = 0; k < n; ++k) sl[k] = map_slot(sl[k], level + 1u);
+        put_wide_node(&lightNodes, index[pi] - lightNode, sl, n);
     }
+    // Roots: 0 = triangle top + light top, 1 = triangle top (shadow rays).
+    WideSlot r0[4], r1[4];
+    int n0 = 0, n1 = 0;
+    for (const WideSlot& c : triTop) r0[n0++] = r1[n1++] = c;
+    for (const WideSlot& c : lightTop) r0[n0++] = c;
+    put_wide_node(&out, kWalkClosestRoot, r0, n0);
+    put_wide_node(&out, kWalkAnyRoot, r1, n1);
+    out.resize(8ull * kWalkFirstPair);
+    out.insert(out.end(), triNodes.begin(), triNodes.end());
+    out.insert(out.end(), lightNodes.begin(), lightNodes.end());
     int rc;
     if ((rc = s->wide_nodes.upload(out.data(), out.size() * sizeof(float4), st)) != MRT_OK) return rc;
     MRT_HIP(hipStreamSynchronize(st));  // `out` is released on return
@@ -241,7 +252,7 @@ int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
     s->dev.walk_light_pair = lightBase;
     s->dev.walk_light_slot = nTri;
     s->depth_need = std::max(s->depth_need, std::max(tri.depth, s->light_bvh.depth) + 1u);
-    if ((rc = build_wide_tree(s, w, lightBase, st)) != MRT_OK) return rc;
+    if ((rc = build_wide_tree(s, w, tri, st)) != MRT_OK) return rc;
     MRT_HIP(hipStreamSynchronize(st));
     return MRT_OK;
 }
@@ -1091,12 +1102,12 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     const bool persistent = useWavefront && d->persistent != 2u && !s->generic &&
                             s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH && s->dev.walk_nodes != nullptr &&
                             s->dev.wide_nodes != nullptr;
-    const uint32_t persistBlocks =
-        persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks) : 0u;
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
     // Wavefront slot groups run concurrently, each with its own spill region.
     const uint32_t slots = useWavefront ? wavefront_slots(p) : 0u;
     const uint32_t groups = useWavefront ? wavefront_groups(slots) : 1u;
+    const uint32_t persistBlocks =
+        persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks, groups) : 0u;
     StackCfg sc;
     if ((rc = ensure_spill(s, blocks * 256u * groups, &sc)) != MRT_OK) return rc;
     std::vector<StackCfg> scs(groups, sc);

@@ -301,3 +301,203 @@ void build_light_bvh(const std::vector<Box>& boxes, BvhOut* out, std::vector<uin
 }
 
 }  // namespace mrt
+
+// ------------------------------------------------------------- wide SAH --
+namespace mrt {
+
+void put_wide_node(std::vector<float4>* out, uint32_t index, const WideSlot* sl, int n) {
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    if (out->size() < 8ull * (index + 1u)) out->resize(8ull * (index + 1u));
+    float4* q = &(*out)[8ull * index];
+    float v[6][4];
+    uint32_t r[8];
+    for (int k = 0; k < 4; ++k) {
+        const bool used = k < n;
+        const Box b = used ? sl[k].box : Box{f3{nan, nan, nan}, f3{nan, nan, nan}};
+        v[0][k] = b.mn.x;
+        v[1][k] = b.mn.y;
+        v[2][k] = b.mn.z;
+        v[3][k] = b.mx.x;
+        v[4][k] = b.mx.y;
+        v[5][k] = b.mx.z;
+        r[2 * k] = used ? sl[k].ref : 0u;
+        r[2 * k + 1] = used ? sl[k].word : 0u;
+    }
+    for (int j = 0; j < 6; ++j) q[j] = make_float4(v[j][0], v[j][1], v[j][2], v[j][3]);
+    std::memcpy(&q[6], r, 16);
+    std::memcpy(&q[7], r + 4, 16);
+}
+
+namespace {
+
+struct BinNode {
+    Box box;
+    int32_t l = -1, r = -1;  // children (interior), -1 for a unit
+    uint32_t unit = 0;
+};
+
+float axis_of(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// Binary tree over the units: binned SAH (16 bins, every axis) on the unit
+// centres, an index median where no bin split separates them.
+void build_binary(const std::vector<WideSlot>& units, std::vector<BinNode>* nodes) {
+    constexpr int kBins = 16;
+    const uint32_t n = static_cast<uint32_t>(units.size());
+    std::vector<uint32_t> idx(n);
+    std::vector<f3> cen(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        idx[i] = i;
+        cen[i] = muls(add(units[i].box.mn, units[i].box.mx), 0.5f);
+    }
+    nodes->clear();
+    nodes->reserve(2ull * n);
+    nodes->push_back(BinNode{});
+    struct Task {
+        int32_t node;
+        uint32_t b, e;
+    };
+    std::vector<Task> stack{{0, 0, n}};
+    while (!stack.empty()) {
+        const Task t = stack.back();
+        stack.pop_back();
+        Box box = units[idx[t.b]].box;
+        Box cb{cen[idx[t.b]], cen[idx[t.b]]};
+        for (uint32_t k = t.b + 1; k < t.e; ++k) {
+            box = surrounding(box, units[idx[k]].box);
+            cb = Box{vmin(cb.mn, cen[idx[k]]), vmax(cb.mx, cen[idx[k]])};
+        }
+        (*nodes)[t.node].box = box;
+        if (t.e - t.b == 1) {
+            (*nodes)[t.node].unit = idx[t.b];
+            continue;
+        }
+        int bestAxis = -1, bestSplit = 0;
+        float bestCost = std::numeric_limits<float>::infinity();
+        for (int a = 0; a < 3; ++a) {
+            const float lo = axis_of(cb.mn, a), ext = axis_of(cb.mx, a) - lo;
+            if (!(ext > 0.0f)) continue;
+            Box bb[kBins];
+            uint32_t cnt[kBins] = {};
+            for (uint32_t k = t.b; k < t.e; ++k) {
+                const int bi = std::min(kBins - 1, static_cast<int>((axis_of(cen[idx[k]], a) - lo) * (kBins / ext)));
+                bb[bi] = cnt[bi] ? surrounding(bb[bi], units[idx[k]].box) : units[idx[k]].box;
+                ++cnt[bi];
+            }
+            float right[kBins];
+            Box rb{};
+            uint32_t rc = 0;
+            for (int i = kBins - 1; i > 0; --i) {
+                if (cnt[i]) {
+                    rb = rc ? surrounding(rb, bb[i]) : bb[i];
+                    rc += cnt[i];
+                }
+                right[i] = rc ? surface_area(rb) * static_cast<float>(rc) : 0.0f;
+            }
+            Box lb{};
+            uint32_t lc = 0;
+            for (int i = 1; i < kBins; ++i) {
+                if (cnt[i - 1]) {
+                    lb = lc ? surrounding(lb, bb[i - 1]) : bb[i - 1];
+                    lc += cnt[i - 1];
+                }
+                if (lc == 0 || lc == t.e - t.b) continue;
+                const float cost = surface_area(lb) * static_cast<float>(lc) + right[i];
+                if (cost < bestCost) {
+                    bestCost = cost;
+                    bestAxis = a;
+                    bestSplit = i;
+                }
+            }
+        }
+        uint32_t mid;
+        if (bestAxis >= 0) {
+            const float lo = axis_of(cb.mn, bestAxis), ext = axis_of(cb.mx, bestAxis) - lo;
+            const auto it = std::partition(idx.begin() + t.b, idx.begin() + t.e, [&](uint32_t u) {
+                return std::min(kBins - 1, static_cast<int>((axis_of(cen[u], bestAxis) - lo) * (kBins / ext))) <
+                       bestSplit;
+            });
+            mid = static_cast<uint32_t>(it - idx.begin());
+        } else {
+            mid = (t.b + t.e) / 2u;  // coincident centres: any halving
+        }
+        if (mid == t.b || mid == t.e) mid = (t.b + t.e) / 2u;
+        const int32_t l = static_cast<int32_t>(nodes->size());
+        nodes->push_back(BinNode{});
+        nodes->push_back(BinNode{});
+        (*nodes)[t.node].l = l;
+        (*nodes)[t.node].r = l + 1;
+        stack.push_back(Task{l, t.b, mid});
+        stack.push_back(Task{l + 1, mid, t.e});
+    }
+}
+
+// Up to `limit` binary nodes covering node b's subtree: b's children, then the
+// interior one with the largest surface area replaced by its children, and so
+// on.
+int expand(const std::vector<BinNode>& bn, int32_t b, int limit, int32_t* out) {
+    if (bn[b].l < 0) {
+        out[0] = b;
+        return 1;
+    }
+    int n = 0;
+    out[n++] = bn[b].l;
+    out[n++] = bn[b].r;
+    while (n < limit) {
+        int best = -1;
+        float bestArea = -1.0f;
+        for (int k = 0; k < n; ++k) {
+            if (bn[out[k]].l < 0) continue;
+            const float a = surface_area(bn[out[k]].box);
+            if (a > bestArea) {
+                bestArea = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        const int32_t c = out[best];
+        out[best] = bn[c].l;
+        out[n++] = bn[c].r;
+    }
+    return n;
+}
+
+}  // namespace
+
+void build_wide_sah(const std::vector<WideSlot>& units, uint32_t base, int max_top, std::vector<float4>* out,
+                    std::vector<WideSlot>* top, uint32_t* depth) {
+    top->clear();
+    *depth = 0;
+    if (units.empty()) return;
+    std::vector<BinNode> bn;
+    build_binary(units, &bn);
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t next = base;
+    // Slot of binary node c: a unit as its own leaf slot, an interior node as
+    // a wide node to be written.
+    struct Pend {
+        int32_t bin;
+        uint32_t index, level;
+    };
+    std::vector<Pend> todo;
+    auto slot_of = [&](int32_t c, uint32_t level) {
+        if (bn[c].l < 0) return units[bn[c].unit];
+        const uint32_t index = next++;
+        todo.push_back(Pend{c, index, level});
+        return WideSlot{bn[c].box, index, 0u};
+    };
+    int32_t sel[4];
+    const int nt = expand(bn, 0, std::max(1, std::min(4, max_top)), sel);
+    for (int k = 0; k < nt; ++k) top->push_back(slot_of(sel[k], 1u));
+    (void)kNone;
+    while (!todo.empty()) {
+        const Pend p = todo.back();
+        todo.pop_back();
+        *depth = std::max(*depth, p.level);
+        const int n = expand(bn, p.bin, 4, sel);
+        WideSlot sl[4];
+        for (int k = 0; k < n; ++k) sl[k] = slot_of(sel[k], p.level + 1u);
+        put_wide_node(out, p.index - base, sl, n);
+    }
+}
+
+}  // namespace mrt

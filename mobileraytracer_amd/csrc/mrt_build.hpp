@@ -60,6 +60,27 @@ void build_grid(uint32_t n, const Box& bounds, int32_t grid_size, BoxFn box_of, 
 // are widened so that the slab test never rejects a point of the triangle).
 void build_light_bvh(const std::vector<Box>& boxes, BvhOut* out, std::vector<uint32_t>* perm);
 
+// One child slot of a 4-wide node: its box, a reference (a wide node for an
+// interior slot, the first primitive slot for a leaf) and the leaf word
+// (0 for an interior slot).
+struct WideSlot {
+    Box box;
+    uint32_t ref, word;
+};
+
+// Writes wide node `index` (8 float4, SoA boxes + (ref, word) pairs, unused
+// slots all-NaN) into `out`.
+void put_wide_node(std::vector<float4>* out, uint32_t index, const WideSlot* slots, int n);
+
+// 4-wide BVH over leaf units (box + leaf word each, every unit kept as its
+// own leaf slot) by binned surface-area heuristic on the unit centres.  Nodes
+// get indices from `base` on and are written into `out` from its start (node
+// `base` first); the tree's top is returned as at most `max_top` slots for the
+// caller's root nodes, and *depth receives the number of wide levels below
+// them.
+void build_wide_sah(const std::vector<WideSlot>& units, uint32_t base, int max_top, std::vector<float4>* out,
+                    std::vector<WideSlot>* top, uint32_t* depth);
+
 }  // namespace mrt
 
 #include "mrt_build_grid.inl"

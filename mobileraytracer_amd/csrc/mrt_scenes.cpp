@@ -210,6 +210,27 @@ MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, m
     return MRT_OK;
 }
 
+MRT_API int mrt_wide_build(const float* boxes6, const uint32_t* words, uint32_t n, float* nodes, uint32_t capacity,
+                           uint32_t* count, uint32_t* depth) {
+    if ((n && (!boxes6 || !words)) || !count || !depth) return MRT_E_INVALID;
+    std::vector<WideSlot> units(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        units[i].box = Box{ld3(boxes6 + 6 * static_cast<size_t>(i)), ld3(boxes6 + 6 * static_cast<size_t>(i) + 3)};
+        units[i].ref = i;
+        units[i].word = words[i];
+        if (words[i] == 0u) return MRT_E_INVALID;
+    }
+    std::vector<float4> out;
+    std::vector<WideSlot> top;
+    build_wide_sah(units, 1u, 4, &out, &top, depth);
+    std::vector<float4> all;
+    put_wide_node(&all, 0u, top.data(), static_cast<int>(top.size()));
+    all.insert(all.end(), out.begin(), out.end());
+    *count = static_cast<uint32_t>(all.size() / 8);
+    if (nodes) std::memcpy(nodes, all.data(), std::min<size_t>(all.size(), 8ull * capacity) * sizeof(float4));
+    return MRT_OK;
+}
+
 // Procedural Cornell box: walls [-1,1]^3 without the front (10 triangles,
 // wound inwards), two axis-aligned boxes (24 triangles, wound outwards), one
 // 0.5 x 0.5 area-light quad at y = 0.99 split in two AreaLights with Le 0.9

@@ -714,7 +714,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     float4* state = s->state.as<float4>();
     const uint32_t ncu = ctx->num_cu ? ctx->num_cu : 256u;
     s->trace_blocks = ncu * 16u / 2u;
-    s->persist_blocks = wavefront_persistent_blocks(ncu, s->trace_blocks);
+    s->persist_blocks = wavefront_persistent_blocks(ncu, s->trace_blocks, G);
     for (uint32_t g = 0; g < G; ++g) {
         const uint32_t w0 = s->pool_waves * g / G, w1 = s->pool_waves * (g + 1) / G;
         WfBuffers& wb = s->wb[g];

@@ -29,6 +29,8 @@
 // megakernel).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "mrt_device_util.hpp"
 #include "mrt_wavefront.hpp"
 
@@ -643,7 +645,7 @@ uint32_t wavefront_slots(const RenderParams& p) {
     return ((covW + 7u) / 8u) * ((covH + 7u) / 8u) * 64u;
 }
 
-uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap) {
+uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups) {
     static int perCu = 0;  // same for every device of this (single-arch) build
     if (perCu == 0) {
         int a = 0;
@@ -653,7 +655,9 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap) {
         }
         perCu = std::max(1, a);
     }
-    return std::min(cap, static_cast<uint32_t>(perCu) * num_cu);
+    const uint32_t resident = static_cast<uint32_t>(perCu) * num_cu;
+    const uint32_t share = groups > 1u ? std::max(1u, resident * 6u / (5u * groups)) : resident;
+    return std::min({cap, resident, share});
 }
 
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,

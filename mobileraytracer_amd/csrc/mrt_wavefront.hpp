@@ -105,9 +105,14 @@ inline size_t wavefront_group_bytes(uint32_t slots) {
 constexpr uint32_t kWfMaxGroups = 3;
 inline uint32_t wavefront_groups(uint32_t slots) { return slots >= (1u << 16) ? kWfMaxGroups : 1u; }
 
-// Workgroups of the persistent trace kernel: as many as can be resident at
-// once on `num_cu` compute units, at most `cap`.
-uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap);
+// Workgroups of one persistent trace launch when `groups` launches run
+// concurrently: 6 / (5 groups) of those that can be resident at once on
+// `num_cu` compute units (all of them for one group), at most `cap`.  A launch
+// carries a few rays per lane only if it does not claim the whole chip; on the
+// 1M-triangle 1080p frame with three groups, launches of 25 / 34 / 40 / 46 /
+// 67 / 100 % of the resident workgroups took 205 / 193 / 187 / 190 / 204 /
+// 206 ms per frame.
+uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups);
 
 // Queues the camera rays of sample `first_sample` of every slot.
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,

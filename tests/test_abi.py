@@ -163,3 +163,64 @@ def test_icosphere_generator():
     step = np.linalg.norm(np.diff(c, axis=0), axis=1).mean()
     assert step < 0.1
     assert mrt.icosphere_scene(3, 5, 3, 64).triangles.tobytes() == t.tobytes()
+
+
+def wide_tree(boxes, words):
+    n = len(boxes)
+    cnt, depth = C.c_uint32(), C.c_uint32()
+    b6 = np.ascontiguousarray(boxes, np.float32)
+    w = np.ascontiguousarray(words, np.uint32)
+    assert abi.lib().mrt_wide_build(b6.ctypes.data, w.ctypes.data, n, None, 0, C.byref(cnt), C.byref(depth)) == 0
+    nodes = np.zeros((cnt.value, 32), np.float32)
+    assert abi.lib().mrt_wide_build(b6.ctypes.data, w.ctypes.data, n, nodes.ctypes.data, cnt.value, C.byref(cnt),
+                                    C.byref(depth)) == 0
+    return nodes, depth.value
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 4000])
+def test_wide_tree_structure(n):
+    """The SAH wide tree over leaves (the persistent trace's tree for rays
+    with a finite inverse direction): every leaf appears exactly once with its
+    own box and word, every interior slot's box is the exact union of the leaf
+    boxes below it (min / max of float32, no rounding), unused slots are
+    all-NaN, and the reported depth is the deepest node level."""
+    rng = np.random.default_rng(n)
+    c = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    h = rng.uniform(0.001, 0.02, (n, 3)).astype(np.float32)
+    boxes = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+    if n == 5:
+        boxes[:] = boxes[0]  # coincident leaves
+    words = (np.arange(n, dtype=np.uint32) % 3 + 1) << 27 | np.arange(n, dtype=np.uint32)
+    nodes, depth = wide_tree(boxes, words)
+    seen = np.zeros(n, int)
+
+    def visit(k, level):
+        nd = nodes[k]
+        mn = nd[0:12].reshape(3, 4).T
+        mx = nd[12:24].reshape(3, 4).T
+        refs = nd[24:32].view(np.uint32).reshape(4, 2)
+        lo = np.full(3, np.inf, np.float32)
+        hi = np.full(3, -np.inf, np.float32)
+        maxl = level
+        for s in range(4):
+            ref, word = refs[s]
+            if np.isnan(mn[s]).all():
+                assert word == 0 and np.isnan(mx[s]).all()
+                continue
+            if word:
+                assert word == words[ref] and np.array_equal(mn[s], boxes[ref, :3]) and np.array_equal(mx[s], boxes[ref, 3:])
+                seen[ref] += 1
+                sub_lo, sub_hi = boxes[ref, :3], boxes[ref, 3:]
+            else:
+                sub_lo, sub_hi, dl = visit(ref, level + 1)
+                assert np.array_equal(mn[s], sub_lo) and np.array_equal(mx[s], sub_hi)
+                maxl = max(maxl, dl)
+            lo = np.minimum(lo, sub_lo)
+            hi = np.maximum(hi, sub_hi)
+        return lo, hi, maxl
+
+    _, _, deepest = visit(0, 0)
+    assert np.all(seen == 1)
+    assert deepest == depth
+    if n == 4000:
+        assert depth <= 12  # SAH over uniform leaves: about log4(n) + a few
