@@ -393,7 +393,8 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
         MRT_HIP(launch_wavefront_start(p, d->first_sample, wbs[g], &L[g], st[g]));
     }
     const uint64_t covered = 256ull * p.bsx * p.bsy;
-    const uint32_t samples = d->samples_pixel - d->first_sample;
+    const uint32_t lanes = wbs[0].lanes;
+    const uint32_t samples = (d->samples_pixel - d->first_sample + lanes - 1u) / lanes;  // per slot
     // A sample takes at most depth_max + 1 iterations (one per vertex, one to
     // finish); more than that means the queues never drained.
     const uint64_t maxIters = static_cast<uint64_t>(samples) * (static_cast<uint64_t>(d->ray_depth_max) + 2u) +
@@ -454,16 +455,28 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
         MRT_HIP(hipEventRecord(gend(g), st[g]));
         MRT_HIP(hipStreamWaitEvent(st[0], gend(g), 0));
     }
+    uint32_t* doneMin = wbs[0].counts + (kCntWords - 1u);
+    if (wbs[0].rad) {
+        MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(doneMin), static_cast<int>(d->samples_pixel), 1,
+                                  st[0]));
+        MRT_HIP(launch_wavefront_resolve(p, d->samples_pixel, wbs[0], dbm, dacc, doneMin, st[0]));
+    }
     MRT_HIP(hipEventRecord(b1, st[0]));
     MRT_HIP(hipEventSynchronize(b1));
-    uint64_t finished = 0;
-    for (uint32_t g = 0; g < G; ++g) {
-        uint32_t c = 0;
-        MRT_HIP(hipMemcpyAsync(&c, wbs[g].counts + kCntDone, sizeof(c), hipMemcpyDeviceToHost, st[0]));
+    uint32_t progress = 0;
+    if (wbs[0].rad) {
+        MRT_HIP(hipMemcpyAsync(&progress, doneMin, sizeof(progress), hipMemcpyDeviceToHost, st[0]));
         MRT_HIP(hipStreamSynchronize(st[0]));
-        finished += c;
+    } else {
+        uint64_t finished = 0;
+        for (uint32_t g = 0; g < G; ++g) {
+            uint32_t c = 0;
+            MRT_HIP(hipMemcpyAsync(&c, wbs[g].counts + kCntDone, sizeof(c), hipMemcpyDeviceToHost, st[0]));
+            MRT_HIP(hipStreamSynchronize(st[0]));
+            finished += c;
+        }
+        progress = d->first_sample + static_cast<uint32_t>(finished / covered);
     }
-    const uint32_t progress = d->first_sample + static_cast<uint32_t>(finished / covered);
     s->progress.store(progress);
     *stopped = stopReq && progress < d->samples_pixel;
     MRT_HIP(hipEventElapsedTime(kernel_ms, b0, b1));
@@ -1103,8 +1116,11 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
                             s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH && s->dev.walk_nodes != nullptr &&
                             s->dev.wide_nodes != nullptr;
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
-    // Wavefront slot groups run concurrently, each with its own spill region.
-    const uint32_t slots = useWavefront ? wavefront_slots(p) : 0u;
+    // Wavefront slot groups run concurrently, each with its own spill region;
+    // sample-parallel frames give every pixel `lanes` slots.
+    const uint32_t pixelSlots = useWavefront ? wavefront_slots(p) : 0u;
+    const uint32_t lanes = useWavefront ? wavefront_lanes(pixelSlots, d->samples_pixel - d->first_sample) : 1u;
+    const uint32_t slots = pixelSlots * lanes;
     const uint32_t groups = useWavefront ? wavefront_groups(slots) : 1u;
     const uint32_t persistBlocks =
         persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks, groups) : 0u;
@@ -1121,6 +1137,13 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         const size_t scanBytes = wavefront_scan_bytes();
         if ((rc = s->wf_state.alloc(wavefront_state_bytes(slots))) != MRT_OK) return rc;
         float4* state = s->wf_state.as<float4>();
+        float* rad = nullptr;
+        if (lanes > 1u) {
+            const size_t n = 3ull * (d->samples_pixel - d->first_sample) * pixelSlots;
+            if ((rc = s->wf_rad.alloc(n * sizeof(float))) != MRT_OK) return rc;
+            rad = s->wf_rad.as<float>();
+            MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(rad), static_cast<int>(kRadPending), n, st));
+        }
         // Key box: the triangle BVH's root box (the unit cube without triangles).
         f3 mn{0.0f, 0.0f, 0.0f}, mx{1.0f, 1.0f, 1.0f};
         const Accel& ta = s->dev.acc[MRT_CAT_TRIANGLE];
@@ -1157,6 +1180,10 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             wb.scan_temp = reinterpret_cast<void*>(t);
             wb.scan_bytes = scanBytes;
             wb.sort = d->ray_sort == 1u ? 1u : 0u;
+            wb.rad = rad;
+            wb.lanes = lanes;
+            wb.pixel_slots = pixelSlots;
+            wb.first = d->first_sample;
             for (int k = 0; k < 3; ++k) {
                 const float e = comp(mx, k) - comp(mn, k);
                 wb.box_mn[k] = comp(mn, k);

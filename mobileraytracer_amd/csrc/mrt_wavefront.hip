@@ -120,15 +120,17 @@ __global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t
     const uint32_t slot = b.slot_base + local;
     bool live = false;
     uint32_t x = 0, y = 0;
+    const uint32_t lane = slot / b.pixel_slots;  // sample lane of a sample-parallel frame
+    const uint32_t s0 = sample + lane;
     if (local < b.slot_count) {
-        slot_pixel(p, slot, &x, &y);
-        live = x < 16u * p.bsx && y < 16u * p.bsy;
+        slot_pixel(p, slot - lane * b.pixel_slots, &x, &y);
+        live = x < 16u * p.bsx && y < 16u * p.bsy && s0 < p.spp;
     }
     f3 o{}, d{};
     if (live) {
         const uint32_t pid = y * p.width + x;
-        primary_ray(p, x, y, pid, sample, &o, &d);
-        reset_path(b, slot, pid, sample);
+        primary_ray(p, x, y, pid, s0, &o, &d);
+        reset_path(b, slot, pid, s0);
     }
     const uint32_t qe = wave_append(count, live);
     if (live) put_ray(b, queue, qe, o, d, MRT_ID_NONE, slot, false);
@@ -486,19 +488,32 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p0, uint32_t s
             }
             const uint32_t pid = __float_as_uint(sbv.w);
             const uint32_t sample = flags_sample(flags);
+            uint32_t step = 1u;
             uint32_t x, y;
-            slot_pixel(p, local, &x, &y);
-            const uint32_t bi = y * p.real_width + x;
-            bitmap[bi] = incremental_avg(c, bitmap[bi], sample + 1u);
-            if (accum) {
-                accum[3u * pid + 0u] += c.x;
-                accum[3u * pid + 1u] += c.y;
-                accum[3u * pid + 2u] += c.z;
+            if (!MULTI && b.rad) {
+                // Sample-parallel frame: keep the radiance for k_wf_resolve.
+                const uint32_t lane = slot / b.pixel_slots;
+                const uint32_t ps = slot - lane * b.pixel_slots;
+                slot_pixel(p, ps, &x, &y);
+                float* r = b.rad + 3ull * ((sample - b.first) * static_cast<uint64_t>(b.pixel_slots) + ps);
+                r[0] = c.x;
+                r[1] = c.y;
+                r[2] = c.z;
+                step = b.lanes;
+            } else {
+                slot_pixel(p, local, &x, &y);
+                const uint32_t bi = y * p.real_width + x;
+                bitmap[bi] = incremental_avg(c, bitmap[bi], sample + 1u);
+                if (accum) {
+                    accum[3u * pid + 0u] += c.x;
+                    accum[3u * pid + 1u] += c.y;
+                    accum[3u * pid + 2u] += c.z;
+                }
             }
             ++finished;
-            if (!stop && sample + 1u < spp) {
-                primary_ray(p, x, y, pid, sample + 1u, &o, &d);
-                reset_path(b, slot, pid, sample + 1u);
+            if (!stop && sample + step < spp) {
+                primary_ray(p, x, y, pid, sample + step, &o, &d);
+                reset_path(b, slot, pid, sample + step);
                 again = true;
             } else {
                 last = true;
@@ -632,7 +647,47 @@ __global__ __launch_bounds__(kBlock) void k_wf_scatter(const uint32_t* __restric
     for (uint32_t k = gtid; k < kSortBins; k += stride) b.hist[k] = 0u;
 }
 
+// ------------------------------------------------------------- resolve --
+// Per covered pixel: the finished samples, in sample order, into the 8-bit
+// running mean (incrementalAvg, Utils.cpp:48-71) and the FP32 sum; the first
+// unfinished sample (stopped frame) ends the pixel's run.
+__global__ __launch_bounds__(kBlock) void k_wf_resolve(RenderParams p, uint32_t spp, WfBuffers b,
+                                                       uint32_t* __restrict__ bitmap, float* __restrict__ accum,
+                                                       uint32_t* done_min) {
+    const uint32_t ps = blockIdx.x * kBlock + threadIdx.x;
+    if (ps >= b.pixel_slots) return;
+    uint32_t x, y;
+    slot_pixel(p, ps, &x, &y);
+    if (x >= 16u * p.bsx || y >= 16u * p.bsy) return;
+    const uint32_t pid = y * p.width + x;
+    const uint32_t bi = y * p.real_width + x;
+    uint32_t bm = bitmap[bi];
+    f3 acc{};
+    if (accum) acc = f3{accum[3u * pid + 0u], accum[3u * pid + 1u], accum[3u * pid + 2u]};
+    uint32_t s = b.first;
+    for (; s < spp; ++s) {
+        const float* r = b.rad + 3ull * ((s - b.first) * static_cast<uint64_t>(b.pixel_slots) + ps);
+        if (__float_as_uint(r[0]) == kRadPending) break;
+        const f3 c{r[0], r[1], r[2]};
+        bm = incremental_avg(c, bm, s + 1u);
+        acc = add(acc, c);
+    }
+    bitmap[bi] = bm;
+    if (accum) {
+        accum[3u * pid + 0u] = acc.x;
+        accum[3u * pid + 1u] = acc.y;
+        accum[3u * pid + 2u] = acc.z;
+    }
+    atomicMin(done_min, s);
+}
+
 // ------------------------------------------------------------- launcher --
+hipError_t launch_wavefront_resolve(const RenderParams& p, uint32_t spp, const WfBuffers& b, uint32_t* bitmap,
+                                    float* accum, uint32_t* done_min, hipStream_t stream) {
+    k_wf_resolve<<<(b.pixel_slots + kBlock - 1u) / kBlock, kBlock, 0, stream>>>(p, spp, b, bitmap, accum, done_min);
+    return hipGetLastError();
+}
+
 size_t wavefront_scan_bytes() {
     size_t bytes = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<const uint32_t*>(nullptr),
@@ -656,7 +711,7 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t gro
         perCu = std::max(1, a);
     }
     const uint32_t resident = static_cast<uint32_t>(perCu) * num_cu;
-    const uint32_t share = groups > 1u ? std::max(1u, resident * 6u / (5u * groups)) : resident;
+    const uint32_t share = groups > 1u ? std::max(1u, resident * 3u / (2u * groups)) : resident;
     return std::min({cap, resident, share});
 }
 

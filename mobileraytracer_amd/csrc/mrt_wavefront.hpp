@@ -46,7 +46,19 @@ struct WfBuffers {
     uint32_t* counts;  // kCntWords counters, zero before launch_wavefront_start
     float box_mn[3], box_inv[3];  // key quantisation box
     uint32_t sort;     // 1: reorder extension rays
+    // Sample-parallel frames: `lanes` slots per pixel, slot k * pixel_slots +
+    // ps running samples first + k, first + k + lanes, ... of pixel slot ps;
+    // each finished sample's radiance goes to rad[(sample - first) *
+    // pixel_slots + ps] (3 floats) and k_wf_resolve folds them into the
+    // outputs in sample order.  lanes == 1 and rad == nullptr: samples are
+    // folded as they finish.
+    float* rad;
+    uint32_t lanes, pixel_slots, first;
 };
+
+// Radiance-buffer entries of samples not finished (x component bits; a NaN
+// payload that arithmetic never produces).
+constexpr uint32_t kRadPending = 0x7FBADBADu;
 
 // A render job of a batch (the render server): the pool waves it owns carry
 // its slots, and its scene's walk trees sit in the batch's shared arena.
@@ -105,13 +117,29 @@ inline size_t wavefront_group_bytes(uint32_t slots) {
 constexpr uint32_t kWfMaxGroups = 3;
 inline uint32_t wavefront_groups(uint32_t slots) { return slots >= (1u << 16) ? kWfMaxGroups : 1u; }
 
+// Slots per pixel of a frame of `samples` samples over `pixel_slots` pixel
+// slots: up to 8, while the frame stays within 2^24 slots (about 3.4 GB of
+// path state and queues) and its per-sample radiance within 2 GiB.  More slots
+// put more rays in each iteration, so the persistent lanes carry more rays per
+// launch and a frame needs fewer iterations.  1M-triangle 1080p frame at 16
+// spp: 1 / 2 / 4 / 8 / 16 slots per pixel took 187 / 169 / 152 / 140 / 137 ms
+// (best launch share each).
+inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples) {
+    uint32_t k = 8u;
+    while (k > 1u && (k > samples || static_cast<uint64_t>(pixel_slots) * k > (1ull << 24) ||
+                      12ull * samples * pixel_slots > (2ull << 30)))
+        k /= 2u;
+    return k;
+}
+
 // Workgroups of one persistent trace launch when `groups` launches run
-// concurrently: 6 / (5 groups) of those that can be resident at once on
+// concurrently: 3 / (2 groups) of those that can be resident at once on
 // `num_cu` compute units (all of them for one group), at most `cap`.  A launch
 // carries a few rays per lane only if it does not claim the whole chip; on the
-// 1M-triangle 1080p frame with three groups, launches of 25 / 34 / 40 / 46 /
-// 67 / 100 % of the resident workgroups took 205 / 193 / 187 / 190 / 204 /
-// 206 ms per frame.
+// 1M-triangle 1080p frame with three groups and one slot per pixel, launches of
+// 25 / 34 / 40 / 46 / 67 / 100 % of the resident workgroups took 205 / 193 /
+// 187 / 190 / 204 / 206 ms per frame; with 8 slots per pixel, 40 / 50 / 60 / 70
+// % took 148 / 140 / 140 / 153 ms.
 uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups);
 
 // Queues the camera rays of sample `first_sample` of every slot.
@@ -126,6 +154,13 @@ hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, 
 hipError_t launch_wavefront_admit(const BatchDev& bd, const JobDev& jd, uint32_t job, uint32_t pool_wave,
                                   uint32_t job_wave, uint32_t n_waves, bool first, const WfBuffers& b,
                                   const WfLoop& L, hipStream_t stream);
+
+// Folds the per-sample radiance of a sample-parallel frame into bitmap and
+// accumulation in sample order (Renderer.cpp:101-104 per sample), stopping at a
+// pixel's first unfinished sample; counts[kCntDone] receives the smallest
+// number of samples any covered pixel completed.
+hipError_t launch_wavefront_resolve(const RenderParams& p, uint32_t spp, const WfBuffers& b, uint32_t* bitmap,
+                                    float* accum, uint32_t* done_min, hipStream_t stream);
 
 // One trace -> regen -> [reorder] -> shade iteration.  persistent: trace with
 // the refilling kernel over the combined walk tree (BVH scenes of triangles
