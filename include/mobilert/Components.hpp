@@ -5,6 +5,9 @@
  * engine classes map onto the C ABI. */
 #pragma once
 
+#include <functional>
+#include <string>
+
 #include "MobileRT.hpp"
 
 namespace Components {
@@ -217,6 +220,33 @@ public:
     void resetSampling() noexcept final;
     bool intersect(::MobileRT::Intersection* intersection, const ::MobileRT::Ray& ray) const noexcept final;
     mrt_light descriptor() const noexcept final;
+};
+
+/* ---- object loaders (app/Components/ObjectLoaders) --------------------- */
+
+/* OBJLoader.hpp: OBJ / MTL files into a Scene (OBJLoader.cpp:23-250), over
+ * this library's parser (mrt_obj_load; the reference uses tinyobjloader). */
+class OBJLoader final {
+private:
+    std::string objFilePath_{};
+    std::string mtlFilePath_{};
+    mrt_obj* obj_{nullptr};
+    bool isProcessed_{false};
+    std::int32_t numberTriangles_{0};
+
+public:
+    explicit OBJLoader(std::string obj, std::string materials) noexcept;
+    OBJLoader(const OBJLoader&) = delete;
+    OBJLoader& operator=(const OBJLoader&) = delete;
+    ~OBJLoader() noexcept;
+    /* Parses the files: the number of triangles (emissive ones included),
+     * -1 on failure. */
+    std::int32_t process() noexcept;
+    bool isProcessed() const noexcept { return isProcessed_; }
+    /* Appends the triangles, materials (each deduplicated against the
+     * scene's, Material.cpp:21-29) and AreaLights (each with a sampler from
+     * lambda) to the scene. */
+    bool fillScene(::MobileRT::Scene* scene, std::function<std::unique_ptr<::MobileRT::Sampler>()> lambda) noexcept;
 };
 
 }  // namespace Components

@@ -6,8 +6,10 @@
  *         (c_wrapper.cpp:444-486).
  * scene:  0 the Cornell box of this build's generator (34 triangles, 2 area
  *         lights, camera at (0, 0, -3.4)), 1 the 1M-triangle soup (990k
- *         diffuse, 10k area lights); the reference's other demo scenes and OBJ
- *         files (pathObj / pathMtl) are refused.
+ *         diffuse, 10k area lights), any other id with pathObj set the OBJ /
+ *         MTL files (Components::OBJLoader) with the reference's default
+ *         camera or its CornellBox pose; the reference's other demo scenes are
+ *         refused.
  * accelerator: 0 Naive, 1 RegularGrid, 2 BVH.
  * threads is accepted and ignored; async renders on a detached host thread
  * and returns at once, as the reference does (the bitmap must outlive the

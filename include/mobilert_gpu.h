@@ -416,6 +416,23 @@ MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, m
 MRT_API int mrt_wide_build(const float* boxes6, const uint32_t* words, uint32_t n, float* nodes, uint32_t capacity,
                            uint32_t* count, uint32_t* depth);
 
+/* ---- OBJ / MTL ingestion --------------------------------------------------- */
+/* OBJLoader::process + fillScene (app/Components/ObjectLoaders/OBJLoader.cpp:
+ * 23-250) over this library's own parser (the reference uses tinyobjloader):
+ * X negated, specular halved, transmittance * (1 - dissolve), emission
+ * normalised by its largest component when above 1, emissive faces as
+ * AreaLights, other materials de-duplicated within Epsilon, polygons split
+ * into fans.  mtl_path may be NULL (no materials). */
+typedef struct mrt_obj mrt_obj;
+MRT_API int mrt_obj_load(const char* obj_path, const char* mtl_path, mrt_obj** out);
+MRT_API int mrt_obj_counts(const mrt_obj* obj, uint32_t* n_triangles, uint32_t* n_materials, uint32_t* n_lights,
+                           uint32_t* n_faces_skipped);
+/* Copies the scene out; every AreaLight gets light_sampler (NULL: RANDOM)
+ * with its seed advanced by the light's index. */
+MRT_API int mrt_obj_fill(const mrt_obj* obj, mrt_triangle* triangles, mrt_material* materials, mrt_light* lights,
+                         const mrt_sampler* light_sampler);
+MRT_API int mrt_obj_destroy(mrt_obj* obj);
+
 /* ---- synthetic scenes ---------------------------------------------------- */
 /* Procedural Cornell box: 10 wall triangles + 24 box triangles, 2 area lights. */
 #define MRT_CORNELL_TRIANGLES 34u

@@ -121,6 +121,27 @@ def icosphere_scene(seed: int, n_spheres: int, subdiv: int, n_emissive: int, dis
     return SceneData(tris, None, lights)
 
 
+def obj_scene(obj_path: str, mtl_path: str | None = None, light_sampler: abi.Sampler | None = None) -> SceneData:
+    """OBJ / MTL files as a scene (OBJLoader::process + fillScene)."""
+    h = C.c_void_p()
+    check(lib().mrt_obj_load(str(obj_path).encode(), str(mtl_path).encode() if mtl_path else None, C.byref(h)),
+          "mrt_obj_load")
+    try:
+        nt, nm, nl, ns = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().mrt_obj_counts(h, C.byref(nt), C.byref(nm), C.byref(nl), C.byref(ns)), "mrt_obj_counts")
+        tris = np.zeros(nt.value, abi.TRIANGLE_DTYPE)
+        mats = np.zeros(nm.value, abi.MATERIAL_DTYPE)
+        lights = np.zeros(nl.value, abi.LIGHT_DTYPE)
+        check(lib().mrt_obj_fill(h, tris.ctypes.data if nt.value else None, mats.ctypes.data if nm.value else None,
+                                 lights.ctypes.data if nl.value else None,
+                                 C.byref(light_sampler) if light_sampler is not None else None), "mrt_obj_fill")
+    finally:
+        lib().mrt_obj_destroy(h)
+    data = SceneData(tris, mats, lights)
+    data.faces_skipped = ns.value
+    return data
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = C.c_void_p()

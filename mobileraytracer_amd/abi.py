@@ -205,6 +205,10 @@ def _proto(lib):
         "mrt_scene_cornell": (C.c_int, [u32, P, P, P]),
         "mrt_scene_soup": (C.c_int, [u32, u32, u32, f32, f32, P, P, C.c_int]),
         "mrt_scene_icospheres": (C.c_int, [u32, u32, u32, f32, u32, f32, P, P, C.c_int]),
+        "mrt_obj_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
+        "mrt_obj_counts": (C.c_int, [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+        "mrt_obj_fill": (C.c_int, [P, P, P, P, C.POINTER(Sampler)]),
+        "mrt_obj_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -3,6 +3,7 @@
 // sampler and shader, then render `repeats` frames into the caller's bitmap.
 #include <chrono>
 #include <cstdio>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -59,11 +60,7 @@ bool make_scene(int32_t id, MobileRT::Scene* scene, std::unique_ptr<MobileRT::Ca
 
 void work(uint32_t* bitmap, int32_t width, int32_t height, int32_t threads, int32_t nShader, int32_t nScene,
           int32_t samplesPixel, int32_t samplesLight, int32_t repeats, int32_t accelerator, bool printStdOut,
-          const char* pathObj) {
-    if (pathObj != nullptr && pathObj[0] != '\0') {
-        std::fprintf(stderr, "RayTrace: OBJ scenes are not supported by this library\n");
-        return;
-    }
+          std::string pathObj, std::string pathMtl) {
     if (accelerator < 0 || accelerator > 2 || width <= 0 || height <= 0 || samplesPixel <= 0 || samplesLight <= 0) {
         std::fprintf(stderr, "RayTrace: invalid arguments\n");
         return;
@@ -75,8 +72,28 @@ void work(uint32_t* bitmap, int32_t width, int32_t height, int32_t threads, int3
     const auto t0 = std::chrono::steady_clock::now();
     MobileRT::Scene scene;
     std::unique_ptr<MobileRT::Camera> camera;
-    if (!make_scene(nScene, &scene, &camera, hfov, vfov)) {
-        std::fprintf(stderr, "RayTrace: scene %d is not available (0 Cornell box, 1 1M-triangle soup)\n", nScene);
+    if (nScene != 0 && nScene != 1 && !pathObj.empty()) {
+        // c_wrapper.cpp:115-135: the OBJ scene (its emissive faces as
+        // MersenneTwister-sampled AreaLights) with the default camera
+        // (c_wrapper.cpp:62-67), or the CornellBox files' pose.  The other
+        // named-file presets (extra lights and poses for the reference's demo
+        // models) are not reproduced.
+        Components::OBJLoader loader{pathObj, pathMtl};
+        loader.process();
+        if (!loader.isProcessed() ||
+            !loader.fillScene(&scene, [] { return std::make_unique<Components::MersenneTwister>(); })) {
+            std::fprintf(stderr, "RayTrace: cannot load %s\n", pathObj.c_str());
+            return;
+        }
+        if (pathObj.find("CornellBox") != std::string::npos)
+            camera = std::make_unique<Components::Perspective>(vec3{0.0f, 0.7f, 3.0f}, vec3{0.0f, 0.7f, -1.0f},
+                                                               vec3{0.0f, 1.0f, 0.0f}, hfov, vfov);
+        else
+            camera = std::make_unique<Components::Perspective>(vec3{0.0f, 0.0f, -3.4f}, vec3{0.0f, 0.0f, 1.0f},
+                                                               vec3{0.0f, 1.0f, 0.0f}, hfov, vfov);
+    } else if (!make_scene(nScene, &scene, &camera, hfov, vfov)) {
+        std::fprintf(stderr, "RayTrace: scene %d is not available (0 Cornell box, 1 1M-triangle soup, other ids "
+                             "with an OBJ path)\n", nScene);
         return;
     }
     const size_t nTriangles = scene.triangles_.size(), nLights = scene.lights_.size();
@@ -128,13 +145,14 @@ void work(uint32_t* bitmap, int32_t width, int32_t height, int32_t threads, int3
 extern "C" void RayTrace(uint32_t* bitmap, int32_t width, int32_t height, int32_t threads, int32_t shader,
                          int32_t scene, int32_t samplesPixel, int32_t samplesLight, int32_t repeats,
                          int32_t accelerator, bool printStdOut, bool async, const char* pathObj,
-                         const char* /*pathMtl*/) {
+                         const char* pathMtl) {
+    const std::string obj = pathObj ? pathObj : "", mtl = pathMtl ? pathMtl : "";
     if (async) {
         std::thread{work, bitmap, width, height, threads, shader, scene, samplesPixel, samplesLight, repeats,
-                    accelerator, printStdOut, pathObj}
+                    accelerator, printStdOut, obj, mtl}
             .detach();
     } else {
         work(bitmap, width, height, threads, shader, scene, samplesPixel, samplesLight, repeats, accelerator,
-             printStdOut, pathObj);
+             printStdOut, obj, mtl);
     }
 }

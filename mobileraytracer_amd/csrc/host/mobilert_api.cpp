@@ -1042,3 +1042,72 @@ mrt_light PointLight::descriptor() const noexcept {
 }
 
 }  // namespace Components
+
+// ------------------------------------------------------------ OBJ loader --
+namespace Components {
+
+OBJLoader::OBJLoader(std::string obj, std::string materials) noexcept
+    : objFilePath_{std::move(obj)}, mtlFilePath_{std::move(materials)} {}
+
+OBJLoader::~OBJLoader() noexcept {
+    if (obj_) mrt_obj_destroy(obj_);
+}
+
+// OBJLoader.cpp:23-64.
+std::int32_t OBJLoader::process() noexcept {
+    if (obj_) {
+        mrt_obj_destroy(obj_);
+        obj_ = nullptr;
+    }
+    if (mrt_obj_load(objFilePath_.c_str(), mtlFilePath_.empty() ? nullptr : mtlFilePath_.c_str(), &obj_) != MRT_OK) {
+        std::fprintf(stderr, "OBJLoader: %s\n", mrt_last_error());
+        isProcessed_ = false;
+        numberTriangles_ = -1;
+        return numberTriangles_;
+    }
+    std::uint32_t nt = 0, nl = 0;
+    mrt_obj_counts(obj_, &nt, nullptr, &nl, nullptr);
+    isProcessed_ = true;
+    numberTriangles_ = static_cast<std::int32_t>(nt + nl);
+    return numberTriangles_;
+}
+
+// OBJLoader.cpp:66-250.
+bool OBJLoader::fillScene(MobileRT::Scene* scene, std::function<std::unique_ptr<MobileRT::Sampler>()> lambda) noexcept {
+    if (!isProcessed_ || !obj_ || !scene) return false;
+    std::uint32_t nt = 0, nm = 0, nl = 0;
+    mrt_obj_counts(obj_, &nt, &nm, &nl, nullptr);
+    std::vector<mrt_triangle> tris(nt);
+    std::vector<mrt_material> mats(nm);
+    std::vector<mrt_light> lights(nl);
+    if (mrt_obj_fill(obj_, tris.data(), mats.data(), lights.data(), nullptr) != MRT_OK) return false;
+    auto material_of = [](const mrt_material& m) {
+        return MobileRT::Material{vec3{m.kd[0], m.kd[1], m.kd[2]}, vec3{m.ks[0], m.ks[1], m.ks[2]},
+                                  vec3{m.kt[0], m.kt[1], m.kt[2]}, m.ior, vec3{m.le[0], m.le[1], m.le[2]}};
+    };
+    // The loader's materials against the scene's (first equal one, else
+    // appended), as the reference's std::find over scene->materials_.
+    std::vector<std::int32_t> index(nm);
+    for (std::uint32_t k = 0; k < nm; ++k) {
+        const MobileRT::Material m = material_of(mats[k]);
+        const auto it = std::find(scene->materials_.begin(), scene->materials_.end(), m);
+        if (it != scene->materials_.end()) {
+            index[k] = static_cast<std::int32_t>(it - scene->materials_.begin());
+        } else {
+            index[k] = static_cast<std::int32_t>(scene->materials_.size());
+            scene->materials_.push_back(m);
+        }
+    }
+    scene->triangles_.reserve(scene->triangles_.size() + nt);
+    for (mrt_triangle t : tris) {
+        if (t.material_id >= 0) t.material_id = index[static_cast<std::size_t>(t.material_id)];
+        scene->triangles_.emplace_back(t);
+    }
+    for (const mrt_light& l : lights)
+        scene->lights_.emplace_back(
+            std::make_unique<AreaLight>(material_of(l.radiance), lambda ? lambda() : std::make_unique<MersenneTwister>(),
+                                        MobileRT::Triangle{l.triangle}));
+    return true;
+}
+
+}  // namespace Components

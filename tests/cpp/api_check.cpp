@@ -6,6 +6,8 @@
 //   api_check raytrace <out>            the same frame through RayTrace
 //   api_check shader <kind> <out> [rt]  the frame with NoShadows / DepthMap /
 //                                       DiffuseMaterial (rt: through RayTrace)
+//   api_check obj <obj> <mtl>           Components::OBJLoader on the fixture
+//   api_check objrt <obj> <mtl> <out>   RayTrace of an OBJ scene
 //   api_check rays                      per-ray methods against the batch C ABI
 // Prints "ok" and exits 0 on success; prints the failed check and exits 1.
 #include <cmath>
@@ -172,6 +174,45 @@ static int render_shader(int kind, const char* out, bool viaRayTrace) {
     return g_failed;
 }
 
+// OBJLoader.cpp:23-250 through the C++ API: counts, and materials
+// deduplicated against one the scene already holds.
+static int obj(const char* objPath, const char* mtlPath) {
+    Components::OBJLoader loader{objPath, mtlPath};
+    CHECK(loader.process() == 9);  // 7 triangles + 2 emissive
+    CHECK(loader.isProcessed());
+    MobileRT::Scene scene;
+    // The fixture's "mirror" after conversion: Ks halved, ior 1.5.
+    scene.materials_.push_back(MobileRT::Material{vec3{0.0f}, vec3{0.45f, 0.4f, 0.35f}, vec3{0.0f}, 1.5f});
+    int made = 0;
+    CHECK(loader.fillScene(&scene, [&made] {
+        ++made;
+        return std::make_unique<Components::Constant>(0.5f);
+    }));
+    CHECK(made == 2 && scene.lights_.size() == 2u);
+    CHECK(scene.triangles_.size() == 7u && scene.materials_.size() == 4u);
+    const int expect[7] = {1, 1, 0, 0, 2, -1, 3};
+    for (int k = 0; k < 7; ++k) CHECK(scene.triangles_[k].materialId_ == expect[k]);
+    CHECK(scene.lights_[0]->radiance_.Le_ == (vec3{1.0f, 0.5f, 0.25f}));
+    Components::OBJLoader missing{"no_such.obj", ""};
+    CHECK(missing.process() == -1 && !missing.isProcessed());
+    return g_failed;
+}
+
+static int obj_raytrace(const char* objPath, const char* mtlPath, const char* out) {
+    const uint32_t w = 64, h = 64;
+    std::vector<uint32_t> bm(w * h, 0u);
+    RayTrace(bm.data(), static_cast<int32_t>(w), static_cast<int32_t>(h), 4, MRT_SHADER_WHITTED, 5, 1, 1, 1, MRT_BVH,
+             false, false, objPath, mtlPath);
+    int lit = 0;
+    for (uint32_t v : bm) lit += (v & 0xFFFFFFu) != 0u;
+    CHECK(lit > 0);
+    FILE* f = std::fopen(out, "wb");
+    if (!f) return 1;
+    std::fwrite(bm.data(), 4, bm.size(), f);
+    std::fclose(f);
+    return g_failed;
+}
+
 // Per-ray methods against the batch C ABI on the same inputs.
 static int rays() {
     auto shader = std::make_unique<Components::Whitted>(cornell(), 1u, MobileRT::Shader::BVH);
@@ -231,6 +272,8 @@ int main(int argc, char** argv) {
     else if (mode == "raytrace" && argc >= 3) rc = render(argv[2], MRT_BVH, true);
     else if (mode == "rays") rc = rays();
     else if (mode == "shader" && argc >= 4) rc = render_shader(std::atoi(argv[2]), argv[3], argc >= 5);
+    else if (mode == "obj" && argc >= 4) rc = obj(argv[2], argv[3]);
+    else if (mode == "objrt" && argc >= 5) rc = obj_raytrace(argv[2], argv[3], argv[4]);
     if (rc == 0) std::printf("ok\n");
     return rc == 0 ? 0 : 1;
 }
