@@ -303,6 +303,14 @@ MRT_API int mrt_render(mrt_scene* scene, const mrt_render_desc* desc, uint32_t* 
 MRT_API int mrt_render_stop(mrt_scene* scene);
 /* Renderer::getSample (Renderer.cpp:118-120). */
 MRT_API int mrt_render_progress(const mrt_scene* scene, uint32_t* samples_done);
+/* Auxiliary outputs of the frame desc describes, for a denoiser (no
+ * counterpart in the reference): per covered pixel, over the camera rays of
+ * samples [first_sample, samples_pixel) (the render's jitter), the mean
+ * first-hit albedo (the material's Kd, the vertex colour without a material,
+ * Le on an emitter), shading normal and hit distance (zeros for a miss).
+ * aov: 7 floats per pixel (albedo.xyz, normal.xyz, depth), width x height,
+ * pixels outside the tile grid zero. */
+MRT_API int mrt_render_aovs(mrt_scene* scene, const mrt_render_desc* desc, float* aov, int mem);
 
 /* ---- render server (continuous batching) -------------------------------- */
 /* No counterpart in the reference, whose Renderer renders one frame at a time

@@ -247,6 +247,13 @@ class Scene:
               "mrt_render")
         return bitmap, stats
 
+    def aovs(self, desc: abi.RenderDesc, out=None, mem: int = MEM_HOST):
+        """First-hit albedo, normal and depth per pixel (height x width x 7)."""
+        if out is None:
+            out = np.zeros((desc.height, desc.width, 7), np.float32)
+        check(lib().mrt_render_aovs(self.h, C.byref(desc), ptr(out), mem), "mrt_render_aovs")
+        return out
+
     def stop(self) -> None:
         check(lib().mrt_render_stop(self.h), "mrt_render_stop")
 

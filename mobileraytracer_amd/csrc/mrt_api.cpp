@@ -1268,6 +1268,34 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     return MRT_OK;
 }
 
+MRT_API int mrt_render_aovs(mrt_scene* s, const mrt_render_desc* d, float* aov, int mem) {
+    if (!s || !aov) return fail(MRT_E_INVALID, "null argument");
+    if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
+    int rc;
+    RenderParams p;
+    std::vector<uint32_t> rank;
+    if ((rc = render_setup(s, d, d ? d->width * 4u : 0u, &p, &rank)) != MRT_OK) return rc;
+    if (d->first_sample >= d->samples_pixel) return fail(MRT_E_INVALID, "first_sample must be below samples_pixel");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    if ((rc = s->tile_rank.upload(rank.data(), rank.size() * sizeof(uint32_t), st)) != MRT_OK) return rc;
+    p.tile_rank = s->tile_rank.as<uint32_t>();
+    const size_t bytes = static_cast<size_t>(d->width) * d->height * 7 * sizeof(float);
+    float* dout = aov;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = s->stage_out.alloc(bytes)) != MRT_OK) return rc;
+        dout = s->stage_out.as<float>();
+    }
+    MRT_HIP(hipMemsetAsync(dout, 0, bytes, st));
+    if ((rc = prepare_aux(s)) != MRT_OK) return rc;
+    StackCfg sc;
+    if ((rc = ensure_spill(s, render_pass_blocks(p) * 256u, &sc)) != MRT_OK) return rc;
+    MRT_HIP(launch_aov(s->dev, s->generic, p, d->first_sample, d->samples_pixel, dout, sc, s->err.as<uint32_t>(), st));
+    if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(aov, dout, bytes, hipMemcpyDeviceToHost, st));
+    return check_err(s);
+}
+
 MRT_API int mrt_render_stop(mrt_scene* s) {
     if (!s) return fail(MRT_E_INVALID, "null scene");
     s->stop.store(true);

@@ -172,6 +172,52 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(SceneDev s, RenderParams
     flush_counts<COUNT>(rc, counters, st.overflow | (pendOverflow << 1), err);
 }
 
+// ------------------------------------------------------------- AOVs --
+// Auxiliary outputs of a frame for a denoiser (no counterpart in the
+// reference): per covered pixel, over the camera rays of samples [first,
+// spp) (the render's own jitter), the mean first-hit albedo (the material's
+// kD, the vertex colour without a material, Le on an emitter), shading normal
+// and hit distance; a miss contributes zeros.  aov: 7 floats per pixel of the
+// width x height frame.
+template <bool GENERIC>
+__global__ __launch_bounds__(kBlock) void k_aov(SceneDev s, RenderParams p, uint32_t first, uint32_t spp,
+                                                float* __restrict__ aov, StackCfg sc, uint32_t* err) {
+    __shared__ uint32_t lds[kStackLds * kBlock];
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    uint32_t x, y;
+    slot_pixel(p, logical * kBlock + threadIdx.x, &x, &y);
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st = make_stack(lds, sc, gtid);
+    TraceStats ts{};
+    if (x < 16u * p.bsx && y < 16u * p.bsy) {
+        const uint32_t pid = y * p.width + x;
+        f3 alb{}, nrm{};
+        float dep = 0.0f;
+        for (uint32_t sample = first; sample < spp; ++sample) {
+            f3 o, d;
+            primary_ray(p, x, y, pid, sample, &o, &d);
+            const RayT r = make_ray(o, d, MRT_ID_NONE);
+            const HitT h = closest_hit<GENERIC, false>(s, r, kRayLengthMax, st, ts);
+            if (h.id == MRT_ID_NONE) continue;
+            const Surf sf = surface(s, r, h);
+            const Shading m = shading_of(s, sf);
+            alb = add(alb, m.emitter ? m.le : m.kd);
+            nrm = add(nrm, sf.normal);
+            dep += h.t;
+        }
+        const float inv = 1.0f / static_cast<float>(spp - first);
+        float* o7 = aov + 7ull * pid;
+        o7[0] = alb.x * inv;
+        o7[1] = alb.y * inv;
+        o7[2] = alb.z * inv;
+        o7[3] = nrm.x * inv;
+        o7[4] = nrm.y * inv;
+        o7[5] = nrm.z * inv;
+        o7[6] = dep * inv;
+    }
+    if (st.overflow) atomicOr(err, 1u);
+}
+
 // ------------------------------------------------------------- launchers --
 static uint32_t grid_for(uint32_t n, uint32_t cap_blocks) {
     uint32_t b = (n + kBlock - 1) / kBlock;
@@ -253,6 +299,14 @@ uint32_t render_pass_blocks(const RenderParams& p) {
     const uint32_t covW = 16u * p.bsx, covH = 16u * p.bsy;
     const uint32_t tiles = ((covW + 7u) / 8u) * ((covH + 7u) / 8u);
     return (tiles + 3u) / 4u;
+}
+
+hipError_t launch_aov(const SceneDev& s, bool generic, const RenderParams& p, uint32_t first, uint32_t spp, float* aov,
+                      const StackCfg& sc, uint32_t* err, hipStream_t stream) {
+    const uint32_t g = render_pass_blocks(p);
+    if (generic) k_aov<true><<<g, kBlock, 0, stream>>>(s, p, first, spp, aov, sc, err);
+    else k_aov<false><<<g, kBlock, 0, stream>>>(s, p, first, spp, aov, sc, err);
+    return hipGetLastError();
 }
 
 hipError_t launch_render_pass(const SceneDev& s, bool generic, bool branches, bool count, const RenderParams& p,

@@ -32,6 +32,8 @@ hipError_t launch_shade_rays(const SceneDev& s, bool generic, bool branches, con
                              uint8_t* flag, const StackCfg& sc, uint32_t cap_blocks, uint32_t* err,
                              hipStream_t stream);
 uint32_t render_pass_blocks(const RenderParams& p);
+hipError_t launch_aov(const SceneDev& s, bool generic, const RenderParams& p, uint32_t first, uint32_t spp, float* aov,
+                      const StackCfg& sc, uint32_t* err, hipStream_t stream);
 hipError_t launch_render_pass(const SceneDev& s, bool generic, bool branches, bool count, const RenderParams& p,
                               uint32_t sample, uint32_t* bitmap, float* accum, const StackCfg& sc,
                               unsigned long long* counters, uint32_t* err, hipStream_t stream);

@@ -267,3 +267,32 @@ def test_obj_scene_exact(gpu_ctx):
     assert (ag > 0).any()
     np.testing.assert_allclose(ag, ao, rtol=1e-3, atol=1e-5)
     assert np.max(np.abs(bg.view(np.uint8).astype(int) - bo.view(np.uint8).astype(int))) <= 1
+
+
+@pytest.mark.parametrize("spp", [1, 3])
+def test_aovs_against_one_hit_shaders(gpu_ctx, spp):
+    """mrt_render_aovs (no reference counterpart) pinned against the
+    oracle-pinned one-hit shaders on the same camera rays: the mean albedo is
+    the mean DiffuseMaterial radiance (Kd, colour or Le first), the mean depth
+    reproduces the DepthMap grey (maxDist - t) / maxDist, normals are unit
+    vectors at 1 spp."""
+    data = mrt.cornell_scene(1)
+    cam = cornell_camera(96, 96)
+    gpu = mrt.Scene(gpu_ctx, data)
+    gpu.build(abi.BVH)
+    kw = dict(pixel=rnd(3), depth_max_point=(1, 1, 1))
+    aov = gpu.aovs(mrt.render_desc(abi.SHADER_DIFFUSE, 96, 96, spp, cam, **kw))
+    acc = np.zeros((96, 96, 3), np.float32)
+    gpu.render(mrt.render_desc(abi.SHADER_DIFFUSE, 96, 96, spp, cam, **kw), accum=acc)
+    np.testing.assert_allclose(aov[..., :3], acc / np.float32(spp), rtol=1e-6, atol=1e-7)
+    if spp == 1:
+        assert np.array_equal(aov[..., :3], acc)
+        n = np.linalg.norm(aov[..., 3:6], axis=2)
+        np.testing.assert_allclose(n[aov[..., 6] > 0], 1.0, rtol=1e-5)
+        depth_acc = np.zeros((96, 96, 3), np.float32)
+        gpu.render(mrt.render_desc(abi.SHADER_DEPTHMAP, 96, 96, 1, cam, **kw), accum=depth_acc)
+        pos = np.array(cam.position, np.float32)
+        max_dist = np.float32(np.linalg.norm(np.float32(1) - pos) * 2)
+        grey = np.maximum((max_dist - aov[..., 6]) / max_dist, 0)
+        hit = aov[..., 6] > 0
+        np.testing.assert_allclose(grey[hit], depth_acc[..., 0][hit], rtol=1e-5)
