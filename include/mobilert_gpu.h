@@ -383,6 +383,61 @@ MRT_API int mrt_server_poll(mrt_server* server, uint64_t job_id, mrt_job_info* i
 MRT_API int mrt_server_cancel(mrt_server* server, uint64_t job_id);
 MRT_API int mrt_server_stats(mrt_server* server, mrt_serving_stats* stats);
 
+/* ---- learned denoiser ------------------------------------------------------ */
+/* No counterpart in the reference (it writes the 8-bit running mean of the
+ * samples, Utils.cpp:48-71, and stops there).  A U-Net over a low-sample
+ * frame: radiance plus the mrt_render_aovs outputs (10 channels) in, radiance
+ * out.  BF16 weights and activations, 3x3 convolutions on MFMA with FP32
+ * accumulation.
+ *
+ * Network, for L levels of c[0..L-1] channels (multiples of 32, at most 256):
+ *   input  x0 = [log(1 + max(r * radiance_scale, 0)) (3), albedo (3),
+ *                normal (3), depth * depth_scale (1)], frame zero-padded
+ *                right and bottom to multiples of 2^(L-1);
+ *   encoder level l: e_l = relu(conv(relu(conv(in_l)))) with c[l] channels,
+ *                in_0 = x0, in_l = maxpool2x2(e_{l-1});
+ *   decoder level l = L-2 .. 0: d_l = relu(conv(relu(conv(
+ *                concat(upsample2x_nearest(d_{l+1}), e_l))))) with c[l]
+ *                channels, d_{L-1} = e_{L-1};
+ *   output y = conv(d_0) (3 channels, no activation); radiance
+ *                max(exp(y) - 1, 0), cropped to the frame.
+ * Weights: FP32, layer after layer in the order encoder 0..L-1 (two convs
+ * each), decoder L-2..0 (two each), output conv: each layer its weight
+ * [cout][cin][3][3] then its bias [cout] (input channels of a decoder's
+ * first conv: the upsampled channels, then the skip channels).  The library
+ * rounds them to BF16 (nearest even). */
+#define MRT_DENOISER_MAX_LEVELS 5
+typedef struct mrt_denoiser mrt_denoiser;
+
+typedef struct mrt_denoiser_desc {
+    uint32_t levels;                            /* 1..5 (0: 5)                       */
+    uint32_t channels[MRT_DENOISER_MAX_LEVELS]; /* per level (all 0: 32 64 96 128 128) */
+    float depth_scale;                          /* multiplies the depth AOV (0: 1)   */
+    uint32_t reserved[5];
+} mrt_denoiser_desc;
+
+/* Number of FP32 weights (with biases) of the network desc describes. */
+MRT_API int mrt_denoiser_weight_count(const mrt_denoiser_desc* desc, uint64_t* count);
+MRT_API int mrt_denoiser_create(mrt_ctx* ctx, const mrt_denoiser_desc* desc, const float* weights, uint64_t count,
+                                mrt_denoiser** out);
+/* Denoises a width x height frame: radiance (3 floats per pixel, multiplied
+ * by radiance_scale, e.g. 1/spp for an mrt_render accumulation) and aov (7
+ * floats per pixel, as mrt_render_aovs writes) in, out (3 floats per pixel)
+ * out; all three in host or device memory per mem.  Activations stay on the
+ * device between calls and grow with the largest frame seen. */
+MRT_API int mrt_denoiser_run(mrt_denoiser* dn, const float* radiance, float radiance_scale, const float* aov,
+                             uint32_t width, uint32_t height, float* out, int mem);
+/* Milliseconds of the last run's device work (convolutions and the
+ * packing kernels), from events around it. */
+MRT_API int mrt_denoiser_last_ms(const mrt_denoiser* dn, float* ms);
+MRT_API int mrt_denoiser_destroy(mrt_denoiser* dn);
+/* One layer of the network on its own: a 3x3 convolution (zero padding 1) of
+ * x (height x width x cin BF16, NHWC) with w ([3][3][cout][cin] BF16, tap
+ * (dy, dx) major) and bias (cout FP32, or NULL), ReLU when relu is set, into
+ * y (height x width x cout BF16).  cin and cout multiples of 32. */
+MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, uint32_t width, uint32_t cin,
+                             const uint16_t* w, const float* bias, uint32_t cout, int relu, uint16_t* y, int mem);
+
 /* ---- host-side construction helpers (no device work) ------------------- */
 /* Triangle constructors (Triangle.cpp:12-55): normals/colors may be NULL. */
 MRT_API int mrt_triangle_make(const float a[3], const float b[3], const float c[3], int32_t material_id,

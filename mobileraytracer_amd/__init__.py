@@ -333,3 +333,83 @@ class RenderServer:
         st = abi.ServingStats()
         check(lib().mrt_server_stats(self.h, C.byref(st)), "mrt_server_stats")
         return st
+
+
+# ---------------------------------------------------------------- denoiser --
+def denoiser_desc(levels: int = 5, channels=(32, 64, 96, 128, 128), depth_scale: float = 1.0) -> abi.DenoiserDesc:
+    d = abi.DenoiserDesc()
+    d.levels = levels
+    ch = list(channels)[:levels] + [0] * (5 - min(levels, len(channels)))
+    d.channels[:] = ch[:5]
+    d.depth_scale = depth_scale
+    return d
+
+
+def denoiser_weight_count(desc: abi.DenoiserDesc) -> int:
+    v = C.c_uint64()
+    check(lib().mrt_denoiser_weight_count(C.byref(desc), C.byref(v)), "mrt_denoiser_weight_count")
+    return v.value
+
+
+class Denoiser:
+    """The U-Net denoiser of mrt_denoiser_* (BF16 on MFMA).  `weights` is the
+    flat FP32 vector in the library's layer order ([cout][cin][3][3] then
+    bias, per layer)."""
+
+    def __init__(self, ctx: Context, weights: np.ndarray, desc: abi.DenoiserDesc | None = None):
+        self.ctx = ctx
+        self.desc = desc if desc is not None else denoiser_desc()
+        w = np.ascontiguousarray(weights, np.float32).ravel()
+        h = C.c_void_p()
+        check(lib().mrt_denoiser_create(ctx.h, C.byref(self.desc), w.ctypes.data, w.size, C.byref(h)),
+              "mrt_denoiser_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mrt_denoiser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, radiance, aov, radiance_scale: float = 1.0, out=None, mem: int = MEM_HOST, width=None,
+            height=None):
+        """Denoised radiance (height x width x 3) of a frame's radiance
+        (height x width x 3, times radiance_scale) and AOVs (height x width x 7)."""
+        if mem == MEM_HOST:
+            radiance = np.ascontiguousarray(radiance, np.float32)
+            aov = np.ascontiguousarray(aov, np.float32)
+            height, width = radiance.shape[:2]
+            if aov.shape[:2] != (height, width) or aov.shape[2] != 7 or radiance.shape[2] != 3:
+                raise ValueError("radiance must be (h, w, 3) and aov (h, w, 7)")
+            if out is None:
+                out = np.zeros((height, width, 3), np.float32)
+        check(lib().mrt_denoiser_run(self.h, ptr(radiance), radiance_scale, ptr(aov), width, height, ptr(out), mem),
+              "mrt_denoiser_run")
+        return out
+
+    def last_ms(self) -> float:
+        v = C.c_float()
+        check(lib().mrt_denoiser_last_ms(self.h, C.byref(v)), "mrt_denoiser_last_ms")
+        return v.value
+
+
+def conv3x3_bf16(ctx: Context, x: np.ndarray, w: np.ndarray, bias: np.ndarray | None, relu: bool) -> np.ndarray:
+    """One 3x3 convolution layer on the GPU: x (h, w, cin) and w (3, 3, cout,
+    cin) as BF16 bit patterns (uint16), bias FP32; returns (h, w, cout) BF16
+    bits."""
+    x = np.ascontiguousarray(x, np.uint16)
+    w = np.ascontiguousarray(w, np.uint16)
+    h, wd, cin = x.shape
+    cout = w.shape[2]
+    if w.shape != (3, 3, cout, cin):
+        raise ValueError("weights must be (3, 3, cout, cin)")
+    b = None if bias is None else np.ascontiguousarray(bias, np.float32)
+    y = np.zeros((h, wd, cout), np.uint16)
+    check(lib().mrt_conv3x3_bf16(ctx.h, x.ctypes.data, h, wd, cin, w.ctypes.data, ptr(b), cout, int(relu),
+                                 y.ctypes.data, MEM_HOST), "mrt_conv3x3_bf16")
+    return y

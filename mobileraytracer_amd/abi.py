@@ -133,6 +133,11 @@ class ServingStats(C.Structure):
                 ("batches", C.c_uint64)]
 
 
+class DenoiserDesc(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("channels", C.c_uint32 * 5), ("depth_scale", C.c_float),
+                ("reserved", C.c_uint32 * 5)]
+
+
 class AccelParams(C.Structure):
     _fields_ = [("camera_min", F3), ("camera_max", F3), ("leaf_size", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
@@ -195,6 +200,12 @@ def _proto(lib):
         "mrt_server_poll": (C.c_int, [P, u64, C.POINTER(JobInfo)]),
         "mrt_server_cancel": (C.c_int, [P, u64]),
         "mrt_server_stats": (C.c_int, [P, C.POINTER(ServingStats)]),
+        "mrt_denoiser_weight_count": (C.c_int, [C.POINTER(DenoiserDesc), C.POINTER(u64)]),
+        "mrt_denoiser_create": (C.c_int, [P, C.POINTER(DenoiserDesc), P, u64, C.POINTER(P)]),
+        "mrt_denoiser_run": (C.c_int, [P, P, f32, P, u32, u32, P, C.c_int]),
+        "mrt_denoiser_last_ms": (C.c_int, [P, C.POINTER(f32)]),
+        "mrt_denoiser_destroy": (C.c_int, [P]),
+        "mrt_conv3x3_bf16": (C.c_int, [P, P, u32, u32, u32, P, P, u32, C.c_int, P, C.c_int]),
         "mrt_triangle_make": (C.c_int, [P, P, P, i32, P, P, P]),
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
         "mrt_tile_values": (C.c_int, [u32, P]),

@@ -1,0 +1,357 @@
+// C ABI of the learned denoiser: weight conversion (FP32 [cout][cin][3][3] to
+// BF16 [tap][cout][cin], channels padded to multiples of 32), the activation
+// arena of a frame and the layer sequence of the U-Net.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "mrt_denoise.hpp"
+#include "mrt_internal.hpp"
+
+using mrt::DevBuf;
+using mrt::fail;
+
+namespace {
+
+constexpr uint32_t kInChannels = 10, kOutChannels = 3, kPad = 32, kMaxChannels = 256;
+
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    return static_cast<uint16_t>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+struct Net {
+    uint32_t levels = 5;
+    uint32_t c[MRT_DENOISER_MAX_LEVELS] = {32, 64, 96, 128, 128};
+    float depth_scale = 1.0f;
+};
+
+int resolve(const mrt_denoiser_desc* d, Net* n) {
+    if (!d) return MRT_OK;
+    if (d->levels > MRT_DENOISER_MAX_LEVELS) return fail(MRT_E_INVALID, "denoiser levels must be 1..5");
+    if (d->levels) n->levels = d->levels;
+    bool any = false;
+    for (uint32_t l = 0; l < MRT_DENOISER_MAX_LEVELS; ++l) any = any || d->channels[l];
+    if (any) {
+        for (uint32_t l = 0; l < n->levels; ++l) {
+            const uint32_t c = d->channels[l];
+            if (c == 0 || c % kPad || c > kMaxChannels)
+                return fail(MRT_E_INVALID, "denoiser channels must be multiples of 32 in 32..256");
+            n->c[l] = c;
+        }
+    }
+    if (d->depth_scale != 0.0f) {
+        if (!std::isfinite(d->depth_scale)) return fail(MRT_E_INVALID, "depth_scale must be finite");
+        n->depth_scale = d->depth_scale;
+    }
+    return MRT_OK;
+}
+
+struct LayerShape {
+    uint32_t cin, cout;  // logical
+};
+
+// Layers in weight order: encoder 0..L-1 (two each), decoder L-2..0 (two
+// each), output.
+std::vector<LayerShape> layer_shapes(const Net& n) {
+    std::vector<LayerShape> v;
+    const uint32_t L = n.levels;
+    for (uint32_t l = 0; l < L; ++l) {
+        v.push_back({l == 0 ? kInChannels : n.c[l - 1], n.c[l]});
+        v.push_back({n.c[l], n.c[l]});
+    }
+    for (int l = static_cast<int>(L) - 2; l >= 0; --l) {
+        v.push_back({n.c[l + 1] + n.c[l], n.c[l]});
+        v.push_back({n.c[l], n.c[l]});
+    }
+    v.push_back({n.c[0], kOutChannels});
+    return v;
+}
+
+uint32_t pad32(uint32_t c) { return (c + kPad - 1) / kPad * kPad; }
+
+uint64_t weight_count(const Net& n) {
+    uint64_t k = 0;
+    for (const LayerShape& s : layer_shapes(n)) k += static_cast<uint64_t>(s.cout) * s.cin * 9 + s.cout;
+    return k;
+}
+
+struct DevLayer {
+    uint32_t cin, cout;  // padded
+    size_t w_off, b_off; // elements into the weight / bias buffers
+};
+
+// Byte offsets of a frame's activations (H x W padded frame).
+struct Arena {
+    size_t x0 = 0, y32 = 0, total = 0;
+    size_t t[MRT_DENOISER_MAX_LEVELS] = {}, d[MRT_DENOISER_MAX_LEVELS] = {}, p[MRT_DENOISER_MAX_LEVELS] = {},
+           o[MRT_DENOISER_MAX_LEVELS] = {}, e_last = 0;
+};
+
+Arena plan(const Net& n, uint32_t H, uint32_t W) {
+    Arena a;
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t r = at;
+        at += (bytes + 255) / 256 * 256;
+        return r;
+    };
+    const uint32_t L = n.levels;
+    a.x0 = take(static_cast<size_t>(H) * W * kPad * 2);
+    for (uint32_t l = 0; l < L; ++l) {
+        const size_t px = static_cast<size_t>(H >> l) * (W >> l);
+        a.t[l] = take(px * n.c[l] * 2);
+        if (l + 1 < L) {
+            a.d[l] = take(px * (n.c[l + 1] + n.c[l]) * 2);
+            a.o[l] = take(px * n.c[l] * 2);
+        }
+        if (l > 0) a.p[l] = take(px * n.c[l - 1] * 2);
+    }
+    a.e_last = take(static_cast<size_t>(H >> (L - 1)) * (W >> (L - 1)) * n.c[L - 1] * 2);
+    a.y32 = take(static_cast<size_t>(H) * W * kOutChannels * 4);
+    a.total = at;
+    return a;
+}
+
+}  // namespace
+
+struct mrt_denoiser {
+    mrt_ctx* ctx = nullptr;
+    Net net;
+    std::vector<DevLayer> layers;
+    DevBuf weights, biases, act, stage_rad, stage_aov, stage_out;
+    hipEvent_t ev[2] = {};
+    float last_ms = 0.0f;
+    ~mrt_denoiser() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (ctx) --ctx->children;
+    }
+};
+
+namespace {
+
+int conv(mrt_denoiser* dn, size_t layer, const uint16_t* x, uint32_t x_stride, uint32_t x_off, uint16_t* y,
+         uint32_t y_stride, uint32_t y_off, float* y32, uint32_t H, uint32_t W, bool relu, hipStream_t st) {
+    const DevLayer& L = dn->layers[layer];
+    mrt::ConvArgs a{};
+    a.x = x;
+    a.x_stride = x_stride;
+    a.x_off = x_off;
+    a.cin = L.cin;
+    a.w = dn->weights.as<uint16_t>() + L.w_off;
+    a.bias = dn->biases.as<float>() + L.b_off;
+    a.cout = L.cout;
+    a.y = y;
+    a.y_stride = y_stride;
+    a.y_off = y_off;
+    a.y32 = y32;
+    a.y32_channels = kOutChannels;
+    a.H = H;
+    a.W = W;
+    a.relu = relu ? 1u : 0u;
+    MRT_HIP(mrt::launch_conv3x3(a, st));
+    return MRT_OK;
+}
+
+int run_net(mrt_denoiser* dn, const float* rad, float scale, const float* aov, uint32_t w, uint32_t h, float* out,
+            hipStream_t st) {
+    const Net& n = dn->net;
+    const uint32_t L = n.levels, m = (1u << (L - 1)) - 1u;
+    const uint32_t H = (h + m) & ~m, W = (w + m) & ~m;
+    const Arena a = plan(n, H, W);
+    int rc;
+    if ((rc = dn->act.alloc(a.total)) != MRT_OK) return rc;
+    char* base = dn->act.as<char>();
+    auto bf = [&](size_t off) { return reinterpret_cast<uint16_t*>(base + off); };
+    MRT_HIP(hipEventRecord(dn->ev[0], st));
+    MRT_HIP(mrt::launch_pack(rad, scale, aov, w, h, n.depth_scale, H, W, bf(a.x0), st));
+    size_t layer = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        const uint32_t Hl = H >> l, Wl = W >> l;
+        const uint16_t* in = l == 0 ? bf(a.x0) : bf(a.p[l]);
+        const uint32_t in_stride = l == 0 ? kPad : n.c[l - 1];
+        if ((rc = conv(dn, layer++, in, in_stride, 0, bf(a.t[l]), n.c[l], 0, nullptr, Hl, Wl, true, st)) != MRT_OK)
+            return rc;
+        if (l + 1 < L) {
+            const uint32_t ds = n.c[l + 1] + n.c[l];
+            if ((rc = conv(dn, layer++, bf(a.t[l]), n.c[l], 0, bf(a.d[l]), ds, n.c[l + 1], nullptr, Hl, Wl, true,
+                           st)) != MRT_OK)
+                return rc;
+            MRT_HIP(mrt::launch_pool2(bf(a.d[l]), ds, n.c[l + 1], n.c[l], Hl, Wl, bf(a.p[l + 1]), st));
+        } else {
+            if ((rc = conv(dn, layer++, bf(a.t[l]), n.c[l], 0, bf(a.e_last), n.c[l], 0, nullptr, Hl, Wl, true,
+                           st)) != MRT_OK)
+                return rc;
+        }
+    }
+    for (int l = static_cast<int>(L) - 2; l >= 0; --l) {
+        const uint32_t Hl = H >> l, Wl = W >> l, ds = n.c[l + 1] + n.c[l];
+        const uint16_t* src = l == static_cast<int>(L) - 2 ? bf(a.e_last) : bf(a.o[l + 1]);
+        MRT_HIP(mrt::launch_up2(src, n.c[l + 1], Hl, Wl, bf(a.d[l]), ds, 0, st));
+        if ((rc = conv(dn, layer++, bf(a.d[l]), ds, 0, bf(a.t[l]), n.c[l], 0, nullptr, Hl, Wl, true, st)) != MRT_OK)
+            return rc;
+        if ((rc = conv(dn, layer++, bf(a.t[l]), n.c[l], 0, bf(a.o[l]), n.c[l], 0, nullptr, Hl, Wl, true, st)) !=
+            MRT_OK)
+            return rc;
+    }
+    const uint16_t* last = L == 1 ? bf(a.e_last) : bf(a.o[0]);
+    float* y32 = reinterpret_cast<float*>(base + a.y32);
+    if ((rc = conv(dn, layer++, last, n.c[0], 0, nullptr, 0, 0, y32, H, W, false, st)) != MRT_OK) return rc;
+    MRT_HIP(mrt::launch_unpack(y32, kOutChannels, W, w, h, out, st));
+    MRT_HIP(hipEventRecord(dn->ev[1], st));
+    return MRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MRT_API int mrt_denoiser_weight_count(const mrt_denoiser_desc* desc, uint64_t* count) {
+    if (!count) return fail(MRT_E_INVALID, "null argument");
+    Net n;
+    int rc;
+    if ((rc = resolve(desc, &n)) != MRT_OK) return rc;
+    *count = weight_count(n);
+    return MRT_OK;
+}
+
+MRT_API int mrt_denoiser_create(mrt_ctx* ctx, const mrt_denoiser_desc* desc, const float* weights, uint64_t count,
+                                mrt_denoiser** out) {
+    if (!ctx || !weights || !out) return fail(MRT_E_INVALID, "null argument");
+    *out = nullptr;
+    Net n;
+    int rc;
+    if ((rc = resolve(desc, &n)) != MRT_OK) return rc;
+    if (count != weight_count(n))
+        return fail(MRT_E_INVALID, "weight count " + std::to_string(count) + " does not match the network's " +
+                                       std::to_string(weight_count(n)));
+    for (uint64_t i = 0; i < count; ++i)
+        if (!std::isfinite(weights[i])) return fail(MRT_E_INVALID, "non-finite denoiser weight");
+    std::vector<uint16_t> wdev;
+    std::vector<float> bdev;
+    std::vector<DevLayer> layers;
+    const float* src = weights;
+    for (const LayerShape& s : layer_shapes(n)) {
+        DevLayer d{pad32(s.cin), pad32(s.cout), wdev.size(), bdev.size()};
+        wdev.resize(wdev.size() + static_cast<size_t>(9) * d.cout * d.cin, 0);
+        bdev.resize(bdev.size() + d.cout, 0.0f);
+        for (uint32_t o = 0; o < s.cout; ++o)
+            for (uint32_t i = 0; i < s.cin; ++i)
+                for (uint32_t tap = 0; tap < 9; ++tap)
+                    wdev[d.w_off + (static_cast<size_t>(tap) * d.cout + o) * d.cin + i] =
+                        bf16_rne(src[(static_cast<size_t>(o) * s.cin + i) * 9 + tap]);
+        src += static_cast<size_t>(s.cout) * s.cin * 9;
+        for (uint32_t o = 0; o < s.cout; ++o) bdev[d.b_off + o] = src[o];
+        src += s.cout;
+        layers.push_back(d);
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    auto dn = std::make_unique<mrt_denoiser>();
+    dn->net = n;
+    dn->layers = std::move(layers);
+    for (hipEvent_t& e : dn->ev) MRT_HIP(hipEventCreate(&e));
+    if ((rc = dn->weights.upload(wdev.data(), wdev.size() * 2, ctx->stream)) != MRT_OK) return rc;
+    if ((rc = dn->biases.upload(bdev.data(), bdev.size() * 4, ctx->stream)) != MRT_OK) return rc;
+    MRT_HIP(hipStreamSynchronize(ctx->stream));
+    dn->ctx = ctx;
+    ++ctx->children;
+    *out = dn.release();
+    return MRT_OK;
+}
+
+MRT_API int mrt_denoiser_run(mrt_denoiser* dn, const float* radiance, float radiance_scale, const float* aov,
+                             uint32_t width, uint32_t height, float* out, int mem) {
+    if (!dn || !radiance || !aov || !out) return fail(MRT_E_INVALID, "null argument");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(MRT_E_INVALID, "frame size must be 1..16384 per side");
+    if (!std::isfinite(radiance_scale)) return fail(MRT_E_INVALID, "radiance_scale must be finite");
+    mrt_ctx* ctx = dn->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    const size_t px = static_cast<size_t>(width) * height;
+    const float *drad = radiance, *daov = aov;
+    float* dout = out;
+    int rc;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = dn->stage_rad.upload(radiance, px * 12, st)) != MRT_OK) return rc;
+        if ((rc = dn->stage_aov.upload(aov, px * 28, st)) != MRT_OK) return rc;
+        if ((rc = dn->stage_out.alloc(px * 12)) != MRT_OK) return rc;
+        drad = dn->stage_rad.as<float>();
+        daov = dn->stage_aov.as<float>();
+        dout = dn->stage_out.as<float>();
+    }
+    if ((rc = run_net(dn, drad, radiance_scale, daov, width, height, dout, st)) != MRT_OK) return rc;
+    if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(out, dout, px * 12, hipMemcpyDeviceToHost, st));
+    MRT_HIP(hipStreamSynchronize(st));
+    MRT_HIP(hipEventElapsedTime(&dn->last_ms, dn->ev[0], dn->ev[1]));
+    return MRT_OK;
+}
+
+MRT_API int mrt_denoiser_last_ms(const mrt_denoiser* dn, float* ms) {
+    if (!dn || !ms) return fail(MRT_E_INVALID, "null argument");
+    *ms = dn->last_ms;
+    return MRT_OK;
+}
+
+MRT_API int mrt_denoiser_destroy(mrt_denoiser* dn) {
+    if (!dn) return MRT_OK;
+    {
+        std::lock_guard<std::mutex> lock(dn->ctx->mu);
+        (void)hipSetDevice(dn->ctx->device);
+        (void)hipStreamSynchronize(dn->ctx->stream);
+    }
+    delete dn;
+    return MRT_OK;
+}
+
+MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, uint32_t width, uint32_t cin,
+                             const uint16_t* w, const float* bias, uint32_t cout, int relu, uint16_t* y, int mem) {
+    if (!ctx || !x || !w || !y) return fail(MRT_E_INVALID, "null argument");
+    if (cin == 0 || cout == 0 || cin % kPad || cout % kPad || cin > 1024 || cout > 1024)
+        return fail(MRT_E_INVALID, "cin and cout must be multiples of 32 up to 1024");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(MRT_E_INVALID, "frame size must be 1..16384 per side");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    const size_t px = static_cast<size_t>(width) * height;
+    DevBuf bx, bw, bb, by;
+    const uint16_t *dx = x, *dw = w;
+    const float* db = bias;
+    uint16_t* dy = y;
+    int rc;
+    if (mem != MRT_MEM_DEVICE) {
+        if ((rc = bx.upload(x, px * cin * 2, st)) != MRT_OK) return rc;
+        if ((rc = bw.upload(w, static_cast<size_t>(9) * cout * cin * 2, st)) != MRT_OK) return rc;
+        if (bias && (rc = bb.upload(bias, cout * 4, st)) != MRT_OK) return rc;
+        if ((rc = by.alloc(px * cout * 2)) != MRT_OK) return rc;
+        dx = bx.as<uint16_t>();
+        dw = bw.as<uint16_t>();
+        db = bias ? bb.as<float>() : nullptr;
+        dy = by.as<uint16_t>();
+    }
+    mrt::ConvArgs a{};
+    a.x = dx;
+    a.x_stride = cin;
+    a.cin = cin;
+    a.w = dw;
+    a.bias = db;
+    a.cout = cout;
+    a.y = dy;
+    a.y_stride = cout;
+    a.H = height;
+    a.W = width;
+    a.relu = relu ? 1u : 0u;
+    MRT_HIP(mrt::launch_conv3x3(a, st));
+    if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(y, dy, px * cout * 2, hipMemcpyDeviceToHost, st));
+    MRT_HIP(hipStreamSynchronize(st));
+    return MRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+"""GPU denoiser (mrt_conv3x3_bf16, mrt_denoiser_* through the C ABI) against
+the PyTorch FP32 statement of the same network on the CPU.
+
+Tolerances:
+  * one convolution layer: |gpu - ref| <= 2^-8 |ref| + 2^-16 S per output,
+    ref the FP32 convolution of the same BF16 inputs and weights, S the same
+    sum over |x| |w| (2^-8 covers the BF16 rounding of the output, 2^-16 S
+    the FP32 accumulation order);
+  * the whole network: reference with BF16 rounding at the library's points;
+    the log-space outputs within 2e-2 relative L2 and 6e-2 of their largest
+    magnitude at any pixel (a rounding that falls the other way in one layer
+    moves later layers by about one BF16 step);
+  * host and device memory runs are bit-identical.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mobileraytracer_amd as mrt
+from helpers import DeviceArray
+from mobileraytracer_amd import abi, denoise
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    # Imported once the library is loaded: torch's wheel carries a HIP
+    # runtime of the same soname, and the first one loaded serves both.
+    return pytest.importorskip("torch")
+
+
+def conv_ref(x, w, bias, relu):
+    """x (h, w, cin) FP32, w (3, 3, cout, cin) FP32: FP32 conv (torch, CPU) and
+    the sum of |x||w| per output."""
+    torch = _torch()
+    xt = torch.from_numpy(np.ascontiguousarray(x.transpose(2, 0, 1)))[None]
+    wt = torch.from_numpy(np.ascontiguousarray(w.transpose(2, 3, 0, 1)))
+    b = None if bias is None else torch.from_numpy(bias)
+    y = torch.nn.functional.conv2d(xt, wt, b, padding=1)
+    s = torch.nn.functional.conv2d(xt.abs(), wt.abs(), None if b is None else b.abs(), padding=1)
+    if relu:
+        y = torch.relu(y)
+    return y[0].numpy().transpose(1, 2, 0), s[0].numpy().transpose(1, 2, 0)
+
+
+@pytest.mark.parametrize("h,w,cin,cout,relu,bias", [
+    (1, 1, 32, 32, False, True),
+    (5, 70, 32, 32, True, True),
+    (9, 64, 64, 96, False, False),
+    (17, 130, 96, 64, True, True),
+    (3, 200, 256, 128, True, True),
+])
+def test_conv3x3_matches_fp32(gpu_ctx, h, w, cin, cout, relu, bias):
+    rng = np.random.default_rng(h * 1000 + cin)
+    x = denoise.bf16_round(rng.normal(size=(h, w, cin)).astype(np.float32))
+    wt = denoise.bf16_round((rng.normal(size=(3, 3, cout, cin)) / np.sqrt(9 * cin)).astype(np.float32))
+    b = rng.normal(size=cout).astype(np.float32) if bias else None
+    y = mrt.conv3x3_bf16(gpu_ctx, denoise.bf16_bits(x), denoise.bf16_bits(wt), b, relu)
+    got = denoise.from_bf16_bits(y).reshape(h, w, cout)
+    ref, s = conv_ref(x, wt, b, relu)
+    err = np.abs(got - ref)
+    tol = 2.0 ** -8 * np.abs(ref) + 2.0 ** -16 * s
+    assert np.all(err <= tol), f"max excess {np.max(err - tol)} at {np.unravel_index(np.argmax(err - tol), err.shape)}"
+    if relu:
+        assert np.all(got >= 0)
+
+
+def test_conv3x3_rejects_bad_shapes(gpu_ctx):
+    x = np.zeros((4, 4, 48), np.uint16)
+    with pytest.raises(abi.MrtError):
+        mrt.conv3x3_bf16(gpu_ctx, x, np.zeros((3, 3, 32, 48), np.uint16), None, False)
+
+
+def random_net(levels, channels, seed):
+    torch = _torch()
+    torch.manual_seed(seed)
+    model = denoise.make_unet(levels, channels)
+    # Biases around zero and He-scaled weights keep the activations O(1).
+    with torch.no_grad():
+        for m in model.convs:
+            torch.nn.init.kaiming_normal_(m.weight, nonlinearity="relu")
+            m.bias.normal_(0.0, 0.05)
+    return model
+
+
+def random_frame(h, w, seed):
+    rng = np.random.default_rng(seed)
+    rad = (rng.exponential(size=(h, w, 3)) * 2.0).astype(np.float32)
+    aov = np.concatenate([rng.uniform(0, 1, (h, w, 3)), rng.normal(size=(h, w, 3)),
+                          rng.uniform(0.2, 3.0, (h, w, 1))], 2).astype(np.float32)
+    return rad, aov
+
+
+@pytest.mark.parametrize("levels,channels,h,w", [
+    (5, denoise.DEFAULT_CHANNELS, 37, 53),
+    (1, (32,), 7, 9),
+    (3, (32, 64, 32), 64, 40),
+    (2, (64, 32), 16, 130),
+])
+def test_denoiser_matches_reference(gpu_ctx, levels, channels, h, w):
+    model = random_net(levels, channels, levels)
+    flat = denoise.flat_weights(model)
+    rad, aov = random_frame(h, w, levels + 10)
+    scale, ds = 0.25, 0.5
+    dn = mrt.Denoiser(gpu_ctx, flat, mrt.denoiser_desc(levels, channels, ds))
+    try:
+        got = dn.run(rad, aov, radiance_scale=scale)
+    finally:
+        dn.close()
+    ref = denoise.reference_denoise(model, rad, aov, scale, ds, bf16=True)
+    assert got.shape == (h, w, 3)
+    lg, lr = np.log1p(got), np.log1p(ref)
+    rel = np.linalg.norm(lg - lr) / max(np.linalg.norm(lr), 1e-6)
+    worst = np.max(np.abs(lg - lr)) / max(np.max(np.abs(lr)), 1e-6)
+    assert rel <= 2e-2 and worst <= 6e-2, (rel, worst)
+
+
+def test_denoiser_device_memory_matches_host(gpu_ctx):
+    model = random_net(5, denoise.DEFAULT_CHANNELS, 3)
+    rad, aov = random_frame(48, 80, 3)
+    dn = mrt.Denoiser(gpu_ctx, denoise.flat_weights(model))
+    try:
+        host = dn.run(rad, aov, radiance_scale=0.5)
+        bufs = [DeviceArray(a.nbytes) for a in (rad, aov, host)]
+        try:
+            bufs[0].upload(rad)
+            bufs[1].upload(aov)
+            dn.run(bufs[0], bufs[1], radiance_scale=0.5, out=bufs[2], mem=mrt.MEM_DEVICE, width=80, height=48)
+            dev = bufs[2].download(np.zeros_like(host))
+        finally:
+            for b in bufs:
+                b.free()
+        assert np.array_equal(host, dev)
+        # A smaller frame after a larger one reuses the activations.
+        small = dn.run(rad[:20, :30], aov[:20, :30], radiance_scale=0.5)
+        again = dn.run(rad[:20, :30], aov[:20, :30], radiance_scale=0.5)
+        assert np.array_equal(small, again)
+        assert dn.last_ms() > 0
+    finally:
+        dn.close()
+
+
+def test_denoiser_rejects_wrong_weight_count(gpu_ctx):
+    with pytest.raises(abi.MrtError):
+        mrt.Denoiser(gpu_ctx, np.zeros(1000, np.float32))
