@@ -20,6 +20,12 @@ window; p50/p99 request latency and the window's Mrays/s go into the line's
 "serving" object.  --serve-sweep runs only the serving workload over a list of
 rates and prints one JSON line per rate.
 
+Denoiser leg (after the timed frames, outside their timing): the BF16 U-Net
+(mrt_denoiser_run, trained weights) over a 1920x1080 frame of FP32 radiance
+and AOVs resident in HBM; device time per frame and the MFMA roofline of the
+network (algorithmic FLOPs / time) go into the line's "denoiser" object.
+--denoise runs only this leg and prints its own line.
+
 Prints one JSON line (see the repository's benchmark contract).
 """
 from __future__ import annotations
@@ -39,6 +45,7 @@ import numpy as np  # noqa: E402
 W, H, SPP = 1920, 1080, 16
 N_TRIS, N_LIGHTS = 1_000_000, 10_000
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg: str) -> None:
@@ -136,6 +143,8 @@ def main() -> int:
     ap.add_argument("--serve-max-tris", type=int, default=8_000_000)
     ap.add_argument("--serve-sweep", type=str, default="",
                     help="comma-separated rates: run only the serving workload, one JSON line per rate")
+    ap.add_argument("--denoise", action="store_true", help="run only the denoiser leg (1920x1080, trained weights)")
+    ap.add_argument("--denoise-reps", type=int, default=20)
     ap.add_argument("--batched", action="store_true",
                     help="run only the batched workload: 64 scenes (10k-4M triangles), one request each, all at once")
     args = ap.parse_args()
@@ -153,6 +162,14 @@ def main() -> int:
 
     if args.batched:
         return batched(mrt, abi)
+
+    if args.denoise:
+        ctx = mrt.Context(0)
+        res = denoiser_leg(mrt, abi, ctx, torch, args.denoise_reps)
+        print(json.dumps({"metric": "denoiser ms per 1920x1080 frame (BF16 U-Net on MFMA)",
+                          "value": res["ms_per_frame"], "unit": "ms", "higher_is_better": False, "denoiser": res}),
+              flush=True)
+        return 0
 
     if args.serve_sweep:
         ctx = mrt.Context(0)
@@ -290,6 +307,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt)
         log(f"cpu baseline took {time.time() - t0:.1f} s")
         out["vs_cpu"] = round(mrays / out["cpu_baseline"]["value"], 1)
+    out["denoiser"] = denoiser_leg(mrt, abi, ctx, torch, args.denoise_reps)
     if not args.no_serve:
         scene.close()
         del bitmap, accum
@@ -298,6 +316,39 @@ def main() -> int:
         srv.close()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def denoiser_leg(mrt, abi, ctx, torch, reps: int) -> dict:
+    """The trained U-Net over a synthetic 1920x1080 frame whose FP32
+    radiance and AOVs are resident in HBM; device time from the library's
+    events around the network's kernels (median of `reps` runs)."""
+    from mobileraytracer_amd import denoise
+
+    flat, levels, channels, ds = denoise.load_weights()
+    dn = mrt.Denoiser(ctx, flat, mrt.denoiser_desc(levels, channels, ds))
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    rad = (torch.rand((H, W, 3), generator=g) * 2.0).to(dev)
+    aov = torch.rand((H, W, 7), generator=g).to(dev)
+    out = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dn.run(rad, aov, 1.0, out=out, mem=abi.MEM_DEVICE, width=W, height=H)
+    times = []
+    for _ in range(reps):
+        dn.run(rad, aov, 1.0, out=out, mem=abi.MEM_DEVICE, width=W, height=H)
+        times.append(dn.last_ms())
+    dn.close()
+    ms = float(np.median(times))
+    flops = denoise.algorithmic_flops(H, W, levels, channels)
+    tflops = flops / (ms / 1000.0) / 1e12
+    log(f"denoiser: {ms:.3f} ms per {W}x{H} frame (median of {reps}), {tflops:.1f} TFLOP/s algorithmic")
+    return {"ms_per_frame": round(ms, 4), "ms_min": round(min(times), 4), "reps": reps,
+            "workload": f"U-Net levels={levels} channels={list(channels)} {W}x{H}, BF16 weights/activations, FP32 "
+                        "accumulation, inputs resident in HBM",
+            "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "flops_per_frame": flops, "kernel": "whole network (k_conv3x3 + pack/pool/up/unpack)"}}
 
 
 def batched(mrt, abi) -> int:

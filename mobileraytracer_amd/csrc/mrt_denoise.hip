@@ -1,20 +1,31 @@
-// Denoiser kernels: BF16 3x3 convolutions on MFMA (FP32 accumulation), 2x2
-// max pooling, 2x nearest upsampling and the input / output packing of a
-// U-Net over the renderer's radiance and first-hit AOVs.
+// Denoiser kernels: BF16 3x3 convolutions on MFMA (FP32 accumulation) and
+// 2x2 max pooling for a U-Net over the renderer's radiance and first-hit
+// AOVs.
 //
 // Tensors are NHWC BF16 (uint16 storage): element (y, x, c) of a buffer with
-// `stride` channels per pixel sits at (y * W + x) * stride + c, so a layer can
-// read or write a channel slice [off, off + channels) of a wider buffer (the
-// skip concatenations are slices of one buffer).
+// `stride` channels per pixel sits at (y * W + x) * stride + c.
 //
-// Convolution = implicit GEMM: C[pixel][cout] = sum over the 9 taps and the
-// input channels of X[neighbour pixel][cin] * Wt[tap][cout][cin].  One wave
-// computes 64 consecutive pixels of a row by 32 output channels as 4 x 2
-// tiles of v_mfma_f32_16x16x32_bf16 (16 pixels x 16 channels, K = 32 input
-// channels of one tap): lane l holds pixel / output channel (l % 16) and the 8
-// consecutive input channels 8 * (l / 16) .. + 7 of the K step, one 16-byte
-// load each; the 4 accumulators of a tile are output rows 4 * (l / 16) .. + 3
-// of column l % 16.
+// Convolution = implicit GEMM: C[cout][pixel] = sum over the 9 taps and the
+// input channels of Wt[tap][cout][cin] * X[neighbour pixel][cin], on
+// v_mfma_f32_16x16x32_bf16 (16 output channels x 16 pixels, K = 32 input
+// channels of one tap).  A 256-thread workgroup computes a tile of 4 rows x
+// 64 pixels x NT output channels (wave w: row w, NT/16 x 4 MFMA tiles).  Per
+// 32-channel chunk the input rows of the tile with a one-pixel halo (6 x 66
+// pixels) are staged in LDS once for the 4 waves, 80 bytes per pixel (64
+// bytes of channels + 16 of padding: consecutive pixels then start 20 banks
+// apart, so the 16-byte fragment reads of 8 or 16 lanes hit distinct banks).
+// The chunk's weights (9 taps x NT output channels) go to LDS the same way
+// for NT = 64; for NT = 32 each wave loads them into registers (72 VGPRs),
+// which leaves the LDS to the input tile and more workgroups per CU.  Lane l
+// holds output channel / pixel (l % 16) and input channels 8 * (l / 16) .. + 7
+// of a K step; the 4 accumulators of a tile are output channels
+// 4 * (l / 16) .. + 3 of pixel l % 16, stored as one 8-byte write.
+//
+// The decoder's concatenation [upsampled coarser level, skip] is never
+// written: chunks of the upsampled part are staged from the coarser tensor at
+// (y / 2, x / 2).  The network input is packed from FP32 radiance and AOVs
+// while staging the first convolution, and the last convolution writes FP32
+// radiance.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -28,13 +39,16 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
-__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+__device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
 
 // Round to nearest even (finite inputs).
-__device__ __forceinline__ uint16_t f_to_bf16(float f) {
+__device__ __forceinline__ uint32_t f_to_bf16(float f) {
     const uint32_t u = __float_as_uint(f);
-    return static_cast<uint16_t>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
 }
+
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return f_to_bf16(lo) | (f_to_bf16(hi) << 16); }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
     bf16x8 r;
@@ -42,154 +56,199 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
     return r;
 }
 
-__global__ __launch_bounds__(64) void k_conv3x3(ConvArgs a) {
-    const uint32_t lane = threadIdx.x;
-    const uint32_t x0 = blockIdx.x * 64u;
-    const uint32_t y = blockIdx.y;
-    const uint32_t n0 = blockIdx.z * 32u;
-    const uint32_t li = lane & 15u, lk = (lane >> 4) * 8u;
-    floatx4 acc[4][2];
-    for (int m = 0; m < 4; ++m)
-        for (int n = 0; n < 2; ++n) acc[m][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int tap = 0; tap < 9; ++tap) {
-        const int yy = static_cast<int>(y) + tap / 3 - 1;
-        const bool rowIn = yy >= 0 && yy < static_cast<int>(a.H);
-        const uint16_t* wt = a.w + (static_cast<size_t>(tap) * a.cout + n0 + li) * a.cin + lk;
-        for (uint32_t kc = 0; kc < a.cin; kc += 32u) {
-            uint4 av[4];
-            for (int m = 0; m < 4; ++m) {
-                const int xx = static_cast<int>(x0 + 16u * m + li) + tap % 3 - 1;
-                const bool in = rowIn && xx >= 0 && xx < static_cast<int>(a.W);
-                av[m] = make_uint4(0u, 0u, 0u, 0u);
-                if (in)
-                    av[m] = *reinterpret_cast<const uint4*>(
-                        a.x + (static_cast<size_t>(yy) * a.W + xx) * a.x_stride + a.x_off + kc + lk);
-            }
-            const uint4 b0 = *reinterpret_cast<const uint4*>(wt + kc);
-            const uint4 b1 = *reinterpret_cast<const uint4*>(wt + static_cast<size_t>(16u) * a.cin + kc);
-            for (int m = 0; m < 4; ++m) {
-                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[m]), as_bf16x8(b0), acc[m][0], 0, 0, 0);
-                acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[m]), as_bf16x8(b1), acc[m][1], 0, 0, 0);
-            }
-        }
-    }
-    for (int n = 0; n < 2; ++n) {
-        const uint32_t co = n0 + 16u * n + li;
-        const float b = a.bias ? a.bias[co] : 0.0f;
-        for (int m = 0; m < 4; ++m) {
-            for (int v = 0; v < 4; ++v) {
-                const uint32_t x = x0 + 16u * m + (lane >> 4) * 4u + v;
-                if (x >= a.W) continue;
-                float r = acc[m][n][v] + b;
-                if (a.relu) r = fmaxf(r, 0.0f);
-                const size_t px = static_cast<size_t>(y) * a.W + x;
-                if (a.y32) {
-                    if (co < a.y32_channels) a.y32[px * a.y32_channels + co] = r;
-                } else {
-                    a.y[px * a.y_stride + a.y_off + co] = f_to_bf16(r);
-                }
-            }
-        }
-    }
-}
+constexpr uint32_t kTileRows = 4, kTileW = 64, kHaloW = kTileW + 2, kRowQuads = 5;
+constexpr uint32_t kTileQuads = (kTileRows + 2) * kHaloW * kRowQuads;
 
-// 2x2 max pooling of channels [off, off + c) of x (H x W) into y (H/2 x W/2,
-// c channels).
-__global__ void k_pool2(const uint16_t* __restrict__ x, uint32_t x_stride, uint32_t x_off, uint32_t c, uint32_t H,
-                        uint32_t W, uint16_t* __restrict__ y) {
-    const uint32_t W2 = W / 2u, H2 = H / 2u;
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= static_cast<size_t>(H2) * W2 * c) return;
-    const uint32_t ch = i % c;
-    const size_t p = i / c;
-    const uint32_t px = p % W2, py = static_cast<uint32_t>(p / W2);
-    float m = -INFINITY;
-    for (uint32_t dy = 0; dy < 2; ++dy)
-        for (uint32_t dx = 0; dx < 2; ++dx)
-            m = fmaxf(m, bf16_to_f(x[((2u * py + dy) * static_cast<size_t>(W) + 2u * px + dx) * x_stride + x_off + ch]));
-    y[i] = f_to_bf16(m);
-}
-
-// 2x nearest upsampling of x (H/2 x W/2, c channels) into channels [off, off +
-// c) of y (H x W).
-__global__ void k_up2(const uint16_t* __restrict__ x, uint32_t c, uint32_t H, uint32_t W, uint16_t* __restrict__ y,
-                      uint32_t y_stride, uint32_t y_off) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= static_cast<size_t>(H) * W * c) return;
-    const uint32_t ch = i % c;
-    const size_t p = i / c;
-    const uint32_t px = p % W, py = static_cast<uint32_t>(p / W);
-    y[p * y_stride + y_off + ch] = x[((py / 2u) * static_cast<size_t>(W / 2u) + px / 2u) * c + ch];
-}
-
-// Network input: radiance * rad_scale (3 floats) as log(1 + max(x, 0)),
-// albedo, normal and depth * depth_scale (7 floats) as BF16 channels 0..9 of
-// a 32-channel NHWC tensor (channels 10..31 zero).  The frame (w x h) sits in
-// the top-left corner of the zero-padded H x W tensor.
-__global__ void k_pack(const float* __restrict__ rad, float rad_scale, const float* __restrict__ aov, uint32_t w,
-                       uint32_t h, float depth_scale, uint32_t H, uint32_t W, uint16_t* __restrict__ x) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= static_cast<size_t>(H) * W) return;
-    const uint32_t px = i % W, py = static_cast<uint32_t>(i / W);
-    uint16_t* o = x + i * 32u;
+// The 32 packed input channels of pixel (yy, xx) as 4 quads of 8 BF16.
+__device__ __forceinline__ void pack_pixel(const ConvArgs& a, int yy, int xx, uint4* q) {
     float v[10] = {};
-    if (px < w && py < h) {
-        const size_t s = static_cast<size_t>(py) * w + px;
-        for (int k = 0; k < 3; ++k) v[k] = log1pf(fmaxf(rad[3 * s + k] * rad_scale, 0.0f));
-        for (int k = 0; k < 6; ++k) v[3 + k] = aov[7 * s + k];
-        v[9] = aov[7 * s + 6] * depth_scale;
+    if (yy < static_cast<int>(a.fh) && xx < static_cast<int>(a.fw)) {
+        const size_t s = static_cast<size_t>(yy) * a.fw + xx;
+        for (int k = 0; k < 3; ++k) v[k] = log1pf(fmaxf(a.rad[3 * s + k] * a.rad_scale, 0.0f));
+        for (int k = 0; k < 6; ++k) v[3 + k] = a.aov[7 * s + k];
+        v[9] = a.aov[7 * s + 6] * a.depth_scale;
     }
-    for (int k = 0; k < 32; ++k) o[k] = k < 10 ? f_to_bf16(v[k]) : static_cast<uint16_t>(0);
+    q[0] = make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+    q[1] = make_uint4(pack2(v[8], v[9]), 0u, 0u, 0u);
+    q[2] = make_uint4(0u, 0u, 0u, 0u);
+    q[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// Network output (FP32, `c` channels per pixel, channels 0..2 used) back to
-// radiance: exp(y) - 1, clamped at 0, cropped to w x h.
-__global__ void k_unpack(const float* __restrict__ y, uint32_t c, uint32_t W, uint32_t w, uint32_t h,
-                         float* __restrict__ out) {
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+    constexpr int NJ = NT / 16;
+    constexpr bool kWeightsLds = NT > 32;
+    __shared__ uint4 smem[kTileQuads + (kWeightsLds ? 9 * NT * kRowQuads : 0)];
+    uint4* tile = smem;
+    uint4* wts = smem + kTileQuads;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const uint32_t li = lane & 15u, lg = lane >> 4;
+    const int x0 = static_cast<int>(blockIdx.x * kTileW), y0 = static_cast<int>(blockIdx.y * kTileRows);
+    const uint32_t n0 = blockIdx.z * NT;
+    const int H = static_cast<int>(a.H), W = static_cast<int>(a.W);
+    floatx4 acc[NJ][4];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[j][m] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t kc = 0; kc < a.cin; kc += 32u) {
+        uint4 wr[kWeightsLds ? 1 : 9][NJ];
+        if constexpr (!kWeightsLds) {
+            const uint16_t* wk = a.w + static_cast<size_t>(n0 + li) * a.cin + kc + lg * 8u;
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+                    wr[tap][j] =
+                        *reinterpret_cast<const uint4*>(wk + (static_cast<size_t>(tap) * a.cout + 16u * j) * a.cin);
+        }
+        __syncthreads();
+        if (a.rad) {
+            for (uint32_t px = tid; px < (kTileRows + 2) * kHaloW; px += 256u) {
+                const uint32_t r = px / kHaloW, c = px % kHaloW;
+                const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
+                uint4 q[4] = {};
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) pack_pixel(a, yy, xx, q);
+                for (int k = 0; k < 4; ++k) tile[px * kRowQuads + k] = q[k];
+            }
+        } else {
+            const bool up = kc < a.cu;
+            for (uint32_t p = tid; p < (kTileRows + 2) * kHaloW * 4u; p += 256u) {
+                const uint32_t q = p & 3u, px = p >> 2;
+                const uint32_t r = px / kHaloW, c = px % kHaloW;
+                const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const uint16_t* src =
+                        up ? a.xu + (static_cast<size_t>(yy >> 1) * (a.W >> 1) + (xx >> 1)) * a.cu + kc
+                           : a.x + (static_cast<size_t>(yy) * a.W + xx) * a.x_stride + a.x_off + (kc - a.cu);
+                    v = *reinterpret_cast<const uint4*>(src + q * 8u);
+                }
+                tile[px * kRowQuads + q] = v;
+            }
+        }
+        if constexpr (kWeightsLds) {
+            for (uint32_t p = tid; p < 9u * NT * 4u; p += 256u) {
+                const uint32_t q = p & 3u, row = p >> 2;  // row = tap * NT + n
+                const uint32_t tap = row / NT, n = row % NT;
+                wts[row * kRowQuads + q] = *reinterpret_cast<const uint4*>(
+                    a.w + (static_cast<size_t>(tap) * a.cout + n0 + n) * a.cin + kc + q * 8u);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+            uint4 bv[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                if constexpr (kWeightsLds)
+                    bv[j] = wts[(tap * NT + 16 * j + li) * kRowQuads + lg];
+                else
+                    bv[j] = wr[tap][j];
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint4 av = tile[((wv + dy) * kHaloW + 16u * m + li + dx) * kRowQuads + lg];
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+                    acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(bv[j]), as_bf16x8(av), acc[j][m], 0,
+                                                                        0, 0);
+            }
+        }
+    }
+    const int y = y0 + static_cast<int>(wv);
+    if (y >= H) return;
+    if (a.rad_out) {
+        // Output channels 0..2 (lanes 0..15 of the first channel tile):
+        // radiance.
+        if (n0 != 0 || lg != 0u || y >= static_cast<int>(a.fh)) return;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int x = x0 + 16 * m + static_cast<int>(li);
+            if (x >= static_cast<int>(a.fw)) continue;
+            float* o = a.rad_out + (static_cast<size_t>(y) * a.fw + x) * 3;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) o[v] = fmaxf(expm1f(acc[0][m][v] + (a.bias ? a.bias[v] : 0.0f)), 0.0f);
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const uint32_t co = n0 + 16u * j + 4u * lg;
+        float b[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) b[v] = a.bias ? a.bias[co + v] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int x = x0 + 16 * m + static_cast<int>(li);
+            if (x >= W) continue;
+            float r[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                r[v] = acc[j][m][v] + b[v];
+                if (a.relu) r[v] = fmaxf(r[v], 0.0f);
+            }
+            *reinterpret_cast<uint2*>(a.y + (static_cast<size_t>(y) * a.W + x) * a.y_stride + a.y_off + co) =
+                make_uint2(pack2(r[0], r[1]), pack2(r[2], r[3]));
+        }
+    }
+}
+
+// 2x2 max pooling, 8 channels (16 bytes) per thread.
+__global__ void k_pool2(const uint16_t* __restrict__ x, uint32_t c, uint32_t H, uint32_t W, uint16_t* __restrict__ y) {
+    const uint32_t W2 = W / 2u, H2 = H / 2u, cq = c / 8u;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= static_cast<size_t>(w) * h) return;
-    const uint32_t px = i % w, py = static_cast<uint32_t>(i / w);
-    const float* s = y + (static_cast<size_t>(py) * W + px) * c;
-    for (int k = 0; k < 3; ++k) out[3 * i + k] = fmaxf(expm1f(s[k]), 0.0f);
+    if (i >= static_cast<size_t>(H2) * W2 * cq) return;
+    const uint32_t q = i % cq;
+    const size_t p = i / cq;
+    const uint32_t px = p % W2, py = static_cast<uint32_t>(p / W2);
+    uint4 s[4];
+    for (uint32_t k = 0; k < 4; ++k)
+        s[k] = *reinterpret_cast<const uint4*>(
+            x + ((2u * py + (k >> 1)) * static_cast<size_t>(W) + 2u * px + (k & 1u)) * c + q * 8u);
+    uint32_t o[4];
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t v0 = (&s[0].x)[e], v1 = (&s[1].x)[e], v2 = (&s[2].x)[e], v3 = (&s[3].x)[e];
+        const float lo = fmaxf(fmaxf(bf16_lo(v0), bf16_lo(v1)), fmaxf(bf16_lo(v2), bf16_lo(v3)));
+        const float hi = fmaxf(fmaxf(bf16_hi(v0), bf16_hi(v1)), fmaxf(bf16_hi(v2), bf16_hi(v3)));
+        // Maxima of BF16 values are BF16 values: no rounding.
+        o[e] = (__float_as_uint(lo) >> 16) | (__float_as_uint(hi) & 0xFFFF0000u);
+    }
+    *reinterpret_cast<uint4*>(y + i * 8u) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-uint32_t blocks_for(size_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
+template <int NT>
+void conv_launch(const ConvArgs& a, dim3 grid, hipStream_t stream) {
+    k_conv3x3<NT><<<grid, 256, 0, stream>>>(a);
+}
 
 }  // namespace
 
-hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t stream) {
-    if (a.cin % 32u || a.cout % 32u || a.x_off % 8u || a.x_stride % 8u || a.x_off + a.cin > a.x_stride)
+hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t stream) {
+    if (a.cin == 0 || a.cin % 32u || a.cout % 32u || a.cu % 32u || a.cu > a.cin || a.x_off % 8u || a.x_stride % 8u)
         return hipErrorInvalidValue;
-    if (!a.y32 && (a.y_off % 8u || a.y_stride % 8u || a.y_off + a.cout > a.y_stride)) return hipErrorInvalidValue;
-    dim3 grid((a.W + 63u) / 64u, a.H, a.cout / 32u);
-    k_conv3x3<<<grid, 64, 0, stream>>>(a);
+    if (a.rad ? (a.cin != 32u || a.cu != 0u || !a.aov || a.fw > a.W || a.fh > a.H)
+              : (a.cin > a.cu && (!a.x || a.x_off + (a.cin - a.cu) > a.x_stride)))
+        return hipErrorInvalidValue;
+    if (a.cu && (a.H % 2u || a.W % 2u || !a.xu)) return hipErrorInvalidValue;
+    if (a.rad_out ? (a.fw > a.W || a.fh > a.H)
+                  : (!a.y || a.y_off % 4u || a.y_stride % 4u || a.y_off + a.cout > a.y_stride))
+        return hipErrorInvalidValue;
+    const uint32_t tiles = ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileRows - 1) / kTileRows);
+    const uint32_t nt = (a.cout % 64u == 0 && tiles * (a.cout / 64u) >= min_blocks) ? 64u : 32u;
+    dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileRows - 1) / kTileRows, a.cout / nt);
+    if (nt == 64u)
+        conv_launch<64>(a, grid, stream);
+    else
+        conv_launch<32>(a, grid, stream);
     return hipGetLastError();
 }
 
-hipError_t launch_pool2(const uint16_t* x, uint32_t x_stride, uint32_t x_off, uint32_t c, uint32_t H, uint32_t W,
-                        uint16_t* y, hipStream_t stream) {
-    const size_t n = static_cast<size_t>(H / 2u) * (W / 2u) * c;
-    k_pool2<<<blocks_for(n, 256), 256, 0, stream>>>(x, x_stride, x_off, c, H, W, y);
-    return hipGetLastError();
-}
-
-hipError_t launch_up2(const uint16_t* x, uint32_t c, uint32_t H, uint32_t W, uint16_t* y, uint32_t y_stride,
-                      uint32_t y_off, hipStream_t stream) {
-    const size_t n = static_cast<size_t>(H) * W * c;
-    k_up2<<<blocks_for(n, 256), 256, 0, stream>>>(x, c, H, W, y, y_stride, y_off);
-    return hipGetLastError();
-}
-
-hipError_t launch_pack(const float* rad, float rad_scale, const float* aov, uint32_t w, uint32_t h,
-                       float depth_scale, uint32_t H, uint32_t W, uint16_t* x, hipStream_t stream) {
-    k_pack<<<blocks_for(static_cast<size_t>(H) * W, 256), 256, 0, stream>>>(rad, rad_scale, aov, w, h, depth_scale, H,
-                                                                           W, x);
-    return hipGetLastError();
-}
-
-hipError_t launch_unpack(const float* y, uint32_t c, uint32_t W, uint32_t w, uint32_t h, float* out,
-                         hipStream_t stream) {
-    k_unpack<<<blocks_for(static_cast<size_t>(w) * h, 256), 256, 0, stream>>>(y, c, W, w, h, out);
+hipError_t launch_pool2(const uint16_t* x, uint32_t c, uint32_t H, uint32_t W, uint16_t* y, hipStream_t stream) {
+    if (c % 8u || H % 2u || W % 2u) return hipErrorInvalidValue;
+    const size_t n = static_cast<size_t>(H / 2u) * (W / 2u) * (c / 8u);
+    k_pool2<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, stream>>>(x, c, H, W, y);
     return hipGetLastError();
 }
 

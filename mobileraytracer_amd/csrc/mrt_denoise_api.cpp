@@ -84,10 +84,12 @@ struct DevLayer {
     size_t w_off, b_off; // elements into the weight / bias buffers
 };
 
-// Byte offsets of a frame's activations (H x W padded frame).
+// Byte offsets of a frame's activations (H x W padded frame): per level l,
+// t = first conv's output, s = encoder output (the skip), p = pooled input
+// (l >= 1), o = decoder output; e_last = the coarsest level's output.
 struct Arena {
-    size_t x0 = 0, y32 = 0, total = 0;
-    size_t t[MRT_DENOISER_MAX_LEVELS] = {}, d[MRT_DENOISER_MAX_LEVELS] = {}, p[MRT_DENOISER_MAX_LEVELS] = {},
+    size_t total = 0;
+    size_t t[MRT_DENOISER_MAX_LEVELS] = {}, s[MRT_DENOISER_MAX_LEVELS] = {}, p[MRT_DENOISER_MAX_LEVELS] = {},
            o[MRT_DENOISER_MAX_LEVELS] = {}, e_last = 0;
 };
 
@@ -100,18 +102,16 @@ Arena plan(const Net& n, uint32_t H, uint32_t W) {
         return r;
     };
     const uint32_t L = n.levels;
-    a.x0 = take(static_cast<size_t>(H) * W * kPad * 2);
     for (uint32_t l = 0; l < L; ++l) {
         const size_t px = static_cast<size_t>(H >> l) * (W >> l);
         a.t[l] = take(px * n.c[l] * 2);
         if (l + 1 < L) {
-            a.d[l] = take(px * (n.c[l + 1] + n.c[l]) * 2);
+            a.s[l] = take(px * n.c[l] * 2);
             a.o[l] = take(px * n.c[l] * 2);
         }
         if (l > 0) a.p[l] = take(px * n.c[l - 1] * 2);
     }
     a.e_last = take(static_cast<size_t>(H >> (L - 1)) * (W >> (L - 1)) * n.c[L - 1] * 2);
-    a.y32 = take(static_cast<size_t>(H) * W * kOutChannels * 4);
     a.total = at;
     return a;
 }
@@ -134,27 +134,26 @@ struct mrt_denoiser {
 
 namespace {
 
-int conv(mrt_denoiser* dn, size_t layer, const uint16_t* x, uint32_t x_stride, uint32_t x_off, uint16_t* y,
-         uint32_t y_stride, uint32_t y_off, float* y32, uint32_t H, uint32_t W, bool relu, hipStream_t st) {
+// One layer: input x (x_channels per pixel) or, with xu, the concatenation
+// [up2(xu) (cu channels), x]; output y (layer's cout channels per pixel).
+mrt::ConvArgs layer_args(const mrt_denoiser* dn, size_t layer, const uint16_t* x, uint32_t x_channels,
+                         const uint16_t* xu, uint32_t cu, uint16_t* y, uint32_t H, uint32_t W) {
     const DevLayer& L = dn->layers[layer];
     mrt::ConvArgs a{};
     a.x = x;
-    a.x_stride = x_stride;
-    a.x_off = x_off;
+    a.x_stride = x_channels;
     a.cin = L.cin;
+    a.xu = xu;
+    a.cu = cu;
     a.w = dn->weights.as<uint16_t>() + L.w_off;
     a.bias = dn->biases.as<float>() + L.b_off;
     a.cout = L.cout;
     a.y = y;
-    a.y_stride = y_stride;
-    a.y_off = y_off;
-    a.y32 = y32;
-    a.y32_channels = kOutChannels;
+    a.y_stride = L.cout;
     a.H = H;
     a.W = W;
-    a.relu = relu ? 1u : 0u;
-    MRT_HIP(mrt::launch_conv3x3(a, st));
-    return MRT_OK;
+    a.relu = 1;
+    return a;
 }
 
 int run_net(mrt_denoiser* dn, const float* rad, float scale, const float* aov, uint32_t w, uint32_t h, float* out,
@@ -162,46 +161,46 @@ int run_net(mrt_denoiser* dn, const float* rad, float scale, const float* aov, u
     const Net& n = dn->net;
     const uint32_t L = n.levels, m = (1u << (L - 1)) - 1u;
     const uint32_t H = (h + m) & ~m, W = (w + m) & ~m;
+    const uint32_t min_blocks = 2u * dn->ctx->num_cu;
     const Arena a = plan(n, H, W);
     int rc;
     if ((rc = dn->act.alloc(a.total)) != MRT_OK) return rc;
     char* base = dn->act.as<char>();
     auto bf = [&](size_t off) { return reinterpret_cast<uint16_t*>(base + off); };
     MRT_HIP(hipEventRecord(dn->ev[0], st));
-    MRT_HIP(mrt::launch_pack(rad, scale, aov, w, h, n.depth_scale, H, W, bf(a.x0), st));
     size_t layer = 0;
     for (uint32_t l = 0; l < L; ++l) {
         const uint32_t Hl = H >> l, Wl = W >> l;
-        const uint16_t* in = l == 0 ? bf(a.x0) : bf(a.p[l]);
-        const uint32_t in_stride = l == 0 ? kPad : n.c[l - 1];
-        if ((rc = conv(dn, layer++, in, in_stride, 0, bf(a.t[l]), n.c[l], 0, nullptr, Hl, Wl, true, st)) != MRT_OK)
-            return rc;
-        if (l + 1 < L) {
-            const uint32_t ds = n.c[l + 1] + n.c[l];
-            if ((rc = conv(dn, layer++, bf(a.t[l]), n.c[l], 0, bf(a.d[l]), ds, n.c[l + 1], nullptr, Hl, Wl, true,
-                           st)) != MRT_OK)
-                return rc;
-            MRT_HIP(mrt::launch_pool2(bf(a.d[l]), ds, n.c[l + 1], n.c[l], Hl, Wl, bf(a.p[l + 1]), st));
-        } else {
-            if ((rc = conv(dn, layer++, bf(a.t[l]), n.c[l], 0, bf(a.e_last), n.c[l], 0, nullptr, Hl, Wl, true,
-                           st)) != MRT_OK)
-                return rc;
+        mrt::ConvArgs c = layer_args(dn, layer++, l == 0 ? nullptr : bf(a.p[l]), l == 0 ? 0u : n.c[l - 1], nullptr,
+                                     0, bf(a.t[l]), Hl, Wl);
+        if (l == 0) {
+            c.rad = rad;
+            c.aov = aov;
+            c.rad_scale = scale;
+            c.depth_scale = n.depth_scale;
+            c.fw = w;
+            c.fh = h;
         }
+        MRT_HIP(mrt::launch_conv3x3(c, min_blocks, st));
+        uint16_t* e = l + 1 < L ? bf(a.s[l]) : bf(a.e_last);
+        MRT_HIP(mrt::launch_conv3x3(layer_args(dn, layer++, bf(a.t[l]), n.c[l], nullptr, 0, e, Hl, Wl), min_blocks,
+                                    st));
+        if (l + 1 < L) MRT_HIP(mrt::launch_pool2(e, n.c[l], Hl, Wl, bf(a.p[l + 1]), st));
     }
     for (int l = static_cast<int>(L) - 2; l >= 0; --l) {
-        const uint32_t Hl = H >> l, Wl = W >> l, ds = n.c[l + 1] + n.c[l];
-        const uint16_t* src = l == static_cast<int>(L) - 2 ? bf(a.e_last) : bf(a.o[l + 1]);
-        MRT_HIP(mrt::launch_up2(src, n.c[l + 1], Hl, Wl, bf(a.d[l]), ds, 0, st));
-        if ((rc = conv(dn, layer++, bf(a.d[l]), ds, 0, bf(a.t[l]), n.c[l], 0, nullptr, Hl, Wl, true, st)) != MRT_OK)
-            return rc;
-        if ((rc = conv(dn, layer++, bf(a.t[l]), n.c[l], 0, bf(a.o[l]), n.c[l], 0, nullptr, Hl, Wl, true, st)) !=
-            MRT_OK)
-            return rc;
+        const uint32_t Hl = H >> l, Wl = W >> l;
+        const uint16_t* coarse = l == static_cast<int>(L) - 2 ? bf(a.e_last) : bf(a.o[l + 1]);
+        MRT_HIP(mrt::launch_conv3x3(layer_args(dn, layer++, bf(a.s[l]), n.c[l], coarse, n.c[l + 1], bf(a.t[l]), Hl, Wl),
+                                    min_blocks, st));
+        MRT_HIP(mrt::launch_conv3x3(layer_args(dn, layer++, bf(a.t[l]), n.c[l], nullptr, 0, bf(a.o[l]), Hl, Wl),
+                                    min_blocks, st));
     }
-    const uint16_t* last = L == 1 ? bf(a.e_last) : bf(a.o[0]);
-    float* y32 = reinterpret_cast<float*>(base + a.y32);
-    if ((rc = conv(dn, layer++, last, n.c[0], 0, nullptr, 0, 0, y32, H, W, false, st)) != MRT_OK) return rc;
-    MRT_HIP(mrt::launch_unpack(y32, kOutChannels, W, w, h, out, st));
+    mrt::ConvArgs c = layer_args(dn, layer++, L == 1 ? bf(a.e_last) : bf(a.o[0]), n.c[0], nullptr, 0, nullptr, H, W);
+    c.relu = 0;
+    c.rad_out = out;
+    c.fw = w;
+    c.fh = h;
+    MRT_HIP(mrt::launch_conv3x3(c, min_blocks, st));
     MRT_HIP(hipEventRecord(dn->ev[1], st));
     return MRT_OK;
 }
@@ -348,7 +347,7 @@ MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, u
     a.H = height;
     a.W = width;
     a.relu = relu ? 1u : 0u;
-    MRT_HIP(mrt::launch_conv3x3(a, st));
+    MRT_HIP(mrt::launch_conv3x3(a, 2u * ctx->num_cu, st));
     if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(y, dy, px * cout * 2, hipMemcpyDeviceToHost, st));
     MRT_HIP(hipStreamSynchronize(st));
     return MRT_OK;

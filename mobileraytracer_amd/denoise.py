@@ -46,6 +46,17 @@ def weight_count(levels: int = 5, channels=DEFAULT_CHANNELS) -> int:
     return sum(co * ci * 9 + co for ci, co in layer_shapes(levels, channels))
 
 
+def algorithmic_flops(height: int, width: int, levels: int = 5, channels=DEFAULT_CHANNELS) -> int:
+    """Multiply-adds x 2 of the network's convolutions over a frame padded to
+    multiples of 2^(levels-1), with the logical channel counts (10 in, 3
+    out; the library pads both to 32)."""
+    m = 1 << (levels - 1)
+    H, W = (height + m - 1) // m * m, (width + m - 1) // m * m
+    shapes = layer_shapes(levels, channels)
+    lvl = [l for l in range(levels) for _ in range(2)] + [l for l in range(levels - 2, -1, -1) for _ in range(2)] + [0]
+    return sum(2 * 9 * ci * co * (H >> l) * (W >> l) for (ci, co), l in zip(shapes, lvl))
+
+
 def bf16_round(a: np.ndarray) -> np.ndarray:
     """FP32 -> nearest-even BF16, returned as FP32 (finite inputs)."""
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
