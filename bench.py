@@ -143,6 +143,8 @@ def main() -> int:
     ap.add_argument("--serve-max-tris", type=int, default=8_000_000)
     ap.add_argument("--serve-sweep", type=str, default="",
                     help="comma-separated rates: run only the serving workload, one JSON line per rate")
+    ap.add_argument("--builder", choices=["auto", "host", "gpu"], default="auto",
+                    help="BVH builder of the scene (auto: GPU from 65536 triangles)")
     ap.add_argument("--denoise", action="store_true", help="run only the denoiser leg (1920x1080, trained weights)")
     ap.add_argument("--denoise-reps", type=int, default=20)
     ap.add_argument("--batched", action="store_true",
@@ -189,9 +191,12 @@ def main() -> int:
     log(f"scene generated in {time.time() - t0:.1f} s")
     ctx = mrt.Context(0)
     scene = mrt.Scene(ctx, data)
-    t0 = time.time()
-    scene.build(abi.BVH)
-    log(f"BVH built and uploaded in {time.time() - t0:.1f} s, footprint {scene.footprint() / 1e6:.1f} MB")
+    builder = {"auto": abi.BUILDER_AUTO, "host": abi.BUILDER_HOST, "gpu": abi.BUILDER_GPU}[args.builder]
+    t0 = time.perf_counter()
+    scene.build(abi.BVH, builder=builder)
+    build_s = time.perf_counter() - t0
+    log(f"BVH built and uploaded in {build_s:.3f} s ({args.builder} builder), footprint "
+        f"{scene.footprint() / 1e6:.1f} MB")
 
     dev = torch.device("cuda:0")
     bitmap = torch.zeros((H, W), dtype=torch.int32, device=dev)
@@ -307,6 +312,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt)
         log(f"cpu baseline took {time.time() - t0:.1f} s")
         out["vs_cpu"] = round(mrays / out["cpu_baseline"]["value"], 1)
+    out["scene_build"] = build_leg(mrt, abi, ctx, data, args.builder, build_s)
     out["denoiser"] = denoiser_leg(mrt, abi, ctx, torch, args.denoise_reps)
     if not args.no_serve:
         scene.close()
@@ -316,6 +322,37 @@ def main() -> int:
         srv.close()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def build_leg(mrt, abi, ctx, data, builder: str, build_s: float) -> dict:
+    """Reference BVH of the benchmark scene's 990k triangles built on the
+    host (mrt_bvh_build) and on the GPU (mrt_bvh_build_gpu, boxes and nodes
+    in host memory, so the copies are included): wall clock and whether the
+    node arrays are identical; and the whole timed scene build above."""
+    import ctypes as C
+
+    n = len(data.triangles)
+    boxes = np.zeros((n, 6), np.float32)
+    abi.check(abi.lib().mrt_primitive_aabbs(abi.CAT_TRIANGLE, data.triangles.ctypes.data, n, boxes.ctypes.data),
+              "mrt_primitive_aabbs")
+    cnt = C.c_uint32()
+    host = np.zeros((2 * n - 1, 8), np.uint32)
+    gpu = np.zeros_like(host)
+    t0 = time.perf_counter()
+    abi.check(abi.lib().mrt_bvh_build(boxes.ctypes.data, n, 2, host.ctypes.data, len(host), C.byref(cnt)),
+              "mrt_bvh_build")
+    host_ms = (time.perf_counter() - t0) * 1e3
+    nh = cnt.value
+    gpu_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        abi.check(abi.lib().mrt_bvh_build_gpu(ctx.h, boxes.ctypes.data, n, 2, gpu.ctypes.data, len(gpu), C.byref(cnt),
+                                              abi.MEM_HOST), "mrt_bvh_build_gpu")
+        gpu_ms.append((time.perf_counter() - t0) * 1e3)
+    same = cnt.value == nh and np.array_equal(host[:nh], gpu[:nh])
+    log(f"reference BVH of {n} triangles: host {host_ms:.1f} ms, GPU {min(gpu_ms):.1f} ms, identical {same}")
+    return {"builder": builder, "scene_build_ms": round(build_s * 1e3, 1), "bvh_nodes": nh,
+            "bvh_host_ms": round(host_ms, 1), "bvh_gpu_ms": round(min(gpu_ms), 1), "bvh_identical": bool(same)}
 
 
 def denoiser_leg(mrt, abi, ctx, torch, reps: int) -> dict:

@@ -229,8 +229,18 @@ typedef struct mrt_accel_params {
     float camera_min[3];       /* Camera::getAABB(), joins the grid bounds     */
     float camera_max[3];       /* (app/MobileRT/Shader.cpp:52-62)              */
     uint32_t leaf_size;        /* BVH maxLeafSize (2 in BVH.hpp:19)            */
-    uint32_t reserved[3];
+    uint32_t builder;          /* mrt_builder: where the BVH is built          */
+    uint32_t reserved[2];
 } mrt_accel_params;
+/* BVH builder of mrt_scene_build.  HOST: the reference builder's restatement
+ * and a binned-SAH 4-wide walk tree over its leaves.  GPU: the same binary
+ * tree built on the GPU (mrt_bvh_build_gpu) and a 4-wide walk tree collapsed
+ * from its pairs (each wide node = a pair with its interior children
+ * expanded); both walk trees give the same hits (every interior box is the
+ * exact union of the reference leaves below it).  AUTO: GPU from 65536
+ * triangles on, HOST below.  A GPU build that declines (NaN coordinates,
+ * more than 1024 levels above 64 primitives) falls back to HOST. */
+enum mrt_builder { MRT_BUILDER_AUTO = 0, MRT_BUILDER_HOST = 1, MRT_BUILDER_GPU = 2 };
 
 typedef struct mrt_ctx mrt_ctx;
 typedef struct mrt_scene mrt_scene;
@@ -468,6 +478,17 @@ typedef struct mrt_bvh_node {
  * node array.  Writes up to `capacity` nodes and the node count. */
 MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, mrt_bvh_node* nodes,
                           uint32_t capacity, uint32_t* count);
+/* The same builder on the GPU (no counterpart in the reference: its build is
+ * sequential): the node array of mrt_bvh_build, bit for bit, built level by
+ * level (segmented scans of the box unions, the SAH cost of every split
+ * position in parallel, one atomic minimum per node) with subtrees of at
+ * most 64 primitives built by one thread each.  boxes6 and nodes in host or
+ * device memory per mem; up to `capacity` nodes are written (nodes may be
+ * NULL) and *count is the array's length.  MRT_E_UNSUPPORTED when a box has
+ * a NaN coordinate or the tree has more than 1024 levels of nodes above 64
+ * primitives (mrt_bvh_build covers both). */
+MRT_API int mrt_bvh_build_gpu(mrt_ctx* ctx, const float* boxes6, uint32_t n, uint32_t leaf_size,
+                              mrt_bvh_node* nodes, uint32_t capacity, uint32_t* count, int mem);
 /* The 4-wide tree the persistent trace walks above a set of leaves (box and
  * non-zero leaf word each; scene builds pass the reference BVH's leaves), as
  * built for rays with a finite inverse direction: 32 floats per node (min x,

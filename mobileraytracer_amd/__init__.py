@@ -199,11 +199,13 @@ class Scene:
         except Exception:
             pass
 
-    def build(self, accelerator: int, camera_min=(0, 0, 0), camera_max=(0, 0, 0), leaf_size: int = 2) -> None:
+    def build(self, accelerator: int, camera_min=(0, 0, 0), camera_max=(0, 0, 0), leaf_size: int = 2,
+              builder: int = abi.BUILDER_AUTO) -> None:
         p = abi.AccelParams()
         p.camera_min[:] = list(camera_min)
         p.camera_max[:] = list(camera_max)
         p.leaf_size = leaf_size
+        p.builder = builder
         check(lib().mrt_scene_build(self.h, accelerator, C.byref(p)), "mrt_scene_build")
 
     def footprint(self) -> int:

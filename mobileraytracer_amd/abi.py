@@ -33,6 +33,7 @@ LIGHT_AREA, LIGHT_POINT = 0, 1
 CAMERA_PERSPECTIVE, CAMERA_ORTHOGRAPHIC = 0, 1
 KERNEL_AUTO, KERNEL_MEGAKERNEL, KERNEL_WAVEFRONT = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
+BUILDER_AUTO, BUILDER_HOST, BUILDER_GPU = 0, 1, 2
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 CORNELL_TRIANGLES, CORNELL_MATERIALS, CORNELL_LIGHTS = 34, 7, 2
 
@@ -139,7 +140,8 @@ class DenoiserDesc(C.Structure):
 
 
 class AccelParams(C.Structure):
-    _fields_ = [("camera_min", F3), ("camera_max", F3), ("leaf_size", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+    _fields_ = [("camera_min", F3), ("camera_max", F3), ("leaf_size", C.c_uint32), ("builder", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 def np_dtype(struct) -> np.dtype:
@@ -214,6 +216,7 @@ def _proto(lib):
         "mrt_primitive_aabbs": (C.c_int, [C.c_int, P, u32, P]),
         "mrt_bvh_build": (C.c_int, [P, u32, u32, P, u32, C.POINTER(u32)]),
         "mrt_wide_build": (C.c_int, [P, P, u32, P, u32, C.POINTER(u32), C.POINTER(u32)]),
+        "mrt_bvh_build_gpu": (C.c_int, [P, P, u32, u32, P, u32, C.POINTER(u32), C.c_int]),
         "mrt_scene_cornell": (C.c_int, [u32, P, P, P]),
         "mrt_scene_soup": (C.c_int, [u32, u32, u32, f32, f32, P, P, C.c_int]),
         "mrt_scene_icospheres": (C.c_int, [u32, u32, u32, f32, u32, f32, P, P, C.c_int]),

@@ -136,7 +136,7 @@ int wide_slots(const std::vector<BNode>& w, uint32_t pi, WideSlot* out) {
 // BVH's leaves (each kept with its own box); the light part collapses the
 // light tree's pairs (each interior child replaced by its two children).
 // Nodes: 0 and 1 the roots, then the triangle nodes, then the light nodes.
-int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri, hipStream_t st) {
+int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri, hipStream_t st, bool collapse) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     // Light slots of root 0: the light root's children (interior root), the
     // root itself (a leaf: the linear light scan), or none.
@@ -150,51 +150,74 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri
             lightTop.push_back(pair_child(w[c.ref], 1));
         }
     }
-    // Triangle units: the reference leaves.
-    std::vector<WideSlot> units;
-    if (!s->tris.empty()) {
-        if (tri.root_count > 0) {
-            units.push_back(WideSlot{tri.root, tri.root_ref, leaf_word(tri.root_ref, tri.root_count)});
-        } else {
-            for (const BNode& p : tri.pairs) {
-                for (int side = 0; side < 2; ++side) {
-                    const WideSlot c = pair_child(p, side);
-                    if (c.word != 0u) units.push_back(WideSlot{c.box, c.ref, leaf_word(c.ref, c.word)});
+    // Depth-first collapse of the pairs below `top` into wide nodes numbered
+    // from `first` (each a pair with its interior children expanded); top
+    // slots sit at level 2, below the roots.
+    std::vector<uint32_t> index(w.size(), kNone);
+    uint32_t maxDepth = 1u;
+    auto collapse_pairs = [&](std::vector<WideSlot>* top, uint32_t first, std::vector<float4>* nodes) {
+        uint32_t next = first;
+        std::vector<std::pair<uint32_t, uint32_t>> todo;  // (pair, level)
+        auto map_slot = [&](WideSlot c, uint32_t level) {
+            if (c.word != 0u || std::isnan(c.box.mn.x)) return c;
+            if (index[c.ref] == kNone) {
+                index[c.ref] = next++;
+                todo.emplace_back(c.ref, level);
+            }
+            c.ref = index[c.ref];
+            return c;
+        };
+        for (WideSlot& c : *top) c = map_slot(c, 2u);
+        while (!todo.empty()) {
+            const auto [pi, level] = todo.back();
+            todo.pop_back();
+            maxDepth = std::max(maxDepth, level);
+            WideSlot sl[4];
+            const int n = wide_slots(w, pi, sl);
+            for (int k = 0; k < n; ++k) sl[k] = map_slot(sl[k], level + 1u);
+            put_wide_node(nodes, index[pi] - first, sl, n);
+        }
+        return next - first;
+    };
+    std::vector<float4> triNodes, out(16, make_float4(0, 0, 0, 0));
+    std::vector<WideSlot> triTop;
+    uint32_t nTriNodes = 0;
+    if (collapse) {
+        // Triangle top: the triangle root of root pair 0, expanded when
+        // interior; then its pairs collapsed.
+        if (!is_nan_box(w[kWalkClosestRoot], 0)) {
+            const WideSlot c = pair_child(w[kWalkClosestRoot], 0);
+            if (c.word != 0u) {
+                triTop.push_back(c);
+            } else {
+                triTop.push_back(pair_child(w[c.ref], 0));
+                triTop.push_back(pair_child(w[c.ref], 1));
+            }
+        }
+        nTriNodes = collapse_pairs(&triTop, kWalkFirstPair, &triNodes);
+    } else {
+        // Triangle units: the reference leaves, under a binned-SAH tree.
+        std::vector<WideSlot> units;
+        if (!s->tris.empty()) {
+            if (tri.root_count > 0) {
+                units.push_back(WideSlot{tri.root, tri.root_ref, leaf_word(tri.root_ref, tri.root_count)});
+            } else {
+                for (const BNode& p : tri.pairs) {
+                    for (int side = 0; side < 2; ++side) {
+                        const WideSlot c = pair_child(p, side);
+                        if (c.word != 0u) units.push_back(WideSlot{c.box, c.ref, leaf_word(c.ref, c.word)});
+                    }
                 }
             }
         }
+        uint32_t triDepth = 0;
+        build_wide_sah(units, kWalkFirstPair, 4 - static_cast<int>(lightTop.size()), &triNodes, &triTop, &triDepth);
+        nTriNodes = static_cast<uint32_t>(triNodes.size() / 8);
+        maxDepth = 1u + triDepth;
     }
-    std::vector<float4> triNodes, out(16, make_float4(0, 0, 0, 0));
-    std::vector<WideSlot> triTop;
-    uint32_t triDepth = 0;
-    build_wide_sah(units, kWalkFirstPair, 4 - static_cast<int>(lightTop.size()), &triNodes, &triTop, &triDepth);
-    const uint32_t nTriNodes = static_cast<uint32_t>(triNodes.size() / 8);
     const uint32_t lightNode = kWalkFirstPair + nTriNodes;
-    // Light nodes: pairs below the light top slots, depth first.
-    std::vector<uint32_t> index(w.size(), kNone);
-    uint32_t next = lightNode;
-    std::vector<std::pair<uint32_t, uint32_t>> todo;  // (pair, level)
-    auto map_slot = [&](WideSlot c, uint32_t level) {
-        if (c.word != 0u || std::isnan(c.box.mn.x)) return c;
-        if (index[c.ref] == kNone) {
-            index[c.ref] = next++;
-            todo.emplace_back(c.ref, level);
-        }
-        c.ref = index[c.ref];
-        return c;
-    };
-    for (WideSlot& c : lightTop) c = map_slot(c, 2u);
-    uint32_t maxDepth = 1u + triDepth;
     std::vector<float4> lightNodes;
-    while (!todo.empty()) {
-        const auto [pi, level] = todo.back();
-        todo.pop_back();
-        maxDepth = std::max(maxDepth, level);
-        WideSlot sl[4];
-        const int n = wide_slots(w, pi, sl);
-        for (int k = 0; k < n; ++k) sl[k] = map_slot(sl[k], level + 1u);
-        put_wide_node(&lightNodes, index[pi] - lightNode, sl, n);
-    }
+    collapse_pairs(&lightTop, lightNode, &lightNodes);
     // Roots: 0 = triangle top + light top, 1 = triangle top (shadow rays).
     WideSlot r0[4], r1[4];
     int n0 = 0, n1 = 0;
@@ -218,7 +241,7 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri
 // Builds the combined walk tree of the persistent kernels (layout in
 // mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 27/5-bit leaf
 // word get none; they run the non-persistent kernels.
-int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
+int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st, bool collapse) {
     s->dev.walk_nodes = nullptr;
     const uint64_t slots = s->tris.size() + static_cast<uint64_t>(s->n_light_slots);
     if (slots >= (1ull << kSlotBits)) return MRT_OK;
@@ -252,7 +275,7 @@ int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st) {
     s->dev.walk_light_pair = lightBase;
     s->dev.walk_light_slot = nTri;
     s->depth_need = std::max(s->depth_need, std::max(tri.depth, s->light_bvh.depth) + 1u);
-    if ((rc = build_wide_tree(s, w, tri, st)) != MRT_OK) return rc;
+    if ((rc = build_wide_tree(s, w, tri, st, collapse)) != MRT_OK) return rc;
     MRT_HIP(hipStreamSynchronize(st));
     return MRT_OK;
 }
@@ -577,6 +600,31 @@ bool wavefront_eligible(const mrt_scene* s, const mrt_render_desc* d) {
 
 }  // namespace mrt
 
+// The reference BVH over `boxes` built on the GPU (the node array of
+// build_bvh_reference, bit for bit) with its child pairs; *done stays false
+// when the GPU build declines (the caller builds on the host).  Callers hold
+// ctx->mu.
+int build_bvh_gpu(mrt_ctx* ctx, const std::vector<Box>& boxes, uint32_t leaf, BvhOut* out, hipStream_t st,
+                  bool* done) {
+    static_assert(sizeof(Box) == 24 && sizeof(RefNode) == 32, "box / node layouts");
+    *done = false;
+    const uint32_t n = static_cast<uint32_t>(boxes.size());
+    int rc;
+    if (n && (rc = ctx->bvh_boxes.upload(boxes.data(), boxes.size() * sizeof(Box), st)) != MRT_OK) return rc;
+    if ((rc = ctx->bvh_nodes.alloc((n ? 2 * static_cast<size_t>(n) - 1 : 1) * sizeof(RefNode))) != MRT_OK) return rc;
+    uint32_t cnt = 0;
+    const hipError_t e =
+        gpu_build_bvh(ctx->bvh_boxes.as<float>(), n, leaf, ctx->bvh_nodes.as<RefNode>(), &cnt, &ctx->bvh_scratch, st);
+    if (e == hipErrorNotSupported) return MRT_OK;
+    MRT_HIP(e);
+    out->ref.resize(cnt);
+    MRT_HIP(hipMemcpyAsync(out->ref.data(), ctx->bvh_nodes.p, cnt * sizeof(RefNode), hipMemcpyDeviceToHost, st));
+    MRT_HIP(hipStreamSynchronize(st));
+    bvh_pairs_from_ref(out);
+    *done = true;
+    return MRT_OK;
+}
+
 extern "C" {
 
 MRT_API const char* mrt_last_error(void) { return g_error.c_str(); }
@@ -622,6 +670,37 @@ MRT_API int mrt_ctx_destroy(mrt_ctx* ctx) {
         (void)hipStreamDestroy(a);
     }
     delete ctx;
+    return MRT_OK;
+}
+
+MRT_API int mrt_bvh_build_gpu(mrt_ctx* ctx, const float* boxes6, uint32_t n, uint32_t leaf_size, mrt_bvh_node* nodes,
+                              uint32_t capacity, uint32_t* count, int mem) {
+    if (!ctx || (n && !boxes6) || !count) return fail(MRT_E_INVALID, "null argument");
+    if (leaf_size == 0 || leaf_size > 16) return fail(MRT_E_INVALID, "leaf size must be 1..16");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    int rc;
+    const float* dbox = boxes6;
+    if (mem != MRT_MEM_DEVICE && n) {
+        if ((rc = ctx->bvh_boxes.upload(boxes6, static_cast<size_t>(n) * 24, st)) != MRT_OK) return rc;
+        dbox = ctx->bvh_boxes.as<float>();
+    }
+    const size_t maxNodes = n ? 2 * static_cast<size_t>(n) - 1 : 1;
+    if ((rc = ctx->bvh_nodes.alloc(maxNodes * sizeof(RefNode))) != MRT_OK) return rc;
+    uint32_t cnt = 0;
+    const hipError_t e = gpu_build_bvh(dbox, n, leaf_size, ctx->bvh_nodes.as<RefNode>(), &cnt, &ctx->bvh_scratch, st);
+    if (e == hipErrorNotSupported)
+        return fail(MRT_E_UNSUPPORTED, "GPU BVH build: NaN box coordinate or more than 1024 levels (use mrt_bvh_build)");
+    MRT_HIP(e);
+    *count = cnt;
+    static_assert(sizeof(RefNode) == sizeof(mrt_bvh_node), "node layouts differ");
+    const size_t copy = static_cast<size_t>(std::min(cnt, capacity)) * sizeof(RefNode);
+    if (nodes && copy) {
+        MRT_HIP(hipMemcpyAsync(nodes, ctx->bvh_nodes.p, copy,
+                               mem == MRT_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+        MRT_HIP(hipStreamSynchronize(st));
+    }
     return MRT_OK;
 }
 
@@ -800,6 +879,8 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
     const hipStream_t st = s->ctx->stream;
     const uint32_t leaf = params && params->leaf_size ? params->leaf_size : 2u;
     if (leaf > 16) return fail(MRT_E_UNSUPPORTED, "leaf size above 16");
+    const uint32_t builder = params ? params->builder : static_cast<uint32_t>(MRT_BUILDER_AUTO);
+    if (builder > MRT_BUILDER_GPU) return fail(MRT_E_INVALID, "unknown builder");
     uint32_t lightDepth = s->dev.lights_linear ? 0u : s->dev.light_acc.depth;
     s->depth_need = lightDepth;
     const uint32_t counts[3] = {static_cast<uint32_t>(s->plas.size()), static_cast<uint32_t>(s->sphs.size()),
@@ -822,7 +903,12 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
                            : c == MRT_CAT_SPHERE ? sphere_box(s->sphs[i])
                                                  : plane_box(s->plas[i]);
             BvhOut b;
-            build_bvh_reference(boxes, leaf, &b);
+            bool onGpu = false;
+            if (c == MRT_CAT_TRIANGLE &&
+                (builder == MRT_BUILDER_GPU || (builder == MRT_BUILDER_AUTO && counts[c] >= kGpuBuildMin))) {
+                if ((rc = build_bvh_gpu(s->ctx, boxes, leaf, &b, st, &onGpu)) != MRT_OK) return rc;
+            }
+            if (!onGpu) build_bvh_reference(boxes, leaf, &b);
             const uint64_t need = buffers_bytes(s) + b.pairs.size() * sizeof(BNode) *
                                                          (c == MRT_CAT_TRIANGLE ? 2u : 1u);
             if (s->ctx->budget && need > s->ctx->budget)
@@ -830,7 +916,8 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
             if ((rc = upload_bvh(s, c, b, st)) != MRT_OK) return rc;
             fill_accel_bvh(&s->dev.acc[c], b, s->nodes[c], nullptr, counts[c]);
             s->depth_need = std::max(s->depth_need, b.depth);
-            if (c == MRT_CAT_TRIANGLE && (rc = build_walk_tree(s, b, st)) != MRT_OK) return rc;
+            if (c == MRT_CAT_TRIANGLE && (rc = build_walk_tree(s, b, st, onGpu)) != MRT_OK) return rc;
+            if (c == MRT_CAT_TRIANGLE) s->gpu_built = onGpu;
         }
     } else if (accelerator == MRT_REGULAR_GRID) {
         const f3 camMin = params ? ld3(params->camera_min) : f3{0, 0, 0};

@@ -43,7 +43,7 @@ uint32_t split_sah(const Box* bx, uint32_t N, uint32_t L, std::vector<float>& le
     return split;
 }
 
-void to_pairs(BvhOut* out) {
+void to_pairs_impl(BvhOut* out) {
     const std::vector<RefNode>& ref = out->ref;
     out->pairs.clear();
     out->depth = 0;
@@ -133,8 +133,10 @@ void build_bvh_reference(const std::vector<Box>& boxes, uint32_t leaf, BvhOut* o
         }
     } while (!sId.empty());
     nodes.resize(static_cast<size_t>(maxId) + 1);
-    to_pairs(out);
+    to_pairs_impl(out);
 }
+
+void bvh_pairs_from_ref(BvhOut* out) { to_pairs_impl(out); }
 
 Box triangle_box(const mrt_triangle& t) {
     const f3 A = ld3(t.a);

@@ -26,6 +26,9 @@ struct BvhOut {
 // The reference builder: BVH<T>::BVH + build + getSplitIndex_SAH
 // (BVH.hpp:64-155, 274-311) over primitive boxes given in scene order.
 void build_bvh_reference(const std::vector<Box>& boxes, uint32_t leaf_size, BvhOut* out);
+// The child pairs, root and depth of out->ref (a reference node array, as
+// the builder or the GPU build leaves it).
+void bvh_pairs_from_ref(BvhOut* out);
 
 // Primitive boxes (getAABB of Triangle.cpp:145-152, Sphere.cpp:91-96,
 // Plane.cpp:80-104).

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "mrt_build.hpp"
+#include "mrt_bvh_gpu.hpp"
 #include "mrt_kernels.hpp"
 #include "mrt_path.hpp"
 #include "mrt_wavefront.hpp"
@@ -62,6 +63,7 @@ struct DevBuf {
 
 constexpr uint32_t kStackLds = 16;      // entries per lane in LDS (mrt_kernels.hip)
 constexpr uint32_t kLinearLights = 16;  // up to this many area lights are scanned linearly
+constexpr uint32_t kGpuBuildMin = 65536; // MRT_BUILDER_AUTO builds triangle BVHs from this size on the GPU
 
 // Render parameters and tile claim ranks of a validated render request
 // (Renderer ctor tile sizes, Renderer.cpp:12-29; claim order,
@@ -88,6 +90,10 @@ struct mrt_ctx {
     uint64_t budget = 0;
     std::atomic<uint32_t> children{0};  // live scenes and servers; the context outlives them
     std::mutex mu;
+    // GPU BVH builds (mrt_bvh_build_gpu, scene builds): scratch, staged
+    // boxes and the node array.
+    mrt::BvhGpuScratch bvh_scratch;
+    mrt::DevBuf bvh_boxes, bvh_nodes;
 };
 
 struct mrt_scene {
@@ -118,6 +124,7 @@ struct mrt_scene {
     }
     bool counted = false;  // holds one of ctx->children
     bool built = false;
+    bool gpu_built = false;  // the triangle BVH came from the GPU build
     bool generic = false;
     uint32_t depth_need = 0;
     uint32_t servers = 0;  // render servers holding this scene (guarded by ctx->mu)

@@ -1,0 +1,177 @@
+"""GPU build of the reference BVH (mrt_bvh_build_gpu) against the host
+restatement of the builder (mrt_bvh_build, itself pinned to the oracle in
+test_abi.py): node arrays bit-identical (boxes, offsets, counts, node order),
+at sizes from one box to the 1M-triangle benchmark soup, for leaf sizes
+1..16, with ties in the split cost, signed zeros and deep unbalanced trees.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mobileraytracer_amd as mrt
+from helpers import DeviceArray
+from mobileraytracer_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def tri_boxes(data) -> np.ndarray:
+    n = len(data.triangles)
+    boxes = np.zeros((n, 6), np.float32)
+    abi.check(abi.lib().mrt_primitive_aabbs(abi.CAT_TRIANGLE, data.triangles.ctypes.data, n, boxes.ctypes.data),
+              "mrt_primitive_aabbs")
+    return boxes
+
+
+def host_build(boxes: np.ndarray, leaf: int) -> np.ndarray:
+    n = len(boxes)
+    cnt = C.c_uint32()
+    abi.check(abi.lib().mrt_bvh_build(boxes.ctypes.data, n, leaf, None, 0, C.byref(cnt)), "mrt_bvh_build")
+    out = np.zeros((cnt.value, 8), np.uint32)
+    abi.check(abi.lib().mrt_bvh_build(boxes.ctypes.data, n, leaf, out.ctypes.data, cnt.value, C.byref(cnt)),
+              "mrt_bvh_build")
+    return out
+
+
+def gpu_build(ctx, boxes: np.ndarray, leaf: int) -> np.ndarray:
+    n = len(boxes)
+    cap = max(1, 2 * n - 1)
+    out = np.zeros((cap, 8), np.uint32)
+    cnt = C.c_uint32()
+    abi.check(abi.lib().mrt_bvh_build_gpu(ctx.h, boxes.ctypes.data, n, leaf, out.ctypes.data, cap, C.byref(cnt),
+                                          abi.MEM_HOST), "mrt_bvh_build_gpu")
+    return out[:cnt.value]
+
+
+def same(ctx, boxes, leaf):
+    boxes = np.ascontiguousarray(boxes, np.float32)
+    h = host_build(boxes, leaf)
+    g = gpu_build(ctx, boxes, leaf)
+    assert g.shape == h.shape
+    if not np.array_equal(g, h):
+        bad = np.nonzero((g != h).any(axis=1))[0]
+        raise AssertionError(f"{len(bad)} of {len(h)} nodes differ, first {bad[:5]}: gpu {g[bad[0]]} host {h[bad[0]]}")
+    return h
+
+
+@pytest.mark.parametrize("leaf", [2, 4, 16])
+def test_icospheres(gpu_ctx, leaf):
+    same(gpu_ctx, tri_boxes(mrt.icosphere_scene(3, 12, 5, 100)), leaf)
+
+
+def test_soup_200k(gpu_ctx):
+    nodes = same(gpu_ctx, tri_boxes(mrt.soup_scene(8, 200_000, 2000)), 2)
+    assert len(nodes) > 100_000
+
+
+def test_benchmark_soup_1m(gpu_ctx):
+    same(gpu_ctx, tri_boxes(mrt.soup_scene(2, 1_000_000, 10_000)), 2)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 64, 65, 66, 130, 1000, 1023, 1024, 1025, 5000])
+def test_small_sizes(gpu_ctx, n):
+    rng = np.random.default_rng(n)
+    c = rng.uniform(0, 1, (n, 3))
+    e = rng.uniform(0.001, 0.05, (n, 3))
+    order = np.argsort(c[:, 0] + 0.1 * c[:, 1])  # a loose spatial order
+    boxes = np.concatenate([c - e, c + e], 1)[order]
+    for leaf in (1, 2, 3) if n <= 1000 else (2, 3):
+        same(gpu_ctx, boxes, leaf)
+
+
+def test_leaf_one_chain_is_refused(gpu_ctx):
+    # With one primitive per leaf the reference's split index (R(0) = 0)
+    # makes the first candidate nearly free, so every node peels off one
+    # primitive: a chain of n levels, quadratic work for the host builder
+    # as for the reference.  Past 1024 levels above 64 primitives the GPU
+    # build declines.
+    rng = np.random.default_rng(11)
+    c = np.sort(rng.uniform(0, 1, (3000, 3)), axis=0)
+    boxes = np.ascontiguousarray(np.concatenate([c - 0.001, c + 0.001], 1), np.float32)
+    cnt = C.c_uint32()
+    rc = abi.lib().mrt_bvh_build_gpu(gpu_ctx.h, boxes.ctypes.data, 3000, 1, None, 0, C.byref(cnt), abi.MEM_HOST)
+    assert rc == abi.MRT_E_UNSUPPORTED
+
+
+def test_cornell(gpu_ctx):
+    same(gpu_ctx, tri_boxes(mrt.cornell_scene()), 2)
+
+
+def test_random_order_deep_tree(gpu_ctx):
+    # Scene order with no locality: the split cost is nearly flat and the
+    # in-order builder peels a few primitives per level (hundreds of levels).
+    rng = np.random.default_rng(7)
+    c = rng.uniform(0, 1, (1500, 3))
+    boxes = np.concatenate([c - 0.01, c + 0.01], 1)
+    same(gpu_ctx, boxes, 2)
+
+
+def test_signed_zeros_and_ties(gpu_ctx):
+    # Coordinates from {-1, -0, +0, 1}: many boxes meet at zero with either
+    # sign, and many split costs tie (ties pick the first position, so the
+    # trees are chains: sizes stay where the GPU build takes them).
+    rng = np.random.default_rng(3)
+    vals = np.array([-1.0, -0.0, 0.0, 1.0], np.float32)
+    a = vals[rng.integers(0, 4, (1000, 3))]
+    b = vals[rng.integers(0, 4, (1000, 3))]
+    boxes = np.concatenate([np.minimum(a, b), np.maximum(a, b)], 1).astype(np.float32)
+    h = same(gpu_ctx, boxes, 2)
+    assert np.signbit(h[:, :6].view(np.float32)).any()
+    same(gpu_ctx, np.tile(np.array([[0, 0, 0, 1, 1, 1]], np.float32), (1000, 1)), 2)
+
+
+def test_nan_box_is_refused(gpu_ctx):
+    boxes = np.zeros((200, 6), np.float32)
+    boxes[:, 3:] = 1.0
+    boxes[77, 2] = np.nan
+    cnt = C.c_uint32()
+    rc = abi.lib().mrt_bvh_build_gpu(gpu_ctx.h, boxes.ctypes.data, 200, 2, None, 0, C.byref(cnt), abi.MEM_HOST)
+    assert rc == abi.MRT_E_UNSUPPORTED
+
+
+def test_device_memory(gpu_ctx):
+    boxes = tri_boxes(mrt.icosphere_scene(4, 6, 4, 50))
+    n = len(boxes)
+    want = host_build(boxes, 2)
+    src, dst = DeviceArray(boxes.nbytes), DeviceArray(want.nbytes)
+    try:
+        src.upload(boxes)
+        cnt = C.c_uint32()
+        abi.check(abi.lib().mrt_bvh_build_gpu(gpu_ctx.h, src.data_ptr(), n, 2, dst.data_ptr(), len(want),
+                                              C.byref(cnt), abi.MEM_DEVICE), "mrt_bvh_build_gpu")
+        assert cnt.value == len(want)
+        got = dst.download(np.zeros_like(want))
+    finally:
+        src.free()
+        dst.free()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("scene", ["soup", "icospheres"])
+@pytest.mark.parametrize("kernel", [abi.KERNEL_WAVEFRONT, abi.KERNEL_MEGAKERNEL])
+def test_scene_builders_render_the_same(gpu_ctx, scene, kernel):
+    """mrt_scene_build with the GPU builder (GPU tree, collapsed wide walk
+    tree) against the host builder (binned-SAH wide walk tree): identical
+    8-bit frames and FP32 accumulations of PathTracer frames (200k-triangle
+    soup with 2000 area lights; 82k-triangle icosphere cluster)."""
+    data = mrt.soup_scene(8, 200_000, 2000) if scene == "soup" else mrt.icosphere_scene(5, 20, 4, 800)
+    cam = mrt.make_camera(abi.CAMERA_PERSPECTIVE, (0.5, 0.5, -1.0), (0.5, 0.5, 0.5), (0, 1, 0),
+                          float(np.float32(45.0) * np.float32(16.0 / 9.0)), 45.0)
+    desc = mrt.render_desc(abi.SHADER_PATHTRACER, 320, 180, 4, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 3),
+                           rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 4), light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 5),
+                           hemi=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 6), depth_min=2, depth_max=6, kernel=kernel)
+    frames = []
+    for builder in (abi.BUILDER_HOST, abi.BUILDER_GPU):
+        sc = mrt.Scene(gpu_ctx, data)
+        try:
+            sc.build(abi.BVH, builder=builder)
+            acc = np.zeros((180, 320, 3), np.float32)
+            bm, _ = sc.render(desc, accum=acc)
+            frames.append((bm, acc))
+        finally:
+            sc.close()
+    assert np.array_equal(frames[0][0], frames[1][0])
+    assert np.array_equal(frames[0][1], frames[1][1])
