@@ -312,7 +312,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt)
         log(f"cpu baseline took {time.time() - t0:.1f} s")
         out["vs_cpu"] = round(mrays / out["cpu_baseline"]["value"], 1)
-    out["scene_build"] = build_leg(mrt, abi, ctx, data, args.builder, build_s)
+    out["scene_build"] = build_leg(mrt, abi, ctx, data, cam, args.builder, build_s)
     out["denoiser"] = denoiser_leg(mrt, abi, ctx, torch, args.denoise_reps)
     if not args.no_serve:
         scene.close()
@@ -324,11 +324,12 @@ def main() -> int:
     return 0
 
 
-def build_leg(mrt, abi, ctx, data, builder: str, build_s: float) -> dict:
+def build_leg(mrt, abi, ctx, data, cam, builder: str, build_s: float) -> dict:
     """Reference BVH of the benchmark scene's 990k triangles built on the
     host (mrt_bvh_build) and on the GPU (mrt_bvh_build_gpu, boxes and nodes
     in host memory, so the copies are included): wall clock and whether the
-    node arrays are identical; and the whole timed scene build above."""
+    node arrays are identical; the whole timed scene build above; and whole
+    RegularGrid scene builds with each builder."""
     import ctypes as C
 
     n = len(data.triangles)
@@ -351,8 +352,20 @@ def build_leg(mrt, abi, ctx, data, builder: str, build_s: float) -> dict:
         gpu_ms.append((time.perf_counter() - t0) * 1e3)
     same = cnt.value == nh and np.array_equal(host[:nh], gpu[:nh])
     log(f"reference BVH of {n} triangles: host {host_ms:.1f} ms, GPU {min(gpu_ms):.1f} ms, identical {same}")
+    # RegularGrid of the same scene (128^3 cells), whole mrt_scene_build
+    # with each builder.
+    cmin, cmax = mrt.camera_aabb(cam)
+    grid_ms = {}
+    for name, b in (("host", abi.BUILDER_HOST), ("gpu", abi.BUILDER_GPU)):
+        sc = mrt.Scene(ctx, data)
+        t0 = time.perf_counter()
+        sc.build(abi.REGULAR_GRID, cmin, cmax, builder=b)
+        grid_ms[name] = (time.perf_counter() - t0) * 1e3
+        sc.close()
+    log(f"RegularGrid scene build: host {grid_ms['host']:.0f} ms, GPU {grid_ms['gpu']:.0f} ms")
     return {"builder": builder, "scene_build_ms": round(build_s * 1e3, 1), "bvh_nodes": nh,
-            "bvh_host_ms": round(host_ms, 1), "bvh_gpu_ms": round(min(gpu_ms), 1), "bvh_identical": bool(same)}
+            "bvh_host_ms": round(host_ms, 1), "bvh_gpu_ms": round(min(gpu_ms), 1), "bvh_identical": bool(same),
+            "grid_scene_build_host_ms": round(grid_ms["host"], 1), "grid_scene_build_gpu_ms": round(grid_ms["gpu"], 1)}
 
 
 def denoiser_leg(mrt, abi, ctx, torch, reps: int) -> dict:

@@ -232,14 +232,16 @@ typedef struct mrt_accel_params {
     uint32_t builder;          /* mrt_builder: where the BVH is built          */
     uint32_t reserved[2];
 } mrt_accel_params;
-/* BVH builder of mrt_scene_build.  HOST: the reference builder's restatement
- * and a binned-SAH 4-wide walk tree over its leaves.  GPU: the same binary
- * tree built on the GPU (mrt_bvh_build_gpu) and a 4-wide walk tree collapsed
- * from its pairs (each wide node = a pair with its interior children
- * expanded); both walk trees give the same hits (every interior box is the
- * exact union of the reference leaves below it).  AUTO: GPU from 65536
- * triangles on, HOST below.  A GPU build that declines (NaN coordinates,
- * more than 1024 levels above 64 primitives) falls back to HOST. */
+/* Where mrt_scene_build builds.  BVH, HOST: the reference builder's
+ * restatement and a binned-SAH 4-wide walk tree over its leaves; GPU: the
+ * same binary tree built on the GPU (mrt_bvh_build_gpu) and a 4-wide walk
+ * tree collapsed from its pairs (each wide node = a pair with its interior
+ * children expanded); both walk trees give the same hits (every interior box
+ * is the exact union of the reference leaves below it).  RegularGrid, GPU:
+ * the cell lists of mrt_grid_build_gpu, identical to the host's.  AUTO: GPU
+ * for categories of 65536 primitives or more, HOST below.  A GPU build that
+ * declines (NaN / non-finite coordinates, more than 1024 BVH levels above 64
+ * primitives) falls back to HOST. */
 enum mrt_builder { MRT_BUILDER_AUTO = 0, MRT_BUILDER_HOST = 1, MRT_BUILDER_GPU = 2 };
 
 typedef struct mrt_ctx mrt_ctx;
@@ -489,6 +491,19 @@ MRT_API int mrt_bvh_build(const float* boxes6, uint32_t n, uint32_t leaf_size, m
  * primitives (mrt_bvh_build covers both). */
 MRT_API int mrt_bvh_build_gpu(mrt_ctx* ctx, const float* boxes6, uint32_t n, uint32_t leaf_size,
                               mrt_bvh_node* nodes, uint32_t capacity, uint32_t* count, int mem);
+/* RegularGrid cell registration on the GPU (no counterpart in the reference:
+ * its grid build is sequential): the cell lists of the grid mrt_scene_build
+ * makes (RegularGrid.hpp:50-154: every primitive in the cells of its box
+ * range whose box it overlaps by intersectBox), identical to the host build:
+ * offsets (grid_size^3 + 1 entries, cell index x + y*gs + z*gs*gs) and refs
+ * (ascending primitive indices per cell; up to refs_capacity written, refs
+ * may be NULL), *n_refs their total.  prims: n records of the category's
+ * struct (mrt_plane, mrt_sphere, mrt_triangle); prims, offsets and refs in
+ * host or device memory per mem; bounds6 (min xyz, max xyz) on the host.
+ * MRT_E_UNSUPPORTED when a primitive box has a non-finite coordinate. */
+MRT_API int mrt_grid_build_gpu(mrt_ctx* ctx, int category, const void* prims, uint32_t n, const float bounds6[6],
+                               int32_t grid_size, uint32_t* offsets, uint32_t* refs, uint32_t refs_capacity,
+                               uint32_t* n_refs, int mem);
 /* The 4-wide tree the persistent trace walks above a set of leaves (box and
  * non-zero leaf word each; scene builds pass the reference BVH's leaves), as
  * built for rays with a finite inverse direction: 32 floats per node (min x,

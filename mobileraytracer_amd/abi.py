@@ -217,6 +217,7 @@ def _proto(lib):
         "mrt_bvh_build": (C.c_int, [P, u32, u32, P, u32, C.POINTER(u32)]),
         "mrt_wide_build": (C.c_int, [P, P, u32, P, u32, C.POINTER(u32), C.POINTER(u32)]),
         "mrt_bvh_build_gpu": (C.c_int, [P, P, u32, u32, P, u32, C.POINTER(u32), C.c_int]),
+        "mrt_grid_build_gpu": (C.c_int, [P, C.c_int, P, u32, P, i32, P, P, u32, C.POINTER(u32), C.c_int]),
         "mrt_scene_cornell": (C.c_int, [u32, P, P, P]),
         "mrt_scene_soup": (C.c_int, [u32, u32, u32, f32, f32, P, P, C.c_int]),
         "mrt_scene_icospheres": (C.c_int, [u32, u32, u32, f32, u32, f32, P, P, C.c_int]),

@@ -600,6 +600,49 @@ bool wavefront_eligible(const mrt_scene* s, const mrt_render_desc* d) {
 
 }  // namespace mrt
 
+// The grid of category c built on the GPU into the scene's cell buffers
+// (identical to build_grid); *done stays false when the GPU build declines.
+// Callers hold ctx->mu.
+int build_grid_gpu(mrt_scene* s, int c, const Box& bounds, int32_t gs, GridOut* g, hipStream_t st, bool* done) {
+    *done = false;
+    mrt_ctx* ctx = s->ctx;
+    int rc;
+    TriLayout L = kTriPacked;
+    uint32_t n = 0;
+    if (c == MRT_CAT_TRIANGLE) {
+        n = static_cast<uint32_t>(s->tris.size());
+        std::vector<float> packed(static_cast<size_t>(n) * 9);
+        for (uint32_t i = 0; i < n; ++i) {
+            std::memcpy(&packed[9 * static_cast<size_t>(i)], s->tris[i].a, 12);
+            std::memcpy(&packed[9 * static_cast<size_t>(i) + 3], s->tris[i].ab, 12);
+            std::memcpy(&packed[9 * static_cast<size_t>(i) + 6], s->tris[i].ac, 12);
+        }
+        if ((rc = ctx->grid_prims.upload(packed.data(), packed.size() * 4, st)) != MRT_OK) return rc;
+        MRT_HIP(hipStreamSynchronize(st));  // `packed` is released on return
+    } else if (c == MRT_CAT_SPHERE) {
+        n = static_cast<uint32_t>(s->sphs.size());
+        if ((rc = ctx->grid_prims.upload(s->sphs.data(), s->sphs.size() * sizeof(mrt_sphere), st)) != MRT_OK) return rc;
+    } else {
+        n = static_cast<uint32_t>(s->plas.size());
+        if ((rc = ctx->grid_prims.upload(s->plas.data(), s->plas.size() * sizeof(mrt_plane), st)) != MRT_OK) return rc;
+    }
+    GridGpuOut o{};
+    const hipError_t e =
+        gpu_build_grid(c, ctx->grid_prims.as<float>(), L, n, bounds, gs, &ctx->grid_scratch, &o, st);
+    if (e == hipErrorNotSupported) return MRT_OK;
+    MRT_HIP(e);
+    const size_t ncell = static_cast<size_t>(gs) * gs * gs;
+    const uint64_t need = buffers_bytes(s) + (ncell + 1 + o.n_refs) * 4ull;
+    if (s->ctx->budget && need > s->ctx->budget) return fail(MRT_E_BUDGET, "grid exceeds the HBM budget");
+    if ((rc = s->cell_off[c].alloc((ncell + 1) * 4)) != MRT_OK) return rc;
+    if ((rc = s->cell_ref[c].alloc(std::max<size_t>(1, o.n_refs) * 4)) != MRT_OK) return rc;
+    MRT_HIP(hipMemcpyAsync(s->cell_off[c].p, o.offsets, (ncell + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (o.n_refs) MRT_HIP(hipMemcpyAsync(s->cell_ref[c].p, o.refs, o.n_refs * 4ull, hipMemcpyDeviceToDevice, st));
+    grid_meta(bounds, gs, g);
+    *done = true;
+    return MRT_OK;
+}
+
 // The reference BVH over `boxes` built on the GPU (the node array of
 // build_bvh_reference, bit for bit) with its child pairs; *done stays false
 // when the GPU build declines (the caller builds on the host).  Callers hold
@@ -701,6 +744,40 @@ MRT_API int mrt_bvh_build_gpu(mrt_ctx* ctx, const float* boxes6, uint32_t n, uin
                                mem == MRT_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         MRT_HIP(hipStreamSynchronize(st));
     }
+    return MRT_OK;
+}
+
+MRT_API int mrt_grid_build_gpu(mrt_ctx* ctx, int category, const void* prims, uint32_t n, const float bounds6[6],
+                               int32_t grid_size, uint32_t* offsets, uint32_t* refs, uint32_t refs_capacity,
+                               uint32_t* n_refs, int mem) {
+    if (!ctx || (n && !prims) || !bounds6 || !offsets || !n_refs) return fail(MRT_E_INVALID, "null argument");
+    if (category < MRT_CAT_PLANE || category > MRT_CAT_TRIANGLE) return fail(MRT_E_INVALID, "unknown category");
+    if (grid_size <= 0 || grid_size > 1024) return fail(MRT_E_INVALID, "grid size must be 1..1024");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    MRT_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    const size_t rec = category == MRT_CAT_TRIANGLE ? sizeof(mrt_triangle)
+                       : category == MRT_CAT_SPHERE ? sizeof(mrt_sphere)
+                                                    : sizeof(mrt_plane);
+    int rc;
+    const void* dp = prims;
+    if (mem != MRT_MEM_DEVICE && n) {
+        if ((rc = ctx->grid_prims.upload(prims, rec * n, st)) != MRT_OK) return rc;
+        dp = ctx->grid_prims.p;
+    }
+    const Box bounds{ld3(bounds6), ld3(bounds6 + 3)};
+    GridGpuOut o{};
+    const hipError_t e = gpu_build_grid(category, static_cast<const float*>(dp), kTriAbi, n, bounds, grid_size,
+                                        &ctx->grid_scratch, &o, st);
+    if (e == hipErrorNotSupported) return fail(MRT_E_UNSUPPORTED, "GPU grid build: non-finite primitive box");
+    MRT_HIP(e);
+    *n_refs = o.n_refs;
+    const hipMemcpyKind kind = mem == MRT_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    const size_t ncell = static_cast<size_t>(grid_size) * grid_size * grid_size;
+    MRT_HIP(hipMemcpyAsync(offsets, o.offsets, (ncell + 1) * sizeof(uint32_t), kind, st));
+    const size_t nr = std::min(o.n_refs, refs_capacity);
+    if (refs && nr) MRT_HIP(hipMemcpyAsync(refs, o.refs, nr * sizeof(uint32_t), kind, st));
+    MRT_HIP(hipStreamSynchronize(st));
     return MRT_OK;
 }
 
@@ -946,12 +1023,19 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
                        : c == MRT_CAT_SPHERE ? sphere_intersect_box(s->sphs[i], cell)
                                              : plane_intersect_box(s->plas[i], cell);
             };
-            build_grid(counts[c], bounds, gs, box_of, hits, &g);
-            const uint64_t need = buffers_bytes(s) + (g.offsets.size() + g.refs.size()) * 4ull;
-            if (s->ctx->budget && need > s->ctx->budget)
-                return fail(MRT_E_BUDGET, "grid exceeds the HBM budget");
-            if ((rc = s->cell_off[c].upload(g.offsets.data(), g.offsets.size() * 4, st)) != MRT_OK) return rc;
-            if ((rc = s->cell_ref[c].upload(g.refs.data(), g.refs.size() * 4, st)) != MRT_OK) return rc;
+            bool onGpu = false;
+            if (counts[c] > 0 &&
+                (builder == MRT_BUILDER_GPU || (builder == MRT_BUILDER_AUTO && counts[c] >= kGpuBuildMin))) {
+                if ((rc = build_grid_gpu(s, c, bounds, gs, &g, st, &onGpu)) != MRT_OK) return rc;
+            }
+            if (!onGpu) {
+                build_grid(counts[c], bounds, gs, box_of, hits, &g);
+                const uint64_t need = buffers_bytes(s) + (g.offsets.size() + g.refs.size()) * 4ull;
+                if (s->ctx->budget && need > s->ctx->budget)
+                    return fail(MRT_E_BUDGET, "grid exceeds the HBM budget");
+                if ((rc = s->cell_off[c].upload(g.offsets.data(), g.offsets.size() * 4, st)) != MRT_OK) return rc;
+                if ((rc = s->cell_ref[c].upload(g.refs.data(), g.refs.size() * 4, st)) != MRT_OK) return rc;
+            }
             Accel& a = s->dev.acc[c];
             a.cell_off = s->cell_off[c].as<uint32_t>();
             a.cell_ref = s->cell_ref[c].as<uint32_t>();

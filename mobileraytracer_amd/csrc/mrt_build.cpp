@@ -1,6 +1,7 @@
 // Host-side builders: the reference BVH builder, the conversion to 64-byte
 // child pairs, the light BVH, primitive boxes and box-overlap tests.
 #include "mrt_build.hpp"
+#include "mrt_grid_reg.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -138,109 +139,22 @@ void build_bvh_reference(const std::vector<Box>& boxes, uint32_t leaf, BvhOut* o
 
 void bvh_pairs_from_ref(BvhOut* out) { to_pairs_impl(out); }
 
-Box triangle_box(const mrt_triangle& t) {
-    const f3 A = ld3(t.a);
-    const f3 B = add(A, ld3(t.ab));
-    const f3 C = add(A, ld3(t.ac));
-    return Box{vmin(A, vmin(B, C)), vmax(A, vmax(B, C))};
-}
+Box triangle_box(const mrt_triangle& t) { return triangle_box_of(ld3(t.a), ld3(t.ab), ld3(t.ac)); }
 
-Box sphere_box(const mrt_sphere& s) {
-    const float r = std::sqrt(s.sq_radius);
-    const f3 c = ld3(s.center);
-    return Box{subs(c, r), adds(c, r)};
-}
+Box sphere_box(const mrt_sphere& s) { return sphere_box_of(ld3(s.center), s.sq_radius); }
 
-Box plane_box(const mrt_plane& p) {
-    const f3 n = ld3(p.normal);
-    f3 right{0.0f, 0.0f, 0.0f};
-    if (n.x >= 1.0f) right = f3{0, 1, 1};
-    else if (n.y >= 1.0f) right = f3{1, 0, 1};
-    else if (n.z >= 1.0f) right = f3{1, 1, 0};
-    else if (n.x <= -1.0f) right = f3{0, 1, 1};
-    else if (n.y <= -1.0f) right = f3{1, 0, 1};
-    else if (n.z <= -1.0f) right = f3{1, 1, 0};
-    right = normalize(right);
-    const f3 pt = ld3(p.point);
-    return Box{add(pt, muls(right, -100.0f)), add(pt, muls(right, 100.0f))};
-}
+Box plane_box(const mrt_plane& p) { return plane_box_of(ld3(p.normal), ld3(p.point)); }
 
 bool triangle_intersect_box(const mrt_triangle& t, const Box& box) {
-    const f3 A = ld3(t.a), AB = ld3(t.ab), AC = ld3(t.ac);
-    auto rayBox = [&](f3 orig, f3 vec) -> bool {
-        float tNear = std::numeric_limits<float>::min();
-        float tFar = std::numeric_limits<float>::max();
-        for (int a = 0; a < 3; ++a) {
-            const float o = comp(orig, a), v = comp(vec, a);
-            const float bmn = comp(box.mn, a), bmx = comp(box.mx, a);
-            if (std::fabs(v) < kEpsilon) {
-                if ((o < bmn) || ((o + v) > bmx)) return false;
-            } else {
-                float t1 = (bmn - o) / v;
-                float t2 = (bmx - o) / v;
-                if (t1 > t2) std::swap(t1, t2);
-                tNear = smax(t1, tNear);
-                tFar = smin(t2, tFar);
-                if ((tNear > tFar) || (tFar < 0.0f)) return false;
-            }
-        }
-        return true;
-    };
-    const f3 vec = sub(box.mx, box.mn);
-    const bool iAB = rayBox(A, AB);
-    const bool iAC = rayBox(A, AC);
-    const f3 B = add(A, AB);
-    const f3 C = add(A, AC);
-    const bool iBC = rayBox(B, sub(C, B));
-    // The box diagonal as a ray from min (Triangle.cpp:230, 236).
-    bool iRay = false;
-    {
-        const f3 d = vec;
-        const f3 perp = cross(d, AC);
-        const float det = dot(AB, perp);
-        if (!(std::fabs(det) < kEpsilon)) {
-            const float inv = 1.0f / det;
-            const f3 s = sub(box.mn, A);
-            const float u = inv * dot(s, perp);
-            if (!(u < 0.0f || u > 1.0f)) {
-                const f3 q = cross(s, AB);
-                const float v = inv * dot(d, q);
-                if (!(v < 0.0f || (u + v) > 1.0f)) {
-                    const float tt = inv * dot(AC, q);
-                    iRay = !(tt < kEpsilon || tt >= kRayLengthMax);
-                }
-            }
-        }
-    }
-    const f3 perp = cross(vec, AC);
-    const bool inside = std::fabs(dot(AB, perp)) < kEpsilon;
-    return iAB || iAC || iBC || iRay || inside;
+    return triangle_box_overlap(ld3(t.a), ld3(t.ab), ld3(t.ac), box);
 }
 
 bool sphere_intersect_box(const mrt_sphere& s, const Box& box) {
-    float dmin = 0.0f;
-    const f3 c = ld3(s.center);
-    for (int a = 0; a < 3; ++a) {
-        const float ca = comp(c, a), mn = comp(box.mn, a), mx = comp(box.mx, a);
-        if (ca < mn) dmin = dmin + (ca - mn) * (ca - mn);
-        else if (ca > mx) dmin = dmin + (ca - mx) * (ca - mx);
-    }
-    return dmin <= s.sq_radius;
-}
-
-// Plane::distance, Plane.cpp:106-121.
-static float plane_distance(const mrt_plane& p, f3 q) {
-    const f3 n = ld3(p.normal), pt = ld3(p.point);
-    const float d = (n.x * -pt.x + n.y * -pt.y) + n.z * -pt.z;
-    const float num = ((n.x * q.x + n.y * q.y) + n.z * q.z) + d;
-    const float den = std::sqrt((n.x * n.x + n.y * n.y) + n.z * n.z);
-    return num / den;
+    return sphere_box_overlap(ld3(s.center), s.sq_radius, box);
 }
 
 bool plane_intersect_box(const mrt_plane& p, const Box& box) {
-    const float dP = plane_distance(p, box.mx);
-    const float dN = plane_distance(p, box.mn);
-    return (dP <= 0.0f && dN >= 0.0f) || (dP >= 0.0f && dN <= 0.0f);
+    return plane_box_overlap(ld3(p.normal), ld3(p.point), box);
 }
 
 int32_t grid_size_for(uint32_t n) {

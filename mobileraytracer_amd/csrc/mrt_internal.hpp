@@ -8,6 +8,7 @@
 
 #include "mrt_build.hpp"
 #include "mrt_bvh_gpu.hpp"
+#include "mrt_grid_gpu.hpp"
 #include "mrt_kernels.hpp"
 #include "mrt_path.hpp"
 #include "mrt_wavefront.hpp"
@@ -94,6 +95,9 @@ struct mrt_ctx {
     // boxes and the node array.
     mrt::BvhGpuScratch bvh_scratch;
     mrt::DevBuf bvh_boxes, bvh_nodes;
+    // GPU grid builds: scratch and the staged primitives.
+    mrt::GridGpuScratch grid_scratch;
+    mrt::DevBuf grid_prims;
 };
 
 struct mrt_scene {
