@@ -59,19 +59,46 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
 constexpr uint32_t kTileRows = 4, kTileW = 64, kHaloW = kTileW + 2, kRowQuads = 5;
 constexpr uint32_t kTileQuads = (kTileRows + 2) * kHaloW * kRowQuads;
 
-// The 32 packed input channels of pixel (yy, xx) as 4 quads of 8 BF16.
-__device__ __forceinline__ void pack_pixel(const ConvArgs& a, int yy, int xx, uint4* q) {
-    float v[10] = {};
-    if (yy < static_cast<int>(a.fh) && xx < static_cast<int>(a.fw)) {
-        const size_t s = static_cast<size_t>(yy) * a.fw + xx;
-        for (int k = 0; k < 3; ++k) v[k] = log1pf(fmaxf(a.rad[3 * s + k] * a.rad_scale, 0.0f));
-        for (int k = 0; k < 6; ++k) v[3 + k] = a.aov[7 * s + k];
-        v[9] = a.aov[7 * s + 6] * a.depth_scale;
+// Stages the packed network input (32 channels, 0..9 used) of the tile rows
+// and halo: log(1 + max(rad * rad_scale, 0)), aov[0..5], aov[6] *
+// depth_scale; zero outside the fw x fh frame.  The loads of both rounds of
+// pixels are issued before any value is converted.
+__device__ __forceinline__ void stage_packed(const ConvArgs& a, uint4* tile, uint32_t tid, int x0, int y0) {
+    constexpr uint32_t kPx = (kTileRows + 2) * kHaloW;
+    constexpr uint32_t kIters = (kPx + 255u) / 256u;
+    float v[kIters][10];
+    bool inside[kIters];
+#pragma unroll
+    for (uint32_t it = 0; it < kIters; ++it) {
+        const uint32_t px = tid + it * 256u;
+        const uint32_t r = px / kHaloW, c = px % kHaloW;
+        const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
+        const bool in = px < kPx && yy >= 0 && xx >= 0 && yy < static_cast<int>(a.fh) && xx < static_cast<int>(a.fw);
+        inside[it] = in;
+        const size_t s = in ? static_cast<size_t>(yy) * a.fw + xx : 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[it][k] = in ? a.rad[3 * s + k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) v[it][3 + k] = in ? a.aov[7 * s + k] : 0.0f;
     }
-    q[0] = make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
-    q[1] = make_uint4(pack2(v[8], v[9]), 0u, 0u, 0u);
-    q[2] = make_uint4(0u, 0u, 0u, 0u);
-    q[3] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (uint32_t it = 0; it < kIters; ++it) {
+        const uint32_t px = tid + it * 256u;
+        if (px >= kPx) continue;
+        float w[10] = {};
+        if (inside[it]) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) w[k] = log1pf(fmaxf(v[it][k] * a.rad_scale, 0.0f));
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[3 + k] = v[it][3 + k];
+            w[9] = v[it][9] * a.depth_scale;
+        }
+        uint4* o = tile + px * kRowQuads;
+        o[0] = make_uint4(pack2(w[0], w[1]), pack2(w[2], w[3]), pack2(w[4], w[5]), pack2(w[6], w[7]));
+        o[1] = make_uint4(pack2(w[8], w[9]), 0u, 0u, 0u);
+        o[2] = make_uint4(0u, 0u, 0u, 0u);
+        o[3] = make_uint4(0u, 0u, 0u, 0u);
+    }
 }
 
 template <int NT>
@@ -91,6 +118,47 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int m = 0; m < 4; ++m) acc[j][m] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // Staging: all of a chunk's loads are issued before any of them is
+    // written to LDS, and the next chunk's loads are in flight while the
+    // current chunk's MFMA loop runs.
+    constexpr uint32_t kStageQuads = (kTileRows + 2) * kHaloW * 4u;
+    constexpr uint32_t kStageIters = (kStageQuads + 255u) / 256u;
+    constexpr uint32_t kWQuads = 9u * NT * 4u;
+    constexpr uint32_t kWIters = kWeightsLds ? (kWQuads + 255u) / 256u : 1u;
+    uint4 sv[kStageIters];
+    uint4 wq[kWIters];
+    auto load_input = [&](uint32_t kc) {
+        const bool up = kc < a.cu;
+#pragma unroll
+        for (uint32_t it = 0; it < kStageIters; ++it) {
+            const uint32_t p = tid + it * 256u;
+            sv[it] = make_uint4(0u, 0u, 0u, 0u);
+            if (p >= kStageQuads) continue;
+            const uint32_t q = p & 3u, px = p >> 2;
+            const uint32_t r = px / kHaloW, c = px % kHaloW;
+            const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const uint16_t* src =
+                    up ? a.xu + (static_cast<size_t>(yy >> 1) * (a.W >> 1) + (xx >> 1)) * a.cu + kc
+                       : a.x + (static_cast<size_t>(yy) * a.W + xx) * a.x_stride + a.x_off + (kc - a.cu);
+                sv[it] = *reinterpret_cast<const uint4*>(src + q * 8u);
+            }
+        }
+    };
+    auto load_weights_lds = [&](uint32_t kc) {
+#pragma unroll
+        for (uint32_t it = 0; it < kWIters; ++it) {
+            const uint32_t p = tid + it * 256u;
+            wq[it] = make_uint4(0u, 0u, 0u, 0u);
+            if (p >= kWQuads) continue;
+            const uint32_t q = p & 3u, row = p >> 2;  // row = tap * NT + n
+            const uint32_t tap = row / NT, n = row % NT;
+            wq[it] = *reinterpret_cast<const uint4*>(a.w + (static_cast<size_t>(tap) * a.cout + n0 + n) * a.cin + kc +
+                                                     q * 8u);
+        }
+    };
+    if (!a.rad) load_input(0);
+    if constexpr (kWeightsLds) load_weights_lds(0);
     for (uint32_t kc = 0; kc < a.cin; kc += 32u) {
         uint4 wr[kWeightsLds ? 1 : 9][NJ];
         if constexpr (!kWeightsLds) {
@@ -104,38 +172,26 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         }
         __syncthreads();
         if (a.rad) {
-            for (uint32_t px = tid; px < (kTileRows + 2) * kHaloW; px += 256u) {
-                const uint32_t r = px / kHaloW, c = px % kHaloW;
-                const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
-                uint4 q[4] = {};
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W) pack_pixel(a, yy, xx, q);
-                for (int k = 0; k < 4; ++k) tile[px * kRowQuads + k] = q[k];
-            }
+            stage_packed(a, tile, tid, x0, y0);
         } else {
-            const bool up = kc < a.cu;
-            for (uint32_t p = tid; p < (kTileRows + 2) * kHaloW * 4u; p += 256u) {
-                const uint32_t q = p & 3u, px = p >> 2;
-                const uint32_t r = px / kHaloW, c = px % kHaloW;
-                const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                    const uint16_t* src =
-                        up ? a.xu + (static_cast<size_t>(yy >> 1) * (a.W >> 1) + (xx >> 1)) * a.cu + kc
-                           : a.x + (static_cast<size_t>(yy) * a.W + xx) * a.x_stride + a.x_off + (kc - a.cu);
-                    v = *reinterpret_cast<const uint4*>(src + q * 8u);
-                }
-                tile[px * kRowQuads + q] = v;
+#pragma unroll
+            for (uint32_t it = 0; it < kStageIters; ++it) {
+                const uint32_t p = tid + it * 256u;
+                if (p < kStageQuads) tile[(p >> 2) * kRowQuads + (p & 3u)] = sv[it];
             }
         }
         if constexpr (kWeightsLds) {
-            for (uint32_t p = tid; p < 9u * NT * 4u; p += 256u) {
-                const uint32_t q = p & 3u, row = p >> 2;  // row = tap * NT + n
-                const uint32_t tap = row / NT, n = row % NT;
-                wts[row * kRowQuads + q] = *reinterpret_cast<const uint4*>(
-                    a.w + (static_cast<size_t>(tap) * a.cout + n0 + n) * a.cin + kc + q * 8u);
+#pragma unroll
+            for (uint32_t it = 0; it < kWIters; ++it) {
+                const uint32_t p = tid + it * 256u;
+                if (p < kWQuads) wts[(p >> 2) * kRowQuads + (p & 3u)] = wq[it];
             }
         }
         __syncthreads();
+        if (kc + 32u < a.cin) {
+            if (!a.rad) load_input(kc + 32u);
+            if constexpr (kWeightsLds) load_weights_lds(kc + 32u);
+        }
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap % 3;
