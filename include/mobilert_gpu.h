@@ -323,6 +323,16 @@ MRT_API int mrt_render_progress(const mrt_scene* scene, uint32_t* samples_done);
  * aov: 7 floats per pixel (albedo.xyz, normal.xyz, depth), width x height,
  * pixels outside the tile grid zero. */
 MRT_API int mrt_render_aovs(mrt_scene* scene, const mrt_render_desc* desc, float* aov, int mem);
+/* mrt_render plus the outputs of mrt_render_aovs for the same desc, in one
+ * call (no counterpart in the reference): the wavefront pipeline records each
+ * sample's first hit when it shades the camera ray (per-sample records, up
+ * to 2 GiB of them) and folds them in sample order after the frame, so aov
+ * is identical to mrt_render_aovs's without tracing the camera rays twice;
+ * the megakernel, or frames beyond that size, run the AOV kernel after the
+ * render.  A stopped render's aov covers the samples finished in every
+ * pixel. */
+MRT_API int mrt_render_with_aovs(mrt_scene* scene, const mrt_render_desc* desc, uint32_t* bitmap,
+                                 uint32_t stride_bytes, float* accum, float* aov, int mem, mrt_render_stats* stats);
 
 /* ---- render server (continuous batching) -------------------------------- */
 /* No counterpart in the reference, whose Renderer renders one frame at a time

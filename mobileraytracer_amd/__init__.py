@@ -249,6 +249,21 @@ class Scene:
               "mrt_render")
         return bitmap, stats
 
+    def render_with_aovs(self, desc: abi.RenderDesc, bitmap=None, accum=None, aov=None,
+                         stride_bytes: int | None = None, mem: int = MEM_HOST):
+        """render() and aovs() of the same frame in one call; returns
+        (bitmap, aov, stats)."""
+        if bitmap is None:
+            bitmap = np.zeros((desc.height, desc.width), np.uint32)
+        if aov is None:
+            aov = np.zeros((desc.height, desc.width, 7), np.float32)
+        if stride_bytes is None:
+            stride_bytes = desc.width * 4
+        stats = abi.RenderStats()
+        check(lib().mrt_render_with_aovs(self.h, C.byref(desc), ptr(bitmap), stride_bytes, ptr(accum), ptr(aov), mem,
+                                         C.byref(stats)), "mrt_render_with_aovs")
+        return bitmap, aov, stats
+
     def aovs(self, desc: abi.RenderDesc, out=None, mem: int = MEM_HOST):
         """First-hit albedo, normal and depth per pixel (height x width x 7)."""
         if out is None:

@@ -194,6 +194,7 @@ def _proto(lib):
         "mrt_render_stop": (C.c_int, [P]),
         "mrt_render_progress": (C.c_int, [P, C.POINTER(u32)]),
         "mrt_render_aovs": (C.c_int, [P, C.POINTER(RenderDesc), P, C.c_int]),
+        "mrt_render_with_aovs": (C.c_int, [P, C.POINTER(RenderDesc), P, u32, P, P, C.c_int, C.POINTER(RenderStats)]),
         "mrt_server_create": (C.c_int, [P, C.POINTER(ServerDesc), C.POINTER(P)]),
         "mrt_server_destroy": (C.c_int, [P]),
         "mrt_server_add_scene": (C.c_int, [P, P, C.POINTER(u32)]),

@@ -382,8 +382,8 @@ uint32_t cap_blocks(const mrt_ctx* c) { return (c->num_cu ? c->num_cu : 256u) * 
 // A stop request lets running paths finish without starting new samples.
 int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p, std::vector<WfBuffers>& wbs,
                   bool persistent, uint32_t trace_blocks, uint32_t persist_blocks, const std::vector<StackCfg>& scs,
-                  uint32_t* dbm, float* dacc, bool* stopped, float* kernel_ms, float* trace_ms, float* busy_ms,
-                  uint32_t* launches) {
+                  uint32_t* dbm, float* dacc, float* daov, bool* stopped, float* kernel_ms, float* trace_ms,
+                  float* busy_ms, uint32_t* launches) {
     constexpr uint32_t kBatch = 4;
     const uint32_t G = static_cast<uint32_t>(wbs.size());
     hipStream_t st[kWfMaxGroups];
@@ -484,6 +484,7 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
                                   st[0]));
         MRT_HIP(launch_wavefront_resolve(p, d->samples_pixel, wbs[0], dbm, dacc, doneMin, st[0]));
     }
+    if (wbs[0].aov_s) MRT_HIP(launch_wavefront_aov_resolve(p, d->samples_pixel, wbs[0], daov, st[0]));
     MRT_HIP(hipEventRecord(b1, st[0]));
     MRT_HIP(hipEventSynchronize(b1));
     uint32_t progress = 0;
@@ -1240,8 +1241,11 @@ MRT_API int mrt_box_rays(mrt_ctx* ctx, const float* boxes6, const mrt_ray* rays,
     return MRT_OK;
 }
 
-MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap, uint32_t stride_bytes, float* accum,
-                       int mem, mrt_render_stats* stats) {
+// Renderer::renderFrame, and with aov the auxiliary outputs of the same
+// frame (recorded by the wavefront pipeline from the camera rays it traces,
+// or by the AOV kernel after the render).
+static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap, uint32_t stride_bytes,
+                       float* accum, float* aov, int mem, mrt_render_stats* stats) {
     if (!s || !bitmap) return fail(MRT_E_INVALID, "null argument");
     if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
     int rc;
@@ -1267,6 +1271,15 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             if ((rc = s->stage_out.upload(accum, accBytes, st)) != MRT_OK) return rc;
             dacc = s->stage_out.as<float>();
         }
+    }
+    const size_t aovBytes = static_cast<size_t>(w) * h * 7 * sizeof(float);
+    float* daov = aov;
+    if (aov) {
+        if (mem != MRT_MEM_DEVICE) {
+            if ((rc = s->stage_aux.alloc(aovBytes)) != MRT_OK) return rc;
+            daov = s->stage_aux.as<float>();
+        }
+        MRT_HIP(hipMemsetAsync(daov, 0, aovBytes, st));
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
     // Kernel shape: the wavefront pipeline where it applies, else the
@@ -1315,6 +1328,17 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             rad = s->wf_rad.as<float>();
             MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(rad), static_cast<int>(kRadPending), n, st));
         }
+        // Auxiliary outputs: one first-hit record per sample and pixel slot
+        // (2 float4) up to 2 GiB; larger frames run the AOV kernel after the
+        // render.
+        float4* aovS = nullptr;
+        const size_t aovSBytes = 32ull * (d->samples_pixel - d->first_sample) * pixelSlots;
+        if (aov && aovSBytes <= (2ull << 30)) {
+            if ((rc = s->wf_aov.alloc(aovSBytes)) != MRT_OK) return rc;
+            aovS = s->wf_aov.as<float4>();
+            MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(aovS), static_cast<int>(kRadPending),
+                                      aovSBytes / 4, st));
+        }
         // Key box: the triangle BVH's root box (the unit cube without triangles).
         f3 mn{0.0f, 0.0f, 0.0f}, mx{1.0f, 1.0f, 1.0f};
         const Accel& ta = s->dev.acc[MRT_CAT_TRIANGLE];
@@ -1352,6 +1376,7 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             wb.scan_bytes = scanBytes;
             wb.sort = d->ray_sort == 1u ? 1u : 0u;
             wb.rad = rad;
+            wb.aov_s = aovS;
             wb.lanes = lanes;
             wb.pixel_slots = pixelSlots;
             wb.first = d->first_sample;
@@ -1378,7 +1403,7 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     float kernel_ms = 0.0f, trace_ms = 0.0f, trace_busy = 0.0f;
     uint32_t traceLaunches = 0;
     if (useWavefront && d->first_sample < d->samples_pixel) {
-        if ((rc = run_wavefront(s, d, p, wbs, persistent, traceBlocks, persistBlocks, scs, dbm, dacc, &stopped,
+        if ((rc = run_wavefront(s, d, p, wbs, persistent, traceBlocks, persistBlocks, scs, dbm, dacc, daov, &stopped,
                                 &kernel_ms, &trace_ms, &trace_busy, &traceLaunches)) != MRT_OK)
             return rc;
     }
@@ -1405,9 +1430,17 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(k0);
     (void)hipEventDestroy(k1);
+    const bool aovDone = useWavefront && !wbs.empty() && wbs[0].aov_s != nullptr;
+    if (aov && !aovDone && d->first_sample < d->samples_pixel) {
+        StackCfg asc;
+        if ((rc = ensure_spill(s, render_pass_blocks(p) * 256u, &asc)) != MRT_OK) return rc;
+        MRT_HIP(launch_aov(s->dev, s->generic, p, d->first_sample, d->samples_pixel, daov, asc, s->err.as<uint32_t>(),
+                           st));
+    }
     if (mem != MRT_MEM_DEVICE) {
         MRT_HIP(hipMemcpyAsync(bitmap, dbm, bmBytes, hipMemcpyDeviceToHost, st));
         if (accum) MRT_HIP(hipMemcpyAsync(accum, dacc, accBytes, hipMemcpyDeviceToHost, st));
+        if (aov) MRT_HIP(hipMemcpyAsync(aov, daov, aovBytes, hipMemcpyDeviceToHost, st));
     }
     unsigned long long cnt[16] = {0};
     MRT_HIP(hipMemcpyAsync(cnt, s->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
@@ -1437,6 +1470,17 @@ MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;
+}
+
+MRT_API int mrt_render(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap, uint32_t stride_bytes, float* accum,
+                       int mem, mrt_render_stats* stats) {
+    return render_impl(s, d, bitmap, stride_bytes, accum, nullptr, mem, stats);
+}
+
+MRT_API int mrt_render_with_aovs(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap, uint32_t stride_bytes,
+                                 float* accum, float* aov, int mem, mrt_render_stats* stats) {
+    if (!aov) return fail(MRT_E_INVALID, "null aov");
+    return render_impl(s, d, bitmap, stride_bytes, accum, aov, mem, stats);
 }
 
 MRT_API int mrt_render_aovs(mrt_scene* s, const mrt_render_desc* d, float* aov, int mem) {

@@ -576,10 +576,25 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s0, RenderParams p
                 const uint32_t sample = flags_sample(flags);
                 if (depth == 1u) ++rc.camera;
                 else ++rc.extension;
+                float4* aovEntry = nullptr;
+                if (!MULTI && b.aov_s && depth == 1u) {
+                    const uint32_t ps = slot % b.pixel_slots;
+                    aovEntry = b.aov_s + 2ull * ((sample - b.first) * static_cast<uint64_t>(b.pixel_slots) + ps);
+                    if (h.id == MRT_ID_NONE) {
+                        aovEntry[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        aovEntry[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+                }
                 if (h.id != MRT_ID_NONE) {
                     const f3 T = f3{sa.x, sa.y, sa.z};
                     const Surf sf = surface(s, r, h);
                     const Shading m = shading_of(s, sf);
+                    if (aovEntry) {
+                        // The first hit of the sample (the quantities of k_aov).
+                        const f3 alb = m.emitter ? m.le : m.kd;
+                        aovEntry[0] = make_float4(alb.x, alb.y, alb.z, h.t);
+                        aovEntry[1] = make_float4(sf.normal.x, sf.normal.y, sf.normal.z, 1.0f);
+                    }
                     const DrawCtx dc{__float_as_uint(b.sB[slot].w), sample, 0u};
                     if (m.emitter) {
                         float4 sbv = b.sB[slot];
@@ -681,7 +696,48 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(RenderParams p, uint32_t 
     atomicMin(done_min, s);
 }
 
+// Folds the per-sample first hits of b.aov_s in sample order, as k_aov does
+// over the same camera rays.
+__global__ __launch_bounds__(kBlock) void k_wf_aov_resolve(RenderParams p, uint32_t spp, WfBuffers b,
+                                                           float* __restrict__ aov) {
+    const uint32_t ps = blockIdx.x * kBlock + threadIdx.x;
+    if (ps >= b.pixel_slots) return;
+    uint32_t x, y;
+    slot_pixel(p, ps, &x, &y);
+    if (x >= 16u * p.bsx || y >= 16u * p.bsy) return;
+    const uint32_t pid = y * p.width + x;
+    f3 alb{}, nrm{};
+    float dep = 0.0f;
+    uint32_t s = b.first;
+    for (; s < spp; ++s) {
+        const float4* e = b.aov_s + 2ull * ((s - b.first) * static_cast<uint64_t>(b.pixel_slots) + ps);
+        const float4 hv = e[1];
+        if (__float_as_uint(hv.w) == kRadPending) break;
+        if (hv.w == 0.0f) continue;
+        const float4 av = e[0];
+        alb = add(alb, f3{av.x, av.y, av.z});
+        nrm = add(nrm, f3{hv.x, hv.y, hv.z});
+        dep += av.w;
+    }
+    if (s == b.first) return;
+    const float inv = 1.0f / static_cast<float>(s - b.first);
+    float* o7 = aov + 7ull * pid;
+    o7[0] = alb.x * inv;
+    o7[1] = alb.y * inv;
+    o7[2] = alb.z * inv;
+    o7[3] = nrm.x * inv;
+    o7[4] = nrm.y * inv;
+    o7[5] = nrm.z * inv;
+    o7[6] = dep * inv;
+}
+
 // ------------------------------------------------------------- launcher --
+hipError_t launch_wavefront_aov_resolve(const RenderParams& p, uint32_t spp, const WfBuffers& b, float* aov,
+                                        hipStream_t stream) {
+    k_wf_aov_resolve<<<(b.pixel_slots + kBlock - 1u) / kBlock, kBlock, 0, stream>>>(p, spp, b, aov);
+    return hipGetLastError();
+}
+
 hipError_t launch_wavefront_resolve(const RenderParams& p, uint32_t spp, const WfBuffers& b, uint32_t* bitmap,
                                     float* accum, uint32_t* done_min, hipStream_t stream) {
     k_wf_resolve<<<(b.pixel_slots + kBlock - 1u) / kBlock, kBlock, 0, stream>>>(p, spp, b, bitmap, accum, done_min);

@@ -54,6 +54,12 @@ struct WfBuffers {
     // folded as they finish.
     float* rad;
     uint32_t lanes, pixel_slots, first;
+    // Frames with auxiliary outputs: each sample's first hit (albedo, t |
+    // normal, hit flag) at aov_s[2 * ((sample - first) * pixel_slots + ps)],
+    // written when its camera ray is shaded and folded by
+    // launch_wavefront_aov_resolve; entries not written keep kRadPending in
+    // the flag.  nullptr: none.
+    float4* aov_s;
 };
 
 // Radiance-buffer entries of samples not finished (x component bits; a NaN
@@ -159,6 +165,12 @@ hipError_t launch_wavefront_admit(const BatchDev& bd, const JobDev& jd, uint32_t
 // accumulation in sample order (Renderer.cpp:101-104 per sample), stopping at a
 // pixel's first unfinished sample; counts[kCntDone] receives the smallest
 // number of samples any covered pixel completed.
+// Auxiliary outputs of a frame from b.aov_s: per covered pixel, the mean
+// over samples [b.first, s) of the first-hit albedo, normal and distance
+// (misses add nothing), s the first sample not written; 7 floats per pixel
+// at aov + 7 * (y * width + x).
+hipError_t launch_wavefront_aov_resolve(const RenderParams& p, uint32_t spp, const WfBuffers& b, float* aov,
+                                        hipStream_t stream);
 hipError_t launch_wavefront_resolve(const RenderParams& p, uint32_t spp, const WfBuffers& b, uint32_t* bitmap,
                                     float* accum, uint32_t* done_min, hipStream_t stream);
 

@@ -185,6 +185,13 @@ def evaluate(mrt, abi, ctx, dn, spp_ref, seed):
         t0 = time.perf_counter()
         aov = scene.aovs(d4)
         ms_aov = (time.perf_counter() - t0) * 1e3
+        # The 4-sample frame and its AOVs in one render.
+        acc = np.zeros((H_FULL, W_FULL, 3), np.float32)
+        t0 = time.perf_counter()
+        _, aov1, _ = scene.render_with_aovs(d4, accum=acc)
+        ms4_aov = (time.perf_counter() - t0) * 1e3
+        if not (np.array_equal(aov1, aov) and np.array_equal(acc / np.float32(SPP_IN), x4)):
+            raise RuntimeError("render_with_aovs differs from render + aovs")
         dn.run(x4, aov)
         t0 = time.perf_counter()
         den = dn.run(x4, aov)
@@ -195,6 +202,7 @@ def evaluate(mrt, abi, ctx, dn, spp_ref, seed):
         scene.close()
         r = {"scene": name, "triangles": n, "relmse_4spp": rel_mse(x4, ref), "relmse_denoised_4spp": rel_mse(den, ref),
              "relmse_64spp": rel_mse(x64, ref), "ms_render_4spp": round(ms4, 2), "ms_aov": round(ms_aov, 2),
+             "ms_render_4spp_with_aovs": round(ms4_aov, 2),
              "ms_denoise_device": round(ms_dn, 3), "ms_denoise_call": round(ms_dn_host, 2),
              "ms_render_64spp": round(ms64, 2), "ms_render_ref": round(ms_ref, 1), "spp_ref": spp_ref}
         log(json.dumps(r))
@@ -217,6 +225,8 @@ def main() -> int:
     ap.add_argument("--crop", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--eval-only", default=None, metavar="WEIGHTS",
+                    help="skip training: evaluate this weights file (--out is ignored)")
     args = ap.parse_args()
 
     torch.cuda.init()
@@ -224,6 +234,18 @@ def main() -> int:
     from mobileraytracer_amd import abi
 
     ctx = mrt.Context(0)
+    if args.eval_only:
+        flat_bf16, levels, channels, ds = denoise.load_weights(args.eval_only)
+        dn = mrt.Denoiser(ctx, flat_bf16, mrt.denoiser_desc(levels, channels, ds))
+        res = evaluate(mrt, abi, ctx, dn, args.spp_ref, args.seed)
+        dn.close()
+        if args.eval_json:
+            with np.load(args.eval_only, allow_pickle=False) as z:
+                meta = str(z["meta"])
+            with open(args.eval_json, "w") as f:
+                json.dump({"training": json.loads(meta) if meta else {}, "evaluation": res}, f, indent=1)
+        ctx.close()
+        return 0
     t0 = time.perf_counter()
     x_all, t_all = render_pairs(mrt, abi, ctx, args.scenes, args.poses, args.spp_target, args.seed, args.tri_lo,
                                 args.tri_hi)

@@ -296,3 +296,32 @@ def test_aovs_against_one_hit_shaders(gpu_ctx, spp):
         grey = np.maximum((max_dist - aov[..., 6]) / max_dist, 0)
         hit = aov[..., 6] > 0
         np.testing.assert_allclose(grey[hit], depth_acc[..., 0][hit], rtol=1e-5)
+
+
+@pytest.mark.parametrize("kernel,spp,first", [(abi.KERNEL_WAVEFRONT, 4, 0), (abi.KERNEL_WAVEFRONT, 1, 0),
+                                              (abi.KERNEL_WAVEFRONT, 7, 2), (abi.KERNEL_MEGAKERNEL, 2, 0)])
+def test_render_with_aovs(gpu_ctx, kernel, spp, first):
+    """mrt_render_with_aovs against mrt_render and mrt_render_aovs of the same
+    desc: bitmap, accumulation and AOVs bit-identical.  The wavefront frames
+    record the first hits while shading (4 and 5 samples: sample-parallel
+    slots; 1 sample: one slot per pixel); the megakernel runs the AOV kernel
+    after the frame."""
+    data = mrt.soup_scene(12, 60_000, 600)
+    cam = soup_camera(320, 180)
+    gpu = mrt.Scene(gpu_ctx, data)
+    try:
+        gpu.build(abi.BVH)
+        desc = mrt.render_desc(abi.SHADER_PATHTRACER, 320, 180, spp, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 3),
+                               rr=rnd(4), light=rnd(5), hemi=rnd(6), depth_min=2, depth_max=6, kernel=kernel,
+                               first_sample=first)
+        acc1 = np.zeros((180, 320, 3), np.float32)
+        bm1, _ = gpu.render(desc, accum=acc1)
+        aov1 = gpu.aovs(desc)
+        acc2 = np.zeros((180, 320, 3), np.float32)
+        bm2, aov2, st = gpu.render_with_aovs(desc, accum=acc2)
+    finally:
+        gpu.close()
+    assert np.array_equal(bm1, bm2)
+    assert np.array_equal(acc1, acc2)
+    assert np.array_equal(aov1, aov2)
+    assert 0.05 < (aov2[..., 6] > 0).mean() < 0.95  # the sparse soup: hits and misses
