@@ -342,13 +342,16 @@ MRT_API int mrt_render_with_aovs(mrt_scene* scene, const mrt_render_desc* desc, 
  * submission order, as soon as enough path slots and job memory are free; its
  * paths then run in the same kernel launches as those of the jobs already
  * running, and its bitmap / accumulation end up exactly as mrt_render leaves
- * them for the same request.  Jobs need the wavefront pipeline (PathTracer,
+ * them for the same request.  A job admitted while at least twice its
+ * pixel slots are free runs up to 8 slots per pixel (its samples in
+ * parallel, each sample's radiance kept in the sample memory and folded in
+ * sample order when the job finishes).  Jobs need the wavefront pipeline (PathTracer,
  * samples_light 1, no reflection/transmission, scenes of triangles and lights
  * with a BVH).  All calls are thread-safe. */
 typedef struct mrt_server mrt_server;
 
 typedef struct mrt_server_desc {
-    uint32_t max_slots;        /* path slots (0: 6291456, three 1920x1080 frames) */
+    uint32_t max_slots;        /* path slots (0: 25165824, twelve 1920x1080 frames) */
     uint32_t max_jobs;         /* jobs admitted at once (0: 256)               */
     uint32_t max_scenes;       /* resident scenes (0: 64)                      */
     uint32_t pipelines;        /* concurrent pipelines, 1..3 (0: 3)            */
@@ -357,7 +360,9 @@ typedef struct mrt_server_desc {
                                   accumulation there) (0: 1 GiB)               */
     uint64_t scene_memory;     /* device bytes for the scene arena (0: no limit
                                   beyond the context budget)                   */
-    uint32_t reserved[4];
+    uint64_t sample_memory;    /* device bytes for the per-sample radiance of
+                                  sample-parallel jobs (0: 8 GiB)              */
+    uint32_t reserved[2];
 } mrt_server_desc;
 
 enum mrt_job_state { MRT_JOB_QUEUED = 0, MRT_JOB_RUNNING = 1, MRT_JOB_DONE = 2, MRT_JOB_FAILED = 3 };
@@ -370,6 +375,9 @@ typedef struct mrt_job_info {
     double t_admit_us;         /* admitted (0 while queued),                   */
     double t_gpu_done_us;      /* last sample seen done on the device,         */
     double t_done_us;          /* result in the caller's buffers               */
+    uint32_t lanes;            /* slots per pixel the job ran with (1 while
+                                  queued; > 1: sample-parallel)               */
+    uint32_t reserved;
 } mrt_job_info;
 
 typedef struct mrt_serving_stats {

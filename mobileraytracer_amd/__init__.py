@@ -287,11 +287,11 @@ class RenderServer:
     bitmap (and accumulation) are in the caller's buffers."""
 
     def __init__(self, ctx: Context, max_slots: int = 0, max_jobs: int = 0, max_scenes: int = 0,
-                 pipelines: int = 0, job_memory: int = 0, scene_memory: int = 0):
+                 pipelines: int = 0, job_memory: int = 0, scene_memory: int = 0, sample_memory: int = 0):
         self.ctx = ctx
         d = abi.ServerDesc()
         d.max_slots, d.max_jobs, d.max_scenes, d.pipelines = max_slots, max_jobs, max_scenes, pipelines
-        d.job_memory, d.scene_memory = job_memory, scene_memory
+        d.job_memory, d.scene_memory, d.sample_memory = job_memory, scene_memory, sample_memory
         h = C.c_void_p()
         check(lib().mrt_server_create(ctx.h, C.byref(d), C.byref(h)), "mrt_server_create")
         self.h = h

@@ -43,10 +43,12 @@ using namespace mrt;
 namespace {
 
 constexpr uint32_t kBatch = 4;  // iterations per pipeline between counter snapshots
-constexpr uint32_t kDefaultSlots = 6291456u;
+constexpr uint32_t kDefaultSlots = 25165824u;
 constexpr uint32_t kDefaultJobs = 256u;
 constexpr uint32_t kDefaultScenes = 64u;
 constexpr uint64_t kDefaultJobMemory = 1ull << 30;
+constexpr uint64_t kDefaultSampleMemory = 8ull << 30;
+constexpr uint32_t kMaxJobLanes = 8;
 constexpr uint64_t kAlign = 256;
 
 double now_us() {
@@ -131,7 +133,10 @@ struct Job {
     float* accum = nullptr;
     int mem = MRT_MEM_HOST;
     size_t bm_bytes = 0, acc_bytes = 0;
-    uint32_t waves = 0, live = 0;
+    uint32_t waves = 0, live = 0;     // admitted: waves_px * lanes waves, covered pixels * lanes slots
+    uint32_t waves_px = 0;            // waves of one slot per pixel
+    uint32_t lanes = 1;               // slots per pixel (sample-parallel jobs: > 1)
+    uint64_t rad_off = 0, rad_units = 0;  // sample-memory block: done word, then per-sample radiance
     JobPhase phase = kQueued;
     int status = MRT_OK;
     std::string error;
@@ -182,6 +187,9 @@ struct mrt_server {
     std::vector<Resident> scenes;
     RangeAlloc waves_free[kWfMaxGroups];
     RangeAlloc mem_free;
+    // Per-sample radiance of sample-parallel jobs (device only).
+    DevBuf sample_mem;
+    RangeAlloc sample_free;
     std::vector<uint32_t> free_slots;
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
     std::deque<uint64_t> queue;
@@ -272,6 +280,8 @@ void fail_job(mrt_server* s, Job* j, int status, const std::string& msg) {
 void release_job(mrt_server* s, Job* j) {
     if (j->mem_units) s->mem_free.release(j->mem_off, j->mem_units);
     j->mem_units = 0;
+    if (j->rad_units) s->sample_free.release(j->rad_off, j->rad_units);
+    j->rad_units = 0;
     s->free_slots.push_back(j->slot);
     if (j->done_ev) (void)hipEventDestroy(j->done_ev);
     j->done_ev = nullptr;
@@ -285,11 +295,29 @@ int admit(mrt_server* s) {
         Job* j = s->jobs[s->queue.front()].get();
         uint64_t freeWaves = 0;
         for (uint32_t g = 0; g < s->G; ++g) freeWaves += s->waves_free[g].free_total();
-        if (freeWaves < j->waves || s->free_slots.empty()) break;
+        if (freeWaves < j->waves_px || s->free_slots.empty()) break;
         uint64_t off = 0;
         if (!s->mem_free.alloc(j->mem_bytes / kAlign, &off)) break;
         j->mem_off = off;
         j->mem_units = j->mem_bytes / kAlign;
+        // Sample-parallel job: up to 8 slots per pixel while they take at
+        // most half of the free waves and the per-sample radiance fits.
+        const uint32_t samples = j->desc.samples_pixel - j->desc.first_sample;
+        uint32_t lanes = std::min(kMaxJobLanes, samples);
+        while (lanes > 1u && static_cast<uint64_t>(j->waves_px) * lanes * 2u > freeWaves) --lanes;
+        if (lanes > 1u) {
+            const uint64_t units = align_up(kAlign + 12ull * samples * j->waves_px * 64u) / kAlign;
+            uint64_t roff = 0;
+            if (s->sample_free.alloc(units, &roff)) {
+                j->rad_off = roff;
+                j->rad_units = units;
+            } else {
+                lanes = 1u;
+            }
+        }
+        j->lanes = lanes;
+        j->waves = j->waves_px * lanes;
+        j->live = 256u * j->p.bsx * j->p.bsy * lanes;
         s->queue.pop_front();
         j->slot = s->free_slots.back();
         s->free_slots.pop_back();
@@ -341,6 +369,15 @@ int admit(mrt_server* s) {
         jd.scene = j->scene;
         jd.wide_root = s->scenes[j->scene].wide_base;
         jd.walk_root = s->scenes[j->scene].walk_base;
+        jd.rad = nullptr;
+        jd.lanes = j->lanes;
+        jd.pixel_slots = j->waves_px * 64u;
+        if (j->lanes > 1u) {
+            uint8_t* block = s->sample_mem.as<uint8_t>() + j->rad_off * kAlign;
+            jd.rad = reinterpret_cast<float*>(block + kAlign);
+            const size_t words = 3ull * (j->desc.samples_pixel - j->desc.first_sample) * jd.pixel_slots;
+            MRT_HIP(hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(jd.rad), static_cast<int>(kRadPending), words));
+        }
         if (j->mem == MRT_MEM_DEVICE) {
             jd.bitmap = j->bitmap;
             jd.accum = j->accum;
@@ -425,6 +462,19 @@ int process_snapshot(mrt_server* s, uint32_t snap) {
         j->t_gpu = now_us();
         for (const Run& r : j->runs) s->waves_free[r.pipe].release(r.pool_wave, r.n);
         j->runs.clear();
+        if (j->lanes > 1u) {
+            // Sample-parallel job: fold its samples into the outputs in
+            // sample order (the kernels that wrote them have completed).
+            const JobDev& jd = s->jobs_host[j->slot];
+            WfBuffers wb{};
+            wb.rad = jd.rad;
+            wb.lanes = jd.lanes;
+            wb.pixel_slots = jd.pixel_slots;
+            wb.first = jd.first_sample;
+            uint32_t* doneMin = reinterpret_cast<uint32_t*>(s->sample_mem.as<uint8_t>() + j->rad_off * kAlign);
+            MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(doneMin), static_cast<int>(jd.spp), 1, s->xfer));
+            MRT_HIP(launch_wavefront_resolve(jd.p, jd.spp, wb, jd.bitmap, jd.accum, doneMin, s->xfer));
+        }
         // The snapshot was copied after the kernels that finished the job,
         // and those have completed: the copy needs no stream dependency.
         if (j->mem != MRT_MEM_DEVICE) {
@@ -557,6 +607,8 @@ void fill_info(const Job* j, mrt_job_info* info) {
     info->t_admit_us = j->t_admit;
     info->t_gpu_done_us = j->t_gpu;
     info->t_done_us = j->t_done;
+    info->lanes = j->lanes;
+    info->reserved = 0;
 }
 
 // Copies a resident scene's walk trees into the arena with rebased references.
@@ -677,9 +729,10 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     const uint32_t maxJobs = d.max_jobs ? d.max_jobs : kDefaultJobs;
     const uint32_t maxScenes = d.max_scenes ? d.max_scenes : kDefaultScenes;
     const uint64_t jobMem = align_up(d.job_memory ? d.job_memory : kDefaultJobMemory);
+    const uint64_t sampleMem = align_up(d.sample_memory ? d.sample_memory : kDefaultSampleMemory);
     if (maxJobs > 65536u || maxScenes > 65536u) return fail(MRT_E_INVALID, "max_jobs and max_scenes must be <= 65536");
     const uint64_t need = wavefront_state_bytes(slots) + G * (wavefront_group_bytes(slots / G + 64u) + 256) +
-                          jobMem + maxJobs * (sizeof(JobDev) + 4ull) + maxScenes * sizeof(SceneDev) +
+                          jobMem + sampleMem + maxJobs * (sizeof(JobDev) + 4ull) + maxScenes * sizeof(SceneDev) +
                           (slots / 64ull) * 8ull;
     if (ctx->budget && need > ctx->budget)
         return fail(MRT_E_BUDGET, "render server needs " + std::to_string(need) + " bytes, budget " +
@@ -755,6 +808,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     if ((rc = s->counters.alloc(16 * sizeof(unsigned long long))) != MRT_OK) return bail(rc);
     if ((rc = s->err.alloc(sizeof(uint32_t))) != MRT_OK) return bail(rc);
     if ((rc = s->job_mem.alloc(jobMem)) != MRT_OK) return bail(rc);
+    if ((rc = s->sample_mem.alloc(sampleMem)) != MRT_OK) return bail(rc);
     MRT_HIP(hipMemset(s->counters.p, 0, 16 * sizeof(unsigned long long)));
     MRT_HIP(hipMemset(s->err.p, 0, sizeof(uint32_t)));
     MRT_HIP(hipMemset(s->finished_dev.p, 0, G * maxJobs * sizeof(uint32_t)));
@@ -764,6 +818,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
         return bail(fail(MRT_E_NOMEM, "pinned host allocation failed"));
     std::memset(s->jobs_host, 0, maxJobs * sizeof(JobDev));
     s->mem_free.reset(0, jobMem / kAlign);
+    s->sample_free.reset(0, sampleMem / kAlign);
     for (uint32_t i = 0; i < maxJobs; ++i) s->free_slots.push_back(maxJobs - 1u - i);
     if ((rc = ensure_spill(s, 0u)) != MRT_OK) return bail(rc);
     for (uint32_t g = 0; g < G; ++g) MRT_HIP(hipStreamSynchronize(s->pipe[g]));
@@ -838,7 +893,8 @@ MRT_API int mrt_server_submit(mrt_server* s, uint32_t scene_id, const mrt_render
     j->mem = mem;
     j->bm_bytes = static_cast<size_t>(d->height) * j->p.real_width * sizeof(uint32_t);
     j->acc_bytes = static_cast<size_t>(d->width) * d->height * 3 * sizeof(float);
-    j->waves = wavefront_slots(j->p) / 64u;
+    j->waves_px = wavefront_slots(j->p) / 64u;
+    j->waves = j->waves_px;
     j->live = 256u * j->p.bsx * j->p.bsy;
     uint64_t bytes = align_up(kBlocks * sizeof(uint32_t));
     if (mem == MRT_MEM_HOST) {
@@ -853,7 +909,7 @@ MRT_API int mrt_server_submit(mrt_server* s, uint32_t scene_id, const mrt_render
     std::lock_guard<std::mutex> lock(s->mu);
     if (s->broken) return fail(MRT_E_HIP, "render server stopped after an error: " + s->broken_msg);
     if (s->shutdown) return fail(MRT_E_INVALID, "render server is shutting down");
-    if (j->waves > s->pool_waves) return fail(MRT_E_BUDGET, "frame needs more path slots than the server has");
+    if (j->waves_px > s->pool_waves) return fail(MRT_E_BUDGET, "frame needs more path slots than the server has");
     if (bytes > s->job_memory) return fail(MRT_E_BUDGET, "frame outputs exceed the server's job memory");
     j->id = s->next_id++;
     j->t_submit = now_us();

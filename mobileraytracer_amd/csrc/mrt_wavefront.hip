@@ -160,13 +160,17 @@ __global__ __launch_bounds__(kBlock) void k_wf_admit(BatchDev bd, JobDev J, uint
         const uint32_t wv = pool_wave + wl;
         if (lane == 0u) bd.waves[wv] = make_uint2(job, job_wave + wl);
         slot = (wv << 6) | lane;
+        // Job slot -> (lane k, pixel slot ps): sample first_sample + k.
+        const uint32_t local = ((job_wave + wl) << 6) | lane;
+        const uint32_t k = J.rad ? local / J.pixel_slots : 0u;
+        const uint32_t ps = local - k * J.pixel_slots;
         uint32_t x, y;
-        slot_pixel(J.p, ((job_wave + wl) << 6) | lane, &x, &y);
+        slot_pixel(J.p, J.rad ? ps : local, &x, &y);
         live = x < 16u * J.p.bsx && y < 16u * J.p.bsy;
         if (live) {
             const uint32_t pid = y * J.p.width + x;
-            primary_ray(J.p, x, y, pid, J.first_sample, &o, &d);
-            reset_path(b, slot, pid, J.first_sample);
+            primary_ray(J.p, x, y, pid, J.first_sample + k, &o, &d);
+            reset_path(b, slot, pid, J.first_sample + k);
         }
     }
     const uint32_t qe = wave_append(count, live);
@@ -500,6 +504,16 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p0, uint32_t s
                 r[1] = c.y;
                 r[2] = c.z;
                 step = b.lanes;
+            } else if (MULTI && J->rad) {
+                // Sample-parallel job: its resolve folds the samples.
+                const uint32_t lane = local / J->pixel_slots;
+                const uint32_t ps = local - lane * J->pixel_slots;
+                slot_pixel(p, ps, &x, &y);
+                float* r = J->rad + 3ull * ((sample - J->first_sample) * static_cast<uint64_t>(J->pixel_slots) + ps);
+                r[0] = c.x;
+                r[1] = c.y;
+                r[2] = c.z;
+                step = J->lanes;
             } else {
                 slot_pixel(p, local, &x, &y);
                 const uint32_t bi = y * p.real_width + x;

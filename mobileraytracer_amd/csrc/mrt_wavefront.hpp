@@ -78,6 +78,13 @@ struct JobDev {
     uint32_t walk_root;     // arena pair of the scene's closest-hit root (any-hit root: + 1)
     uint32_t* bitmap;       // RGBA8, p.real_width pixels per row
     float* accum;           // optional FP32 radiance sums
+    // Sample-parallel jobs: job slot k * pixel_slots + ps runs samples
+    // first_sample + k, + k + lanes, ... of pixel slot ps and leaves each
+    // sample's radiance at rad[3 * ((sample - first_sample) * pixel_slots +
+    // ps)] for the job's resolve.  rad == nullptr: lanes == 1, samples are
+    // folded into bitmap / accum as they finish.
+    float* rad;
+    uint32_t lanes, pixel_slots;
 };
 
 // Device tables of a batch.  Every slot of pool wave w belongs to job

@@ -80,9 +80,12 @@ def test_server_mix_equals_render_alone(gpu_ctx, scenes, pipelines, max_slots):
             jobs.append((si, desc, srv.submit(ids[si], desc, accum=acc)))
         print(f"submitted {len(jobs)} jobs (pipelines {pipelines}, slots {max_slots})", flush=True)
         rays = np.zeros(3, np.int64)
+        lanes = []
         for k, (si, desc, jid) in enumerate(jobs):
             bm, acc, info = srv.wait(jid)
             assert info.state == abi.JOB_DONE and info.status == abi.MRT_OK
+            assert 1 <= info.lanes <= min(8, desc.samples_pixel)
+            lanes.append(info.lanes)
             assert info.t_submit_us <= info.t_admit_us <= info.t_gpu_done_us <= info.t_done_us
             wb, wa, st = alone(scenes[si], desc)
             rays += (st.rays_camera, st.rays_extension, st.rays_shadow)
@@ -92,6 +95,10 @@ def test_server_mix_equals_render_alone(gpu_ctx, scenes, pipelines, max_slots):
               f"batches {stt.batches}", flush=True)
         assert (stt.rays_camera, stt.rays_extension, stt.rays_shadow) == tuple(int(r) for r in rays)
         assert stt.jobs_done == len(MIX) and stt.jobs_failed == 0 and stt.slots_used == 0
+        print(f"slots per pixel of the jobs: {lanes}", flush=True)
+        if max_slots == 0:
+            # The default pool: multi-sample jobs ran sample-parallel.
+            assert max(lanes) > 1
     finally:
         srv.close()
 
