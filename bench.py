@@ -445,7 +445,8 @@ def denoiser_leg(mrt, abi, ctx, torch, reps: int, scene=None, desc4=None) -> dic
 def batched(mrt, abi) -> int:
     """64 independent scenes (log-uniform 10k-4M triangles, alternately soups
     and displaced-icosphere clusters) rendered as one batch through the
-    render server, whose shape the autotuner picks on a 1-spp probe batch;
+    render server, whose shape the autotuner picks on a probe batch (the
+    requests at up to 4 spp);
     compared with the same requests rendered one after another by mrt_render."""
     from mobileraytracer_amd import serving
 

@@ -170,11 +170,15 @@ def batch_requests(pool: list[PoolScene], seed: int) -> list:
 
 
 def run_batch(srv: RenderServer, pool: list[PoolScene], reqs: list) -> dict:
-    """Submits every request at once and waits for all: makespan and Mrays/s."""
+    """Submits every request at once, largest first (pixels x samples: the
+    longest jobs start while the pool is emptiest, the usual makespan
+    heuristic; no pixel depends on the order), and waits for all: makespan
+    and Mrays/s."""
     st0 = srv.stats()
+    order = sorted(reqs, key=lambda r: -int(r[1].width) * int(r[1].height) * int(r[1].samples_pixel))
     t0 = time.perf_counter()
     ids = []
-    for si, desc in reqs:
+    for si, desc in order:
         bm = np.zeros((desc.height, desc.width), np.uint32)
         acc = np.zeros((desc.height, desc.width, 3), np.float32)
         ids.append(srv.submit(pool[si].server_id, desc, bitmap=bm, accum=acc))
@@ -192,15 +196,17 @@ def run_batch(srv: RenderServer, pool: list[PoolScene], reqs: list) -> dict:
 
 def tune_server(ctx, pool: list[PoolScene], reqs: list, candidates=None) -> tuple[dict, list]:
     """Autotuner of the server shape (pipelines, slot-pool size) on a probe
-    batch: the requests at 1 spp.  These choices change speed only, never a
-    pixel.  Returns the best configuration and every probe's result."""
+    batch: the requests at min(spp, 4) samples (enough for jobs to run
+    sample-parallel where the pool allows).  These choices change speed only,
+    never a pixel.  Returns the best configuration and every probe's
+    result."""
     if candidates is None:
         candidates = [dict(pipelines=p, max_slots=s) for p in (1, 2, 3)
-                      for s in (2_097_152, 6_291_456, 12_582_912)]
+                      for s in (2_097_152, 6_291_456, 12_582_912, 25_165_824)]
     probe = []
     for si, d in reqs:
         q = abi.RenderDesc.from_buffer_copy(d)
-        q.samples_pixel = 1
+        q.samples_pixel = min(int(d.samples_pixel), 4)
         probe.append((si, q))
     results = []
     for cand in candidates:
