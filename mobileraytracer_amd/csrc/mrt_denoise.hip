@@ -42,13 +42,17 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 __device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
 
-// Round to nearest even (finite inputs).
-__device__ __forceinline__ uint32_t f_to_bf16(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return f_to_bf16(lo) | (f_to_bf16(hi) << 16); }
+// Two floats to packed BF16 (round to nearest even: v_cvt_pk_bf16_f32), lo in
+// the low half.
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+    const bf16x2 b = __builtin_convertvector(floatx2{lo, hi}, bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &b, sizeof(u));
+    return u;
+}
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
     bf16x8 r;
@@ -56,8 +60,20 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
     return r;
 }
 
+// 16-byte load at a byte offset from a (wave-uniform) base.
+__device__ __forceinline__ uint4 ld16(const uint16_t* base, uint32_t byte_off) {
+    return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Wave-uniform value in a scalar register.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 constexpr uint32_t kTileRows = 4, kTileW = 64, kHaloW = kTileW + 2, kRowQuads = 5;
 constexpr uint32_t kTileQuads = (kTileRows + 2) * kHaloW * kRowQuads;
+
+// What a convolution reads and writes: BF16 tensors; the packed network input
+// (first layer); FP32 radiance out (last layer).
+enum : int { kModeTensor = 0, kModePackedIn = 1, kModeRadOut = 2 };
 
 // Stages the packed network input (32 channels, 0..9 used) of the tile rows
 // and halo: log(1 + max(rad * rad_scale, 0)), aov[0..5], aov[6] *
@@ -101,95 +117,117 @@ __device__ __forceinline__ void stage_packed(const ConvArgs& a, uint4* tile, uin
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+// Addressing: every row of the tile (input row, output row of a wave) is a
+// wave-uniform base pointer in scalar registers plus a 32-bit per-lane
+// element offset within the row, so that loads and stores take the
+// scalar-base + vector-offset form and no lane does 64-bit index arithmetic.
+// Occupancy: NT = 32 workgroups (31.7 KB of LDS) are held to 168 registers
+// per lane, 3 waves per SIMD (2 with the compiler's default register use);
+// NT = 64 workgroups (77.8 KB of LDS) fit twice per compute unit anyway.
+template <int NT, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 3 : 2))) void k_conv3x3(ConvArgs a) {
     constexpr int NJ = NT / 16;
     constexpr bool kWeightsLds = NT > 32;
+    static_assert(NT == 32 || NT == 64, "32 or 64 output channels per workgroup");
     __shared__ uint4 smem[kTileQuads + (kWeightsLds ? 9 * NT * kRowQuads : 0)];
     uint4* tile = smem;
     uint4* wts = smem + kTileQuads;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = uniform(tid >> 6);
     const uint32_t li = lane & 15u, lg = lane >> 4;
     const int x0 = static_cast<int>(blockIdx.x * kTileW), y0 = static_cast<int>(blockIdx.y * kTileRows);
     const uint32_t n0 = blockIdx.z * NT;
     const int H = static_cast<int>(a.H), W = static_cast<int>(a.W);
+    // Accumulators start at the bias: output channels n0 + 16 j + 4 lg + v.
     floatx4 acc[NJ][4];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j)
+    for (int j = 0; j < NJ; ++j) {
+        floatx4 b{0.0f, 0.0f, 0.0f, 0.0f};
+        if (a.bias) {
+            const float* bj = a.bias + n0 + 16u * j + 4u * lg;
+            b = floatx4{bj[0], bj[1], bj[2], bj[3]};
+        }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) acc[j][m] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    // Staging: all of a chunk's loads are issued before any of them is
-    // written to LDS, and the next chunk's loads are in flight while the
-    // current chunk's MFMA loop runs.
-    constexpr uint32_t kStageQuads = (kTileRows + 2) * kHaloW * 4u;
-    constexpr uint32_t kStageIters = (kStageQuads + 255u) / 256u;
-    constexpr uint32_t kWQuads = 9u * NT * 4u;
-    constexpr uint32_t kWIters = kWeightsLds ? (kWQuads + 255u) / 256u : 1u;
-    uint4 sv[kStageIters];
-    uint4 wq[kWIters];
+        for (int m = 0; m < 4; ++m) acc[j][m] = b;
+    }
+    // Input staging: thread t carries quad (t & 3) of halo column t >> 2 of
+    // each of the 6 tile rows; threads 0..47 also carry the 2 remaining
+    // columns (quad t & 3 of column 64 + ((t >> 2) & 1) of row t >> 3).  All
+    // of a chunk's loads are issued before any of them is written to LDS, and
+    // the next chunk's loads are in flight while the current chunk's MFMA
+    // loop runs.
+    constexpr uint32_t kRows = kTileRows + 2;
+    const uint32_t q = tid & 3u, cm = tid >> 2;
+    const int xm = x0 - 1 + static_cast<int>(cm);
+    const bool xm_in = xm >= 0 && xm < W;
+    const bool has_e = tid < 8u * kRows;
+    const uint32_t re = tid >> 3, ce = kTileW + ((tid >> 2) & 1u);
+    const int xe = x0 - 1 + static_cast<int>(ce), ye = y0 - 1 + static_cast<int>(re);
+    const bool e_in = has_e && xe < W && ye >= 0 && ye < H;
+    // Per-lane element offsets within a row (tensor part, upsampled part).
+    const uint32_t xs = a.x_stride, cu = a.cu;
+    const uint32_t om = 2u * ((xm_in ? static_cast<uint32_t>(xm) : 0u) * xs + 8u * q);
+    const uint32_t oum = 2u * ((xm_in ? static_cast<uint32_t>(xm) >> 1 : 0u) * cu + 8u * q);
+    const uint32_t oe = 2u * ((e_in ? static_cast<uint32_t>(xe) : 0u) * xs + 8u * q);
+    const uint32_t oue = 2u * ((e_in ? static_cast<uint32_t>(xe) >> 1 : 0u) * cu + 8u * q);
+    const uint32_t ye_c = e_in ? static_cast<uint32_t>(ye) : 0u;
+    uint4 sv[kRows + 1];
     auto load_input = [&](uint32_t kc) {
-        const bool up = kc < a.cu;
+        const bool up = kc < cu;
+        const uint16_t* xb = up ? a.xu + kc : a.x + a.x_off + (kc - cu);
+        const size_t row_el = up ? static_cast<size_t>(a.W >> 1) * cu : static_cast<size_t>(a.W) * xs;
+        const uint32_t o_m = up ? oum : om;
 #pragma unroll
-        for (uint32_t it = 0; it < kStageIters; ++it) {
-            const uint32_t p = tid + it * 256u;
-            sv[it] = make_uint4(0u, 0u, 0u, 0u);
-            if (p >= kStageQuads) continue;
-            const uint32_t q = p & 3u, px = p >> 2;
-            const uint32_t r = px / kHaloW, c = px % kHaloW;
-            const int yy = y0 - 1 + static_cast<int>(r), xx = x0 - 1 + static_cast<int>(c);
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                const uint16_t* src =
-                    up ? a.xu + (static_cast<size_t>(yy >> 1) * (a.W >> 1) + (xx >> 1)) * a.cu + kc
-                       : a.x + (static_cast<size_t>(yy) * a.W + xx) * a.x_stride + a.x_off + (kc - a.cu);
-                sv[it] = *reinterpret_cast<const uint4*>(src + q * 8u);
+        for (uint32_t r = 0; r < kRows; ++r) {
+            const int yy = y0 - 1 + static_cast<int>(r);
+            sv[r] = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < H) {
+                const uint16_t* rb = xb + static_cast<size_t>(up ? yy >> 1 : yy) * row_el;
+                if (xm_in) sv[r] = ld16(rb, o_m);
             }
         }
+        sv[kRows] = make_uint4(0u, 0u, 0u, 0u);
+        if (e_in)
+            sv[kRows] = ld16(xb + static_cast<size_t>(up ? ye_c >> 1 : ye_c) * row_el, up ? oue : oe);
     };
+    // Weights of a chunk.  NT = 64: LDS, row tap * 64 + n of the chunk from
+    // thread quads (tap = round, n = t >> 2); NT = 32: registers, lane (li,
+    // lg) holding cin 8 lg .. + 7 of output channel n0 + 16 j + li per tap.
+    const uint32_t cin = a.cin, cout = a.cout;
+    constexpr uint32_t kWIters = kWeightsLds ? 9u : 1u;
+    uint4 wq[kWIters];
+    const uint32_t ow = 2u * (cm * cin + 8u * q);
     auto load_weights_lds = [&](uint32_t kc) {
 #pragma unroll
-        for (uint32_t it = 0; it < kWIters; ++it) {
-            const uint32_t p = tid + it * 256u;
-            wq[it] = make_uint4(0u, 0u, 0u, 0u);
-            if (p >= kWQuads) continue;
-            const uint32_t q = p & 3u, row = p >> 2;  // row = tap * NT + n
-            const uint32_t tap = row / NT, n = row % NT;
-            wq[it] = *reinterpret_cast<const uint4*>(a.w + (static_cast<size_t>(tap) * a.cout + n0 + n) * a.cin + kc +
-                                                     q * 8u);
-        }
+        for (uint32_t tap = 0; tap < kWIters; ++tap)
+            wq[tap] = ld16(a.w + kc + static_cast<size_t>(tap * cout + n0) * cin, ow);
     };
-    if (!a.rad) load_input(0);
+    const uint32_t owr = 2u * (li * cin + 8u * lg);
+    if constexpr (MODE != kModePackedIn) load_input(0);
     if constexpr (kWeightsLds) load_weights_lds(0);
-    for (uint32_t kc = 0; kc < a.cin; kc += 32u) {
+    for (uint32_t kc = 0; kc < cin; kc += 32u) {
         uint4 wr[kWeightsLds ? 1 : 9][NJ];
         if constexpr (!kWeightsLds) {
-            const uint16_t* wk = a.w + static_cast<size_t>(n0 + li) * a.cin + kc + lg * 8u;
 #pragma unroll
             for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j)
-                    wr[tap][j] =
-                        *reinterpret_cast<const uint4*>(wk + (static_cast<size_t>(tap) * a.cout + 16u * j) * a.cin);
+                    wr[tap][j] = ld16(a.w + kc + static_cast<size_t>(tap * cout + n0 + 16u * j) * cin, owr);
         }
         __syncthreads();
-        if (a.rad) {
+        if constexpr (MODE == kModePackedIn) {
             stage_packed(a, tile, tid, x0, y0);
         } else {
 #pragma unroll
-            for (uint32_t it = 0; it < kStageIters; ++it) {
-                const uint32_t p = tid + it * 256u;
-                if (p < kStageQuads) tile[(p >> 2) * kRowQuads + (p & 3u)] = sv[it];
-            }
+            for (uint32_t r = 0; r < kRows; ++r) tile[(r * kHaloW + cm) * kRowQuads + q] = sv[r];
+            if (has_e) tile[(re * kHaloW + ce) * kRowQuads + q] = sv[kRows];
         }
         if constexpr (kWeightsLds) {
 #pragma unroll
-            for (uint32_t it = 0; it < kWIters; ++it) {
-                const uint32_t p = tid + it * 256u;
-                if (p < kWQuads) wts[(p >> 2) * kRowQuads + (p & 3u)] = wq[it];
-            }
+            for (uint32_t tap = 0; tap < kWIters; ++tap) wts[(tap * NT + cm) * kRowQuads + q] = wq[tap];
         }
         __syncthreads();
-        if (kc + 32u < a.cin) {
-            if (!a.rad) load_input(kc + 32u);
+        if (kc + 32u < cin) {
+            if constexpr (MODE != kModePackedIn) load_input(kc + 32u);
             if constexpr (kWeightsLds) load_weights_lds(kc + 32u);
         }
 #pragma unroll
@@ -215,38 +253,38 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     }
     const int y = y0 + static_cast<int>(wv);
     if (y >= H) return;
-    if (a.rad_out) {
+    if constexpr (MODE == kModeRadOut) {
         // Output channels 0..2 (lanes 0..15 of the first channel tile):
         // radiance.
         if (n0 != 0 || lg != 0u || y >= static_cast<int>(a.fh)) return;
+        float* ob = a.rad_out + static_cast<size_t>(y) * a.fw * 3u;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const int x = x0 + 16 * m + static_cast<int>(li);
-            if (x >= static_cast<int>(a.fw)) continue;
-            float* o = a.rad_out + (static_cast<size_t>(y) * a.fw + x) * 3;
+            const uint32_t x = static_cast<uint32_t>(x0) + 16u * m + li;
+            if (x >= a.fw) continue;
+            float* o = ob + 3u * x;
 #pragma unroll
-            for (int v = 0; v < 3; ++v) o[v] = fmaxf(expm1f(acc[0][m][v] + (a.bias ? a.bias[v] : 0.0f)), 0.0f);
+            for (int v = 0; v < 3; ++v) o[v] = fmaxf(expm1f(acc[0][m][v]), 0.0f);
         }
-        return;
-    }
+    } else {
+        if (a.relu) {
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-        const uint32_t co = n0 + 16u * j + 4u * lg;
-        float b[4];
+            for (int j = 0; j < NJ; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) b[v] = a.bias ? a.bias[co + v] : 0.0f;
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[j][m][v] = fmaxf(acc[j][m][v], 0.0f);
+        }
+        uint16_t* yb = a.y + static_cast<size_t>(y) * a.W * a.y_stride + a.y_off + n0;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const int x = x0 + 16 * m + static_cast<int>(li);
-            if (x >= W) continue;
-            float r[4];
+            const uint32_t x = static_cast<uint32_t>(x0) + 16u * m + li;
+            if (x >= a.W) continue;
+            const uint32_t o = x * a.y_stride + 4u * lg;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                r[v] = acc[j][m][v] + b[v];
-                if (a.relu) r[v] = fmaxf(r[v], 0.0f);
-            }
-            *reinterpret_cast<uint2*>(a.y + (static_cast<size_t>(y) * a.W + x) * a.y_stride + a.y_off + co) =
-                make_uint2(pack2(r[0], r[1]), pack2(r[2], r[3]));
+            for (int j = 0; j < NJ; ++j)
+                *reinterpret_cast<uint2*>(yb + o + 16u * j) =
+                    make_uint2(pack2(acc[j][m][0], acc[j][m][1]), pack2(acc[j][m][2], acc[j][m][3]));
         }
     }
 }
@@ -274,9 +312,9 @@ __global__ void k_pool2(const uint16_t* __restrict__ x, uint32_t c, uint32_t H, 
     *reinterpret_cast<uint4*>(y + i * 8u) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-template <int NT>
+template <int NT, int MODE>
 void conv_launch(const ConvArgs& a, dim3 grid, hipStream_t stream) {
-    k_conv3x3<NT><<<grid, 256, 0, stream>>>(a);
+    k_conv3x3<NT, MODE><<<grid, 256, 0, stream>>>(a);
 }
 
 }  // namespace
@@ -288,16 +326,25 @@ hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t st
               : (a.cin > a.cu && (!a.x || a.x_off + (a.cin - a.cu) > a.x_stride)))
         return hipErrorInvalidValue;
     if (a.cu && (a.H % 2u || a.W % 2u || !a.xu)) return hipErrorInvalidValue;
+    // 16-byte loads of inputs and weights, 8-byte stores of outputs.
+    const auto misaligned = [](const void* p, uintptr_t n) { return (reinterpret_cast<uintptr_t>(p) & (n - 1)) != 0; };
+    if (misaligned(a.x, 16) || misaligned(a.xu, 16) || misaligned(a.w, 16) || misaligned(a.y, 8) ||
+        misaligned(a.bias, 4))
+        return hipErrorInvalidValue;
     if (a.rad_out ? (a.fw > a.W || a.fh > a.H)
                   : (!a.y || a.y_off % 4u || a.y_stride % 4u || a.y_off + a.cout > a.y_stride))
         return hipErrorInvalidValue;
     const uint32_t tiles = ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileRows - 1) / kTileRows);
     const uint32_t nt = (a.cout % 64u == 0 && tiles * (a.cout / 64u) >= min_blocks) ? 64u : 32u;
     dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileRows - 1) / kTileRows, a.cout / nt);
-    if (nt == 64u)
-        conv_launch<64>(a, grid, stream);
+    if (a.rad)
+        conv_launch<32, kModePackedIn>(a, dim3(grid.x, grid.y, a.cout / 32u), stream);
+    else if (a.rad_out)
+        conv_launch<32, kModeRadOut>(a, dim3(grid.x, grid.y, a.cout / 32u), stream);
+    else if (nt == 64u)
+        conv_launch<64, kModeTensor>(a, grid, stream);
     else
-        conv_launch<32>(a, grid, stream);
+        conv_launch<32, kModeTensor>(a, grid, stream);
     return hipGetLastError();
 }
 
