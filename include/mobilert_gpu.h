@@ -464,7 +464,9 @@ MRT_API int mrt_denoiser_destroy(mrt_denoiser* dn);
 /* One layer of the network on its own: a 3x3 convolution (zero padding 1) of
  * x (height x width x cin BF16, NHWC) with w ([3][3][cout][cin] BF16, tap
  * (dy, dx) major) and bias (cout FP32, or NULL), ReLU when relu is set, into
- * y (height x width x cout BF16).  cin and cout multiples of 32. */
+ * y (height x width x cout BF16).  cin and cout multiples of 32.  Device
+ * buffers (mem = MRT_MEM_DEVICE): x and w 16-byte aligned, y 8-byte aligned,
+ * bias 4-byte aligned (MRT_E_INVALID otherwise). */
 MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, uint32_t width, uint32_t cin,
                              const uint16_t* w, const float* bias, uint32_t cout, int relu, uint16_t* y, int mem);
 

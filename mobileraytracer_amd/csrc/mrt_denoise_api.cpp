@@ -316,6 +316,10 @@ MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, u
         return fail(MRT_E_INVALID, "cin and cout must be multiples of 32 up to 1024");
     if (width == 0 || height == 0 || width > 16384 || height > 16384)
         return fail(MRT_E_INVALID, "frame size must be 1..16384 per side");
+    if (mem == MRT_MEM_DEVICE &&
+        ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(w)) % 16 || reinterpret_cast<uintptr_t>(y) % 8 ||
+         reinterpret_cast<uintptr_t>(bias) % 4))
+        return fail(MRT_E_INVALID, "device buffers must be aligned: x and w to 16 bytes, y to 8, bias to 4");
     std::lock_guard<std::mutex> lock(ctx->mu);
     MRT_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;

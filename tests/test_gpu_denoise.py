@@ -72,6 +72,25 @@ def test_conv3x3_rejects_bad_shapes(gpu_ctx):
         mrt.conv3x3_bf16(gpu_ctx, x, np.zeros((3, 3, 32, 48), np.uint16), None, False)
 
 
+def test_conv3x3_device_buffers_checked_for_alignment(gpu_ctx):
+    """Device buffers off the 16-byte (x, w) or 8-byte (y) alignment the
+    kernel's vector loads and stores need are refused before any launch."""
+    h, w, cin, cout = 4, 8, 32, 32
+    bufs = [DeviceArray(h * w * cin * 2 + 64), DeviceArray(9 * cout * cin * 2 + 64), DeviceArray(h * w * cout * 2 + 64)]
+    try:
+        base = [b.data_ptr() for b in bufs]
+        lib = abi.lib()
+        for off in ((8, 0, 0), (0, 4, 0), (0, 0, 4)):
+            x, wt, y = (p + o for p, o in zip(base, off))
+            rc = lib.mrt_conv3x3_bf16(gpu_ctx.h, x, h, w, cin, wt, None, cout, 0, y, abi.MEM_DEVICE)
+            assert rc == abi.MRT_E_INVALID, off
+        rc = lib.mrt_conv3x3_bf16(gpu_ctx.h, base[0], h, w, cin, base[1], None, cout, 0, base[2], abi.MEM_DEVICE)
+        assert rc == abi.MRT_OK
+    finally:
+        for b in bufs:
+            b.free()
+
+
 def random_net(levels, channels, seed):
     torch = _torch()
     torch.manual_seed(seed)
