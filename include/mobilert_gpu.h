@@ -236,7 +236,8 @@ typedef struct mrt_accel_params {
  * restatement and a binned-SAH 4-wide walk tree over its leaves; GPU: the
  * same binary tree built on the GPU (mrt_bvh_build_gpu) and a 4-wide walk
  * tree collapsed from its pairs (each wide node = a pair with its interior
- * children expanded); both walk trees give the same hits (every interior box
+ * children expanded), all written on the device (the host adds the roots
+ * and the light part); both walk trees give the same hits (every interior box
  * is the exact union of the reference leaves below it).  RegularGrid, GPU:
  * the cell lists of mrt_grid_build_gpu, identical to the host's.  AUTO: GPU
  * for categories of 65536 primitives or more, HOST below.  A GPU build that

@@ -29,7 +29,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 GPU_DEVICE_SOURCES = ["mrt_kernels.hip", "mrt_wavefront.hip", "mrt_denoise.hip", "mrt_bvh_gpu.hip",
-                      "mrt_grid_gpu.hip"]
+                      "mrt_grid_gpu.hip", "mrt_walk_gpu.hip"]
 GPU_HOST_SOURCES = ["mrt_api.cpp", "mrt_build.cpp", "mrt_scenes.cpp", "mrt_server.cpp", "mrt_obj.cpp",
                     "mrt_denoise_api.cpp"]
 HOST_API_SOURCES = ["host/mobilert_api.cpp", "host/c_wrapper.cpp"]

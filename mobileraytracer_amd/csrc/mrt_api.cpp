@@ -109,9 +109,21 @@ WideSlot pair_child(const BNode& p, int side) {
     return {Box{f3{p.q1.z, p.q1.w, p.q2.x}, f3{p.q2.y, p.q2.z, p.q2.w}}, p.q3.z, p.q3.w};
 }
 
+// Walk-tree pairs by index: [0, lo_n) from lo and, when hi is set, [hi_base,
+// hi_base + hi_n) from hi (the root pairs and the light pairs of a walk tree
+// whose triangle pairs live on the device only).
+struct PairView {
+    const BNode* lo;
+    uint32_t lo_n;
+    const BNode* hi = nullptr;
+    uint32_t hi_base = 0, hi_n = 0;
+    const BNode& operator[](uint32_t i) const { return i < lo_n ? lo[i] : hi[i - hi_base]; }
+    uint32_t size() const { return hi ? hi_base + hi_n : lo_n; }
+};
+
 // Child slots of the wide node made from pair `pi`: each leaf or empty child
 // as it is, each interior child replaced by its two children.
-int wide_slots(const std::vector<BNode>& w, uint32_t pi, WideSlot* out) {
+int wide_slots(const PairView& w, uint32_t pi, WideSlot* out) {
     int n = 0;
     for (int side = 0; side < 2; ++side) {
         const WideSlot c = pair_child(w[pi], side);
@@ -125,6 +137,54 @@ int wide_slots(const std::vector<BNode>& w, uint32_t pi, WideSlot* out) {
     return n;
 }
 
+// Depth-first collapse of the pairs below `top` into wide nodes numbered
+// from `first` (each a pair with its interior children expanded), written to
+// `nodes` from its start; top slots sit at level 2, below the roots, and
+// *max_level receives the deepest level reached.  `index` maps pairs to
+// their wide nodes (kNone: not yet numbered).
+uint32_t collapse_pairs(const PairView& w, std::vector<WideSlot>* top, uint32_t first, std::vector<float4>* nodes,
+                        std::vector<uint32_t>* index, uint32_t* max_level) {
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t next = first;
+    std::vector<std::pair<uint32_t, uint32_t>> todo;  // (pair, level)
+    auto map_slot = [&](WideSlot c, uint32_t level) {
+        if (c.word != 0u || std::isnan(c.box.mn.x)) return c;
+        if ((*index)[c.ref] == kNone) {
+            (*index)[c.ref] = next++;
+            todo.emplace_back(c.ref, level);
+        }
+        c.ref = (*index)[c.ref];
+        return c;
+    };
+    for (WideSlot& c : *top) c = map_slot(c, 2u);
+    while (!todo.empty()) {
+        const auto [pi, level] = todo.back();
+        todo.pop_back();
+        *max_level = std::max(*max_level, level);
+        WideSlot sl[4];
+        const int n = wide_slots(w, pi, sl);
+        for (int k = 0; k < n; ++k) sl[k] = map_slot(sl[k], level + 1u);
+        put_wide_node(nodes, (*index)[pi] - first, sl, n);
+    }
+    return next - first;
+}
+
+// The light slots of root 0: the light root's children (interior root), the
+// root itself (a leaf: the linear light scan), or none.
+std::vector<WideSlot> light_top(const PairView& w) {
+    std::vector<WideSlot> top;
+    if (!is_nan_box(w[kWalkClosestRoot], 1)) {
+        const WideSlot c = pair_child(w[kWalkClosestRoot], 1);
+        if (c.word != 0u) {
+            top.push_back(c);
+        } else {
+            top.push_back(pair_child(w[c.ref], 0));
+            top.push_back(pair_child(w[c.ref], 1));
+        }
+    }
+    return top;
+}
+
 // Builds the wide walk tree (layout in mrt_scene.hpp) for rays whose inverse
 // direction is finite in every axis.  For such rays the reference slab test
 // is monotone under box inclusion (the products (bound - o) * inv round
@@ -133,52 +193,16 @@ int wide_slots(const std::vector<BNode>& w, uint32_t pi, WideSlot* out) {
 // those whose own boxes it passes, whatever tree is built above them, as long
 // as every interior box is the exact union of the leaf boxes below it.  The
 // triangle part is therefore a binned-SAH 4-wide tree over the reference
-// BVH's leaves (each kept with its own box); the light part collapses the
-// light tree's pairs (each interior child replaced by its two children).
+// BVH's leaves (each kept with its own box), or (collapse) the reference
+// pairs collapsed; the light part collapses the light tree's pairs (each
+// interior child replaced by its two children).
 // Nodes: 0 and 1 the roots, then the triangle nodes, then the light nodes.
-int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri, hipStream_t st, bool collapse) {
+int build_wide_tree(mrt_scene* s, const std::vector<BNode>& wv, const BvhOut& tri, hipStream_t st, bool collapse) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
-    // Light slots of root 0: the light root's children (interior root), the
-    // root itself (a leaf: the linear light scan), or none.
-    std::vector<WideSlot> lightTop;
-    if (!is_nan_box(w[kWalkClosestRoot], 1)) {
-        const WideSlot c = pair_child(w[kWalkClosestRoot], 1);
-        if (c.word != 0u) {
-            lightTop.push_back(c);
-        } else {
-            lightTop.push_back(pair_child(w[c.ref], 0));
-            lightTop.push_back(pair_child(w[c.ref], 1));
-        }
-    }
-    // Depth-first collapse of the pairs below `top` into wide nodes numbered
-    // from `first` (each a pair with its interior children expanded); top
-    // slots sit at level 2, below the roots.
+    const PairView w{wv.data(), static_cast<uint32_t>(wv.size())};
+    std::vector<WideSlot> lightTop = light_top(w);
     std::vector<uint32_t> index(w.size(), kNone);
     uint32_t maxDepth = 1u;
-    auto collapse_pairs = [&](std::vector<WideSlot>* top, uint32_t first, std::vector<float4>* nodes) {
-        uint32_t next = first;
-        std::vector<std::pair<uint32_t, uint32_t>> todo;  // (pair, level)
-        auto map_slot = [&](WideSlot c, uint32_t level) {
-            if (c.word != 0u || std::isnan(c.box.mn.x)) return c;
-            if (index[c.ref] == kNone) {
-                index[c.ref] = next++;
-                todo.emplace_back(c.ref, level);
-            }
-            c.ref = index[c.ref];
-            return c;
-        };
-        for (WideSlot& c : *top) c = map_slot(c, 2u);
-        while (!todo.empty()) {
-            const auto [pi, level] = todo.back();
-            todo.pop_back();
-            maxDepth = std::max(maxDepth, level);
-            WideSlot sl[4];
-            const int n = wide_slots(w, pi, sl);
-            for (int k = 0; k < n; ++k) sl[k] = map_slot(sl[k], level + 1u);
-            put_wide_node(nodes, index[pi] - first, sl, n);
-        }
-        return next - first;
-    };
     std::vector<float4> triNodes, out(16, make_float4(0, 0, 0, 0));
     std::vector<WideSlot> triTop;
     uint32_t nTriNodes = 0;
@@ -194,7 +218,7 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri
                 triTop.push_back(pair_child(w[c.ref], 1));
             }
         }
-        nTriNodes = collapse_pairs(&triTop, kWalkFirstPair, &triNodes);
+        nTriNodes = collapse_pairs(w, &triTop, kWalkFirstPair, &triNodes, &index, &maxDepth);
     } else {
         // Triangle units: the reference leaves, under a binned-SAH tree.
         std::vector<WideSlot> units;
@@ -217,7 +241,7 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri
     }
     const uint32_t lightNode = kWalkFirstPair + nTriNodes;
     std::vector<float4> lightNodes;
-    collapse_pairs(&lightTop, lightNode, &lightNodes);
+    collapse_pairs(w, &lightTop, lightNode, &lightNodes, &index, &maxDepth);
     // Roots: 0 = triangle top + light top, 1 = triangle top (shadow rays).
     WideSlot r0[4], r1[4];
     int n0 = 0, n1 = 0;
@@ -238,27 +262,18 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& w, const BvhOut& tri
     return MRT_OK;
 }
 
-// Builds the combined walk tree of the persistent kernels (layout in
-// mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 27/5-bit leaf
-// word get none; they run the non-persistent kernels.
-int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st, bool collapse) {
-    s->dev.walk_nodes = nullptr;
+// The root pairs of a walk tree (triangle root `tri` at pair_base, lights
+// after it) and whether the tree's slots and leaves fit the leaf word.
+bool walk_roots(mrt_scene* s, const BvhOut& tri, uint32_t lightBase, BNode w[kWalkFirstPair]) {
     const uint64_t slots = s->tris.size() + static_cast<uint64_t>(s->n_light_slots);
-    if (slots >= (1ull << kSlotBits)) return MRT_OK;
-    uint32_t maxLeaf = std::max(tri.root_count, s->n_light_slots <= kLinearLights ? s->n_light_slots : 0u);
-    for (const BNode& p : tri.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
-    for (const BNode& p : s->light_bvh.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
-    if (maxLeaf >= (1u << (32u - kSlotBits))) return MRT_OK;
+    if (slots >= (1ull << kSlotBits)) return false;
     const float nan = std::numeric_limits<float>::quiet_NaN();
     const float inf = std::numeric_limits<float>::infinity();
     const Box empty{f3{nan, nan, nan}, f3{nan, nan, nan}};
     const uint32_t nTri = static_cast<uint32_t>(s->tris.size());
-    const uint32_t triBase = kWalkFirstPair;
-    const uint32_t lightBase = triBase + static_cast<uint32_t>(tri.pairs.size());
-    std::vector<BNode> w(kWalkFirstPair);
-    std::memset(w.data(), 0, w.size() * sizeof(BNode));
+    std::memset(w, 0, kWalkFirstPair * sizeof(BNode));
     for (uint32_t r = 0; r < kWalkFirstPair; ++r) {
-        if (nTri > 0u) set_root_child(&w[r], 0, tri, triBase, 0u);
+        if (nTri > 0u) set_root_child(&w[r], 0, tri, kWalkFirstPair, 0u);
         else set_child(&w[r], 0, empty, 0u, 0u);
         set_child(&w[r], 1, empty, 0u, 0u);
     }
@@ -267,6 +282,28 @@ int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st, bool collap
                                             leaf_word(nTri, s->n_light_slots));
         else set_root_child(&w[kWalkClosestRoot], 1, s->light_bvh, lightBase, nTri);
     }
+    return true;
+}
+
+uint32_t light_max_leaf(const mrt_scene* s) {
+    uint32_t m = s->n_light_slots <= kLinearLights ? s->n_light_slots : 0u;
+    for (const BNode& p : s->light_bvh.pairs) m = std::max({m, p.q3.y, p.q3.w});
+    return m;
+}
+
+// Builds the combined walk tree of the persistent kernels (layout in
+// mrt_scene.hpp).  Scenes whose slots or leaves do not fit the 27/5-bit leaf
+// word get none; they run the non-persistent kernels.
+int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st, bool collapse) {
+    s->dev.walk_nodes = nullptr;
+    uint32_t maxLeaf = std::max(tri.root_count, light_max_leaf(s));
+    for (const BNode& p : tri.pairs) maxLeaf = std::max({maxLeaf, p.q3.y, p.q3.w});
+    if (maxLeaf >= (1u << (32u - kSlotBits))) return MRT_OK;
+    const uint32_t nTri = static_cast<uint32_t>(s->tris.size());
+    const uint32_t triBase = kWalkFirstPair;
+    const uint32_t lightBase = triBase + static_cast<uint32_t>(tri.pairs.size());
+    std::vector<BNode> w(kWalkFirstPair);
+    if (!walk_roots(s, tri, lightBase, w.data())) return MRT_OK;
     append_pairs(&w, tri, triBase, 0u);
     append_pairs(&w, s->light_bvh, lightBase, nTri);
     int rc;
@@ -277,6 +314,117 @@ int build_walk_tree(mrt_scene* s, const BvhOut& tri, hipStream_t st, bool collap
     s->depth_need = std::max(s->depth_need, std::max(tri.depth, s->light_bvh.depth) + 1u);
     if ((rc = build_wide_tree(s, w, tri, st, collapse)) != MRT_OK) return rc;
     MRT_HIP(hipStreamSynchronize(st));
+    return MRT_OK;
+}
+
+// The triangle BVH of a scene built on the GPU and kept there: boxes from the
+// uploaded triangles, the reference node array (gpu_build_bvh), then the
+// child pairs, the walk tree's triangle pairs and its wide nodes written by
+// the device (mrt_walk_gpu.hip); the host adds the root pairs, the light
+// pairs and the light part of the wide tree (the collapse of build_wide_tree
+// in both parts, the triangle wide nodes numbered in pair order).  *resident
+// stays false when the GPU build declines (*b untouched) or the root is a
+// leaf (*b then holds the host form of the one-node tree); the caller goes
+// on with the host steps.  Callers hold ctx->mu.
+int build_tri_bvh_resident(mrt_scene* s, uint32_t leaf, hipStream_t st, BvhOut* b, bool* built, bool* resident) {
+    *built = false;
+    *resident = false;
+    mrt_ctx* ctx = s->ctx;
+    const uint32_t n = static_cast<uint32_t>(s->tris.size());
+    if (n == 0) return MRT_OK;
+    int rc;
+    if ((rc = ctx->bvh_boxes.alloc(static_cast<size_t>(n) * sizeof(Box))) != MRT_OK) return rc;
+    MRT_HIP(gpu_tri_boxes(s->tri_geo.as<float4>(), n, ctx->bvh_boxes.as<float>(), st));
+    if ((rc = ctx->bvh_nodes.alloc((2 * static_cast<size_t>(n) - 1) * sizeof(RefNode))) != MRT_OK) return rc;
+    uint32_t cnt = 0;
+    const RefNode* ref = ctx->bvh_nodes.as<RefNode>();
+    const hipError_t e =
+        gpu_build_bvh(ctx->bvh_boxes.as<float>(), n, leaf, ctx->bvh_nodes.as<RefNode>(), &cnt, &ctx->bvh_scratch, st);
+    if (e == hipErrorNotSupported) return MRT_OK;
+    MRT_HIP(e);
+    *built = true;
+    TriTreeInfo info{};
+    if (cnt < 3 || gpu_tri_tree_plan(ref, cnt, &ctx->walk_scratch, &info, st) != hipSuccess) {
+        // A leaf root (or a tree the device passes decline): the host steps.
+        b->ref.resize(cnt);
+        MRT_HIP(hipMemcpyAsync(b->ref.data(), ref, cnt * sizeof(RefNode), hipMemcpyDeviceToHost, st));
+        MRT_HIP(hipStreamSynchronize(st));
+        bvh_pairs_from_ref(b);
+        return MRT_OK;
+    }
+    b->ref.clear();
+    b->pairs.clear();
+    b->root = info.root.box;
+    b->root_ref = 0;
+    b->root_count = 0;
+    b->depth = info.depth;
+    const uint64_t need = buffers_bytes(s) + 2ull * info.pairs * sizeof(BNode);
+    if (ctx->budget && need > ctx->budget) return fail(MRT_E_BUDGET, "BVH exceeds the HBM budget");
+    const int c = MRT_CAT_TRIANGLE;
+    if ((rc = s->nodes[c].alloc(static_cast<size_t>(info.pairs) * sizeof(BNode))) != MRT_OK) return rc;
+    fill_accel_bvh(&s->dev.acc[c], *b, s->nodes[c], nullptr, n);
+    s->depth_need = std::max(s->depth_need, b->depth);
+    *resident = true;
+
+    // Walk tree: roots, triangle pairs (device), light pairs.
+    s->dev.walk_nodes = nullptr;
+    const uint32_t nTri = n;
+    const uint32_t lightBase = kWalkFirstPair + info.pairs;
+    BNode roots[kWalkFirstPair];
+    const bool walk = leaf < (1u << (32u - kSlotBits)) && light_max_leaf(s) < (1u << (32u - kSlotBits)) &&
+                      walk_roots(s, *b, lightBase, roots);
+    if (!walk) {
+        MRT_HIP(gpu_tri_tree_write(ref, cnt, ctx->walk_scratch, info, kWalkFirstPair, kWalkFirstPair,
+                                   s->nodes[c].as<BNode>(), nullptr, nullptr, st));
+        MRT_HIP(hipStreamSynchronize(st));
+        return MRT_OK;
+    }
+    std::vector<BNode> lp;
+    append_pairs(&lp, s->light_bvh, lightBase, nTri);
+    const size_t nWalk = kWalkFirstPair + static_cast<size_t>(info.pairs) + lp.size();
+    if ((rc = s->walk_nodes.alloc(nWalk * sizeof(BNode))) != MRT_OK) return rc;
+    BNode* wdev = s->walk_nodes.as<BNode>();
+    MRT_HIP(hipMemcpyAsync(wdev, roots, sizeof(roots), hipMemcpyHostToDevice, st));
+    if (!lp.empty())
+        MRT_HIP(hipMemcpyAsync(wdev + lightBase, lp.data(), lp.size() * sizeof(BNode), hipMemcpyHostToDevice, st));
+
+    // Wide tree: roots, triangle nodes (device), light nodes (host collapse).
+    const PairView w{roots, kWalkFirstPair, lp.data(), lightBase, static_cast<uint32_t>(lp.size())};
+    std::vector<WideSlot> lightTop = light_top(w);
+    std::vector<WideSlot> triTop;
+    for (int k = 0; k < 2; ++k) {
+        const RefNode& ch = info.child[k];
+        if (ch.count != 0u) triTop.push_back(WideSlot{ch.box, ch.offset, leaf_word(ch.offset, ch.count)});
+        else triTop.push_back(WideSlot{ch.box, kWalkFirstPair + info.child_wide[k], 0u});
+    }
+    const uint32_t lightNode = kWalkFirstPair + info.wide;
+    std::vector<uint32_t> index(w.size(), 0xFFFFFFFFu);
+    uint32_t maxDepth = info.wide_levels;
+    std::vector<float4> lightNodes, top(16, make_float4(0, 0, 0, 0));
+    collapse_pairs(w, &lightTop, lightNode, &lightNodes, &index, &maxDepth);
+    WideSlot r0[4], r1[4];
+    int n0 = 0, n1 = 0;
+    for (const WideSlot& t : triTop) r0[n0++] = r1[n1++] = t;
+    for (const WideSlot& t : lightTop) r0[n0++] = t;
+    put_wide_node(&top, kWalkClosestRoot, r0, n0);
+    put_wide_node(&top, kWalkAnyRoot, r1, n1);
+    const size_t nWide = 8ull * (kWalkFirstPair + info.wide) + lightNodes.size();
+    if ((rc = s->wide_nodes.alloc(nWide * sizeof(float4))) != MRT_OK) return rc;
+    float4* wide = s->wide_nodes.as<float4>();
+    MRT_HIP(hipMemcpyAsync(wide, top.data(), 8ull * kWalkFirstPair * sizeof(float4), hipMemcpyHostToDevice, st));
+    if (!lightNodes.empty())
+        MRT_HIP(hipMemcpyAsync(wide + 8ull * lightNode, lightNodes.data(), lightNodes.size() * sizeof(float4),
+                               hipMemcpyHostToDevice, st));
+    MRT_HIP(gpu_tri_tree_write(ref, cnt, ctx->walk_scratch, info, kWalkFirstPair, kWalkFirstPair,
+                               s->nodes[c].as<BNode>(), wdev + kWalkFirstPair, wide + 8ull * kWalkFirstPair, st));
+    MRT_HIP(hipStreamSynchronize(st));  // the host arrays are released on return
+    s->dev.walk_nodes = wdev;
+    s->dev.walk_light_pair = lightBase;
+    s->dev.walk_light_slot = nTri;
+    s->depth_need = std::max(s->depth_need, std::max(b->depth, s->light_bvh.depth) + 1u);
+    s->dev.wide_nodes = wide;
+    s->dev.wide_light_node = lightNode;
+    s->depth_need = std::max(s->depth_need, 3u * maxDepth + 1u);
     return MRT_OK;
 }
 
@@ -644,31 +792,6 @@ int build_grid_gpu(mrt_scene* s, int c, const Box& bounds, int32_t gs, GridOut* 
     return MRT_OK;
 }
 
-// The reference BVH over `boxes` built on the GPU (the node array of
-// build_bvh_reference, bit for bit) with its child pairs; *done stays false
-// when the GPU build declines (the caller builds on the host).  Callers hold
-// ctx->mu.
-int build_bvh_gpu(mrt_ctx* ctx, const std::vector<Box>& boxes, uint32_t leaf, BvhOut* out, hipStream_t st,
-                  bool* done) {
-    static_assert(sizeof(Box) == 24 && sizeof(RefNode) == 32, "box / node layouts");
-    *done = false;
-    const uint32_t n = static_cast<uint32_t>(boxes.size());
-    int rc;
-    if (n && (rc = ctx->bvh_boxes.upload(boxes.data(), boxes.size() * sizeof(Box), st)) != MRT_OK) return rc;
-    if ((rc = ctx->bvh_nodes.alloc((n ? 2 * static_cast<size_t>(n) - 1 : 1) * sizeof(RefNode))) != MRT_OK) return rc;
-    uint32_t cnt = 0;
-    const hipError_t e =
-        gpu_build_bvh(ctx->bvh_boxes.as<float>(), n, leaf, ctx->bvh_nodes.as<RefNode>(), &cnt, &ctx->bvh_scratch, st);
-    if (e == hipErrorNotSupported) return MRT_OK;
-    MRT_HIP(e);
-    out->ref.resize(cnt);
-    MRT_HIP(hipMemcpyAsync(out->ref.data(), ctx->bvh_nodes.p, cnt * sizeof(RefNode), hipMemcpyDeviceToHost, st));
-    MRT_HIP(hipStreamSynchronize(st));
-    bvh_pairs_from_ref(out);
-    *done = true;
-    return MRT_OK;
-}
-
 extern "C" {
 
 MRT_API const char* mrt_last_error(void) { return g_error.c_str(); }
@@ -975,18 +1098,25 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
     s->dev.wide_nodes = nullptr;
     if (accelerator == MRT_BVH) {
         for (int c = 0; c < 3; ++c) {
-            std::vector<Box> boxes(counts[c]);
-            for (uint32_t i = 0; i < counts[c]; ++i)
-                boxes[i] = c == MRT_CAT_TRIANGLE ? triangle_box(s->tris[i])
-                           : c == MRT_CAT_SPHERE ? sphere_box(s->sphs[i])
-                                                 : plane_box(s->plas[i]);
             BvhOut b;
             bool onGpu = false;
             if (c == MRT_CAT_TRIANGLE &&
                 (builder == MRT_BUILDER_GPU || (builder == MRT_BUILDER_AUTO && counts[c] >= kGpuBuildMin))) {
-                if ((rc = build_bvh_gpu(s->ctx, boxes, leaf, &b, st, &onGpu)) != MRT_OK) return rc;
+                bool resident = false;
+                if ((rc = build_tri_bvh_resident(s, leaf, st, &b, &onGpu, &resident)) != MRT_OK) return rc;
+                if (resident) {
+                    s->gpu_built = true;
+                    continue;
+                }
             }
-            if (!onGpu) build_bvh_reference(boxes, leaf, &b);
+            if (!onGpu) {
+                std::vector<Box> boxes(counts[c]);
+                for (uint32_t i = 0; i < counts[c]; ++i)
+                    boxes[i] = c == MRT_CAT_TRIANGLE ? triangle_box(s->tris[i])
+                               : c == MRT_CAT_SPHERE ? sphere_box(s->sphs[i])
+                                                     : plane_box(s->plas[i]);
+                build_bvh_reference(boxes, leaf, &b);
+            }
             const uint64_t need = buffers_bytes(s) + b.pairs.size() * sizeof(BNode) *
                                                          (c == MRT_CAT_TRIANGLE ? 2u : 1u);
             if (s->ctx->budget && need > s->ctx->budget)

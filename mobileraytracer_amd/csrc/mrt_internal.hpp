@@ -11,6 +11,7 @@
 #include "mrt_grid_gpu.hpp"
 #include "mrt_kernels.hpp"
 #include "mrt_path.hpp"
+#include "mrt_walk_gpu.hpp"
 #include "mrt_wavefront.hpp"
 
 namespace mrt {
@@ -95,6 +96,7 @@ struct mrt_ctx {
     // boxes and the node array.
     mrt::BvhGpuScratch bvh_scratch;
     mrt::DevBuf bvh_boxes, bvh_nodes;
+    mrt::WalkGpuScratch walk_scratch;  // device-side walk trees of GPU-built BVHs
     // GPU grid builds: scratch and the staged primitives.
     mrt::GridGpuScratch grid_scratch;
     mrt::DevBuf grid_prims;

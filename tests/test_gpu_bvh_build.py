@@ -175,3 +175,32 @@ def test_scene_builders_render_the_same(gpu_ctx, scene, kernel):
             sc.close()
     assert np.array_equal(frames[0][0], frames[1][0])
     assert np.array_equal(frames[0][1], frames[1][1])
+
+
+@pytest.mark.parametrize("n,lights,leaf", [(3, 1, 2), (4, 1, 1), (400, 3, 2), (5000, 0, 4), (30000, 60, 2),
+                                            (30000, 60, 16)])
+def test_resident_builds_render_the_same(gpu_ctx, n, lights, leaf):
+    """The GPU builder keeps the triangle trees on the device (pairs, walk
+    pairs and wide nodes written by the device, roots and the light part
+    from the host): identical wavefront frames to the host builder for a
+    leaf root (the host steps), a single interior root, linear light scans
+    (16 lights or fewer), no lights, a light tree, and leaf sizes 1-16."""
+    data = mrt.soup_scene(n + 11, n, lights)
+    cam = mrt.make_camera(abi.CAMERA_PERSPECTIVE, (0.5, 0.5, -1.0), (0.5, 0.5, 0.5), (0, 1, 0),
+                          float(np.float32(45.0) * np.float32(16.0 / 9.0)), 45.0)
+    desc = mrt.render_desc(abi.SHADER_PATHTRACER, 256, 144, 2, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 3),
+                           rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 4), light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 5),
+                           hemi=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 6), depth_min=2, depth_max=6,
+                           kernel=abi.KERNEL_WAVEFRONT)
+    frames = []
+    for builder in (abi.BUILDER_HOST, abi.BUILDER_GPU):
+        sc = mrt.Scene(gpu_ctx, data)
+        try:
+            sc.build(abi.BVH, leaf_size=leaf, builder=builder)
+            acc = np.zeros((144, 256, 3), np.float32)
+            bm, _ = sc.render(desc, accum=acc)
+            frames.append((bm, acc))
+        finally:
+            sc.close()
+    assert np.array_equal(frames[0][0], frames[1][0])
+    assert np.array_equal(frames[0][1], frames[1][1])
