@@ -30,13 +30,16 @@ def rnd(seed):
 
 @pytest.fixture(scope="module")
 def scenes(gpu_ctx):
-    datas = [mrt.soup_scene(11, 40000, 400),   # 400 area lights: light tree
-             mrt.soup_scene(12, 20000, 8),     # 8 area lights: linear scan
-             mrt.cornell_scene(1)]
+    # The first scene's triangle trees are built and written on the device
+    # (GPU builder), the others' on the host: the server relocates both
+    # kinds into its arena.
+    datas = [(mrt.soup_scene(11, 40000, 400), abi.BUILDER_GPU),   # 400 area lights: light tree
+             (mrt.soup_scene(12, 20000, 8), abi.BUILDER_HOST),    # 8 area lights: linear scan
+             (mrt.cornell_scene(1), abi.BUILDER_HOST)]
     out = []
-    for d in datas:
+    for d, builder in datas:
         s = mrt.Scene(gpu_ctx, d)
-        s.build(abi.BVH)
+        s.build(abi.BVH, builder=builder)
         out.append(s)
     return out
 
