@@ -344,8 +344,12 @@ int build_tri_bvh_resident(mrt_scene* s, uint32_t leaf, hipStream_t st, BvhOut* 
     MRT_HIP(e);
     *built = true;
     TriTreeInfo info{};
-    if (cnt < 3 || gpu_tri_tree_plan(ref, cnt, &ctx->walk_scratch, &info, st) != hipSuccess) {
-        // A leaf root (or a tree the device passes decline): the host steps.
+    // hipErrorInvalidValue: a leaf root or a tree the device passes decline;
+    // any other error is reported.
+    const hipError_t pe = cnt < 3 ? hipErrorInvalidValue : gpu_tri_tree_plan(ref, cnt, &ctx->walk_scratch, &info, st);
+    if (pe != hipSuccess && pe != hipErrorInvalidValue) MRT_HIP(pe);
+    if (pe != hipSuccess) {
+        // The host steps.
         b->ref.resize(cnt);
         MRT_HIP(hipMemcpyAsync(b->ref.data(), ref, cnt * sizeof(RefNode), hipMemcpyDeviceToHost, st));
         MRT_HIP(hipStreamSynchronize(st));

@@ -278,7 +278,7 @@ hipError_t gpu_tri_tree_plan(const RefNode* ref, uint32_t cnt, WalkGpuScratch* s
 hipError_t gpu_tri_tree_write(const RefNode* ref, uint32_t cnt, const WalkGpuScratch& sc, const TriTreeInfo& info,
                               uint32_t pair_base, uint32_t wide_base, BNode* pairs, BNode* walk, float4* wide,
                               hipStream_t st) {
-    (void)info;
+    if (info.pairs == 0u || info.pairs > cnt) return hipErrorInvalidValue;
     const size_t tmp = scan_bytes(cnt, st);
     Pieces P;
     if (carve(nullptr, cnt, tmp, &P) > sc.bytes) return hipErrorInvalidValue;
