@@ -492,14 +492,17 @@ def serving_setup(mrt, ctx, args):
 
     t0 = time.time()
     sizes = serving.pool_sizes(args.serve_scenes, args.serve_min_tris, args.serve_max_tris, seed=4)
-    pool = serving.build_pool(ctx, sizes, seed=400)
+    times = {}
+    pool = serving.build_pool(ctx, sizes, seed=400, times=times)
     srv = mrt.RenderServer(ctx)
+    t1 = time.time()
     for p in pool:
         p.server_id = srv.add_scene(p.scene)
+    times["add_s"] = time.time() - t1
     st = srv.stats()
     log(f"serving pool: {len(pool)} scenes, {sum(p.triangles for p in pool)} triangles "
         f"({', '.join(f'{p.kind} {p.triangles}' for p in pool)}), arena {st.scene_bytes / 1e6:.0f} MB, "
-        f"built in {time.time() - t0:.1f} s")
+        f"built in {time.time() - t0:.1f} s (" + ", ".join(f"{k[:-2]} {v:.2f} s" for k, v in times.items()) + ")")
     return srv, pool
 
 

@@ -41,10 +41,11 @@ def pool_sizes(n: int, lo: int, hi: int, seed: int) -> list[int]:
     return [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(n)]
 
 
-def build_pool(ctx, sizes: list[int], seed: int, threads: int = 8) -> list[PoolScene]:
+def build_pool(ctx, sizes: list[int], seed: int, threads: int = 8, times: dict | None = None) -> list[PoolScene]:
     """Scenes of the given triangle counts (icosphere clusters round theirs
     to whole spheres), built with the BVH.  The generators run on `threads`
-    host threads."""
+    host threads.  `times`, when given, receives the seconds spent
+    generating, creating (packing and upload) and building the scenes."""
     from concurrent.futures import ThreadPoolExecutor
 
     def generate(i, n):
@@ -55,14 +56,24 @@ def build_pool(ctx, sizes: list[int], seed: int, threads: int = 8) -> list[PoolS
         total = spheres * 20 * 4 ** sub
         return icosphere_scene(seed + i, spheres, sub, max(1, total // 100)), "icospheres", total
 
+    t = {"generate_s": 0.0, "create_s": 0.0, "build_s": 0.0}
+    t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
         datas = list(ex.map(lambda a: generate(*a), enumerate(sizes)))
+    t["generate_s"] = time.perf_counter() - t0
     out = []
     for data, kind, n in datas:
+        t0 = time.perf_counter()
         s = Scene(ctx, data)
+        t1 = time.perf_counter()
         s.build(abi.BVH)
+        t2 = time.perf_counter()
+        t["create_s"] += t1 - t0
+        t["build_s"] += t2 - t1
         s.data = None  # the library keeps its own copy
         out.append(PoolScene(s, kind, n))
+    if times is not None:
+        times.update(t)
     return out
 
 
