@@ -611,56 +611,19 @@ void fill_info(const Job* j, mrt_job_info* info) {
     info->reserved = 0;
 }
 
-// Copies a resident scene's walk trees into the arena with rebased references.
+// Copies a resident scene's walk trees into the arena with rebased references
+// (rebased on the device, on the transfer stream: the pipelines keep running,
+// and the new nodes land past every node they read).
 int add_to_arena(mrt_server* s, mrt_scene* sc, Resident* out) {
     const size_t nWide = sc->wide_nodes.bytes / (8 * sizeof(float4));
     const size_t nWalk = sc->walk_nodes.bytes / sizeof(BNode);
     const size_t nGeo = sc->tri_geo.bytes / (3 * sizeof(float4));
-    std::vector<float4> wide(8 * nWide), geo(3 * nGeo);
-    std::vector<BNode> walk(nWalk);
-    MRT_HIP(hipMemcpy(wide.data(), sc->wide_nodes.p, sc->wide_nodes.bytes, hipMemcpyDeviceToHost));
-    MRT_HIP(hipMemcpy(walk.data(), sc->walk_nodes.p, sc->walk_nodes.bytes, hipMemcpyDeviceToHost));
-    MRT_HIP(hipMemcpy(geo.data(), sc->tri_geo.p, sc->tri_geo.bytes, hipMemcpyDeviceToHost));
     if (s->geo_used + nGeo >= (1ull << kSlotBits) || s->wide_used + nWide >= 0xFFFFFFF0ull ||
         s->walk_used + nWalk >= 0xFFFFFFF0ull)
         return fail(MRT_E_BUDGET, "the scene arena's 27-bit triangle slots are used up");
     const uint32_t wideBase = static_cast<uint32_t>(s->wide_used);
     const uint32_t walkBase = static_cast<uint32_t>(s->walk_used);
     const uint32_t triBase = static_cast<uint32_t>(s->geo_used);
-    // Wide nodes: 4 (ref, word) slots in q[6], q[7]; a leaf word carries the
-    // slot in its low bits, an interior slot (word 0, finite box) a wide node.
-    for (size_t i = 0; i < nWide; ++i) {
-        float4* q = &wide[8 * i];
-        uint32_t r[8];
-        std::memcpy(r, &q[6], 16);
-        std::memcpy(r + 4, &q[7], 16);
-        const float mnx[4] = {q[0].x, q[0].y, q[0].z, q[0].w};
-        for (int k = 0; k < 4; ++k) {
-            if (r[2 * k + 1] != 0u) {
-                r[2 * k] += triBase;
-                r[2 * k + 1] += triBase;
-            } else if (!std::isnan(mnx[k])) {
-                r[2 * k] += wideBase;
-            }
-        }
-        std::memcpy(&q[6], r, 16);
-        std::memcpy(&q[7], r + 4, 16);
-    }
-    for (BNode& p : walk) {
-        const bool nanL = std::isnan(p.q0.x), nanR = std::isnan(p.q1.z);
-        if (p.q3.y != 0u) {
-            p.q3.x += triBase;
-            p.q3.y += triBase;
-        } else if (!nanL) {
-            p.q3.x += walkBase;
-        }
-        if (p.q3.w != 0u) {
-            p.q3.z += triBase;
-            p.q3.w += triBase;
-        } else if (!nanR) {
-            p.q3.z += walkBase;
-        }
-    }
     const size_t wideBytes = (s->wide_used + nWide) * 8 * sizeof(float4);
     const size_t walkBytes = (s->walk_used + nWalk) * sizeof(BNode);
     const size_t geoBytes = (s->geo_used + nGeo) * 3 * sizeof(float4);
@@ -676,12 +639,19 @@ int add_to_arena(mrt_server* s, mrt_scene* sc, Resident* out) {
         return rc;
     if ((rc = grow(s, s->arena_geo, cap(geoBytes, s->arena_geo.bytes), s->geo_used * 3 * sizeof(float4))) != MRT_OK)
         return rc;
-    MRT_HIP(hipMemcpy(s->arena_wide.as<float4>() + 8 * s->wide_used, wide.data(), wide.size() * sizeof(float4),
-                      hipMemcpyHostToDevice));
-    MRT_HIP(hipMemcpy(s->arena_walk.as<BNode>() + s->walk_used, walk.data(), walk.size() * sizeof(BNode),
-                      hipMemcpyHostToDevice));
-    MRT_HIP(hipMemcpy(s->arena_geo.as<float4>() + 3 * s->geo_used, geo.data(), geo.size() * sizeof(float4),
-                      hipMemcpyHostToDevice));
+    // Wide nodes: 4 (ref, word) slots; a leaf word carries the slot in its low
+    // bits, an interior slot (word 0, finite box) a wide node.  Walk pairs:
+    // leaf children (count != 0) move with the slots, interior ones (finite
+    // box) with the pairs.
+    const hipStream_t st = s->xfer;
+    MRT_HIP(gpu_rebase_wide(sc->wide_nodes.as<float4>(), nWide, s->arena_wide.as<float4>() + 8 * s->wide_used, triBase,
+                            wideBase, st));
+    MRT_HIP(gpu_rebase_walk(sc->walk_nodes.as<BNode>(), nWalk, s->arena_walk.as<BNode>() + s->walk_used, triBase,
+                            walkBase, st));
+    if (nGeo)
+        MRT_HIP(hipMemcpyAsync(s->arena_geo.as<float4>() + 3 * s->geo_used, sc->tri_geo.p,
+                               nGeo * 3 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    MRT_HIP(hipStreamSynchronize(st));
     s->wide_used += nWide;
     s->walk_used += nWalk;
     s->geo_used += nGeo;

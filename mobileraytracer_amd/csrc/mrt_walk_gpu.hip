@@ -217,6 +217,55 @@ __global__ __launch_bounds__(kThreads) void k_tri_boxes(const float4* __restrict
     o[5] = bx.mx.z;
 }
 
+__global__ __launch_bounds__(kThreads) void k_rebase_wide(const float4* __restrict__ src, size_t n,
+                                                          float4* __restrict__ dst, uint32_t tri_base,
+                                                          uint32_t wide_base) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float4* q = src + 8 * i;
+    float4* o = dst + 8 * i;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) o[j] = q[j];
+    const float mnx[4] = {q[0].x, q[0].y, q[0].z, q[0].w};
+    uint32_t r[8];
+    const float4 a = q[6], b = q[7];
+    r[0] = __float_as_uint(a.x), r[1] = __float_as_uint(a.y), r[2] = __float_as_uint(a.z), r[3] = __float_as_uint(a.w);
+    r[4] = __float_as_uint(b.x), r[5] = __float_as_uint(b.y), r[6] = __float_as_uint(b.z), r[7] = __float_as_uint(b.w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (r[2 * k + 1] != 0u) {
+            r[2 * k] += tri_base;
+            r[2 * k + 1] += tri_base;
+        } else if (!isnan(mnx[k])) {
+            r[2 * k] += wide_base;
+        }
+    }
+    o[6] = make_float4(__uint_as_float(r[0]), __uint_as_float(r[1]), __uint_as_float(r[2]), __uint_as_float(r[3]));
+    o[7] = make_float4(__uint_as_float(r[4]), __uint_as_float(r[5]), __uint_as_float(r[6]), __uint_as_float(r[7]));
+}
+
+__global__ __launch_bounds__(kThreads) void k_rebase_walk(const BNode* __restrict__ src, size_t n,
+                                                          BNode* __restrict__ dst, uint32_t tri_base,
+                                                          uint32_t walk_base) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    BNode p = src[i];
+    const bool nanL = isnan(p.q0.x), nanR = isnan(p.q1.z);
+    if (p.q3.y != 0u) {
+        p.q3.x += tri_base;
+        p.q3.y += tri_base;
+    } else if (!nanL) {
+        p.q3.x += walk_base;
+    }
+    if (p.q3.w != 0u) {
+        p.q3.z += tri_base;
+        p.q3.w += tri_base;
+    } else if (!nanR) {
+        p.q3.z += walk_base;
+    }
+    dst[i] = p;
+}
+
 #define WALK_HIP(expr)                   \
     do {                                 \
         const hipError_t e_ = (expr);    \
@@ -285,6 +334,20 @@ hipError_t gpu_tri_tree_write(const RefNode* ref, uint32_t cnt, const WalkGpuScr
     carve(static_cast<char*>(sc.p), cnt, tmp, &P);
     k_write<<<blocks_for(cnt), kThreads, 0, st>>>(ref, cnt, P.rank, P.depth, P.widx, pair_base, wide_base, pairs, walk,
                                                   wide);
+    return hipGetLastError();
+}
+
+hipError_t gpu_rebase_wide(const float4* src, size_t n, float4* dst, uint32_t tri_base, uint32_t wide_base,
+                           hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_rebase_wide<<<blocks_for(n), kThreads, 0, st>>>(src, n, dst, tri_base, wide_base);
+    return hipGetLastError();
+}
+
+hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_base, uint32_t walk_base,
+                           hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_rebase_walk<<<blocks_for(n), kThreads, 0, st>>>(src, n, dst, tri_base, walk_base);
     return hipGetLastError();
 }
 

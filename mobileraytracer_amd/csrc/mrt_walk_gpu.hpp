@@ -52,6 +52,16 @@ hipError_t gpu_tri_tree_write(const RefNode* ref, uint32_t cnt, const WalkGpuScr
                               uint32_t pair_base, uint32_t wide_base, BNode* pairs, BNode* walk, float4* wide,
                               hipStream_t st);
 
+// Copies of walk trees with their references rebased (a render server's
+// scene arena): wide nodes (8 float4 each) with leaf slots and leaf words +
+// tri_base and interior slots (word 0, finite box) + wide_base; walk pairs
+// with leaf references and words + tri_base and interior references (count 0,
+// finite box) + walk_base.  src and dst may not overlap.
+hipError_t gpu_rebase_wide(const float4* src, size_t n, float4* dst, uint32_t tri_base, uint32_t wide_base,
+                           hipStream_t st);
+hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_base, uint32_t walk_base,
+                           hipStream_t st);
+
 // Boxes (6 floats: min xyz, max xyz) of the first n triangles of a tri_geo
 // array (3 float4 per triangle: A.xyz AB.x | AB.yz AC.xy | AC.z id), the
 // same arithmetic as the host's triangle_box.
