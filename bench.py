@@ -103,10 +103,10 @@ def soup_camera(mrt, abi):
                            float(np.float32(45.0) * ratio), 45.0)
 
 
-def cpu_baseline(data, cam, abi, mrt) -> dict:
+def cpu_baseline(data, cam, abi, mrt, accel: str = "bvh") -> dict:
     """The oracle (reference-faithful CPU restatement) on a bounded sample of
-    the same workload: the 16 central tiles of the 16x16 tile grid, the first
-    2 sample passes, on min(16, cores) threads."""
+    the same workload (same accelerator): the 16 central tiles of the 16x16
+    tile grid, the first 2 sample passes, on min(16, cores) threads."""
     import oracle
 
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -114,7 +114,11 @@ def cpu_baseline(data, cam, abi, mrt) -> dict:
     sc = oracle.OracleScene(data.desc())
     p = abi.AccelParams()
     p.leaf_size = 2
-    sc.build(abi.BVH, p)
+    if accel == "grid":
+        cmin, cmax = mrt.camera_aabb(cam)
+        p.camera_min[:] = list(cmin)
+        p.camera_max[:] = list(cmax)
+    sc.build(abi.REGULAR_GRID if accel == "grid" else abi.BVH, p)
     vals = np.frombuffer(bytes(desc.tile_values), np.float32)
     tiles = np.rint(vals * 256).astype(int)
     central = {ty * 16 + tx for ty in range(6, 10) for tx in range(6, 10)}
@@ -122,7 +126,8 @@ def cpu_baseline(data, cam, abi, mrt) -> dict:
     bm, counters, secs = sc.render(desc, threads=threads, tile_mask=mask)
     rays = int(counters.sum())
     return {"value": rays / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle render, 16 central tiles x 2 spp of the 1920x1080 frame, {rays} rays in {secs:.2f} s"}
+            "sample": f"oracle render ({'RegularGrid' if accel == 'grid' else 'BVH'}), 16 central tiles x 2 spp "
+                      f"of the 1920x1080 frame, {rays} rays in {secs:.2f} s"}
 
 
 def main() -> int:
@@ -145,6 +150,8 @@ def main() -> int:
     ap.add_argument("--serve-max-tris", type=int, default=8_000_000)
     ap.add_argument("--serve-sweep", type=str, default="",
                     help="comma-separated rates: run only the serving workload, one JSON line per rate")
+    ap.add_argument("--accel", choices=["bvh", "grid"], default="bvh",
+                    help="accelerator of the scene: the reference BVH (default) or the 128^3 RegularGrid")
     ap.add_argument("--builder", choices=["auto", "host", "gpu"], default="auto",
                     help="BVH builder of the scene (auto: GPU from 65536 triangles)")
     ap.add_argument("--denoise", action="store_true", help="run only the denoiser leg (1920x1080, trained weights)")
@@ -200,10 +207,14 @@ def main() -> int:
     scene = mrt.Scene(ctx, data)
     builder = {"auto": abi.BUILDER_AUTO, "host": abi.BUILDER_HOST, "gpu": abi.BUILDER_GPU}[args.builder]
     t0 = time.perf_counter()
-    scene.build(abi.BVH, builder=builder)
+    if args.accel == "grid":
+        cmin, cmax = mrt.camera_aabb(cam)
+        scene.build(abi.REGULAR_GRID, cmin, cmax, builder=builder)
+    else:
+        scene.build(abi.BVH, builder=builder)
     build_s = time.perf_counter() - t0
-    log(f"BVH built and uploaded in {build_s:.3f} s ({args.builder} builder), footprint "
-        f"{scene.footprint() / 1e6:.1f} MB")
+    log(f"{'RegularGrid' if args.accel == 'grid' else 'BVH'} built and uploaded in {build_s:.3f} s "
+        f"({args.builder} builder), footprint {scene.footprint() / 1e6:.1f} MB")
 
     dev = torch.device("cuda:0")
     bitmap = torch.zeros((H, W), dtype=torch.int32, device=dev)
@@ -217,6 +228,7 @@ def main() -> int:
                          bitmap=bitmap,
                          accum=accum,
                          mem=abi.MEM_DEVICE)
+    st_instr = st
     rays1 = st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
     closest1 = st.rays_camera + st.rays_extension + st.rays_specular
     pairs_per_ray = st.node_visits / rays1
@@ -271,10 +283,20 @@ def main() -> int:
     # launches concurrently on separate streams, so a launch's duration
     # includes time shared with the other groups' launches; device_achieved
     # divides all trace bytes by the time at least one trace launch ran.
+    # RegularGrid scenes: 8 bytes (a cell's two CSR offsets) per DDA step and
+    # 52 per cell primitive test (its 4-byte reference and the triangle);
+    # the light tree's pairs and tests as above.  The persistent walk needs
+    # the BVH's walk tree, so grid scenes trace with the one-ray-per-lane
+    # wavefront kernels.
     wavefront = launches > 0
-    persistent = wavefront and not args.no_persistent
+    persistent = wavefront and not args.no_persistent and args.accel == "bvh"
     node_bytes = 128 if persistent else 64
     bytes_per_ray = 48 + node_bytes * pairs_per_ray + 48 * prims_per_ray
+    if args.accel == "grid":
+        light_pairs = st_instr.light_node_visits / rays1
+        light_tests = st_instr.light_prim_tests / rays1
+        bytes_per_ray = (48 + 8 * (pairs_per_ray - light_pairs) + 64 * light_pairs +
+                         52 * (prims_per_ray - light_tests) + 48 * light_tests)
     if wavefront:
         kernel_name = "k_wf_trace_p" if persistent else "k_wf_extend + k_wf_shadow"
         bytes_total = rays * bytes_per_ray
@@ -302,7 +324,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded 1M-triangle soup (10k emissive), deterministic per-pixel sampling",
-        "config": {"workload": f"soup-1M 1920x1080 PathTracer 16spp depth6 RR3 Halton BVH {args.kernel}",
+        "config": {"workload": f"soup-1M 1920x1080 PathTracer 16spp depth6 RR3 Halton "
+                               f"{'RegularGrid128' if args.accel == 'grid' else 'BVH'} {args.kernel}",
                    "triangles": N_TRIS, "lights": N_LIGHTS, "width": W, "height": H, "spp": SPP,
                    "rays_per_step": rays // args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -316,7 +339,7 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         t0 = time.time()
-        out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt)
+        out["cpu_baseline"] = cpu_baseline(data, cam, abi, mrt, args.accel)
         log(f"cpu baseline took {time.time() - t0:.1f} s")
         out["vs_cpu"] = round(mrays / out["cpu_baseline"]["value"], 1)
     out["scene_build"] = build_leg(mrt, abi, ctx, data, cam, args.builder, build_s)
