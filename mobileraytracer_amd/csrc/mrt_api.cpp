@@ -960,17 +960,12 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
     s->counted = true;
     const hipStream_t st = ctx->stream;
     int rc;
-    // Scene triangles, then (below) one slot per area light in light-tree order.
-    std::vector<float4> geo(3 * s->tris.size());
-    {
-        std::vector<float4> shade(5 * s->tris.size());
-        for (size_t i = 0; i < s->tris.size(); ++i) {
-            pack_geo(s->tris[i], MRT_MAKE_ID(MRT_CAT_TRIANGLE, static_cast<uint32_t>(i)), &geo[3 * i]);
-            pack_shade(s->tris[i], &shade[5 * i]);
-        }
-        if ((rc = s->tri_shade.upload(shade.data(), shade.size() * sizeof(float4), st)) != MRT_OK) return rc;
-        MRT_HIP(hipStreamSynchronize(st));
-    }
+    // Scene triangles, packed on the device from one upload of their records
+    // (below, once the light slots are known), then one slot per area light
+    // in light-tree order.
+    const size_t nTri = s->tris.size();
+    DevBuf raw;
+    if ((rc = raw.upload(s->tris.data(), nTri * sizeof(mrt_triangle), st)) != MRT_OK) return rc;
     {
         std::vector<float4> sp(s->sphs.size());
         std::vector<int32_t> sm(s->sphs.size());
@@ -1038,11 +1033,16 @@ MRT_API int mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* d, mrt_scene** 
             slotLight = perm;
         }
         s->n_light_slots = static_cast<uint32_t>(slotLight.size());
-        const size_t base = geo.size();
-        geo.resize(base + 3 * slotLight.size());
+        std::vector<float4> tail(3 * slotLight.size());
         for (size_t k = 0; k < slotLight.size(); ++k)
-            std::copy(&lg[3 * slotLight[k]], &lg[3 * slotLight[k]] + 3, &geo[base + 3 * k]);
-        if ((rc = s->tri_geo.upload(geo.data(), geo.size() * sizeof(float4), st)) != MRT_OK) return rc;
+            std::copy(&lg[3 * slotLight[k]], &lg[3 * slotLight[k]] + 3, &tail[3 * k]);
+        if ((rc = s->tri_geo.alloc((3 * nTri + tail.size()) * sizeof(float4))) != MRT_OK) return rc;
+        if ((rc = s->tri_shade.alloc(5 * nTri * sizeof(float4))) != MRT_OK) return rc;
+        MRT_HIP(gpu_pack_triangles(raw.as<mrt_triangle>(), static_cast<uint32_t>(nTri), s->tri_geo.as<float4>(),
+                                   s->tri_shade.as<float4>(), st));
+        if (!tail.empty())
+            MRT_HIP(hipMemcpyAsync(s->tri_geo.as<float4>() + 3 * nTri, tail.data(), tail.size() * sizeof(float4),
+                                   hipMemcpyHostToDevice, st));
         MRT_HIP(hipStreamSynchronize(st));
     }
     s->dev.tri_geo = s->tri_geo.as<float4>();

@@ -202,6 +202,31 @@ __global__ __launch_bounds__(kThreads) void k_write(const RefNode* __restrict__ 
     q[7] = make_float4(__uint_as_float(r[4]), __uint_as_float(r[5]), __uint_as_float(r[6]), __uint_as_float(r[7]));
 }
 
+__global__ __launch_bounds__(kThreads) void k_pack_triangles(const float4* __restrict__ raw, uint32_t n,
+                                                             float4* __restrict__ geo, float4* __restrict__ shade) {
+    static_assert(sizeof(mrt_triangle) == 7 * sizeof(float4), "mrt_triangle is 112 bytes");
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    // mrt_triangle: ac[3] ab[3] a[3] na[3] nb[3] nc[3] ca[3] cb[3] cc[3] material_id.
+    float f[28];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const float4 q = raw[7ull * i + k];
+        f[4 * k] = q.x, f[4 * k + 1] = q.y, f[4 * k + 2] = q.z, f[4 * k + 3] = q.w;
+    }
+    const float id = __uint_as_float(MRT_MAKE_ID(MRT_CAT_TRIANGLE, i));
+    float4* g = geo + 3ull * i;
+    g[0] = make_float4(f[6], f[7], f[8], f[3]);
+    g[1] = make_float4(f[4], f[5], f[0], f[1]);
+    g[2] = make_float4(f[2], id, 0.0f, 0.0f);
+    float4* sh = shade + 5ull * i;
+    sh[0] = make_float4(f[9], f[10], f[11], f[12]);
+    sh[1] = make_float4(f[13], f[14], f[15], f[16]);
+    sh[2] = make_float4(f[17], f[18], f[19], f[20]);
+    sh[3] = make_float4(f[21], f[22], f[23], f[24]);
+    sh[4] = make_float4(f[25], f[26], f[27], 0.0f);
+}
+
 __global__ __launch_bounds__(kThreads) void k_tri_boxes(const float4* __restrict__ geo, uint32_t n,
                                                         float* __restrict__ boxes6) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
@@ -348,6 +373,12 @@ hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_
                            hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_rebase_walk<<<blocks_for(n), kThreads, 0, st>>>(src, n, dst, tri_base, walk_base);
+    return hipGetLastError();
+}
+
+hipError_t gpu_pack_triangles(const mrt_triangle* tris, uint32_t n, float4* geo, float4* shade, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_pack_triangles<<<blocks_for(n), kThreads, 0, st>>>(reinterpret_cast<const float4*>(tris), n, geo, shade);
     return hipGetLastError();
 }
 

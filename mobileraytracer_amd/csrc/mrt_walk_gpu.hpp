@@ -62,6 +62,12 @@ hipError_t gpu_rebase_wide(const float4* src, size_t n, float4* dst, uint32_t tr
 hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_base, uint32_t walk_base,
                            hipStream_t st);
 
+// The device records of n scene triangles from their mrt_triangle array
+// (device memory): tri_geo (3 float4: A.xyz AB.x | AB.yz AC.xy | AC.z id 0 0,
+// id = MRT_MAKE_ID(MRT_CAT_TRIANGLE, index)) and tri_shade (5 float4: nA nB
+// nC cA cB cC materialId), the same bits as the host packing.
+hipError_t gpu_pack_triangles(const mrt_triangle* tris, uint32_t n, float4* geo, float4* shade, hipStream_t st);
+
 // Boxes (6 floats: min xyz, max xyz) of the first n triangles of a tri_geo
 // array (3 float4 per triangle: A.xyz AB.x | AB.yz AC.xy | AC.z id), the
 // same arithmetic as the host's triangle_box.
