@@ -785,7 +785,7 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t gro
         perCu = std::max(1, a);
     }
     const uint32_t resident = static_cast<uint32_t>(perCu) * num_cu;
-    const uint32_t share = groups > 1u ? std::max(1u, resident * 3u / (2u * groups)) : resident;
+    const uint32_t share = groups > 1u ? std::max(1u, resident * 9u / (5u * groups)) : resident;
     return std::min({cap, resident, share});
 }
 

@@ -146,13 +146,15 @@ inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples) {
 }
 
 // Workgroups of one persistent trace launch when `groups` launches run
-// concurrently: 3 / (2 groups) of those that can be resident at once on
+// concurrently: 9 / (5 groups) of those that can be resident at once on
 // `num_cu` compute units (all of them for one group), at most `cap`.  A launch
 // carries a few rays per lane only if it does not claim the whole chip; on the
 // 1M-triangle 1080p frame with three groups and one slot per pixel, launches of
 // 25 / 34 / 40 / 46 / 67 / 100 % of the resident workgroups took 205 / 193 /
 // 187 / 190 / 204 / 206 ms per frame; with 8 slots per pixel, 40 / 50 / 60 / 70
-// % took 148 / 140 / 140 / 153 ms.
+// % took 148 / 140 / 140 / 153 ms when waves refilled at 8 idle lanes, and 50
+// / 60 / 65 / 70 % took 138.7 / 133.9 / 140.1 / 147.0 ms with the refill at 32
+// idle lanes.
 uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups);
 
 // Queues the camera rays of sample `first_sample` of every slot.
