@@ -197,6 +197,11 @@ typedef struct mrt_render_desc {
     uint32_t persistent;       /* wavefront: 0 default (on), 1 on, 2 off: lanes
                                   refill from the queue (BVH, triangles+lights) */
     float depth_max_point[3];  /* DepthMap maxPoint_ (DepthMap.cpp:12-16)       */
+    uint32_t tile_skip[8];     /* bit t (t = tile row * 16 + tile column) set:
+                                  tile t is not rendered (region renders and
+                                  tile shards; no counterpart in the
+                                  reference); all zero: every tile.  Other
+                                  pixels are unchanged by it.                 */
 } mrt_render_desc;
 
 typedef struct mrt_render_stats {
@@ -222,6 +227,8 @@ typedef struct mrt_render_stats {
     float ms_trace_busy;       /* wavefront: time at least one trace launch ran
                                   (launches of slot groups overlap)            */
     uint32_t trace_groups;     /* wavefront: slot groups run concurrently      */
+    uint32_t slot_lanes;       /* wavefront: path slots per pixel (samples of a
+                                  pixel in flight together)                    */
 } mrt_render_stats;
 
 /* Accelerator build parameters (Shader::initializeAccelerators inputs). */

@@ -48,7 +48,10 @@ def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Came
                 depth_min: int = 4, depth_max: int = 8, pixel=None, rr=None, light=None, hemi=None,
                 tile_seed: int = 0, kernel: int = abi.KERNEL_MEGAKERNEL, first_sample: int = 0,
                 instrument: bool = False, ray_sort: int = 0, persistent: int = 0,
-                depth_max_point=(1.0, 1.0, 1.0)) -> abi.RenderDesc:
+                depth_max_point=(1.0, 1.0, 1.0), tiles=None) -> abi.RenderDesc:
+    """Render descriptor.  tiles: None renders every tile of the 16x16
+    grid; else an iterable of tile ids (row * 16 + column) to render, the
+    others are skipped (tile_skip)."""
     d = abi.RenderDesc()
     d.shader = shader
     d.width, d.height, d.samples_pixel, d.samples_light = width, height, spp, samples_light
@@ -66,6 +69,10 @@ def render_desc(shader: int, width: int, height: int, spp: int, camera: abi.Came
     d.ray_sort = ray_sort
     d.persistent = persistent
     d.depth_max_point[:] = [float(x) for x in depth_max_point]
+    if tiles is not None:
+        keep = set(int(t) for t in tiles)
+        for k in range(8):
+            d.tile_skip[k] = sum(1 << b for b in range(32) if 32 * k + b not in keep)
     return d
 
 

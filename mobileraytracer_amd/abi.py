@@ -101,7 +101,8 @@ class RenderDesc(C.Structure):
                 ("sampler_hemi", Sampler),
                 ("tile_values", C.c_float * 256),
                 ("kernel", C.c_int32), ("first_sample", C.c_uint32), ("instrument", C.c_uint32),
-                ("ray_sort", C.c_uint32), ("persistent", C.c_uint32), ("depth_max_point", F3)]
+                ("ray_sort", C.c_uint32), ("persistent", C.c_uint32), ("depth_max_point", F3),
+                ("tile_skip", C.c_uint32 * 8)]
 
 
 class RenderStats(C.Structure):
@@ -112,7 +113,8 @@ class RenderStats(C.Structure):
                 ("samples_done", C.c_uint32), ("covered_pixels", C.c_uint32),
                 ("ms_render", C.c_float), ("ms_kernel", C.c_float), ("ms_trace", C.c_float),
                 ("trace_launches", C.c_uint32), ("trace_lane_iters", C.c_uint64),
-                ("trace_lane_slots", C.c_uint64), ("ms_trace_busy", C.c_float), ("trace_groups", C.c_uint32)]
+                ("trace_lane_slots", C.c_uint64), ("ms_trace_busy", C.c_float), ("trace_groups", C.c_uint32),
+                ("slot_lanes", C.c_uint32)]
 
 
 class ServerDesc(C.Structure):

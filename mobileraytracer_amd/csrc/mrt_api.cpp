@@ -433,6 +433,13 @@ int build_tri_bvh_resident(mrt_scene* s, uint32_t leaf, hipStream_t st, BvhOut* 
 }
 
 // Render parameters common to mrt_render and mrt_shade_rays.
+// Tiles of the 16x16 grid not skipped by tile_skip.
+uint32_t rendered_tiles(const RenderParams& p) {
+    uint32_t n = 0;
+    for (int k = 0; k < 8; ++k) n += static_cast<uint32_t>(__builtin_popcount(~p.tile_skip[k]));
+    return n;
+}
+
 RenderParams params_from(const mrt_render_desc& d) {
     RenderParams p{};
     p.shader = d.shader;
@@ -448,6 +455,7 @@ RenderParams params_from(const mrt_render_desc& d) {
     p.s_light = d.sampler_light;
     p.s_hemi = d.sampler_hemi;
     for (int k = 0; k < 3; ++k) p.max_point[k] = d.depth_max_point[k];
+    for (int k = 0; k < 8; ++k) p.tile_skip[k] = d.tile_skip[k];
     return p;
 }
 
@@ -567,7 +575,7 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
         MRT_HIP(hipMemsetAsync(wbs[g].counts, 0, kCntWords * sizeof(uint32_t), st[g]));
         MRT_HIP(launch_wavefront_start(p, d->first_sample, wbs[g], &L[g], st[g]));
     }
-    const uint64_t covered = 256ull * p.bsx * p.bsy;
+    const uint64_t covered = static_cast<uint64_t>(rendered_tiles(p)) * p.bsx * p.bsy;
     const uint32_t lanes = wbs[0].lanes;
     const uint32_t samples = (d->samples_pixel - d->first_sample + lanes - 1u) / lanes;  // per slot
     // A sample takes at most depth_max + 1 iterations (one per vertex, one to
@@ -740,6 +748,7 @@ int render_setup(const mrt_scene* s, const mrt_render_desc* d, uint32_t stride_b
     p.compat = compat ? 1u : 0u;
     p.pixels_per_pass = kBlocks * bsx * bsy;
     p.tile_rank = nullptr;
+    if (rendered_tiles(p) == 0u) return fail(MRT_E_INVALID, "tile_skip skips every tile");
     *out = p;
     *rank_out = std::move(rank);
     return MRT_OK;
@@ -1592,7 +1601,7 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->shadow_node_visits = cnt[6];
         stats->shadow_prim_tests = cnt[9];
         stats->samples_done = s->progress.load();
-        stats->covered_pixels = 256u * bsx * bsy;
+        stats->covered_pixels = rendered_tiles(p) * bsx * bsy;
         stats->ms_render = ms;
         stats->ms_kernel = kernel_ms;
         stats->ms_trace = trace_ms;
@@ -1601,6 +1610,7 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->trace_lane_slots = cnt[11];
         stats->ms_trace_busy = trace_busy;
         stats->trace_groups = useWavefront ? static_cast<uint32_t>(wbs.size()) : 0u;
+        stats->slot_lanes = useWavefront ? lanes : 0u;
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;

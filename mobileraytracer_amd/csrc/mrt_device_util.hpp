@@ -84,6 +84,14 @@ __device__ __forceinline__ void slot_pixel(const RenderParams& p, uint32_t slot,
     *y = (wave / tilesX) * 8u + (lane >> 3);
 }
 
+// Pixel (x, y) is rendered: inside the 16x16 tile grid (Renderer.cpp:20-25,
+// 82-93) and its tile not skipped.
+__device__ __forceinline__ bool pixel_covered(const RenderParams& p, uint32_t x, uint32_t y) {
+    if (x >= 16u * p.bsx || y >= 16u * p.bsy) return false;
+    const uint32_t t = (y / p.bsy) * 16u + x / p.bsx;
+    return ((p.tile_skip[t >> 5] >> (t & 31u)) & 1u) == 0u;
+}
+
 // Pixel jitter draws of Renderer.cpp:96-99 for pixel (x, y), sample pass
 // `sample`; *_COMPAT kinds replay the frame-wide counter in single-thread
 // claim order (HaltonSeq.cpp:14-22, Stratified.cpp:14-23).

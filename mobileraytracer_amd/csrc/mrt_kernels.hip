@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(SceneDev s, RenderParams
     Stack st = make_stack(lds, sc, gtid);
     RayCounts rc{};
     uint32_t pendOverflow = 0;
-    if (x < 16u * p.bsx && y < 16u * p.bsy) {
+    if (pixel_covered(p, x, y)) {
         const uint32_t pid = y * p.width + x;
         f3 o, d;
         primary_ray(p, x, y, pid, sample, &o, &d);
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_aov(SceneDev s, RenderParams p, uint
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     Stack st = make_stack(lds, sc, gtid);
     TraceStats ts{};
-    if (x < 16u * p.bsx && y < 16u * p.bsy) {
+    if (pixel_covered(p, x, y)) {
         const uint32_t pid = y * p.width + x;
         f3 alb{}, nrm{};
         float dep = 0.0f;

@@ -41,6 +41,7 @@ struct RenderParams {
     uint32_t compat;              // pixel sampler is a *_COMPAT kind
     uint32_t pixels_per_pass;     // 256 * bsx * bsy
     float max_point[3];           // DepthMap maxPoint_
+    uint32_t tile_skip[8];        // bit ty * 16 + tx: tile not rendered
 };
 
 // Ray counters of one lane, reduced per wave at the end of a launch.

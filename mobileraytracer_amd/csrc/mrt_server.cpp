@@ -856,6 +856,8 @@ MRT_API int mrt_server_submit(mrt_server* s, uint32_t scene_id, const mrt_render
                                        "samples_light 1, no reflective or transmissive materials, "
                                        "ray_depth_max >= 1 and at most 65535 samples per pixel");
     if (d->first_sample >= d->samples_pixel) return fail(MRT_E_INVALID, "first_sample must be below samples_pixel");
+    for (uint32_t k = 0; k < 8; ++k)
+        if (d->tile_skip[k] != 0u) return fail(MRT_E_UNSUPPORTED, "render servers render whole frames (tile_skip)");
     j->scene = scene_id;
     j->desc = *d;
     j->bitmap = bitmap;

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_generate(RenderParams p, uint32_t
     const uint32_t s0 = sample + lane;
     if (local < b.slot_count) {
         slot_pixel(p, slot - lane * b.pixel_slots, &x, &y);
-        live = x < 16u * p.bsx && y < 16u * p.bsy && s0 < p.spp;
+        live = pixel_covered(p, x, y) && s0 < p.spp;
     }
     f3 o{}, d{};
     if (live) {
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(RenderParams p, uint32_t 
     if (ps >= b.pixel_slots) return;
     uint32_t x, y;
     slot_pixel(p, ps, &x, &y);
-    if (x >= 16u * p.bsx || y >= 16u * p.bsy) return;
+    if (!pixel_covered(p, x, y)) return;
     const uint32_t pid = y * p.width + x;
     const uint32_t bi = y * p.real_width + x;
     uint32_t bm = bitmap[bi];
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_aov_resolve(RenderParams p, uint3
     if (ps >= b.pixel_slots) return;
     uint32_t x, y;
     slot_pixel(p, ps, &x, &y);
-    if (x >= 16u * p.bsx || y >= 16u * p.bsy) return;
+    if (!pixel_covered(p, x, y)) return;
     const uint32_t pid = y * p.width + x;
     f3 alb{}, nrm{};
     float dep = 0.0f;

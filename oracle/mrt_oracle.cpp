@@ -539,6 +539,16 @@ int render(const Scene<R>& scene, const mrt_render_desc& d, const RenderArgs& a,
                 const uint32_t pixel = roundBlock * bsx % resolution;
                 const uint32_t startY = ((pixel / w) * bsy) % h;
                 const uint32_t endY = startY + bsy;
+                // tile_skip (no counterpart in the reference): the tile is
+                // not rendered; a compat counter still advances by the
+                // tile's draws, so every other pixel draws what it draws in
+                // the whole frame.
+                const uint32_t tileId = (startY / bsy) * 16u + (pixel % w) / bsx;
+                if ((d.tile_skip[tileId >> 5] >> (tileId & 31u)) & 1u) {
+                    if (isCompat)
+                        for (uint32_t k = 0; k < 2u * bsx * bsy; ++k) (void)pixelDraw(0, sample, SLOT_JITTER_U);
+                    continue;
+                }
                 for (uint32_t y = startY; y < endY; ++y) {
                     const float v = static_cast<float>(y) * invH;
                     const uint32_t yWidth = y * realWidth;
