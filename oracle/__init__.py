@@ -55,6 +55,8 @@ def lib():
             "oracle_triangle_aabb": (C.c_int, [P, P]),
             "oracle_triangle_intersect_box": (C.c_int, [P, P, P]),
             "oracle_plane_intersect_box": (C.c_int, [P, P, P]),
+            "oracle_sphere_intersect_box": (C.c_int, [P, P, P]),
+            "oracle_grid_walk": (C.c_int, [P, P, P, u32, P, P]),
             "oracle_plane_distance": (f32, [P, P]),
             "oracle_plane_make": (C.c_int, [P, P, i32, P]),
             "oracle_slab": (C.c_int, [P, P, P]),
@@ -116,6 +118,19 @@ class OracleScene:
         refs = np.zeros(max(1, nref.value), np.uint32)
         lib().oracle_grid_cells(self.h, category, off.ctypes.data, refs.ctypes.data)
         return gs.value, bounds, off, refs[: nref.value]
+
+    def grid_walk(self, ray: np.ndarray, hit_dtype, cap: int = 4096):
+        """Triangle-grid walk of one ray: (visited cell indices in walk
+        order, hit record)."""
+        cells = np.zeros(cap, np.uint32)
+        n = C.c_uint32()
+        hit = np.zeros(1, hit_dtype)
+        rc = lib().oracle_grid_walk(self.h, ray.ctypes.data, cells.ctypes.data, cap, C.byref(n), hit.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"oracle_grid_walk failed: {rc}")
+        if n.value > cap:
+            raise RuntimeError(f"grid walk visited {n.value} cells, more than {cap}")
+        return cells[: n.value].copy(), hit[0]
 
     def trace_closest(self, rays: np.ndarray, threads: int = 8, hit_dtype=None):
         out = np.zeros(len(rays), hit_dtype)

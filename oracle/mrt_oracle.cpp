@@ -920,6 +920,27 @@ int oracle_triangle_intersect_box(const mrt_triangle* t, const float* mn, const 
     const Triangle<float> tri = Triangle<float>::from(*t, MRT_MAKE_ID(MRT_CAT_TRIANGLE, 0));
     return tri.intersect_box(AABB<float>{from3<float>(mn), from3<float>(mx)}) ? 1 : 0;
 }
+int oracle_sphere_intersect_box(const mrt_sphere* sp, const float* mn, const float* mx) {
+    const Sphere<float> sph = Sphere<float>::from(*sp, MRT_MAKE_ID(MRT_CAT_SPHERE, 0));
+    return sph.intersect_box(AABB<float>{from3<float>(mn), from3<float>(mx)}) ? 1 : 0;
+}
+// RegularGrid<T>::trace of the triangle grid alone for one ray: the hit and
+// the cells the walk visited, in order (at most cap are written; *n_cells is
+// the full count).
+int oracle_grid_walk(void* hv, const mrt_ray* ray, uint32_t* cells, uint32_t cap, uint32_t* n_cells, mrt_hit* hit) {
+    auto* h = static_cast<Handle*>(hv);
+    if (h->f->accel != MRT_REGULAR_GRID) return MRT_E_INVALID;
+    std::vector<uint32_t> visited;
+    TraceCounters tc;
+    tc.cells = &visited;
+    Isect<float> is;
+    is.length = ray->tmax;
+    const bool hv2 = h->f->gridT.trace(&is, ray_f(*ray), &tc);
+    fill_hit(is, hv2, hit);
+    *n_cells = static_cast<uint32_t>(visited.size());
+    for (uint32_t i = 0; i < visited.size() && i < cap; ++i) cells[i] = visited[i];
+    return 0;
+}
 int oracle_plane_intersect_box(const mrt_plane* p, const float* mn, const float* mx) {
     const Plane<float> pl = Plane<float>::from(*p, MRT_MAKE_ID(MRT_CAT_PLANE, 0));
     return pl.intersect_box(AABB<float>{from3<float>(mn), from3<float>(mx)}) ? 1 : 0;

@@ -157,6 +157,7 @@ struct Isect {
 struct TraceCounters {
     uint64_t pairs = 0;
     uint64_t prims = 0;
+    std::vector<uint32_t>* cells = nullptr;  // optional: grid cells visited, in walk order
 };
 
 // --------------------------------------------------------------- Triangle --
@@ -735,6 +736,7 @@ struct RegularGrid {
         bool hit = false;
         bool phase2 = false;
         while (!phase2) {
+            if (tc && tc->cells) tc->cells->push_back(cell_index(g));
             const std::vector<uint32_t> list = cells[cell_index(g)];  // copied as in :243
             for (uint32_t pi : list) {
                 if (tc) ++tc->prims;
@@ -766,6 +768,8 @@ struct RegularGrid {
             }
         }
         while (true) {
+            // Phase 2 starts in the cell phase 1 ended in (logged there).
+            if (tc && tc->cells && tc->cells->back() != cell_index(g)) tc->cells->push_back(cell_index(g));
             const std::vector<uint32_t> list = cells[cell_index(g)];  // copied as in :290
             for (uint32_t pi : list) {
                 if (tc) ++tc->prims;
