@@ -485,6 +485,12 @@ MRT_API int mrt_triangle_make(const float a[3], const float b[3], const float c[
 /* Camera constructor basis (Camera.cpp:47-54) and Perspective/Orthographic fields. */
 MRT_API int mrt_camera_make(int kind, const float position[3], const float look_at[3], const float up[3],
                             float fov_h_or_size_h, float fov_v_or_size_v, mrt_camera* out);
+/* Camera::getAABB, the camera's share of the RegularGrid bounds
+ * (Shader.cpp:52-62): the position for a perspective camera (Camera.cpp:39-45);
+ * for an orthographic one the view rectangle widened by half a unit, with the
+ * reference's vertical order (its min takes the top edge, its max the bottom
+ * one, Orthographic.cpp:33-45).  out: min xyz, max xyz. */
+MRT_API int mrt_camera_aabb(const mrt_camera* camera, float out[6]);
 /* Camera tile table (Camera.cpp:16-26): haltonSequence(i, 2) for i < 256,
  * shuffled by a seeded Fisher-Yates (seed 0 keeps the unshuffled order). */
 MRT_API int mrt_tile_values(uint32_t seed, float out[256]);

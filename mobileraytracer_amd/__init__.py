@@ -33,9 +33,10 @@ def make_camera(kind, position, look_at, up, a, b) -> abi.Camera:
 
 
 def camera_aabb(cam: abi.Camera) -> tuple[np.ndarray, np.ndarray]:
-    """Camera::getAABB for a perspective camera (Camera.cpp:39-45): its position."""
-    p = np.array(cam.position, np.float32)
-    return p, p.copy()
+    """Camera::getAABB (Camera.cpp:39-45, Orthographic.cpp:33-45): (min, max)."""
+    out = np.zeros(6, np.float32)
+    check(lib().mrt_camera_aabb(C.byref(cam), out.ctypes.data), "mrt_camera_aabb")
+    return out[:3].copy(), out[3:].copy()
 
 
 def tile_values(seed: int = 0) -> np.ndarray:

@@ -214,6 +214,7 @@ def _proto(lib):
         "mrt_conv3x3_bf16": (C.c_int, [P, P, u32, u32, u32, P, P, u32, C.c_int, P, C.c_int]),
         "mrt_triangle_make": (C.c_int, [P, P, P, i32, P, P, P]),
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
+        "mrt_camera_aabb": (C.c_int, [C.POINTER(Camera), P]),
         "mrt_tile_values": (C.c_int, [u32, P]),
         "mrt_triangle_intersect_box": (C.c_int, [P, P, P]),
         "mrt_primitive_intersect_box": (C.c_int, [C.c_int, P, P, P]),

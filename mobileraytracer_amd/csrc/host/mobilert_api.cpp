@@ -975,11 +975,10 @@ MobileRT::Ray Orthographic::generateRay(float u, float v, float deviationU, floa
 
 // Orthographic.cpp:33-45 (the view rectangle widened by half a unit).
 MobileRT::AABB Orthographic::getAABB() const noexcept {
-    const vec3 mn = ((position_ + right_ * (0.0f - 0.5f) * sizeH_) + right_ * -0.5f + up_ * (0.5f - 0.0f) * sizeV_) +
-                    up_ * -0.5f;
-    const vec3 mx = ((position_ + right_ * (1.0f - 0.5f) * sizeH_) + right_ * 0.5f + up_ * (0.5f - 1.0f) * sizeV_) +
-                    up_ * 0.5f;
-    return MobileRT::AABB{mn, mx};
+    const mrt_camera d = descriptor();
+    float b[6] = {};
+    MobileRT::ok(mrt_camera_aabb(&d, b), "mrt_camera_aabb");
+    return MobileRT::AABB{MobileRT::ld3(b), MobileRT::ld3(b + 3)};
 }
 
 float Orthographic::getSizeH() const noexcept { return sizeH_; }

@@ -1095,10 +1095,29 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
     if (leaf > 16) return fail(MRT_E_UNSUPPORTED, "leaf size above 16");
     const uint32_t builder = params ? params->builder : static_cast<uint32_t>(MRT_BUILDER_AUTO);
     if (builder > MRT_BUILDER_GPU) return fail(MRT_E_INVALID, "unknown builder");
-    uint32_t lightDepth = s->dev.lights_linear ? 0u : s->dev.light_acc.depth;
-    s->depth_need = lightDepth;
     const uint32_t counts[3] = {static_cast<uint32_t>(s->plas.size()), static_cast<uint32_t>(s->sphs.size()),
                                 static_cast<uint32_t>(s->tris.size())};
+    if (s->ctx->budget) {
+        // Refused before any device work when the geometry and the smallest
+        // accelerator of this kind already exceed the budget: a BVH has at
+        // least ceil(n / leaf) - 1 interior nodes (64-byte child pairs, the
+        // triangle tree twice with its walk pairs), a grid its cell offsets.
+        uint64_t need = s->tri_geo.bytes + s->tri_shade.bytes + s->sph.bytes + s->sph_mat.bytes + s->pla.bytes +
+                        s->mat.bytes + s->light_geo.bytes + s->light_shade.bytes + s->light_info.bytes;
+        for (int c = 0; c < 3; ++c) {
+            if (counts[c] == 0) continue;
+            if (accelerator == MRT_BVH)
+                need += ((counts[c] + leaf - 1) / leaf - 1) * sizeof(BNode) * (c == MRT_CAT_TRIANGLE ? 2u : 1u);
+            else if (accelerator == MRT_REGULAR_GRID)
+                need += (static_cast<uint64_t>(grid_size_for(counts[c])) * grid_size_for(counts[c]) *
+                             grid_size_for(counts[c]) + 1) * 4u;
+        }
+        if (need > s->ctx->budget)
+            return fail(MRT_E_BUDGET, "accelerator needs at least " + std::to_string(need) + " bytes, budget " +
+                                          std::to_string(s->ctx->budget));
+    }
+    uint32_t lightDepth = s->dev.lights_linear ? 0u : s->dev.light_acc.depth;
+    s->depth_need = lightDepth;
     int rc;
     for (int c = 0; c < 3; ++c) {
         Accel& a = s->dev.acc[c];

@@ -141,6 +141,25 @@ MRT_API int mrt_camera_make(int kind, const float position[3], const float look_
     return MRT_OK;
 }
 
+MRT_API int mrt_camera_aabb(const mrt_camera* c, float out[6]) {
+    if (!c || !out) return MRT_E_INVALID;
+    const f3 P = ld3(c->position);
+    f3 mn = P, mx = P;
+    if (c->kind == MRT_CAMERA_ORTHOGRAPHIC) {
+        const f3 R = ld3(c->right), U = ld3(c->up);
+        const float sh = c->fov_h, sv = c->fov_v;  // sizeH_, sizeV_
+        mn = add(add(add(add(P, muls(muls(R, 0.0f - 0.5f), sh)), muls(R, -0.5f)), muls(muls(U, 0.5f - 0.0f), sv)),
+                 muls(U, -0.5f));
+        mx = add(add(add(add(P, muls(muls(R, 1.0f - 0.5f), sh)), muls(R, 0.5f)), muls(muls(U, 0.5f - 1.0f), sv)),
+                 muls(U, 0.5f));
+    } else if (c->kind != MRT_CAMERA_PERSPECTIVE) {
+        return MRT_E_INVALID;
+    }
+    st3(out, mn);
+    st3(out + 3, mx);
+    return MRT_OK;
+}
+
 // FillThings (Camera.cpp:16-26) with a seeded shuffle in place of the
 // /dev/urandom-seeded std::shuffle.
 MRT_API int mrt_tile_values(uint32_t seed, float out[256]) {

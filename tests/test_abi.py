@@ -77,6 +77,29 @@ def test_camera_make_matches_oracle():
         assert bytes(pc) == bytes(oc)
 
 
+def test_camera_aabb_closed_forms():
+    """Camera::getAABB (Camera.cpp:39-45): the position; Orthographic::getAABB
+    (Orthographic.cpp:33-45): position + R * (-0.5 sizeH_ - 0.5) + U *
+    (0.5 sizeV_ - 0.5) as min, position + R * (0.5 sizeH_ + 0.5) + U *
+    (-0.5 sizeV_ + 0.5) as max (the vertical order inverted for sizeV_ >
+    1).  Dyadic sizes make the float32 results exact."""
+    cam = mrt.make_camera(abi.CAMERA_PERSPECTIVE, (0.5, 0.25, -1.0), (0.5, 0.5, 0.5), (0, 1, 0), 45.0, 45.0)
+    mn, mx = mrt.camera_aabb(cam)
+    assert mn.tolist() == mx.tolist() == np.array(cam.position, np.float32).tolist()
+    cam = mrt.make_camera(abi.CAMERA_ORTHOGRAPHIC, (0, 0, -3.5), (0, 0, 1), (0, 1, 0), 3.0, 4.0)
+    assert (cam.fov_h, cam.fov_v) == (1.5, 2.0)
+    mn, mx = mrt.camera_aabb(cam)
+    assert mn.tolist() == [-1.25, 0.5, -3.5]
+    assert mx.tolist() == [1.25, -0.5, -3.5]
+    # A general pose against float64.
+    cam = mrt.make_camera(abi.CAMERA_ORTHOGRAPHIC, (1, 2, 3), (-1, 0.5, 7), (0.1, 1, 0), 5.0, 3.0)
+    P, R, U = (np.array(v, np.float64) for v in (cam.position, cam.right, cam.up))
+    sh, sv = float(cam.fov_h), float(cam.fov_v)
+    mn, mx = mrt.camera_aabb(cam)
+    np.testing.assert_allclose(mn, P + R * (-0.5 * sh - 0.5) + U * (0.5 * sv - 0.5), atol=2e-6)
+    np.testing.assert_allclose(mx, P + R * (0.5 * sh + 0.5) + U * (-0.5 * sv + 0.5), atol=2e-6)
+
+
 def test_intersect_box_matches_oracle():
     rng = np.random.default_rng(1)
     hits = 0

@@ -50,6 +50,11 @@ def conv_ref(x, w, bias, relu):
     (9, 64, 64, 96, False, False),
     (17, 130, 96, 64, True, True),
     (3, 200, 256, 128, True, True),
+    # Enough tiles for the 64-output-channel kernel (at least two workgroups
+    # per compute unit), the variant the 1080p network's full-resolution
+    # layers run on.
+    (544, 960, 64, 64, True, True),
+    (136, 512, 96, 128, False, True),
 ])
 def test_conv3x3_matches_fp32(gpu_ctx, h, w, cin, cout, relu, bias):
     rng = np.random.default_rng(h * 1000 + cin)
@@ -213,3 +218,25 @@ def test_trained_weights_on_a_rendered_frame(gpu_ctx):
 
     plain, den = rel_mse(acc4 / 4.0), rel_mse(got)
     assert den <= 0.25 * plain, (plain, den)
+
+
+def test_trained_network_full_hd_frame(gpu_ctx):
+    """The shipped network over a whole 1920x1080 frame (the benchmark's
+    denoiser size: every level at its real shape, the full-resolution layers
+    on the 64-channel kernel) against the PyTorch statement."""
+    flat, levels, channels, ds = denoise.load_weights()
+    rad, aov = random_frame(1080, 1920, 77)
+    dn = mrt.Denoiser(gpu_ctx, flat, mrt.denoiser_desc(levels, channels, ds))
+    try:
+        got = dn.run(rad, aov, radiance_scale=0.25)
+    finally:
+        dn.close()
+    _torch()
+    model = denoise.make_unet(levels, channels)
+    denoise.set_flat_weights(model, flat)
+    ref = denoise.reference_denoise(model, rad, aov, 0.25, ds, bf16=True)
+    lg, lr = np.log1p(got), np.log1p(ref)
+    rel = np.linalg.norm(lg - lr) / max(np.linalg.norm(lr), 1e-6)
+    worst = np.max(np.abs(lg - lr)) / max(np.max(np.abs(lr)), 1e-6)
+    print(f"1080p network: relative L2 {rel:.2e}, worst {worst:.2e}", flush=True)
+    assert rel <= 2e-2 and worst <= 6e-2, (rel, worst)
