@@ -287,6 +287,17 @@ MRT_API int mrt_scene_footprint(const mrt_scene* scene, uint64_t* bytes);
  * planes, spheres, triangles (through the built accelerator) and lights
  * (Scene::traceLights, Scene.cpp:31-37). */
 MRT_API int mrt_trace_closest(mrt_scene* scene, const mrt_ray* rays, uint32_t n, mrt_hit* hits, int mem);
+/* mrt_trace_closest on device memory that allocates nothing.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  rays and hits are
+ * 16-byte-aligned device pointers.  The traversal-stack spill and the error
+ * flag live in a 16 MiB array the library holds on each GPU until the process
+ * ends, so the call allocates nothing; trees deeper than it supports (about
+ * 16,000 levels) are refused with MRT_E_BUDGET.  Results equal
+ * mrt_trace_closest's.  Calls on one GPU share that array: calls that overlap
+ * on one GPU (scenes of different contexts on it, used from several threads)
+ * race. */
+MRT_API int mrt_trace_closest_noalloc(mrt_scene* scene, const mrt_ray* rays, uint32_t n, mrt_hit* hits);
 /* Shader::shadowTrace (Shader.cpp:162-187): any hit in (1e-6, tmax), lights excluded. */
 MRT_API int mrt_trace_any(mrt_scene* scene, const mrt_ray* rays, uint32_t n, uint8_t* occluded, int mem);
 /* Shader::rayTrace including shade(): radiance and the "hit light" flag for a

@@ -188,6 +188,7 @@ def _proto(lib):
         "mrt_scene_build": (C.c_int, [P, C.c_int, C.POINTER(AccelParams)]),
         "mrt_scene_footprint": (C.c_int, [P, C.POINTER(u64)]),
         "mrt_trace_closest": (C.c_int, [P, P, u32, P, C.c_int]),
+        "mrt_trace_closest_noalloc": (C.c_int, [P, P, u32, P]),
         "mrt_trace_any": (C.c_int, [P, P, u32, P, C.c_int]),
         "mrt_shade_rays": (C.c_int, [P, C.POINTER(RenderDesc), P, P, u32, u32, P, P, C.c_int]),
         "mrt_camera_rays": (C.c_int, [P, C.POINTER(Camera), P, u32, P, C.c_int]),

@@ -826,8 +826,11 @@ MRT_API int mrt_ctx_create(int device, mrt_ctx** out) {
     MRT_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     MRT_HIP(hipGetDeviceProperties(&prop, device));
+    uint32_t* scratch = nullptr;
+    MRT_HIP(noalloc_scratch(device, &scratch));
     auto* c = new mrt_ctx;
     c->device = device;
+    c->scratch = scratch;
     c->num_cu = static_cast<uint32_t>(prop.multiProcessorCount);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -1244,6 +1247,44 @@ MRT_API int mrt_trace_closest(mrt_scene* s, const mrt_ray* rays, uint32_t n, mrt
                                  s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), st));
     if (mem != MRT_MEM_DEVICE) MRT_HIP(hipMemcpyAsync(hits, dout, n * sizeof(mrt_hit), hipMemcpyDeviceToHost, st));
     return check_err(s);
+}
+
+MRT_API int mrt_trace_closest_noalloc(mrt_scene* s, const mrt_ray* rays, uint32_t n, mrt_hit* hits) {
+    if (!s || (n && (!rays || !hits))) return fail(MRT_E_INVALID, "null argument");
+    if (!s->built) return fail(MRT_E_INVALID, "scene has no accelerator (call mrt_scene_build)");
+    if ((reinterpret_cast<uintptr_t>(rays) | reinterpret_cast<uintptr_t>(hits)) & 15u)
+        return fail(MRT_E_INVALID, "rays and hits must be 16-byte aligned device memory");
+    if (n == 0) return MRT_OK;
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    MRT_HIP(hipSetDevice(s->ctx->device));
+    const hipStream_t st = s->ctx->stream;
+    uint32_t* scr = s->ctx->scratch;
+    if (!scr) return fail(MRT_E_HIP, "no scratch address for this device");
+    // The spill of one launch must fit the scratch: as many 256-lane
+    // workgroups as it holds, each lane walking rays grid-stride.
+    uint32_t blocks = std::min<uint32_t>(cap_blocks(s->ctx), (n + 255u) / 256u);
+    StackCfg sc{nullptr, 0u, 0u};
+    const uint32_t need = s->depth_need + 1u;
+    if (need > kStackLds) {
+        sc.cap_spill = need - kStackLds;
+        const uint64_t fit = (kNoAllocScratchWords - kNoAllocReserved) / (256ull * sc.cap_spill);
+        if (fit == 0)
+            return fail(MRT_E_BUDGET, "tree depth " + std::to_string(s->depth_need) + " needs " +
+                                          std::to_string(256ull * sc.cap_spill) +
+                                          " scratch words per workgroup; the allocation-free scratch holds " +
+                                          std::to_string(kNoAllocScratchWords - kNoAllocReserved));
+        blocks = static_cast<uint32_t>(std::min<uint64_t>(blocks, fit));
+        sc.spill = scr + kNoAllocReserved;
+        sc.spill_stride = blocks * 256u;
+    }
+    uint32_t* err = scr;
+    MRT_HIP(launch_fill_u32(err, 1u, 0u, st));
+    MRT_HIP(launch_trace_closest(s->dev, s->generic, rays, n, hits, sc, blocks, false, nullptr, err, st));
+    uint32_t e = 0;
+    MRT_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, st));
+    MRT_HIP(hipStreamSynchronize(st));
+    if (e & 1u) return fail(MRT_E_INVALID, "traversal stack overflow (BVH deeper than the stack budget)");
+    return MRT_OK;
 }
 
 MRT_API int mrt_trace_any(mrt_scene* s, const mrt_ray* rays, uint32_t n, uint8_t* occluded, int mem) {

@@ -100,6 +100,8 @@ struct mrt_ctx {
     // GPU grid builds: scratch and the staged primitives.
     mrt::GridGpuScratch grid_scratch;
     mrt::DevBuf grid_prims;
+    // This GPU's copy of the allocation-free entry point's scratch array.
+    uint32_t* scratch = nullptr;
 };
 
 struct mrt_scene {

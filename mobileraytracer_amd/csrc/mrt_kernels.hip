@@ -8,9 +8,46 @@
 // Render passes map one wave to an 8x8 pixel block and renumber workgroups so
 // that each of the 8 XCDs (which receive workgroups round-robin) works on a
 // contiguous band of the image, keeping neighbouring rays in one L2.
+#include <mutex>
+
 #include "mrt_device_util.hpp"
 
 namespace mrt {
+
+// Scratch of mrt_trace_closest_noalloc (see noalloc_scratch).  Host code
+// never uses this name as an address: it takes each GPU's copy from
+// hipGetSymbolAddress.
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, uint32_t** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_fill_u32<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(p, n, v);
+    return hipGetLastError();
+}
 
 // ------------------------------------------------------------- trace --
 template <bool GENERIC, bool COUNT>
