@@ -103,14 +103,47 @@ def soup_camera(mrt, abi):
                            float(np.float32(45.0) * ratio), 45.0)
 
 
-def cpu_baseline(data, cam, abi, mrt, accel: str = "bvh") -> dict:
-    """The oracle (reference-faithful CPU restatement) on a bounded sample of
-    the same workload (same accelerator): the 16 central tiles of the 16x16
-    tile grid, the first 2 sample passes, on min(16, cores) threads."""
+# Tile ids (row * 16 + column) of the CPU baseline's sample: a 4 x 4 lattice
+# over the frame (the cube's image and the background beside it).  The
+# reference's tile loop hands whole tiles to its threads, so the sample needs
+# at least as many tiles as threads.
+BASELINE_TILES = tuple(r * 16 + c for r in (2, 6, 9, 13) for c in (3, 6, 9, 12))
+BASELINE_SPP = 2
+ONE_THREAD_TILE = 7 * 16 + 7
+
+
+def host_threads() -> int:
+    """Host threads of the CPU baseline: the logical CPUs this process may
+    run on, at most 16 (one GPU's share of the GPU box's host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(data, cam, abi, mrt, accel: str = "bvh", runs: int = 3) -> dict:
+    """The oracle (reference-faithful CPU restatement: AoS triangles,
+    unordered un-culled BVH walk, linear light scan, std::thread tile loop)
+    on a bounded sample of the same workload and accelerator: the first 2 of
+    the 16 samples of 16 of the 256 tiles (a lattice over the frame), median
+    of `runs` runs on every host thread available (at most 16); and one
+    thread on one tile at 2 samples.  All 16 samples of 16 tiles would take
+    about 45 s per run on 16 threads; the sample passes differ only in their
+    random numbers, so the rate per ray is that of the whole frame."""
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    desc = make_desc(mrt, abi, cam, spp=2)
+    threads = host_threads()
     sc = oracle.OracleScene(data.desc())
     p = abi.AccelParams()
     p.leaf_size = 2
@@ -119,15 +152,57 @@ def cpu_baseline(data, cam, abi, mrt, accel: str = "bvh") -> dict:
         p.camera_min[:] = list(cmin)
         p.camera_max[:] = list(cmax)
     sc.build(abi.REGULAR_GRID if accel == "grid" else abi.BVH, p)
-    vals = np.frombuffer(bytes(desc.tile_values), np.float32)
-    tiles = np.rint(vals * 256).astype(int)
-    central = {ty * 16 + tx for ty in range(6, 10) for tx in range(6, 10)}
-    mask = np.array([1 if t in central else 0 for t in tiles], np.uint8)
-    bm, counters, secs = sc.render(desc, threads=threads, tile_mask=mask)
-    rays = int(counters.sum())
-    return {"value": rays / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle render ({'RegularGrid' if accel == 'grid' else 'BVH'}), 16 central tiles x 2 spp "
-                      f"of the 1920x1080 frame, {rays} rays in {secs:.2f} s"}
+
+    def mask(desc, tiles):
+        ids = np.rint(np.frombuffer(bytes(desc.tile_values), np.float32) * 256).astype(int)
+        return np.array([1 if t in tiles else 0 for t in ids], np.uint8)
+
+    desc = make_desc(mrt, abi, cam, spp=BASELINE_SPP)
+    rates, rays = [], 0
+    for _ in range(runs):
+        _, counters, secs = sc.render(desc, threads=threads, tile_mask=mask(desc, BASELINE_TILES))
+        rays = int(counters.sum())
+        rates.append(rays / secs / 1e6)
+        log(f"cpu baseline run: {rays} rays in {secs:.2f} s on {threads} threads")
+    d1 = make_desc(mrt, abi, cam, spp=2)
+    _, c1, s1 = sc.render(d1, threads=1, tile_mask=mask(d1, (ONE_THREAD_TILE,)))
+    one = int(c1.sum()) / s1 / 1e6
+    name = "RegularGrid" if accel == "grid" else "BVH"
+    return {"value": float(np.median(rates)), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle render ({name}), 16 tiles (a 4x4 lattice of the 16x16 grid) x samples 0-1 of the "
+                      f"1920x1080 {SPP}-spp frame ({rays} rays), median of {runs} runs",
+            "runs_mrays_per_s": [round(r, 4) for r in rates], "cpu_model": cpu_model(),
+            "one_thread": {"value": round(one, 4), "unit": "Mrays/s",
+                           "sample": f"tile {ONE_THREAD_TILE} x 2 samples, {int(c1.sum())} rays in {s1:.2f} s"},
+            "build": "g++ -O3 -march=x86-64-v3 (built off the GPU box)"}
+
+
+def ordered_traffic(data, cam, abi, mrt) -> dict:
+    """Algorithmic bytes per ray of the benchmark frame, independent of this
+    implementation: the oracle's ordered (near-first, culled) traversal of
+    the same reference BVH over the rays of every 256th pixel (all 16
+    samples): 32 bytes of ray in, 16 of hit out (4 for a shadow ray), 64 per
+    interior visit (the node's child pair) and 36 per triangle test (A, AB,
+    AC)."""
+    import oracle
+
+    sc = oracle.OracleScene(data.desc())
+    p = abi.AccelParams()
+    p.leaf_size = 2
+    sc.build(abi.BVH, p)
+    t0 = time.time()
+    r = sc.ordered_traffic(make_desc(mrt, abi, cam, spp=SPP), 256, host_threads())
+    rc, rs = r["rays_closest"], r["rays_shadow"]
+    pairs = r["pairs_closest"] + r["pairs_shadow"] + r["pairs_lights"]
+    tests = r["tests_closest"] + r["tests_shadow"] + r["tests_lights"]
+    per_ray = (48 * rc + 36 * rs + 64 * pairs + 36 * tests) / (rc + rs)
+    log(f"ordered traversal over 1/256 of the pixels ({time.time() - t0:.1f} s): {rc:.0f} closest-hit rays "
+        f"({r['pairs_closest'] / rc:.1f} pairs, {r['tests_closest'] / rc:.1f} tests; lights "
+        f"{r['pairs_lights'] / rc:.1f} pairs, {r['tests_lights'] / rc:.1f} tests), {rs:.0f} shadow rays "
+        f"({r['pairs_shadow'] / rs:.1f} pairs, {r['tests_shadow'] / rs:.1f} tests): {per_ray:.1f} bytes per ray")
+    return {"bytes_per_ray": round(per_ray, 1), "rays": int(rc + rs),
+            "pairs_per_ray": round(pairs / (rc + rs), 2), "tests_per_ray": round(tests / (rc + rs), 2),
+            "sample": "rays of every 256th pixel, all 16 samples"}
 
 
 def main() -> int:
@@ -136,6 +211,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-bytes", action="store_true",
+                    help="roofline bytes from the GPU's own visit counts instead of the oracle's ordered traversal")
     ap.add_argument("--kernel", choices=sorted(KERNELS), default="auto")
     ap.add_argument("--sort", choices=["default", "on", "off"], default="default",
                     help="wavefront: reorder extension rays by origin cell (default: off)")
@@ -221,10 +298,12 @@ def main() -> int:
     accum = torch.zeros((H, W, 3), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
 
-    # Instrumented frame (1 spp): node visits and primitive tests per ray.
+    # Instrumented frame (the timed frame's configuration with the traversal
+    # counters on, outside the timed region): node visits and primitive tests
+    # per ray, lane occupancy of the persistent trace.
     sort = {"default": 0, "on": 1, "off": 2}[args.sort]
     pers = 2 if args.no_persistent else 0
-    _, st = scene.render(make_desc(mrt, abi, cam, spp=1, instrument=True, kernel=args.kernel, ray_sort=sort, persistent=pers),
+    _, st = scene.render(make_desc(mrt, abi, cam, instrument=True, kernel=args.kernel, ray_sort=sort, persistent=pers),
                          bitmap=bitmap,
                          accum=accum,
                          mem=abi.MEM_DEVICE)
@@ -240,9 +319,11 @@ def main() -> int:
         f"{(st.prim_tests - st.light_prim_tests - st.shadow_prim_tests) / closest1:.1f} tests, lights "
         f"{st.light_node_visits / closest1:.1f} pairs {st.light_prim_tests / closest1:.1f} tests; per shadow ray: "
         f"{st.shadow_node_visits / shadow1:.1f} pairs {st.shadow_prim_tests / shadow1:.1f} tests")
+    occupancy = None
     if st.trace_lane_slots:
+        occupancy = st.trace_lane_iters / st.trace_lane_slots
         log(f"persistent trace: {st.trace_lane_iters / rays1:.1f} lane iterations per ray, lanes holding a ray "
-            f"{100.0 * st.trace_lane_iters / st.trace_lane_slots:.1f}% of lane iterations, visits+tests per lane "
+            f"{100.0 * occupancy:.1f}% of lane iterations, visits+tests per lane "
             f"iteration {(st.node_visits + st.prim_tests) / max(1, st.trace_lane_iters):.2f}")
 
     desc = make_desc(mrt, abi, cam, kernel=args.kernel, ray_sort=sort, persistent=pers)
@@ -274,10 +355,13 @@ def main() -> int:
     mrays = rays / elapsed / 1e6
 
     # Roofline of the dominant kernel.  Wavefront: the trace kernel, timed
-    # with HIP events around each of its launches on the render stream;
-    # algorithmic bytes per ray = node bytes per visit (128 per wide node of
-    # the persistent walk, 64 per node pair otherwise) + 48 per triangle test
-    # (three float4 of tri_geo) + 48 of ray record in and result out.
+    # with HIP events around each of its launches on the render stream.
+    # Algorithmic bytes per ray (BVH): the oracle's ordered traversal of the
+    # same reference tree over a 1/256 pixel subsample of the frame
+    # (ordered_traffic), independent of this implementation; beside it the
+    # GPU's own visits: 128 bytes per wide node of the persistent walk (64 per
+    # node pair otherwise) + 48 per triangle test (three float4 of tri_geo) +
+    # 48 of ray record in and result out.
     # Megakernel: the render-pass kernel, one launch per sample pass, plus 256
     # bytes of shading per closest-hit ray.  Slot groups run their trace
     # launches concurrently on separate streams, so a launch's duration
@@ -292,6 +376,11 @@ def main() -> int:
     persistent = wavefront and not args.no_persistent and args.accel == "bvh"
     node_bytes = 128 if persistent else 64
     bytes_per_ray = 48 + node_bytes * pairs_per_ray + 48 * prims_per_ray
+    gpu_bytes_per_ray = bytes_per_ray
+    ordered = None
+    if args.accel == "bvh" and not args.no_oracle_bytes:
+        ordered = ordered_traffic(data, cam, abi, mrt)
+        bytes_per_ray = ordered["bytes_per_ray"]
     if args.accel == "grid":
         light_pairs = st_instr.light_node_visits / rays1
         light_tests = st_instr.light_prim_tests / rays1
@@ -335,7 +424,10 @@ def main() -> int:
                      "concurrent_groups": groups, "device_achieved": round(device_achieved, 1),
                      "device_frac": round(device_achieved / HBM_PEAK_GBS, 4),
                      "bytes_per_launch": round(bytes_per_launch), "bytes_per_ray": round(bytes_per_ray, 1),
-                     "node_visits_per_ray": round(pairs_per_ray, 2), "tests_per_ray": round(prims_per_ray, 2)},
+                     "bytes_per_ray_source": "oracle ordered traversal" if ordered else "GPU visit counts",
+                     "oracle_ordered": ordered, "gpu_bytes_per_ray": round(gpu_bytes_per_ray, 1),
+                     "node_visits_per_ray": round(pairs_per_ray, 2), "tests_per_ray": round(prims_per_ray, 2),
+                     "lane_occupancy": round(occupancy, 4) if occupancy is not None else None},
     }
     if not args.no_cpu_baseline:
         t0 = time.time()

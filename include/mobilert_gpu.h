@@ -268,6 +268,23 @@ MRT_API int mrt_ctx_destroy(mrt_ctx* ctx);
  * admission of app/System_dependent/Android_JNI/DrawView.cpp:373-416. */
 MRT_API int mrt_ctx_set_budget(mrt_ctx* ctx, uint64_t bytes);
 MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx);
+/* Launch-shape knobs of the wavefront renderer and the render servers a
+ * context creates afterwards (no counterpart in the reference; they change
+ * speed only, never a pixel).  Zero fields keep the defaults. */
+typedef struct mrt_tuning {
+    uint32_t refill_lanes;     /* persistent trace: a wave takes new rays once this
+                                  many of its lanes are idle (1-64; 32)        */
+    uint32_t fetch_chunk;      /* queue entries a wave reserves per atomic
+                                  (64-4096; 128)                               */
+    uint32_t launch_share_pct; /* workgroups of one slot group's trace launch, %
+                                  of those the chip holds at once (1-100; 100
+                                  for one group, 180 / groups for more)        */
+    uint32_t max_slot_lanes;   /* path slots per pixel at most (1, 2, 4, 8, 16; 8) */
+    uint32_t max_groups;       /* slot groups of a frame at most (1-3; 3)      */
+    uint32_t reserved[3];
+} mrt_tuning;
+MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* tuning);
+MRT_API int mrt_ctx_get_tuning(const mrt_ctx* ctx, mrt_tuning* tuning);
 
 /* ---- scene -------------------------------------------------------------- */
 /* Copies the scene to the device (Shader(Scene&&, ...), Shader.cpp:18-25). */

@@ -171,6 +171,19 @@ class Context:
     def set_budget(self, nbytes: int) -> None:
         check(lib().mrt_ctx_set_budget(self.h, nbytes), "mrt_ctx_set_budget")
 
+    def set_tuning(self, **knobs) -> None:
+        """Launch-shape knobs (mrt_tuning): refill_lanes, fetch_chunk,
+        launch_share_pct, max_slot_lanes, max_groups; 0 keeps a default."""
+        t = abi.Tuning()
+        for k, v in knobs.items():
+            setattr(t, k, int(v))
+        check(lib().mrt_ctx_set_tuning(self.h, C.byref(t)), "mrt_ctx_set_tuning")
+
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        check(lib().mrt_ctx_get_tuning(self.h, C.byref(t)), "mrt_ctx_get_tuning")
+        return {k: getattr(t, k) for k, _ in abi.Tuning._fields_ if k != "reserved"}
+
     def camera_rays(self, cam: abi.Camera, uvdd: np.ndarray) -> np.ndarray:
         uvdd = np.ascontiguousarray(uvdd, np.float32)
         out = np.zeros(len(uvdd), abi.RAY_DTYPE)

@@ -117,6 +117,11 @@ class RenderStats(C.Structure):
                 ("slot_lanes", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("refill_lanes", C.c_uint32), ("fetch_chunk", C.c_uint32), ("launch_share_pct", C.c_uint32),
+                ("max_slot_lanes", C.c_uint32), ("max_groups", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("max_slots", C.c_uint32), ("max_jobs", C.c_uint32), ("max_scenes", C.c_uint32),
                 ("pipelines", C.c_uint32), ("job_memory", C.c_uint64), ("scene_memory", C.c_uint64),
@@ -216,6 +221,8 @@ def _proto(lib):
         "mrt_triangle_make": (C.c_int, [P, P, P, i32, P, P, P]),
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
         "mrt_camera_aabb": (C.c_int, [C.POINTER(Camera), P]),
+        "mrt_ctx_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "mrt_ctx_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "mrt_tile_values": (C.c_int, [u32, P]),
         "mrt_triangle_intersect_box": (C.c_int, [P, P, P]),
         "mrt_primitive_intersect_box": (C.c_int, [C.c_int, P, P, P]),

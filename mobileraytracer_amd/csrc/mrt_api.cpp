@@ -927,6 +927,26 @@ MRT_API int mrt_ctx_set_budget(mrt_ctx* ctx, uint64_t bytes) {
     return MRT_OK;
 }
 
+MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* t) {
+    if (!ctx || !t) return fail(MRT_E_INVALID, "null argument");
+    if (t->refill_lanes > 64u) return fail(MRT_E_INVALID, "refill_lanes must be 0-64");
+    if (t->fetch_chunk && (t->fetch_chunk < 64u || t->fetch_chunk > 4096u))
+        return fail(MRT_E_INVALID, "fetch_chunk must be 0 or 64-4096");
+    if (t->launch_share_pct > 100u) return fail(MRT_E_INVALID, "launch_share_pct must be 0-100");
+    if (t->max_slot_lanes && (t->max_slot_lanes > 16u || (t->max_slot_lanes & (t->max_slot_lanes - 1u))))
+        return fail(MRT_E_INVALID, "max_slot_lanes must be 0 or a power of two up to 16");
+    if (t->max_groups > kWfMaxGroups) return fail(MRT_E_INVALID, "max_groups must be 0-3");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuning = *t;
+    return MRT_OK;
+}
+
+MRT_API int mrt_ctx_get_tuning(const mrt_ctx* ctx, mrt_tuning* t) {
+    if (!ctx || !t) return fail(MRT_E_INVALID, "null argument");
+    *t = ctx->tuning;
+    return MRT_OK;
+}
+
 MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx) {
     if (!ctx) return fail(MRT_E_INVALID, "null context");
     MRT_HIP(hipSetDevice(ctx->device));
@@ -1506,11 +1526,18 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     // Wavefront slot groups run concurrently, each with its own spill region;
     // sample-parallel frames give every pixel `lanes` slots.
     const uint32_t pixelSlots = useWavefront ? wavefront_slots(p) : 0u;
-    const uint32_t lanes = useWavefront ? wavefront_lanes(pixelSlots, d->samples_pixel - d->first_sample) : 1u;
+    const mrt_tuning& tune = s->ctx->tuning;
+    const uint32_t lanes =
+        useWavefront ? wavefront_lanes(pixelSlots, d->samples_pixel - d->first_sample,
+                                       tune.max_slot_lanes ? tune.max_slot_lanes : kDefaultSlotLanes)
+                     : 1u;
     const uint32_t slots = pixelSlots * lanes;
-    const uint32_t groups = useWavefront ? wavefront_groups(slots) : 1u;
+    const uint32_t groups = useWavefront ? wavefront_groups(slots, tune.max_groups ? tune.max_groups : kWfMaxGroups)
+                                         : 1u;
     const uint32_t persistBlocks =
-        persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks, groups) : 0u;
+        persistent ? wavefront_persistent_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u, traceBlocks, groups,
+                                                 tune.launch_share_pct)
+                   : 0u;
     StackCfg sc;
     if ((rc = ensure_spill(s, blocks * 256u * groups, &sc)) != MRT_OK) return rc;
     std::vector<StackCfg> scs(groups, sc);
@@ -1578,6 +1605,8 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             wb.scan_temp = reinterpret_cast<void*>(t);
             wb.scan_bytes = scanBytes;
             wb.sort = d->ray_sort == 1u ? 1u : 0u;
+            wb.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultRefillLanes;
+            wb.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultFetchChunk;
             wb.rad = rad;
             wb.aov_s = aovS;
             wb.lanes = lanes;

@@ -102,6 +102,7 @@ struct mrt_ctx {
     mrt::DevBuf grid_prims;
     // This GPU's copy of the allocation-free entry point's scratch array.
     uint32_t* scratch = nullptr;
+    mrt_tuning tuning{};  // zero fields: defaults
 };
 
 struct mrt_scene {

@@ -737,7 +737,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     float4* state = s->state.as<float4>();
     const uint32_t ncu = ctx->num_cu ? ctx->num_cu : 256u;
     s->trace_blocks = ncu * 16u / 2u;
-    s->persist_blocks = wavefront_persistent_blocks(ncu, s->trace_blocks, G);
+    s->persist_blocks = wavefront_persistent_blocks(ncu, s->trace_blocks, G, ctx->tuning.launch_share_pct);
     for (uint32_t g = 0; g < G; ++g) {
         const uint32_t w0 = s->pool_waves * g / G, w1 = s->pool_waves * (g + 1) / G;
         WfBuffers& wb = s->wb[g];
@@ -763,6 +763,8 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
         wb.scan_temp = nullptr;
         wb.scan_bytes = 0;
         wb.sort = 0u;
+        wb.refill_lanes = ctx->tuning.refill_lanes ? ctx->tuning.refill_lanes : kDefaultRefillLanes;
+        wb.fetch_chunk = ctx->tuning.fetch_chunk ? ctx->tuning.fetch_chunk : kDefaultFetchChunk;
         MRT_HIP(hipMemsetAsync(wb.counts, 0, kCntWords * sizeof(uint32_t), s->pipe[g]));
         s->L[g].qin = wb.queue_a;
         s->L[g].qout = wb.queue_b;

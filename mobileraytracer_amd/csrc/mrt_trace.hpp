@@ -649,12 +649,21 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& ra, float& kb, uint32
 // reached leaves go to l0..l3, reached interior slots are ordered by entry
 // distance; the nearest is visited next and the others are pushed, farthest
 // first.
+struct WideNode {
+    float4 mnx, mny, mnz, mxx, mxy, mxz, ra, rb;
+};
+__device__ __forceinline__ WideNode load_wide(const SceneDev& s, uint32_t node) {
+    const float4* nd = s.wide_nodes + 8u * node;
+    return WideNode{nd[0], nd[1], nd[2], nd[3], nd[4], nd[5], nd[6], nd[7]};
+}
+
+// The visit of node w.node with its record already loaded (so that callers
+// can issue the load before other work).
 template <bool COUNT>
-__device__ __forceinline__ void walk_visit_wide(const SceneDev& s, Walk& w, float lim, Stack& st, TraceStats& ts,
-                                                int kind) {
-    const float4* nd = s.wide_nodes + 8u * w.node;
-    const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5];
-    const float4 ra = nd[6], rb = nd[7];
+__device__ __forceinline__ void walk_visit_wide(const SceneDev& s, Walk& w, const WideNode& wn, float lim, Stack& st,
+                                                TraceStats& ts, int kind) {
+    const float4 mnx = wn.mnx, mny = wn.mny, mnz = wn.mnz, mxx = wn.mxx, mxy = wn.mxy, mxz = wn.mxz;
+    const float4 ra = wn.ra, rb = wn.rb;
     if (COUNT) ++ts.pairs[kind == 2 ? 2 : (w.node >= s.wide_light_node ? 1 : 0)];
     float e0, e1, e2, e3;
     const bool h0 = slab_finite(f3{mnx.x, mny.x, mnz.x}, f3{mxx.x, mxy.x, mxz.x}, w.r.o, w.r.inv, &e0) && e0 <= lim;

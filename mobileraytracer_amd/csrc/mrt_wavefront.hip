@@ -272,9 +272,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow(SceneDev s, const uint32_t
 // 783 / 800 / 811 / 815 / 791 Mrays/s (a refill is a ballot, a fetch and a
 // ray setup for the whole wave, so waiting for more idle lanes amortises it
 // until the idle lanes cost more than it).
-__device__ __forceinline__ bool wave_wants_refill(bool need, bool has) {
+__device__ __forceinline__ bool wave_wants_refill(bool need, bool has, uint32_t refill_lanes) {
     const unsigned long long needMask = __ballot(need);
-    return needMask != 0ull && (__popcll(needMask) >= 32 || __ballot(has) == 0ull);
+    return needMask != 0ull && (static_cast<uint32_t>(__popcll(needMask)) >= refill_lanes || __ballot(has) == 0ull);
 }
 
 // Test of the first queued triangle for a closest-hit lane (the (t, id) key)
@@ -282,11 +282,10 @@ __device__ __forceinline__ bool wave_wants_refill(bool need, bool has) {
 // dist held in h.t); a blocker ends the shadow lane's walk.  Returns true when
 // a shadow lane is occluded.
 template <bool COUNT>
-__device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, bool shadow, TraceStats& ts) {
+__device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, const TriG& g, uint32_t id, bool shadow,
+                                          TraceStats& ts) {
     const uint32_t slot = w.cur & kSlotMask;
     if (COUNT) ++ts.prims[shadow ? 2 : (slot >= s.walk_light_slot ? 1 : 0)];
-    uint32_t id;
-    const TriG g = walk_tri(s, slot, &id);
     float t, u, v;
     const bool hit = tri_test_flat(g, w.r, &t, &u, &v) && id != w.r.excl;
     const bool better = !shadow && hit && key_better(t, id, w.h);
@@ -310,22 +309,31 @@ __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, 
                                           TraceStats& ts) {
     const bool canVisit = has && walk_can_visit(w);
     const bool canTest = has && w.cur != 0u;
-    const bool occluded = canTest && walk_test<COUNT>(s, w, shadow, ts);
+    // The test's triangle and the visit's node are loaded where they are
+    // used: loading both up front overlaps their latencies but holds the
+    // node's 32 registers across the test (86 instead of 68 registers, 5
+    // instead of 7 waves per SIMD), which measured slower (818 against 844
+    // Mrays/s on the 1M-triangle 1080p frame).
+    bool occluded = false;
+    if (canTest) {
+        uint32_t id;
+        const TriG g = walk_tri(s, w.cur & kSlotMask, &id);
+        occluded = walk_test<COUNT>(s, w, g, id, shadow, ts);
+    }
     if (canVisit && !occluded) {
         const float lim = cull_limit(w.h.t);
         const int kind = shadow ? 2 : 0;
-        if (finite) walk_visit_wide<COUNT>(s, w, lim, st, ts, kind);
+        if (finite) walk_visit_wide<COUNT>(s, w, load_wide(s, w.node), lim, st, ts, kind);
         else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
     }
     return occluded;
 }
 
 // Wave-private run of trace-queue entries [next, end): a wave reserves
-// kFetchChunk entries with one atomic and hands them to its lanes as they go
-// idle.  Same-address atomics serialise device-wide (about 10 ns each), so one
-// atomic per refill of a few lanes would cap the whole trace at ~100 M
+// `chunk` (>= 64) entries with one atomic and hands them to its lanes as they
+// go idle.  Same-address atomics serialise device-wide (about 10 ns each), so
+// one atomic per refill of a few lanes would cap the whole trace at ~100 M
 // refills per second.
-constexpr uint32_t kFetchChunk = 128;
 struct FetchRun {
     uint32_t next, end;  // equal in every lane of the wave
     bool drained;        // a run reached the queue's end: reserve no more
@@ -333,7 +341,8 @@ struct FetchRun {
 
 // Next queue entry for each lane that wants one (n entries in all);
 // kDeadEntry once the queue is used up.
-__device__ __forceinline__ uint32_t fetch_entry(uint32_t* cursor, uint32_t n, bool want, FetchRun& run) {
+__device__ __forceinline__ uint32_t fetch_entry(uint32_t* cursor, uint32_t n, bool want, FetchRun& run,
+                                                uint32_t chunk) {
     const unsigned long long mask = __ballot(want);
     const uint32_t total = static_cast<uint32_t>(__popcll(mask));
     const uint32_t lane = threadIdx.x & 63u;
@@ -344,15 +353,15 @@ __device__ __forceinline__ uint32_t fetch_entry(uint32_t* cursor, uint32_t n, bo
         run.next += total;
     } else if (!run.drained) {
         // The run's rest goes to the first `avail` lanes, a new run to the
-        // others (total <= 64 <= kFetchChunk).
+        // others (total <= 64 <= chunk).
         const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask))) - 1u;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(cursor, kFetchChunk);
+        if (lane == leader) base = atomicAdd(cursor, chunk);
         base = __shfl(base, static_cast<int>(leader), 64);
         if (rank >= avail) e = base + (rank - avail);
         run.next = base + (total - avail);
-        run.end = base + kFetchChunk;
-        run.drained = base + kFetchChunk >= n;
+        run.end = base + chunk;
+        run.drained = base + chunk >= n;
     } else {
         if (rank >= avail) e = kDeadEntry;
         run.next = run.end;
@@ -391,8 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
     FetchRun run{0u, 0u, false};
     while (true) {
         const bool need = !has && !exhausted;
-        if (wave_wants_refill(need, has)) {
-            const uint32_t e = fetch_entry(fetch, n, need, run);
+        if (wave_wants_refill(need, has, b.refill_lanes)) {
+            const uint32_t e = fetch_entry(fetch, n, need, run, b.fetch_chunk);
             if (need) {
                 if (e < nq) {
                     const float4 a = queue[2u * e], c = queue[2u * e + 1u];
@@ -774,7 +783,7 @@ uint32_t wavefront_slots(const RenderParams& p) {
     return ((covW + 7u) / 8u) * ((covH + 7u) / 8u) * 64u;
 }
 
-uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups) {
+uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups, uint32_t share_pct) {
     static int perCu = 0;  // same for every device of this (single-arch) build
     if (perCu == 0) {
         int a = 0;
@@ -785,7 +794,8 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t gro
         perCu = std::max(1, a);
     }
     const uint32_t resident = static_cast<uint32_t>(perCu) * num_cu;
-    const uint32_t share = groups > 1u ? std::max(1u, resident * 9u / (5u * groups)) : resident;
+    const uint32_t share = share_pct ? std::max(1u, resident * std::min(share_pct, 100u) / 100u)
+                           : groups > 1u ? std::max(1u, resident * 9u / (5u * groups)) : resident;
     return std::min({cap, resident, share});
 }
 

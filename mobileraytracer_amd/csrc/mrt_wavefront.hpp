@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mrt_kernels.hpp"
 #include "mrt_path.hpp"
 
@@ -60,7 +62,15 @@ struct WfBuffers {
     // launch_wavefront_aov_resolve; entries not written keep kRadPending in
     // the flag.  nullptr: none.
     float4* aov_s;
+    // Persistent trace: a wave refills once refill_lanes lanes are idle and
+    // reserves fetch_chunk (>= 64) queue entries per atomic.
+    uint32_t refill_lanes, fetch_chunk;
 };
+
+// Defaults of the launch-shape knobs (mrt_tuning).
+constexpr uint32_t kDefaultRefillLanes = 32;
+constexpr uint32_t kDefaultFetchChunk = 128;
+constexpr uint32_t kDefaultSlotLanes = 8;
 
 // Radiance-buffer entries of samples not finished (x component bits; a NaN
 // payload that arithmetic never produces).
@@ -128,7 +138,9 @@ inline size_t wavefront_group_bytes(uint32_t slots) {
 // 208 ms, 4 groups 299 ms (more streams than the process's 4 hardware queues
 // serialise).
 constexpr uint32_t kWfMaxGroups = 3;
-inline uint32_t wavefront_groups(uint32_t slots) { return slots >= (1u << 16) ? kWfMaxGroups : 1u; }
+inline uint32_t wavefront_groups(uint32_t slots, uint32_t max_groups = kWfMaxGroups) {
+    return slots >= (1u << 16) ? std::max(1u, std::min(max_groups, kWfMaxGroups)) : 1u;
+}
 
 // Slots per pixel of a frame of `samples` samples over `pixel_slots` pixel
 // slots: up to 8, while the frame stays within 2^24 slots (about 3.4 GB of
@@ -137,8 +149,8 @@ inline uint32_t wavefront_groups(uint32_t slots) { return slots >= (1u << 16) ? 
 // launch and a frame needs fewer iterations.  1M-triangle 1080p frame at 16
 // spp: 1 / 2 / 4 / 8 / 16 slots per pixel took 187 / 169 / 152 / 140 / 137 ms
 // (best launch share each).
-inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples) {
-    uint32_t k = 8u;
+inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples, uint32_t max_lanes = kDefaultSlotLanes) {
+    uint32_t k = max_lanes;
     while (k > 1u && (k > samples || static_cast<uint64_t>(pixel_slots) * k > (1ull << 24) ||
                       12ull * samples * pixel_slots > (2ull << 30)))
         k /= 2u;
@@ -155,7 +167,7 @@ inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples) {
 // % took 148 / 140 / 140 / 153 ms when waves refilled at 8 idle lanes, and 50
 // / 60 / 65 / 70 % took 138.7 / 133.9 / 140.1 / 147.0 ms with the refill at 32
 // idle lanes.
-uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups);
+uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups, uint32_t share_pct = 0);
 
 // Queues the camera rays of sample `first_sample` of every slot.
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,

@@ -57,6 +57,7 @@ def lib():
             "oracle_plane_intersect_box": (C.c_int, [P, P, P]),
             "oracle_sphere_intersect_box": (C.c_int, [P, P, P]),
             "oracle_grid_walk": (C.c_int, [P, P, P, u32, P, P]),
+            "oracle_ordered_traffic": (C.c_int, [P, P, u32, u32, P]),
             "oracle_plane_distance": (f32, [P, P]),
             "oracle_plane_make": (C.c_int, [P, P, i32, P]),
             "oracle_slab": (C.c_int, [P, P, P]),
@@ -131,6 +132,18 @@ class OracleScene:
         if n.value > cap:
             raise RuntimeError(f"grid walk visited {n.value} cells, more than {cap}")
         return cells[: n.value].copy(), hit[0]
+
+    def ordered_traffic(self, desc, pixel_mod: int = 256, threads: int = 8) -> dict:
+        """Node-pair visits and triangle tests per ray of an ordered
+        (near-first, culled) traversal of the same trees, over the rays of
+        the pixels whose id is a multiple of pixel_mod (BVH scenes)."""
+        out = np.zeros(8, np.float64)
+        rc = lib().oracle_ordered_traffic(self.h, C.addressof(desc), pixel_mod, threads, out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"oracle_ordered_traffic failed: {rc}")
+        keys = ["rays_closest", "pairs_closest", "tests_closest", "rays_shadow", "pairs_shadow", "tests_shadow",
+                "pairs_lights", "tests_lights"]
+        return dict(zip(keys, out.tolist()))
 
     def trace_closest(self, rays: np.ndarray, threads: int = 8, hit_dtype=None):
         out = np.zeros(len(rays), hit_dtype)

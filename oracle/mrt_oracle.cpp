@@ -62,6 +62,7 @@ struct Scene {
     RegularGrid<R, Plane<R>> gridP;
     RegularGrid<R, Sphere<R>> gridS;
     RegularGrid<R, Triangle<R>> gridT;
+    BVH<R, Triangle<R>> lightBvh;  // reference-built tree over the area lights (ordered accounting only)
     Naive<R, Plane<R>> naiveP;
     Naive<R, Sphere<R>> naiveS;
     Naive<R, Triangle<R>> naiveT;
@@ -161,6 +162,29 @@ struct Shader {
     const Scene<R>* scene = nullptr;
     const mrt_render_desc* desc = nullptr;
     Counters<R>* counters = nullptr;
+    OrderedCounts* ordered = nullptr;  // optional: ordered-traversal accounting of every counted ray
+
+    void account_closest(const Ray<R>& ray) const {
+        if (!ordered || scene->accel != MRT_BVH) return;
+        uint64_t p = 0, t = 0, lp = 0, lt = 0;
+        Isect<R> is;
+        is.length = static_cast<R>(kRayLengthMax);
+        ordered_walk(scene->bvhT, ray, &is, false, static_cast<R>(0), &p, &t);
+        ordered_walk(scene->lightBvh, ray, &is, false, static_cast<R>(0), &lp, &lt);
+        ordered->rays_closest.fetch_add(1, std::memory_order_relaxed);
+        ordered->pairs_closest.fetch_add(p, std::memory_order_relaxed);
+        ordered->tests_closest.fetch_add(t, std::memory_order_relaxed);
+        ordered->pairs_lights.fetch_add(lp, std::memory_order_relaxed);
+        ordered->tests_lights.fetch_add(lt, std::memory_order_relaxed);
+    }
+    void account_shadow(const Ray<R>& ray, R dist) const {
+        if (!ordered || scene->accel != MRT_BVH) return;
+        uint64_t p = 0, t = 0;
+        ordered_walk(scene->bvhT, ray, static_cast<Isect<R>*>(nullptr), true, dist, &p, &t);
+        ordered->rays_shadow.fetch_add(1, std::memory_order_relaxed);
+        ordered->pairs_shadow.fetch_add(p, std::memory_order_relaxed);
+        ordered->tests_shadow.fetch_add(t, std::memory_order_relaxed);
+    }
 
     float draw(const mrt_sampler& s, const DrawCtx& c, uint32_t slot, uint32_t ls, int32_t depth) const {
         return orc::draw(s, c.pixel, c.sample, make_dim(slot, ls, static_cast<uint32_t>(depth), c.branch),
@@ -220,6 +244,7 @@ struct Shader {
                 sr.depth = depth + 1;
                 sr.exclude = is.prim;
                 if (counters) counters->shadow.fetch_add(1, std::memory_order_relaxed);
+                account_shadow(sr, dist);
                 if (!scene->shadow(sr, dist, nullptr)) L = L + light.radiance.Le * cosNL;
             }
         }
@@ -230,6 +255,7 @@ struct Shader {
     bool ray_trace(V3<R>* rgb, const Ray<R>& ray, const DrawCtx& c, int kind) const {
         Isect<R> is;
         const bool hit = scene->closest(&is, ray, nullptr);
+        if (ray.depth <= desc->ray_depth_max) account_closest(ray);
         if (counters && ray.depth <= desc->ray_depth_max) {
             if (kind == 0) counters->camera.fetch_add(1, std::memory_order_relaxed);
             else if (kind == 1) counters->extension.fetch_add(1, std::memory_order_relaxed);
@@ -487,6 +513,8 @@ struct RenderArgs {
     double* accum;            // optional, 3 per pixel (width stride)
     uint32_t nthreads;
     const uint8_t* tile_mask; // optional: only tiles (claim rank) with mask != 0
+    uint32_t pixel_mod = 0;   // optional: only pixels whose id is a multiple of it
+    OrderedCounts* ordered = nullptr;
 };
 
 // Renderer::renderFrame + renderScene, app/MobileRT/Renderer.cpp:31-116.
@@ -510,6 +538,7 @@ int render(const Scene<R>& scene, const mrt_render_desc& d, const RenderArgs& a,
     sh.scene = &scene;
     sh.desc = &d;
     sh.counters = &counters;
+    sh.ordered = a.ordered;
     std::atomic<uint32_t> compat{0};  // Sampler::sample_, reset per frame
     const bool isCompat = d.sampler_pixel.kind == MRT_SAMPLER_HALTON_COMPAT ||
                           d.sampler_pixel.kind == MRT_SAMPLER_STRATIFIED_COMPAT;
@@ -557,6 +586,7 @@ int render(const Scene<R>& scene, const mrt_render_desc& d, const RenderArgs& a,
                     for (uint32_t x = startX; x < endX; ++x) {
                         const float u = static_cast<float>(x) * invW;
                         const uint32_t pid = y * w + x;
+                        if (a.pixel_mod && pid % a.pixel_mod != 0u) continue;
                         const float r1 = pixelDraw(pid, sample, SLOT_JITTER_U);
                         const float r2 = pixelDraw(pid, sample, SLOT_JITTER_V);
                         const float du = ((r1 - 0.5f) * 2.0f) * pixW;
@@ -826,6 +856,38 @@ int oracle_render(void* hv, const mrt_render_desc* d, uint32_t* bitmap, uint32_t
     else rc = render<float>(*h->f, *d, a, counters);
     const auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return rc;
+}
+
+// Traffic of an ordered traversal of the scene's trees (measurement): renders
+// the pixels whose id is a multiple of pixel_mod and, for every counted ray,
+// walks the same reference BVH near-first with culling (closest hit: the
+// triangle tree, then a reference-built tree over the area lights; any hit:
+// the triangle tree).  out: closest-hit rays, their pair visits, triangle
+// tests; shadow rays, pair visits, tests; light-tree pair visits, tests.
+int oracle_ordered_traffic(void* hv, const mrt_render_desc* d, uint32_t pixel_mod, uint32_t nthreads, double out[8]) {
+    auto* h = static_cast<Handle*>(hv);
+    Scene<float>& sc = *h->f;
+    if (sc.accel != MRT_BVH) return MRT_E_INVALID;
+    if (sc.lightBvh.boxes.empty()) {
+        std::vector<Triangle<float>> lt;
+        for (const auto& l : sc.lights)
+            if (l.kind == MRT_LIGHT_AREA) lt.push_back(l.tri);
+        sc.lightBvh.build(std::move(lt), 2);
+    }
+    std::vector<uint32_t> bitmap(static_cast<size_t>(d->width) * d->height, 0u);
+    OrderedCounts oc;
+    RenderArgs a{bitmap.data(), d->width * 4u, nullptr, nthreads, nullptr, pixel_mod, &oc};
+    uint64_t cnt[4] = {};
+    const int rc = render<float>(sc, *d, a, cnt);
+    out[0] = static_cast<double>(oc.rays_closest.load());
+    out[1] = static_cast<double>(oc.pairs_closest.load());
+    out[2] = static_cast<double>(oc.tests_closest.load());
+    out[3] = static_cast<double>(oc.rays_shadow.load());
+    out[4] = static_cast<double>(oc.pairs_shadow.load());
+    out[5] = static_cast<double>(oc.tests_shadow.load());
+    out[6] = static_cast<double>(oc.pairs_lights.load());
+    out[7] = static_cast<double>(oc.tests_lights.load());
     return rc;
 }
 

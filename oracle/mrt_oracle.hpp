@@ -585,6 +585,73 @@ struct BVH {
     }
 };
 
+// Near-first walk of the same tree with culling by the nearest hit so far
+// (closest hit) or an exit at the first blocker (any hit).  Measurement only:
+// it counts the interior visits (each fetches the node's pair of children,
+// two 32-byte BVHNodes) and primitive tests an ordered traversal of the
+// reference tree needs, the implementation-independent traffic of a ray; it
+// returns nothing the renderer uses.
+struct OrderedCounts {
+    std::atomic<uint64_t> rays_closest{0}, pairs_closest{0}, tests_closest{0};
+    std::atomic<uint64_t> rays_shadow{0}, pairs_shadow{0}, tests_shadow{0};
+    std::atomic<uint64_t> pairs_lights{0}, tests_lights{0};
+};
+
+template <class R, class T>
+bool ordered_walk(const BVH<R, T>& b, const Ray<R>& ray, Isect<R>* is, bool any, R dist, uint64_t* pairs,
+                  uint64_t* tests) {
+    if (b.prims.empty()) return false;
+    const V3<R> inv{static_cast<R>(1) / ray.d.x, static_cast<R>(1) / ray.d.y, static_cast<R>(1) / ray.d.z};
+    auto entry = [&](const AABB<R>& bx, R limit, R* e) {
+        R t0 = 0, t1 = limit;
+        for (int a = 0; a < 3; ++a) {
+            R n = (bx.mn[a] - ray.o[a]) * inv[a], f = (bx.mx[a] - ray.o[a]) * inv[a];
+            if (n > f) std::swap(n, f);
+            if (!(n <= t1) || !(f >= t0)) return false;  // NaN (0 * inf) keeps the box
+            t0 = std::max(t0, n);
+            t1 = std::min(t1, f);
+        }
+        *e = t0;
+        return t0 <= t1;
+    };
+    auto limit = [&]() { return any ? dist : is->length; };
+    R e0;
+    if (!entry(b.boxes[0].box, limit(), &e0)) return false;
+    std::vector<std::pair<uint32_t, R>> stack{{0u, e0}};
+    bool hit = false;
+    while (!stack.empty()) {
+        const auto [id, e] = stack.back();
+        stack.pop_back();
+        if (e > limit()) continue;
+        const auto& node = b.boxes[id];
+        if (node.numberPrimitives > 0) {
+            for (uint32_t i = 0; i < node.numberPrimitives; ++i) {
+                ++*tests;
+                if (any) {
+                    if (b.prims[node.indexOffset + i].intersect(ray, dist)) return true;
+                } else {
+                    hit |= b.prims[node.indexOffset + i].intersect(is, ray);
+                }
+            }
+            continue;
+        }
+        ++*pairs;
+        const uint32_t l = node.indexOffset, r = l + 1;
+        R el, er;
+        const bool hl = entry(b.boxes[l].box, limit(), &el), hr = entry(b.boxes[r].box, limit(), &er);
+        if (hl && hr) {
+            const bool lFirst = el <= er;
+            stack.emplace_back(lFirst ? r : l, lFirst ? er : el);
+            stack.emplace_back(lFirst ? l : r, lFirst ? el : er);
+        } else if (hl) {
+            stack.emplace_back(l, el);
+        } else if (hr) {
+            stack.emplace_back(r, er);
+        }
+    }
+    return hit;
+}
+
 // ------------------------------------------------------------------ Naive --
 // app/MobileRT/Accelerators/Naive.hpp:55-74.
 template <class R, class T>
