@@ -281,7 +281,18 @@ typedef struct mrt_tuning {
                                   for one group, 180 / groups for more)        */
     uint32_t max_slot_lanes;   /* path slots per pixel at most (1, 2, 4, 8, 16; 8) */
     uint32_t max_groups;       /* slot groups of a frame at most (1-3; 3)      */
-    uint32_t reserved[3];
+    uint32_t overlap_loads;    /* persistent trace issues each step's triangle and
+                                  node loads together: 0 when a launch cannot
+                                  give each lane more than one ray, 1 always,
+                                  2 never                                      */
+    uint32_t graph_loop;       /* 1: mrt_render's wavefront loop as a captured
+                                  HIP graph of 4 iterations of every slot
+                                  group, replayed until the frame drains; the
+                                  scene keeps it, and a frame with the same
+                                  launch arguments replays it without a new
+                                  capture (graph frames report no per-launch
+                                  trace times).  0, 2: launched iterations    */
+    uint32_t reserved[1];
 } mrt_tuning;
 MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* tuning);
 MRT_API int mrt_ctx_get_tuning(const mrt_ctx* ctx, mrt_tuning* tuning);

@@ -588,9 +588,112 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
     uint32_t checks = 0;
     bool stopReq = false;
     size_t ev = 0;
+    // Graph loop: one batch (kBatch iterations of every group, the groups
+    // forked from and joined back into the first stream) is captured once and
+    // replayed until the frame drains.  An even number of iterations returns
+    // every queue and list to its starting role, so each replay repeats the
+    // captured launches exactly; a stop request recaptures with the stop
+    // flag set.  The counter snapshots and check events stay outside.
+    // A later frame whose launch arguments are all the same (same scene,
+    // descriptor, buffers and launch sizes: a repeated or progressive view)
+    // replays the scene's kept graph without capturing it again.
+    const bool useGraph = s->ctx->tuning.graph_loop == 1u;
+    hipGraph_t& graph = s->wf_graph;
+    hipGraphExec_t& exec = s->wf_exec;
+    bool graphStop = false;
+    uint64_t graphIters = 0;
+    std::vector<hipEvent_t> gev(useGraph ? G + 1 : 0, nullptr);
+    struct EventGuard {
+        std::vector<hipEvent_t>& e;
+        ~EventGuard() {
+            for (hipEvent_t v : e)
+                if (v) (void)hipEventDestroy(v);
+        }
+    } graphGuard{gev};
+    for (hipEvent_t& e : gev) MRT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    auto graph_key = [&](bool stopFlag) {
+        std::vector<uint8_t> k;
+        auto put = [&](const void* v, size_t n) {
+            const uint8_t* b = static_cast<const uint8_t*>(v);
+            k.insert(k.end(), b, b + n);
+        };
+        const uint32_t flags[4] = {persistent ? 1u : 0u, d->instrument, d->samples_pixel, stopFlag ? 1u : 0u};
+        put(flags, sizeof(flags));
+        put(&s->dev, sizeof(s->dev));
+        put(&p, sizeof(p));
+        const void* ptrs[4] = {dbm, dacc, s->counters.p, s->err.p};
+        put(ptrs, sizeof(ptrs));
+        for (uint32_t g = 0; g < G; ++g) {
+            put(&wbs[g], sizeof(WfBuffers));
+            put(&scs[g], sizeof(StackCfg));
+            put(&st[g], sizeof(hipStream_t));
+            const uint32_t blocks[2] = {sized_blocks(trace_blocks, wbs[g].slot_count),
+                                        sized_trace_blocks(persist_blocks, wbs[g].slot_count)};
+            put(blocks, sizeof(blocks));
+        }
+        return k;
+    };
+    auto capture = [&]() -> int {
+        std::vector<uint8_t> key = graph_key(stopReq);
+        if (exec && key == s->wf_key) {
+            // Same arguments: replay the kept batch (the loop state advances
+            // as the captured launches would have advanced it).
+            for (uint32_t k = 0; k < kBatch; ++k)
+                for (uint32_t g = 0; g < G; ++g) wavefront_loop_advance(&L[g], wbs[g].sort != 0u);
+            graphStop = stopReq;
+            return MRT_OK;
+        }
+        s->wf_key.clear();
+        if (exec) MRT_HIP(hipGraphExecDestroy(exec));
+        exec = nullptr;
+        if (graph) MRT_HIP(hipGraphDestroy(graph));
+        graph = nullptr;
+        MRT_HIP(hipStreamBeginCapture(st[0], hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipEventRecord(gev[0], st[0]);
+        for (uint32_t g = 1; g < G && e == hipSuccess; ++g) e = hipStreamWaitEvent(st[g], gev[0], 0);
+        for (uint32_t k = 0; k < kBatch && e == hipSuccess; ++k)
+            for (uint32_t g = 0; g < G && e == hipSuccess; ++g)
+                e = launch_wavefront_iteration(s->dev, s->generic, persistent, d->instrument != 0u, p,
+                                               d->samples_pixel, stopReq, dbm, dacc, wbs[g], &L[g],
+                                               sized_blocks(trace_blocks, wbs[g].slot_count),
+                                               sized_trace_blocks(persist_blocks, wbs[g].slot_count), scs[g],
+                                               s->counters.as<unsigned long long>(), s->err.as<uint32_t>(),
+                                               nullptr, nullptr, st[g]);
+        for (uint32_t g = 1; g < G && e == hipSuccess; ++g) {
+            e = hipEventRecord(gev[g], st[g]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st[0], gev[g], 0);
+        }
+        const hipError_t ec = hipStreamEndCapture(st[0], &graph);
+        MRT_HIP(e);
+        MRT_HIP(ec);
+        MRT_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        s->wf_key = std::move(key);
+        graphStop = stopReq;
+        return MRT_OK;
+    };
+    bool graphReady = false;
     while (true) {
         if (!stopReq && s->stop.load()) stopReq = true;
-        for (uint32_t k = 0; k < kBatch; ++k) {
+        if (useGraph) {
+            if (!graphReady || graphStop != stopReq) {
+                const int rc = capture();
+                if (rc != MRT_OK) return rc;
+                graphReady = true;
+            } else {
+                for (uint32_t k = 0; k < kBatch; ++k)  // the replay repeats the captured batch
+                    for (uint32_t g = 0; g < G; ++g) wavefront_loop_advance(&L[g], wbs[g].sort != 0u);
+            }
+            MRT_HIP(hipGraphLaunch(exec, st[0]));
+            graphIters += kBatch * G;
+            const uint32_t slot = checks & 1u;
+            for (uint32_t g = 0; g < G; ++g) {
+                MRT_HIP(hipMemcpyAsync(s->counts_host + (2 * g + slot) * kCntWords, wbs[g].counts,
+                                       kCntWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st[0]));
+                MRT_HIP(hipEventRecord(chk(g, slot), st[0]));
+                std::copy(L[g].pending, L[g].pending + 3, pend[g][slot]);
+            }
+        }
+        for (uint32_t k = 0; k < kBatch && !useGraph; ++k) {
             for (uint32_t g = 0; g < G; ++g) {
                 if (done[g]) continue;
                 while (ev + 2 > s->trace_events.size()) {
@@ -599,15 +702,17 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
                     s->trace_events.push_back(e);
                 }
                 MRT_HIP(launch_wavefront_iteration(s->dev, s->generic, persistent, d->instrument != 0u, p,
-                                                   d->samples_pixel, stopReq, dbm, dacc, wbs[g], &L[g], trace_blocks,
-                                                   persist_blocks, scs[g], s->counters.as<unsigned long long>(),
+                                                   d->samples_pixel, stopReq, dbm, dacc, wbs[g], &L[g],
+                                                   sized_blocks(trace_blocks, wbs[g].slot_count),
+                                                   sized_trace_blocks(persist_blocks, wbs[g].slot_count), scs[g],
+                                                   s->counters.as<unsigned long long>(),
                                                    s->err.as<uint32_t>(), s->trace_events[ev],
                                                    s->trace_events[ev + 1], st[g]));
                 ev += 2;
             }
         }
         const uint32_t slot = checks & 1u;
-        for (uint32_t g = 0; g < G; ++g) {
+        for (uint32_t g = 0; g < G && !useGraph; ++g) {
             if (done[g]) continue;
             MRT_HIP(hipMemcpyAsync(s->counts_host + (2 * g + slot) * kCntWords, wbs[g].counts,
                                    kCntWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st[g]));
@@ -687,7 +792,7 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
     }
     *trace_ms = total;
     *busy_ms = busy;
-    *launches = static_cast<uint32_t>(ev / 2);
+    *launches = useGraph ? static_cast<uint32_t>(graphIters) : static_cast<uint32_t>(ev / 2);
     return MRT_OK;
 }
 
@@ -936,6 +1041,8 @@ MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* t) {
     if (t->max_slot_lanes && (t->max_slot_lanes > 16u || (t->max_slot_lanes & (t->max_slot_lanes - 1u))))
         return fail(MRT_E_INVALID, "max_slot_lanes must be 0 or a power of two up to 16");
     if (t->max_groups > kWfMaxGroups) return fail(MRT_E_INVALID, "max_groups must be 0-3");
+    if (t->overlap_loads > 2u) return fail(MRT_E_INVALID, "overlap_loads must be 0-2");
+    if (t->graph_loop > 2u) return fail(MRT_E_INVALID, "graph_loop must be 0-2");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->tuning = *t;
     return MRT_OK;
@@ -1607,6 +1714,7 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
             wb.sort = d->ray_sort == 1u ? 1u : 0u;
             wb.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultRefillLanes;
             wb.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultFetchChunk;
+            wb.overlap = wavefront_overlap(2ull * wb.slot_count, persistBlocks, tune.overlap_loads) ? 1u : 0u;
             wb.rad = rad;
             wb.aov_s = aovS;
             wb.lanes = lanes;

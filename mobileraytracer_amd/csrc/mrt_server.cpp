@@ -415,12 +415,21 @@ int issue_batch(mrt_server* s, uint32_t snap, bool* issued) {
     for (uint32_t g = 0; g < s->G; ++g)
         if (pipe_active(s, g)) mask |= 1u << g;
     if (!mask) return MRT_OK;
+    // Launches sized by the pipeline's slots that hold jobs (admitted before
+    // this batch; finished ones not yet released only make it larger).
+    uint32_t tb[kWfMaxGroups] = {}, pb[kWfMaxGroups] = {};
+    for (uint32_t g = 0; g < s->G; ++g) {
+        const uint64_t used = 64u * (s->waves_free[g].total() - s->waves_free[g].free_total());
+        tb[g] = sized_blocks(s->trace_blocks, used);
+        pb[g] = sized_trace_blocks(s->persist_blocks, used);
+        s->wb[g].overlap = wavefront_overlap(2u * used, s->persist_blocks, s->ctx->tuning.overlap_loads) ? 1u : 0u;
+    }
     for (uint32_t k = 0; k < kBatch; ++k) {
         for (uint32_t g = 0; g < s->G; ++g) {
             if (!(mask & (1u << g))) continue;
             const BatchDev bd = batch_dev(s, g);
             MRT_HIP(launch_wavefront_iteration(arena, false, true, false, none, 0u, false, nullptr, nullptr, s->wb[g],
-                                               &s->L[g], s->trace_blocks, s->persist_blocks, s->sc[g],
+                                               &s->L[g], tb[g], pb[g], s->sc[g],
                                                s->counters.as<unsigned long long>(), s->err.as<uint32_t>(), nullptr,
                                                nullptr, s->pipe[g], &bd));
         }

@@ -304,17 +304,35 @@ __device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, const TriG
 // pending leaf.  Lanes with a finite inverse direction walk the wide tree,
 // the others the pair tree with the reference slab test.  Returns true when
 // a shadow lane found a blocker.
-template <bool COUNT>
+// OVERLAP: both loads of the iteration go out before either result is used
+// (the node a lane visits does not depend on its test), so a ray's two
+// memory round trips per iteration overlap; the node's 32 registers are then
+// held across the test (86 instead of 68 registers, 5 instead of 7 waves per
+// SIMD).  That wins when a launch holds too few rays to fill the chip (a
+// ray's walk latency sets the launch time) and loses on full launches (818
+// against 844 Mrays/s on the 1M-triangle 1080p frame), so both are built and
+// the host picks by queue size.
+template <bool COUNT, bool OVERLAP>
 __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, bool shadow, bool finite, Stack& st,
                                           TraceStats& ts) {
     const bool canVisit = has && walk_can_visit(w);
     const bool canTest = has && w.cur != 0u;
-    // The test's triangle and the visit's node are loaded where they are
-    // used: loading both up front overlaps their latencies but holds the
-    // node's 32 registers across the test (86 instead of 68 registers, 5
-    // instead of 7 waves per SIMD), which measured slower (818 against 844
-    // Mrays/s on the 1M-triangle 1080p frame).
     bool occluded = false;
+    if (OVERLAP) {
+        TriG g{};
+        uint32_t id = 0;
+        if (canTest) g = walk_tri(s, w.cur & kSlotMask, &id);
+        WideNode wn{};
+        if (canVisit && finite) wn = load_wide(s, w.node);
+        occluded = canTest && walk_test<COUNT>(s, w, g, id, shadow, ts);
+        if (canVisit && !occluded) {
+            const float lim = cull_limit(w.h.t);
+            const int kind = shadow ? 2 : 0;
+            if (finite) walk_visit_wide<COUNT>(s, w, wn, lim, st, ts, kind);
+            else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
+        }
+        return occluded;
+    }
     if (canTest) {
         uint32_t id;
         const TriG g = walk_tri(s, w.cur & kSlotMask, &id);
@@ -375,7 +393,7 @@ __device__ __forceinline__ uint32_t fetch_entry(uint32_t* cursor, uint32_t n, bo
 // counters that regen and shade fill next.
 // MULTI: the rays belong to the jobs of a batch and start at their scenes'
 // roots in the walk arena `s`.
-template <bool COUNT, bool MULTI>
+template <bool COUNT, bool MULTI, bool OVERLAP>
 __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4* __restrict__ queue,
                                                        const uint32_t* __restrict__ count_q,
                                                        const uint32_t* __restrict__ count_s, float4* __restrict__ hits,
@@ -443,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
             ++rc.ts.lane_slots;
             if (has) ++rc.ts.lane_iters;
         }
-        const bool occluded = walk_work<COUNT>(s, w, has, shadow, finite, st, rc.ts);
+        const bool occluded = walk_work<COUNT, OVERLAP>(s, w, has, shadow, finite, st, rc.ts);
         if (has && walk_done(w)) {
             if (!shadow) hits[idx] = make_float4(w.h.t, __uint_as_float(w.h.id), w.h.u, w.h.v);
             else if (!occluded) apply_light(b, b.shadow + 4u * idx);
@@ -787,7 +805,7 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t gro
     static int perCu = 0;  // same for every device of this (single-arch) build
     if (perCu == 0) {
         int a = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace_p<false, false>, kBlock, 0) != hipSuccess) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace_p<false, false, false>, kBlock, 0) != hipSuccess) {
             (void)hipGetLastError();
             return cap;
         }
@@ -841,18 +859,18 @@ hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool pers
         if (e != hipSuccess) return e;
     }
     if (persistent) {
-        if (batch)
-            k_wf_trace_p<false, true><<<persist_blocks, kBlock, 0, stream>>>(
-                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
-                err, bd);
-        else if (count)
-            k_wf_trace_p<true, false><<<persist_blocks, kBlock, 0, stream>>>(
-                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
-                err, bd);
-        else
-            k_wf_trace_p<false, false><<<persist_blocks, kBlock, 0, stream>>>(
-                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
-                err, bd);
+#define MRT_TRACE_P(C, M, O)                                                                                        \
+    k_wf_trace_p<C, M, O><<<persist_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, c + kCntShadow, b.hits, b,  \
+                                                                c + L->cout, c + eCur, c + kCntFetch, sc, counters, \
+                                                                err, bd)
+        if (batch) {
+            if (b.overlap) MRT_TRACE_P(false, true, true); else MRT_TRACE_P(false, true, false);
+        } else if (count) {
+            if (b.overlap) MRT_TRACE_P(true, false, true); else MRT_TRACE_P(true, false, false);
+        } else {
+            if (b.overlap) MRT_TRACE_P(false, false, true); else MRT_TRACE_P(false, false, false);
+        }
+#undef MRT_TRACE_P
     } else {
 #define MRT_EXTEND(G, C)                                                                                            \
     k_wf_extend<G, C><<<trace_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, b.hits, c + L->cout, c + eCur, sc, \
@@ -894,15 +912,19 @@ hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool pers
         const hipError_t e = hipcub::DeviceScan::ExclusiveSum(b.scan_temp, bytes, b.hist, b.offs, kSortBins, stream);
         if (e != hipSuccess) return e;
         k_wf_scatter<<<trace_blocks, kBlock, 0, stream>>>(c + L->cout, b);
-    } else {
-        std::swap(L->qin, L->qout);
     }
+    wavefront_loop_advance(L, b.sort != 0u);
+    return hipGetLastError();
+}
+
+void wavefront_loop_advance(WfLoop* L, bool sort) {
+    const uint32_t eCur = (L->iter & 1u) ? kCntEnded1 : kCntEnded0;
+    if (!sort) std::swap(L->qin, L->qout);
     std::swap(L->cin, L->cout);
     L->pending[0] = L->cin;
     L->pending[1] = kCntShadow;
     L->pending[2] = eCur;
     ++L->iter;
-    return hipGetLastError();
 }
 
 }  // namespace mrt

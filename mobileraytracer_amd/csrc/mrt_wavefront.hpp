@@ -65,7 +65,21 @@ struct WfBuffers {
     // Persistent trace: a wave refills once refill_lanes lanes are idle and
     // reserves fetch_chunk (>= 64) queue entries per atomic.
     uint32_t refill_lanes, fetch_chunk;
+    // Persistent trace: 1 issues each iteration's triangle and node loads
+    // together (for launches too small to fill the chip).
+    uint32_t overlap;
 };
+
+// Whether a persistent trace launch of `blocks` workgroups over at most
+// `rays` rays runs the overlapped-load walk: when its rays cannot give every
+// lane of the launch more than one ray (a ray's own walk latency then sets
+// the launch time), unless `mode` (mrt_tuning::overlap_loads) forces it on
+// (1) or off (2).
+inline bool wavefront_overlap(uint64_t rays, uint32_t blocks, uint32_t mode) {
+    if (mode == 1u) return true;
+    if (mode == 2u) return false;
+    return rays <= static_cast<uint64_t>(blocks) * 256u;
+}
 
 // Defaults of the launch-shape knobs (mrt_tuning).
 constexpr uint32_t kDefaultRefillLanes = 32;
@@ -169,6 +183,18 @@ inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples, uint32_t
 // idle lanes.
 uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups, uint32_t share_pct = 0);
 
+// Launch sizes of one slot group's iteration when at most `slots` of its
+// slots hold paths: its queues hold at most one extension and one shadow ray
+// per slot, so workgroups beyond ceil(2 * slots / 256) (trace) and
+// ceil(slots / 256) (regen, shade) would find nothing to do.  A 256x256
+// frame's iteration otherwise launches thousands of idle workgroups.
+inline uint32_t sized_trace_blocks(uint32_t cap, uint64_t slots) {
+    return static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>(cap, (2u * slots + 255u) / 256u)));
+}
+inline uint32_t sized_blocks(uint32_t cap, uint64_t slots) {
+    return static_cast<uint32_t>(std::max<uint64_t>(1u, std::min<uint64_t>(cap, (slots + 255u) / 256u)));
+}
+
 // Queues the camera rays of sample `first_sample` of every slot.
 hipError_t launch_wavefront_start(const RenderParams& p, uint32_t first_sample, const WfBuffers& b, WfLoop* L,
                                   hipStream_t stream);
@@ -202,6 +228,11 @@ hipError_t launch_wavefront_resolve(const RenderParams& p, uint32_t spp, const W
 // launches.  With `batch`, the slots belong to the jobs of a batch (the job
 // tables give each slot's parameters, scene and outputs; `s` holds the walk
 // arena); p, spp, stop, bitmap and accum are then unused.
+// The host-side bookkeeping of one iteration (queue and counter roles,
+// pending counters, iteration count), as launch_wavefront_iteration does it;
+// for replays of captured iterations.
+void wavefront_loop_advance(WfLoop* L, bool sort);
+
 hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool persistent, bool count,
                                       const RenderParams& p, uint32_t spp, bool stop, uint32_t* bitmap, float* accum,
                                       const WfBuffers& b, WfLoop* L, uint32_t trace_blocks, uint32_t persist_blocks,
