@@ -1626,9 +1626,14 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         useWavefront = wfEligible;
     }
     const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
-    const bool persistent = useWavefront && d->persistent != 2u && !s->generic &&
-                            s->dev.acc[MRT_CAT_TRIANGLE].kind == MRT_BVH && s->dev.walk_nodes != nullptr &&
-                            s->dev.wide_nodes != nullptr;
+    // Persistent refilling trace: over the combined walk tree for BVH scenes,
+    // over a resumable grid walk for RegularGrid scenes (triangles and lights
+    // only either way).
+    const Accel& triAcc = s->dev.acc[MRT_CAT_TRIANGLE];
+    const bool persistent =
+        useWavefront && d->persistent != 2u && !s->generic &&
+        ((triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr) ||
+         (triAcc.kind == MRT_REGULAR_GRID && triAcc.count > 0 && triAcc.cell_off != nullptr));
     const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
     // Wavefront slot groups run concurrently, each with its own spill region;
     // sample-parallel frames give every pixel `lanes` slots.

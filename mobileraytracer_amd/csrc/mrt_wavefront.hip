@@ -472,6 +472,127 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
     else flush_counts<false>(rc, counters, st.overflow, err);
 }
 
+// ------------------------------------------- trace (persistent, grid) --
+// Persistent trace of both queues for RegularGrid scenes of triangles and
+// lights: the refilling lanes of k_wf_trace_p over a resumable 3D-DDA.  A
+// lane's unit of work is one primitive test of its current cell's list or
+// one DDA step to the next cell (RegularGrid<T>::trace / shadowTrace,
+// RegularGrid.hpp:157-465, in their order).  Closest hits: phase 1 tests the
+// cell's list until a primitive hits; the rest of that cell's list is then
+// tested in phase 2 (the reference's phase 2 re-tests the whole list, but a
+// primitive already tested misses again or returns t equal to the kept one,
+// which the strict t < length_ rejects); phase 2 steps on only while the
+// nearest t is not below the next wall.  The area lights are tested after the
+// walk (Scene::traceLights after the accelerators, Shader.cpp:144-149).  Any
+// hits return at the first blocker.
+struct GridWalk {
+    Dda g;
+    uint32_t pos, end;  // untested part of the current cell's list
+    bool phase2, done;
+};
+
+__device__ __forceinline__ void grid_walk_begin(const Accel& a, const RayT& r, GridWalk& w) {
+    dda_setup(a, r, w.g);
+    const uint32_t c = dda_cell(a, w.g);
+    w.pos = a.cell_off[c];
+    w.end = a.cell_off[c + 1u];
+    w.phase2 = false;
+    w.done = false;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_wf_trace_grid_p(SceneDev s, const float4* __restrict__ queue,
+                                                            const uint32_t* __restrict__ count_q,
+                                                            const uint32_t* __restrict__ count_s,
+                                                            float4* __restrict__ hits, WfBuffers b, uint32_t* clear_q,
+                                                            uint32_t* clear_e, uint32_t* fetch, StackCfg sc,
+                                                            unsigned long long* counters, uint32_t* err) {
+    __shared__ uint32_t lds[kStackLds * kBlock];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    if (gtid == 0) {
+        *clear_q = 0u;
+        *clear_e = 0u;
+    }
+    Stack st = make_stack(lds, sc, gtid);
+    RayCounts rc{};
+    const Accel& a = s.acc[MRT_CAT_TRIANGLE];
+    const uint32_t nq = *count_q;
+    const uint32_t n = nq + *count_s;
+    RayT r{};
+    HitT h{};
+    GridWalk w{};
+    uint32_t idx = 0;
+    bool has = false, exhausted = false, shadow = false;
+    FetchRun run{0u, 0u, false};
+    while (true) {
+        const bool need = !has && !exhausted;
+        if (wave_wants_refill(need, has, b.refill_lanes)) {
+            const uint32_t e = fetch_entry(fetch, n, need, run, b.fetch_chunk);
+            if (need) {
+                if (e < nq) {
+                    const float4 qa = queue[2u * e], qc = queue[2u * e + 1u];
+                    if (__float_as_uint(qc.w) != kDeadEntry) {
+                        idx = e;
+                        shadow = false;
+                        r = make_ray(f3{qa.x, qa.y, qa.z}, f3{qa.w, qc.x, qc.y}, __float_as_uint(qc.z));
+                        h = HitT{kRayLengthMax, MRT_ID_NONE, 0.0f, 0.0f};
+                        grid_walk_begin(a, r, w);
+                        has = true;
+                    }
+                } else if (e < n) {
+                    idx = e - nq;
+                    const float4* sh = b.shadow + 4u * idx;
+                    const float4 qa = sh[0], qc = sh[1];
+                    shadow = true;
+                    r = make_ray(f3{qa.x, qa.y, qa.z}, f3{qa.w, qc.x, qc.y}, __float_as_uint(qc.w));
+                    h = HitT{qc.z, MRT_ID_NONE, 0.0f, 0.0f};  // h.t holds the shadow distance
+                    grid_walk_begin(a, r, w);
+                    ++rc.shadow;
+                    has = true;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(has || !exhausted) == 0ull) break;
+        if (COUNT) {
+            ++rc.ts.lane_slots;
+            if (has) ++rc.ts.lane_iters;
+        }
+        bool occluded = false;
+        if (has && w.pos < w.end) {
+            const uint32_t pi = a.cell_ref[w.pos++];
+            if (COUNT) ++rc.ts.prims[shadow ? 2 : 0];
+            if (shadow) {
+                occluded = prim_any<MRT_CAT_TRIANGLE>(s, pi, r, h.t);
+                w.done = occluded;
+            } else if (prim_closest<MRT_CAT_TRIANGLE, true>(s, pi, r, h)) {
+                w.phase2 = true;
+            }
+        } else if (has) {
+            if (COUNT) ++rc.ts.pairs[shadow ? 2 : 0];
+            if (!dda_step(w.g, !shadow && w.phase2, h.t)) {
+                w.done = true;
+            } else {
+                const uint32_t c = dda_cell(a, w.g);
+                w.pos = a.cell_off[c];
+                w.end = a.cell_off[c + 1u];
+            }
+        }
+        if (has && w.done) {
+            if (!shadow) {
+                lights_closest<COUNT>(s, r, h, st, rc.ts);
+                hits[idx] = make_float4(h.t, __uint_as_float(h.id), h.u, h.v);
+            } else if (!occluded) {
+                apply_light(b, b.shadow + 4u * idx);
+            }
+            has = false;
+        }
+    }
+    if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
+    else flush_counts<false>(rc, counters, st.overflow, err);
+}
+
 // ------------------------------------------------------------- regen --
 // Finishes the paths that ended in the previous shade (PathTracer.cpp:144-164
 // result selection, then incrementalAvg of Renderer.cpp:101-104 with the
@@ -858,7 +979,17 @@ hipError_t launch_wavefront_iteration(const SceneDev& s, bool generic, bool pers
         const hipError_t e = hipEventRecord(trace_begin, stream);
         if (e != hipSuccess) return e;
     }
-    if (persistent) {
+    if (persistent && s.acc[MRT_CAT_TRIANGLE].kind == MRT_REGULAR_GRID) {
+        if (batch) return hipErrorInvalidValue;
+        if (count)
+            k_wf_trace_grid_p<true><<<persist_blocks, kBlock, 0, stream>>>(
+                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
+                err);
+        else
+            k_wf_trace_grid_p<false><<<persist_blocks, kBlock, 0, stream>>>(
+                s, L->qin, c + L->cin, c + kCntShadow, b.hits, b, c + L->cout, c + eCur, c + kCntFetch, sc, counters,
+                err);
+    } else if (persistent) {
 #define MRT_TRACE_P(C, M, O)                                                                                        \
     k_wf_trace_p<C, M, O><<<persist_blocks, kBlock, 0, stream>>>(s, L->qin, c + L->cin, c + kCntShadow, b.hits, b,  \
                                                                 c + L->cout, c + eCur, c + kCntFetch, sc, counters, \

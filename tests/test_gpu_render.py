@@ -364,3 +364,33 @@ def test_graph_loop_equals_launched_loop(size, spp, generic):
     assert c1 == c2 and g1 == g2
     assert np.array_equal(b1, b2)
     assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32))
+
+
+@pytest.mark.parametrize("spp", [1, 3])
+def test_grid_persistent_equals_one_ray_per_lane(gpu_ctx, spp):
+    """RegularGrid scenes: the persistent refilling grid walk, the one-ray-
+    per-lane wavefront trace and the megakernel give identical frames and
+    ray counts (512x288: three slot groups)."""
+    W, H = 512, 288
+    data = mrt.soup_scene(13, 120000, 1200)
+    cam = soup_camera(W, H)
+    cmin, cmax = mrt.camera_aabb(cam)
+    gpu = mrt.Scene(gpu_ctx, data)
+    res = []
+    try:
+        gpu.build(abi.REGULAR_GRID, cmin, cmax)
+        for kernel, pers in ((abi.KERNEL_MEGAKERNEL, 0), (abi.KERNEL_WAVEFRONT, 2), (abi.KERNEL_WAVEFRONT, 1)):
+            desc = mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
+                                   rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6, kernel=kernel,
+                                   persistent=pers)
+            acc = np.zeros((H, W, 3), np.float32)
+            bm, st = gpu.render(desc, accum=acc)
+            res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow)))
+    finally:
+        gpu.close()
+    b0, a0, c0 = res[0]
+    for b1, a1, c1 in res[1:]:
+        print(f"grid frames: rays {c0} {c1}", flush=True)
+        assert c0 == c1
+        assert np.array_equal(b0, b1)
+        assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
