@@ -174,6 +174,11 @@ typedef struct mrt_camera {
 } mrt_camera;
 
 /* Kernel shape for mrt_render. */
+/* MEGAKERNEL: every path on one lane.  Frames the wavefront kernels can run
+ * (PathTracer, one light sample, no reflection or transmission, BVH) run the
+ * persistent megakernel, one launch for the whole frame (desc.persistent 2:
+ * one launch per sample pass instead); everything else runs one launch per
+ * sample pass. */
 enum mrt_kernel { MRT_KERNEL_AUTO = 0, MRT_KERNEL_MEGAKERNEL = 1, MRT_KERNEL_WAVEFRONT = 2 };
 
 typedef struct mrt_render_desc {
@@ -292,7 +297,9 @@ typedef struct mrt_tuning {
                                   launch arguments replays it without a new
                                   capture (graph frames report no per-launch
                                   trace times).  0, 2: launched iterations    */
-    uint32_t reserved[1];
+    uint32_t service_lanes;    /* persistent megakernel: a wave shades once this
+                                  many of its lanes finished their walks (1-64;
+                                  16)                                          */
 } mrt_tuning;
 MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* tuning);
 MRT_API int mrt_ctx_get_tuning(const mrt_ctx* ctx, mrt_tuning* tuning);

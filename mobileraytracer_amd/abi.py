@@ -120,7 +120,7 @@ class RenderStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("refill_lanes", C.c_uint32), ("fetch_chunk", C.c_uint32), ("launch_share_pct", C.c_uint32),
                 ("max_slot_lanes", C.c_uint32), ("max_groups", C.c_uint32), ("overlap_loads", C.c_uint32),
-                ("graph_loop", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("graph_loop", C.c_uint32), ("service_lanes", C.c_uint32)]
 
 
 class ServerDesc(C.Structure):

@@ -1043,6 +1043,7 @@ MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* t) {
     if (t->max_groups > kWfMaxGroups) return fail(MRT_E_INVALID, "max_groups must be 0-3");
     if (t->overlap_loads > 2u) return fail(MRT_E_INVALID, "overlap_loads must be 0-2");
     if (t->graph_loop > 2u) return fail(MRT_E_INVALID, "graph_loop must be 0-2");
+    if (t->service_lanes > 64u) return fail(MRT_E_INVALID, "service_lanes must be 0-64");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->tuning = *t;
     return MRT_OK;
@@ -1634,15 +1635,20 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         useWavefront && d->persistent != 2u && !s->generic &&
         ((triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr) ||
          (triAcc.kind == MRT_REGULAR_GRID && triAcc.count > 0 && triAcc.cell_off != nullptr));
-    const uint32_t blocks = useWavefront ? traceBlocks : render_pass_blocks(p);
+    // The persistent megakernel: megakernel frames the wavefront could run,
+    // over the combined walk tree.
+    const bool megaP = !useWavefront && wfEligible && d->persistent != 2u && !s->generic &&
+                       triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr;
+    const uint32_t megaBlocks = megaP ? mega_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u) : 0u;
+    const uint32_t blocks = useWavefront ? traceBlocks : megaP ? megaBlocks : render_pass_blocks(p);
     // Wavefront slot groups run concurrently, each with its own spill region;
     // sample-parallel frames give every pixel `lanes` slots.
-    const uint32_t pixelSlots = useWavefront ? wavefront_slots(p) : 0u;
+    const uint32_t pixelSlots = useWavefront || megaP ? wavefront_slots(p) : 0u;
     const mrt_tuning& tune = s->ctx->tuning;
     const uint32_t lanes =
-        useWavefront ? wavefront_lanes(pixelSlots, d->samples_pixel - d->first_sample,
-                                       tune.max_slot_lanes ? tune.max_slot_lanes : kDefaultSlotLanes)
-                     : 1u;
+        useWavefront || megaP ? wavefront_lanes(pixelSlots, d->samples_pixel - d->first_sample,
+                                                tune.max_slot_lanes ? tune.max_slot_lanes : kDefaultSlotLanes)
+                              : 1u;
     const uint32_t slots = pixelSlots * lanes;
     const uint32_t groups = useWavefront ? wavefront_groups(slots, tune.max_groups ? tune.max_groups : kWfMaxGroups)
                                          : 1u;
@@ -1752,7 +1758,55 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
                                 &kernel_ms, &trace_ms, &trace_busy, &traceLaunches)) != MRT_OK)
             return rc;
     }
-    for (uint32_t sample = d->first_sample; !useWavefront && sample < d->samples_pixel; ++sample) {
+    if (megaP && d->first_sample < d->samples_pixel && !s->stop.load()) {
+        // One launch for the frame; per-sample radiance and the resolve
+        // when a pixel's samples run on several lanes.
+        if ((rc = s->mega_ctl.alloc(256)) != MRT_OK) return rc;
+        uint32_t* ctl = s->mega_ctl.as<uint32_t>();
+        MRT_HIP(hipMemsetAsync(ctl, 0, 256, st));
+        float* rad = nullptr;
+        if (lanes > 1u) {
+            const size_t n = 3ull * (d->samples_pixel - d->first_sample) * pixelSlots;
+            if ((rc = s->wf_rad.alloc(n * sizeof(float))) != MRT_OK) return rc;
+            rad = s->wf_rad.as<float>();
+            MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(rad), static_cast<int>(kRadPending), n, st));
+        }
+        MegaArgs a{};
+        a.p = p;
+        a.first = d->first_sample;
+        a.spp = d->samples_pixel;
+        a.lanes = lanes;
+        a.pixel_slots = pixelSlots;
+        a.rad = rad;
+        a.bitmap = dbm;
+        a.accum = dacc;
+        a.fetch = ctl;
+        a.done = ctl + 16;
+        a.service_lanes = tune.service_lanes ? tune.service_lanes : kDefaultServiceLanes;
+        a.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultRefillLanes;
+        a.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultFetchChunk;
+        MRT_HIP(hipEventRecord(k0, st));
+        MRT_HIP(launch_mega_p(s->dev, d->instrument != 0u, a, megaBlocks, sc, s->counters.as<unsigned long long>(),
+                              s->err.as<uint32_t>(), st));
+        if (rad) {
+            WfBuffers wb{};
+            wb.rad = rad;
+            wb.lanes = lanes;
+            wb.pixel_slots = pixelSlots;
+            wb.first = d->first_sample;
+            uint32_t* doneMin = ctl + 32;
+            MRT_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(doneMin), static_cast<int>(d->samples_pixel),
+                                      1, st));
+            MRT_HIP(launch_wavefront_resolve(p, d->samples_pixel, wb, dbm, dacc, doneMin, st));
+        }
+        MRT_HIP(hipEventRecord(k1, st));
+        MRT_HIP(hipEventSynchronize(k1));
+        MRT_HIP(hipEventElapsedTime(&kernel_ms, k0, k1));
+        s->progress.store(d->samples_pixel);
+    } else if (megaP && s->stop.load()) {
+        stopped = true;
+    }
+    for (uint32_t sample = d->first_sample; !useWavefront && !megaP && sample < d->samples_pixel; ++sample) {
         if (s->stop.load()) {
             stopped = true;
             break;

@@ -593,6 +593,209 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_grid_p(SceneDev s, const fl
     else flush_counts<false>(rc, counters, st.overflow, err);
 }
 
+// ----------------------------------------------- persistent megakernel --
+// One launch renders the whole frame: every lane takes work items (a pixel
+// slot and a sample lane k, whose samples first + k, first + k + lanes, ...
+// it renders in order) and keeps its path in registers, walking the combined
+// walk tree one unit per iteration like the persistent trace.  A lane whose
+// walk ended needs service (shade the vertex and start the shadow or
+// extension walk, apply a light sample, finish a sample and start the next,
+// or take a new item); a wave runs the service step once `service_lanes` of
+// its lanes need it, or when no lane is walking, so the shading code runs for
+// many lanes at once instead of for each lane whose walk just ended.  Each
+// path's operations and their order are those of PathTracer::shade as the
+// wavefront kernels run it (shade, then its shadow ray, then its extension
+// ray), so frames are identical to theirs.
+
+enum : uint32_t { kMegaIdle = 0, kMegaExt = 1, kMegaShadow = 2 };
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, StackCfg sc, unsigned long long* counters,
+                                                   uint32_t* err) {
+    __shared__ uint32_t lds[kStackLds * kBlock];
+    const RenderParams& p = A.p;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st = make_stack(lds, sc, gtid);
+    RayCounts rc{};
+    const uint32_t nItems = A.lanes * A.pixel_slots;
+    uint32_t ps = 0, x = 0, y = 0, pid = 0, sample = 0;  // the lane's item and current sample
+    f3 T{}, accAll{}, accPrefix{};                       // path state (sA, sB, sC of the wavefront)
+    bool found = false, ends = false;
+    uint32_t depth = 0;
+    f3 vO{}, shD{}, extD{}, contrib{};                   // the shaded vertex's shadow and extension rays
+    uint32_t vExcl = MRT_ID_NONE;
+    float shDist = 0.0f;
+    bool ldPos = false, pendExt = false;
+    Walk w;
+    walk_clear(w);
+    w.h.t = 0.0f;
+    uint32_t mode = kMegaIdle;
+    bool finite = true, occluded = false, exhausted = false;
+    uint32_t finished = 0;
+    FetchRun run{0u, 0u, false};
+    auto start_walk = [&](f3 o, f3 d, uint32_t excl, bool shadowRay, float dist) {
+        w.r = make_ray(o, d, excl);
+        finite = inv_finite(w.r);
+        walk_begin(w, shadowRay ? kWalkAnyRoot : kWalkClosestRoot, shadowRay ? dist : kRayLengthMax, st);
+        mode = shadowRay ? kMegaShadow : kMegaExt;
+        occluded = false;
+    };
+    auto start_sample = [&]() {
+        f3 o, d;
+        primary_ray(p, x, y, pid, sample, &o, &d);
+        T = f3{1.0f, 1.0f, 1.0f};
+        accAll = accPrefix = f3{0.0f, 0.0f, 0.0f};
+        found = ends = false;
+        depth = 1u;
+        start_walk(o, d, MRT_ID_NONE, false, 0.0f);
+    };
+    while (true) {
+        const bool walking = mode != kMegaIdle && !walk_done(w);
+        const bool needs = (mode != kMegaIdle && !walking) || (mode == kMegaIdle && !exhausted);
+        if (__ballot(walking || needs) == 0ull) break;
+        const unsigned long long svc = __ballot(needs);
+        if (svc != 0ull && (static_cast<uint32_t>(__popcll(svc)) >= A.service_lanes || __ballot(walking) == 0ull)) {
+            bool finish = false;
+            if (needs && mode == kMegaShadow) {
+                // The light sample's shadow ray is done (k_wf_trace_p's
+                // apply_light), then the vertex's extension ray, if any.
+                if (!occluded) {
+                    accAll = add(accAll, contrib);
+                    if (!found && ldPos) {
+                        found = true;
+                        accPrefix = accAll;
+                    }
+                }
+                if (pendExt) start_walk(vO, extD, vExcl, false, 0.0f);
+                else finish = true;
+            } else if (needs && mode == kMegaExt) {
+                // PathTracer::shade of the vertex the ray reached
+                // (PathTracer.cpp:25-133), as k_wf_shade runs it.
+                if (depth == 1u) ++rc.camera;
+                else ++rc.extension;
+                bool wantShadow = false;
+                pendExt = false;
+                if (w.h.id != MRT_ID_NONE) {
+                    const Surf sf = surface(s, w.r, w.h);
+                    const Shading m = shading_of(s, sf);
+                    const DrawCtx dc{pid, sample, 0u};
+                    if (m.emitter) {
+                        accAll = add(accAll, mul(T, m.le));
+                        ends = true;
+                    } else if (any_gt0(m.kd)) {
+                        if (s.n_lights > 0u) {
+                            f3 lc;
+                            if (light_sample(s, p, sf, static_cast<int32_t>(depth), dc, 0u, &shD, &shDist, &lc)) {
+                                const f3 Ld = divs(mul(lc, m.kd), static_cast<float>(p.samples_light));
+                                contrib = mul(T, Ld);
+                                ldPos = any_gt0(Ld);
+                                wantShadow = true;
+                            }
+                        }
+                        bool cont = static_cast<int32_t>(depth) <= p.depth_min;
+                        if (!cont) cont = draw_at(p, p.s_rr, dc, kSlotRR, 0, static_cast<int32_t>(depth)) > 0.5f;
+                        if (cont && static_cast<int32_t>(depth) + 1 <= p.depth_max) {
+                            extD = cosine_hemisphere(p, sf.normal, dc, static_cast<int32_t>(depth));
+                            f3 Tn = mul(T, m.kd);
+                            if (static_cast<int32_t>(depth) > p.depth_min) Tn = divs(Tn, 0.5f * 0.5f);
+                            T = Tn;
+                            ++depth;
+                            pendExt = true;
+                        }
+                        vO = sf.point;
+                        vExcl = sf.excl;
+                    }
+                }
+                if (wantShadow) {
+                    ++rc.shadow;
+                    start_walk(vO, shD, vExcl, true, shDist);
+                } else if (pendExt) {
+                    start_walk(vO, extD, vExcl, false, 0.0f);
+                } else {
+                    finish = true;
+                }
+            }
+            if (finish) {
+                // The sample is done (k_wf_regen's result selection and
+                // running mean, Renderer.cpp:101-104).
+                const f3 c = (ends && found) ? accPrefix : accAll;
+                if (A.rad) {
+                    float* r = A.rad + 3ull * ((sample - A.first) * static_cast<uint64_t>(A.pixel_slots) + ps);
+                    r[0] = c.x;
+                    r[1] = c.y;
+                    r[2] = c.z;
+                } else {
+                    const uint32_t bi = y * p.real_width + x;
+                    A.bitmap[bi] = incremental_avg(c, A.bitmap[bi], sample + 1u);
+                    if (A.accum) {
+                        A.accum[3u * pid + 0u] += c.x;
+                        A.accum[3u * pid + 1u] += c.y;
+                        A.accum[3u * pid + 2u] += c.z;
+                    }
+                }
+                ++finished;
+                sample += A.lanes;
+                if (sample < A.spp) {
+                    start_sample();
+                } else {
+                    mode = kMegaIdle;
+                    walk_clear(w);
+                }
+            }
+            // New items for idle lanes (every lane of the wave takes part in
+            // the fetch's ballot).
+            const bool want = mode == kMegaIdle && !exhausted;
+            const uint32_t e = fetch_entry(A.fetch, nItems, want, run, A.fetch_chunk);
+            if (want) {
+                if (e == kDeadEntry) {
+                    exhausted = true;
+                } else {
+                    const uint32_t k = e / A.pixel_slots;
+                    ps = e - k * A.pixel_slots;
+                    slot_pixel(p, ps, &x, &y);
+                    sample = A.first + k;
+                    if (pixel_covered(p, x, y) && sample < A.spp) {
+                        pid = y * p.width + x;
+                        start_sample();
+                    }
+                }
+            }
+            continue;
+        }
+        if (COUNT) {
+            ++rc.ts.lane_slots;
+            if (walking) ++rc.ts.lane_iters;
+        }
+        if (walk_work<COUNT, false>(s, w, walking, mode == kMegaShadow, finite, st, rc.ts)) occluded = true;
+    }
+    const uint32_t total = wave_sum(finished);
+    if ((threadIdx.x & 63u) == 0u && total) atomicAdd(A.done, total);
+    if (COUNT) flush_counts<true>(rc, counters, st.overflow, err);
+    else flush_counts<false>(rc, counters, st.overflow, err);
+}
+
+uint32_t mega_blocks(uint32_t num_cu) {
+    static int perCu = 0;
+    if (perCu == 0) {
+        int a = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_mega_p<false>, kBlock, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            a = 2;
+        }
+        perCu = std::max(1, a);
+    }
+    return static_cast<uint32_t>(perCu) * num_cu;
+}
+
+hipError_t launch_mega_p(const SceneDev& s, bool count, const MegaArgs& a, uint32_t blocks, const StackCfg& sc,
+                         unsigned long long* counters, uint32_t* err, hipStream_t stream) {
+    if (a.lanes == 0u || a.pixel_slots == 0u || (a.lanes > 1u) != (a.rad != nullptr) || a.fetch_chunk < 64u)
+        return hipErrorInvalidValue;
+    if (count) k_mega_p<true><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+    else k_mega_p<false><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------- regen --
 // Finishes the paths that ended in the previous shade (PathTracer.cpp:144-164
 // result selection, then incrementalAvg of Renderer.cpp:101-104 with the

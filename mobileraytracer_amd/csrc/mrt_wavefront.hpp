@@ -228,6 +228,30 @@ hipError_t launch_wavefront_resolve(const RenderParams& p, uint32_t spp, const W
 // launches.  With `batch`, the slots belong to the jobs of a batch (the job
 // tables give each slot's parameters, scene and outputs; `s` holds the walk
 // arena); p, spp, stop, bitmap and accum are then unused.
+// Persistent megakernel (k_mega_p): one launch renders samples [first, spp)
+// of every covered pixel; each lane takes work items (pixel slot ps, sample
+// lane k: samples first + k, + k + lanes, ...) and keeps its path in
+// registers.  lanes > 1: each sample's radiance goes to rad for
+// launch_wavefront_resolve; lanes == 1: the lane folds its pixel's samples
+// into bitmap / accum in order.  fetch and done are zero before the launch.
+struct MegaArgs {
+    RenderParams p;
+    uint32_t first, spp;
+    uint32_t lanes;
+    uint32_t pixel_slots;
+    float* rad;
+    uint32_t* bitmap;
+    float* accum;
+    uint32_t* fetch;
+    uint32_t* done;
+    uint32_t service_lanes, refill_lanes, fetch_chunk;
+};
+hipError_t launch_mega_p(const SceneDev& s, bool count, const MegaArgs& a, uint32_t blocks, const StackCfg& sc,
+                         unsigned long long* counters, uint32_t* err, hipStream_t stream);
+// Workgroups of k_mega_p the chip holds at once.
+uint32_t mega_blocks(uint32_t num_cu);
+constexpr uint32_t kDefaultServiceLanes = 16;
+
 // The host-side bookkeeping of one iteration (queue and counter roles,
 // pending counters, iteration count), as launch_wavefront_iteration does it;
 // for replays of captured iterations.

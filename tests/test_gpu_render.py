@@ -196,8 +196,11 @@ def test_render_deterministic(gpu_ctx):
 @pytest.mark.parametrize("generic,size", [(False, (256, 144)), (True, (256, 144)), (False, (512, 288))])
 def test_wavefront_equals_megakernel(gpu_ctx, generic, size):
     """Both kernel shapes evaluate every path with the same operations in the
-    same order: bitmaps, FP32 radiance and ray counts are identical.  At
-    512x288 the wavefront frame runs as two slot groups on two streams."""
+    same order: bitmaps, FP32 radiance and ray counts are identical (the
+    megakernel per sample pass and, on scenes of triangles and lights, as one
+    persistent launch; the wavefront with and without reordering and
+    refilling lanes).  At 512x288 the wavefront frame runs as three slot
+    groups on three streams."""
     W, H = size
     data = mrt.soup_scene(9, 80000, 800)
     if generic:
@@ -215,7 +218,7 @@ def test_wavefront_equals_megakernel(gpu_ctx, generic, size):
         data.materials[0]["ior"] = 1.0
     cam = soup_camera(W, H)
     res = []
-    for kernel, sort, pers in ((abi.KERNEL_MEGAKERNEL, 0, 0), (abi.KERNEL_WAVEFRONT, 2, 2),
+    for kernel, sort, pers in ((abi.KERNEL_MEGAKERNEL, 0, 2), (abi.KERNEL_MEGAKERNEL, 0, 0), (abi.KERNEL_WAVEFRONT, 2, 2),
                                (abi.KERNEL_WAVEFRONT, 1, 2), (abi.KERNEL_WAVEFRONT, 2, 1),
                                (abi.KERNEL_WAVEFRONT, 1, 1)):
         desc = mrt.render_desc(abi.SHADER_PATHTRACER, W, H, 3, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
@@ -394,3 +397,32 @@ def test_grid_persistent_equals_one_ray_per_lane(gpu_ctx, spp):
         assert c0 == c1
         assert np.array_equal(b0, b1)
         assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+
+
+@pytest.mark.parametrize("spp,tiles", [(1, None), (3, None), (9, None), (4, (0, 17, 120, 255))])
+def test_persistent_megakernel_equals_per_pass(gpu_ctx, spp, tiles):
+    """The persistent megakernel (one launch per frame; 1, 2 or 8 sample
+    lanes per pixel, the latter two through the per-sample radiance and the
+    resolve) against the per-pass megakernel: identical frames and counts,
+    also on a tile subset."""
+    W, H = 320, 176
+    data = mrt.soup_scene(14, 100000, 1000)
+    cam = soup_camera(W, H)
+    gpu = mrt.Scene(gpu_ctx, data)
+    res = []
+    try:
+        gpu.build(abi.BVH)
+        for pers in (2, 0):
+            desc = mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0, 5),
+                                   rr=rnd(6), light=rnd(7), hemi=rnd(8), depth_min=2, depth_max=6,
+                                   kernel=abi.KERNEL_MEGAKERNEL, persistent=pers, tiles=tiles)
+            acc = np.zeros((H, W, 3), np.float32)
+            bm, st = gpu.render(desc, accum=acc)
+            res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow), st.samples_done))
+    finally:
+        gpu.close()
+    (b0, a0, c0, d0), (b1, a1, c1, d1) = res
+    print(f"megakernel per pass / persistent: rays {c0} {c1}", flush=True)
+    assert c0 == c1 and d0 == d1 == spp
+    assert np.array_equal(b0, b1)
+    assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
