@@ -216,7 +216,7 @@ def main() -> int:
     ap.add_argument("--kernel", choices=sorted(KERNELS), default="auto")
     ap.add_argument("--sort", choices=["default", "on", "off"], default="default",
                     help="wavefront: reorder extension rays by origin cell (default: off)")
-    ap.add_argument("--no-persistent", action="store_true", help="wavefront: one ray per lane, no refill")
+    ap.add_argument("--no-persistent", action="store_true", help="no persistent kernels: the wavefront with one ray per lane, no refill")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
@@ -338,6 +338,7 @@ def main() -> int:
     busy_ms = 0.0
     launches = 0
     groups = 0
+    mega_p = False
     for _ in range(args.steps):
         _, st = scene.render(desc, bitmap=bitmap, accum=accum, mem=abi.MEM_DEVICE)
         rays += st.rays_camera + st.rays_extension + st.rays_shadow + st.rays_specular
@@ -347,15 +348,25 @@ def main() -> int:
         busy_ms += st.ms_trace_busy
         launches += st.trace_launches
         groups = st.trace_groups
-        log(f"step: {st.ms_render:.1f} ms, {st.rays_camera} camera / {st.rays_extension} extension / "
-            f"{st.rays_shadow} shadow rays; trace {st.ms_trace:.1f} ms in {st.trace_launches} launches "
-            f"({st.trace_groups} slot groups, some trace launch running {st.ms_trace_busy:.1f} ms)")
+        mega_p = st.persistent_megakernel != 0
+        if mega_p:
+            log(f"step: {st.ms_render:.1f} ms, {st.rays_camera} camera / {st.rays_extension} extension / "
+                f"{st.rays_shadow} shadow rays; persistent megakernel {st.ms_kernel:.1f} ms "
+                f"({st.slot_lanes} sample lanes per pixel)")
+        else:
+            log(f"step: {st.ms_render:.1f} ms, {st.rays_camera} camera / {st.rays_extension} extension / "
+                f"{st.rays_shadow} shadow rays; trace {st.ms_trace:.1f} ms in {st.trace_launches} launches "
+                f"({st.trace_groups} slot groups, some trace launch running {st.ms_trace_busy:.1f} ms)")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     mrays = rays / elapsed / 1e6
 
-    # Roofline of the dominant kernel.  Wavefront: the trace kernel, timed
-    # with HIP events around each of its launches on the render stream.
+    # Roofline of the dominant kernel.  Persistent megakernel (the AUTO
+    # kernel of the BVH frame): one launch per frame, timed with HIP events
+    # around it on the render stream; its bytes are the rays' traversal bytes
+    # plus 256 bytes of shading per closest-hit ray (hit triangle's shading
+    # record, material, light sample, path state).  Wavefront: the trace
+    # kernel, timed with HIP events around each of its launches.
     # Algorithmic bytes per ray (BVH): the oracle's ordered traversal of the
     # same reference tree over a 1/256 pixel subsample of the frame
     # (ordered_traffic), independent of this implementation; beside it the
@@ -373,7 +384,7 @@ def main() -> int:
     # the BVH's walk tree, so grid scenes trace with the one-ray-per-lane
     # wavefront kernels.
     wavefront = launches > 0
-    persistent = wavefront and not args.no_persistent and args.accel == "bvh"
+    persistent = (wavefront or mega_p) and not args.no_persistent and args.accel == "bvh"
     node_bytes = 128 if persistent else 64
     bytes_per_ray = 48 + node_bytes * pairs_per_ray + 48 * prims_per_ray
     gpu_bytes_per_ray = bytes_per_ray
@@ -386,7 +397,12 @@ def main() -> int:
         light_tests = st_instr.light_prim_tests / rays1
         bytes_per_ray = (48 + 8 * (pairs_per_ray - light_pairs) + 64 * light_pairs +
                          52 * (prims_per_ray - light_tests) + 48 * light_tests)
-    if wavefront:
+    if mega_p:
+        kernel_name = "k_mega_p"
+        launches = args.steps
+        bytes_total = rays * bytes_per_ray + closest * 256
+        avg_launch_s = kernel_ms / 1000.0 / launches
+    elif wavefront:
         kernel_name = "k_wf_trace_p" if persistent else "k_wf_extend + k_wf_shadow"
         bytes_total = rays * bytes_per_ray
         avg_launch_s = trace_ms / 1000.0 / launches

@@ -232,8 +232,10 @@ typedef struct mrt_render_stats {
     float ms_trace_busy;       /* wavefront: time at least one trace launch ran
                                   (launches of slot groups overlap)            */
     uint32_t trace_groups;     /* wavefront: slot groups run concurrently      */
-    uint32_t slot_lanes;       /* wavefront: path slots per pixel (samples of a
-                                  pixel in flight together)                    */
+    uint32_t slot_lanes;       /* wavefront, persistent megakernel: path slots
+                                  (sample lanes) per pixel                     */
+    uint32_t persistent_megakernel; /* 1: the frame ran as one persistent
+                                  megakernel launch (ms_kernel is its time)    */
 } mrt_render_stats;
 
 /* Accelerator build parameters (Shader::initializeAccelerators inputs). */

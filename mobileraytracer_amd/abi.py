@@ -114,7 +114,7 @@ class RenderStats(C.Structure):
                 ("ms_render", C.c_float), ("ms_kernel", C.c_float), ("ms_trace", C.c_float),
                 ("trace_launches", C.c_uint32), ("trace_lane_iters", C.c_uint64),
                 ("trace_lane_slots", C.c_uint64), ("ms_trace_busy", C.c_float), ("trace_groups", C.c_uint32),
-                ("slot_lanes", C.c_uint32)]
+                ("slot_lanes", C.c_uint32), ("persistent_megakernel", C.c_uint32)]
 
 
 class Tuning(C.Structure):

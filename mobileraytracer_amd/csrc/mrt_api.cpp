@@ -1613,9 +1613,18 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         MRT_HIP(hipMemsetAsync(daov, 0, aovBytes, st));
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
-    // Kernel shape: the wavefront pipeline where it applies, else the
-    // megakernel.
+    // Kernel shape.  AUTO: the persistent megakernel for PathTracer frames
+    // of BVH scenes of triangles and lights (one launch, paths in registers;
+    // on the 1M-triangle soup it took 4.5 / 5.0 / 10.2 / 19.2 / 132 ms for
+    // 256x256 x 1 and x 4, 640x480 x 4, 1280x720 x 4 and 1920x1080 x 16
+    // samples against 6.5 / 8.6 / 14.2 / 22.6 / 134 ms for the wavefront),
+    // the wavefront for the other frames it can run and for frames with
+    // auxiliary outputs (it records them while shading), else the
+    // megakernel per sample pass.
     const bool wfEligible = wavefront_eligible(s, d);
+    const Accel& triAcc = s->dev.acc[MRT_CAT_TRIANGLE];
+    const bool megaEligible = wfEligible && d->persistent != 2u && !s->generic && triAcc.kind == MRT_BVH &&
+                              s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr;
     bool useWavefront = false;
     if (d->kernel == MRT_KERNEL_WAVEFRONT) {
         if (!wfEligible)
@@ -1624,21 +1633,19 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
                                            "65535 samples per pixel");
         useWavefront = true;
     } else if (d->kernel == MRT_KERNEL_AUTO) {
-        useWavefront = wfEligible;
+        useWavefront = wfEligible && (!megaEligible || aov != nullptr);
     }
     const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
     // Persistent refilling trace: over the combined walk tree for BVH scenes,
     // over a resumable grid walk for RegularGrid scenes (triangles and lights
     // only either way).
-    const Accel& triAcc = s->dev.acc[MRT_CAT_TRIANGLE];
     const bool persistent =
         useWavefront && d->persistent != 2u && !s->generic &&
         ((triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr) ||
          (triAcc.kind == MRT_REGULAR_GRID && triAcc.count > 0 && triAcc.cell_off != nullptr));
     // The persistent megakernel: megakernel frames the wavefront could run,
     // over the combined walk tree.
-    const bool megaP = !useWavefront && wfEligible && d->persistent != 2u && !s->generic &&
-                       triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr;
+    const bool megaP = !useWavefront && megaEligible;
     const uint32_t megaBlocks = megaP ? mega_blocks(s->ctx->num_cu ? s->ctx->num_cu : 256u) : 0u;
     const uint32_t blocks = useWavefront ? traceBlocks : megaP ? megaBlocks : render_pass_blocks(p);
     // Wavefront slot groups run concurrently, each with its own spill region;
@@ -1785,9 +1792,17 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         a.service_lanes = tune.service_lanes ? tune.service_lanes : kDefaultServiceLanes;
         a.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultRefillLanes;
         a.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultFetchChunk;
+        // Path state between walks in device memory rather than registers:
+        // the kernel needs about 160 registers either way (3 waves per SIMD),
+        // and this way measured 857 against 836 Mrays/s on the 1M-triangle
+        // 1080p frame.
+        if ((rc = s->mega_state.alloc(7ull * sizeof(float4) * megaBlocks * 256u)) != MRT_OK) return rc;
+        a.state = s->mega_state.as<float4>();
+        // ms_kernel: the megakernel alone (the resolve follows it).
         MRT_HIP(hipEventRecord(k0, st));
         MRT_HIP(launch_mega_p(s->dev, d->instrument != 0u, a, megaBlocks, sc, s->counters.as<unsigned long long>(),
                               s->err.as<uint32_t>(), st));
+        MRT_HIP(hipEventRecord(k1, st));
         if (rad) {
             WfBuffers wb{};
             wb.rad = rad;
@@ -1799,7 +1814,6 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
                                       1, st));
             MRT_HIP(launch_wavefront_resolve(p, d->samples_pixel, wb, dbm, dacc, doneMin, st));
         }
-        MRT_HIP(hipEventRecord(k1, st));
         MRT_HIP(hipEventSynchronize(k1));
         MRT_HIP(hipEventElapsedTime(&kernel_ms, k0, k1));
         s->progress.store(d->samples_pixel);
@@ -1866,7 +1880,8 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         stats->trace_lane_slots = cnt[11];
         stats->ms_trace_busy = trace_busy;
         stats->trace_groups = useWavefront ? static_cast<uint32_t>(wbs.size()) : 0u;
-        stats->slot_lanes = useWavefront ? lanes : 0u;
+        stats->slot_lanes = useWavefront || megaP ? lanes : 0u;
+        stats->persistent_megakernel = megaP ? 1u : 0u;
     }
     if (stopped) return fail(MRT_E_STOPPED, "render stopped");
     return MRT_OK;

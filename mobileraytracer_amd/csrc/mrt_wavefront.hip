@@ -609,23 +609,179 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_grid_p(SceneDev s, const fl
 
 enum : uint32_t { kMegaIdle = 0, kMegaExt = 1, kMegaShadow = 2 };
 
-template <bool COUNT>
+// A lane's item, path and pending rays between its walks.
+struct MegaState {
+    f3 T, accAll, accPrefix;  // path state (sA, sB, sC of the wavefront)
+    f3 vO, shD, extD, contrib;  // the shaded vertex's shadow and extension rays
+    float shDist;
+    uint32_t vExcl, ps, x, y, pid, sample, depth;
+    bool found, ends, ldPos, pendExt;
+};
+
+// MEM: the state lives in device memory (7 float4 per lane) between
+// services instead of registers.
+__device__ __forceinline__ void mega_store(float4* q, uint32_t stride, const MegaState& m) {
+    const uint32_t fl = (m.found ? 1u : 0u) | (m.ends ? 2u : 0u) | (m.ldPos ? 4u : 0u) | (m.pendExt ? 8u : 0u) |
+                        (m.depth << 8);
+    q[0] = make_float4(m.T.x, m.T.y, m.T.z, __uint_as_float(fl));
+    q[stride] = make_float4(m.accAll.x, m.accAll.y, m.accAll.z, __uint_as_float(m.pid));
+    q[2u * stride] = make_float4(m.accPrefix.x, m.accPrefix.y, m.accPrefix.z, __uint_as_float(m.sample));
+    q[3u * stride] = make_float4(m.contrib.x, m.contrib.y, m.contrib.z, m.shDist);
+    q[4u * stride] = make_float4(m.shD.x, m.shD.y, m.shD.z, __uint_as_float(m.vExcl));
+    q[5u * stride] = make_float4(m.extD.x, m.extD.y, m.extD.z, __uint_as_float(m.ps));
+    q[6u * stride] = make_float4(m.vO.x, m.vO.y, m.vO.z, __uint_as_float(m.x | (m.y << 16)));
+}
+__device__ __forceinline__ void mega_load(const float4* q, uint32_t stride, MegaState& m) {
+    const float4 a = q[0], b = q[stride], c = q[2u * stride], d = q[3u * stride];
+    const float4 e = q[4u * stride], f = q[5u * stride], g = q[6u * stride];
+    const uint32_t fl = __float_as_uint(a.w);
+    m.T = f3{a.x, a.y, a.z};
+    m.found = fl & 1u;
+    m.ends = fl & 2u;
+    m.ldPos = fl & 4u;
+    m.pendExt = fl & 8u;
+    m.depth = fl >> 8;
+    m.accAll = f3{b.x, b.y, b.z};
+    m.pid = __float_as_uint(b.w);
+    m.accPrefix = f3{c.x, c.y, c.z};
+    m.sample = __float_as_uint(c.w);
+    m.contrib = f3{d.x, d.y, d.z};
+    m.shDist = d.w;
+    m.shD = f3{e.x, e.y, e.z};
+    m.vExcl = __float_as_uint(e.w);
+    m.extD = f3{f.x, f.y, f.z};
+    m.ps = __float_as_uint(f.w);
+    m.vO = f3{g.x, g.y, g.z};
+    const uint32_t xy = __float_as_uint(g.w);
+    m.x = xy & 0xFFFFu;
+    m.y = xy >> 16;
+}
+
+__device__ __forceinline__ void mega_walk(Walk& w, bool& finite, uint32_t& mode, bool& occluded, f3 o, f3 d,
+                                          uint32_t excl, bool shadowRay, float dist, Stack& st) {
+    w.r = make_ray(o, d, excl);
+    finite = inv_finite(w.r);
+    walk_begin(w, shadowRay ? kWalkAnyRoot : kWalkClosestRoot, shadowRay ? dist : kRayLengthMax, st);
+    mode = shadowRay ? kMegaShadow : kMegaExt;
+    occluded = false;
+}
+
+__device__ __forceinline__ void mega_start_sample(const RenderParams& p, MegaState& m, Walk& w, bool& finite,
+                                                  uint32_t& mode, bool& occluded, Stack& st) {
+    f3 o, d;
+    primary_ray(p, m.x, m.y, m.pid, m.sample, &o, &d);
+    m.T = f3{1.0f, 1.0f, 1.0f};
+    m.accAll = m.accPrefix = f3{0.0f, 0.0f, 0.0f};
+    m.found = m.ends = false;
+    m.depth = 1u;
+    mega_walk(w, finite, mode, occluded, o, d, MRT_ID_NONE, false, 0.0f, st);
+}
+
+// The service of a lane whose walk ended: applies a light sample and starts
+// the pending extension walk, or shades the vertex the extension (or camera)
+// ray reached (PathTracer::shade, PathTracer.cpp:25-133, as k_wf_shade runs
+// it) and starts its shadow or extension walk, or finishes the sample
+// (k_wf_regen's result selection and running mean, Renderer.cpp:101-104)
+// and starts the lane's next one.
+__device__ __forceinline__ void mega_service(const SceneDev& s, const MegaArgs& A, MegaState& m, Walk& w,
+                                             bool& finite, uint32_t& mode, bool& occluded, Stack& st, RayCounts& rc,
+                                             uint32_t& finished) {
+    const RenderParams& p = A.p;
+    bool finish = false;
+    if (mode == kMegaShadow) {
+        if (!occluded) {
+            m.accAll = add(m.accAll, m.contrib);
+            if (!m.found && m.ldPos) {
+                m.found = true;
+                m.accPrefix = m.accAll;
+            }
+        }
+        if (m.pendExt) mega_walk(w, finite, mode, occluded, m.vO, m.extD, m.vExcl, false, 0.0f, st);
+        else finish = true;
+    } else {
+        if (m.depth == 1u) ++rc.camera;
+        else ++rc.extension;
+        bool wantShadow = false;
+        m.pendExt = false;
+        if (w.h.id != MRT_ID_NONE) {
+            const Surf sf = surface(s, w.r, w.h);
+            const Shading sh = shading_of(s, sf);
+            const DrawCtx dc{m.pid, m.sample, 0u};
+            const int32_t depth = static_cast<int32_t>(m.depth);
+            if (sh.emitter) {
+                m.accAll = add(m.accAll, mul(m.T, sh.le));
+                m.ends = true;
+            } else if (any_gt0(sh.kd)) {
+                if (s.n_lights > 0u) {
+                    f3 lc;
+                    if (light_sample(s, p, sf, depth, dc, 0u, &m.shD, &m.shDist, &lc)) {
+                        const f3 Ld = divs(mul(lc, sh.kd), static_cast<float>(p.samples_light));
+                        m.contrib = mul(m.T, Ld);
+                        m.ldPos = any_gt0(Ld);
+                        wantShadow = true;
+                    }
+                }
+                bool cont = depth <= p.depth_min;
+                if (!cont) cont = draw_at(p, p.s_rr, dc, kSlotRR, 0, depth) > 0.5f;
+                if (cont && depth + 1 <= p.depth_max) {
+                    m.extD = cosine_hemisphere(p, sf.normal, dc, depth);
+                    f3 Tn = mul(m.T, sh.kd);
+                    if (depth > p.depth_min) Tn = divs(Tn, 0.5f * 0.5f);
+                    m.T = Tn;
+                    ++m.depth;
+                    m.pendExt = true;
+                }
+                m.vO = sf.point;
+                m.vExcl = sf.excl;
+            }
+        }
+        if (wantShadow) {
+            ++rc.shadow;
+            mega_walk(w, finite, mode, occluded, m.vO, m.shD, m.vExcl, true, m.shDist, st);
+        } else if (m.pendExt) {
+            mega_walk(w, finite, mode, occluded, m.vO, m.extD, m.vExcl, false, 0.0f, st);
+        } else {
+            finish = true;
+        }
+    }
+    if (finish) {
+        const f3 c = (m.ends && m.found) ? m.accPrefix : m.accAll;
+        if (A.rad) {
+            float* r = A.rad + 3ull * ((m.sample - A.first) * static_cast<uint64_t>(A.pixel_slots) + m.ps);
+            r[0] = c.x;
+            r[1] = c.y;
+            r[2] = c.z;
+        } else {
+            const uint32_t bi = m.y * p.real_width + m.x;
+            A.bitmap[bi] = incremental_avg(c, A.bitmap[bi], m.sample + 1u);
+            if (A.accum) {
+                A.accum[3u * m.pid + 0u] += c.x;
+                A.accum[3u * m.pid + 1u] += c.y;
+                A.accum[3u * m.pid + 2u] += c.z;
+            }
+        }
+        ++finished;
+        m.sample += A.lanes;
+        if (m.sample < A.spp) {
+            mega_start_sample(p, m, w, finite, mode, occluded, st);
+        } else {
+            mode = kMegaIdle;
+            walk_clear(w);
+        }
+    }
+}
+
+template <bool COUNT, bool MEM>
 __global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, StackCfg sc, unsigned long long* counters,
                                                    uint32_t* err) {
     __shared__ uint32_t lds[kStackLds * kBlock];
-    const RenderParams& p = A.p;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lanesAll = gridDim.x * kBlock;
     Stack st = make_stack(lds, sc, gtid);
     RayCounts rc{};
     const uint32_t nItems = A.lanes * A.pixel_slots;
-    uint32_t ps = 0, x = 0, y = 0, pid = 0, sample = 0;  // the lane's item and current sample
-    f3 T{}, accAll{}, accPrefix{};                       // path state (sA, sB, sC of the wavefront)
-    bool found = false, ends = false;
-    uint32_t depth = 0;
-    f3 vO{}, shD{}, extD{}, contrib{};                   // the shaded vertex's shadow and extension rays
-    uint32_t vExcl = MRT_ID_NONE;
-    float shDist = 0.0f;
-    bool ldPos = false, pendExt = false;
+    MegaState m{};
+    float4* mq = MEM ? A.state + gtid : nullptr;
     Walk w;
     walk_clear(w);
     w.h.t = 0.0f;
@@ -633,114 +789,16 @@ __global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, Stack
     bool finite = true, occluded = false, exhausted = false;
     uint32_t finished = 0;
     FetchRun run{0u, 0u, false};
-    auto start_walk = [&](f3 o, f3 d, uint32_t excl, bool shadowRay, float dist) {
-        w.r = make_ray(o, d, excl);
-        finite = inv_finite(w.r);
-        walk_begin(w, shadowRay ? kWalkAnyRoot : kWalkClosestRoot, shadowRay ? dist : kRayLengthMax, st);
-        mode = shadowRay ? kMegaShadow : kMegaExt;
-        occluded = false;
-    };
-    auto start_sample = [&]() {
-        f3 o, d;
-        primary_ray(p, x, y, pid, sample, &o, &d);
-        T = f3{1.0f, 1.0f, 1.0f};
-        accAll = accPrefix = f3{0.0f, 0.0f, 0.0f};
-        found = ends = false;
-        depth = 1u;
-        start_walk(o, d, MRT_ID_NONE, false, 0.0f);
-    };
     while (true) {
         const bool walking = mode != kMegaIdle && !walk_done(w);
         const bool needs = (mode != kMegaIdle && !walking) || (mode == kMegaIdle && !exhausted);
         if (__ballot(walking || needs) == 0ull) break;
         const unsigned long long svc = __ballot(needs);
         if (svc != 0ull && (static_cast<uint32_t>(__popcll(svc)) >= A.service_lanes || __ballot(walking) == 0ull)) {
-            bool finish = false;
-            if (needs && mode == kMegaShadow) {
-                // The light sample's shadow ray is done (k_wf_trace_p's
-                // apply_light), then the vertex's extension ray, if any.
-                if (!occluded) {
-                    accAll = add(accAll, contrib);
-                    if (!found && ldPos) {
-                        found = true;
-                        accPrefix = accAll;
-                    }
-                }
-                if (pendExt) start_walk(vO, extD, vExcl, false, 0.0f);
-                else finish = true;
-            } else if (needs && mode == kMegaExt) {
-                // PathTracer::shade of the vertex the ray reached
-                // (PathTracer.cpp:25-133), as k_wf_shade runs it.
-                if (depth == 1u) ++rc.camera;
-                else ++rc.extension;
-                bool wantShadow = false;
-                pendExt = false;
-                if (w.h.id != MRT_ID_NONE) {
-                    const Surf sf = surface(s, w.r, w.h);
-                    const Shading m = shading_of(s, sf);
-                    const DrawCtx dc{pid, sample, 0u};
-                    if (m.emitter) {
-                        accAll = add(accAll, mul(T, m.le));
-                        ends = true;
-                    } else if (any_gt0(m.kd)) {
-                        if (s.n_lights > 0u) {
-                            f3 lc;
-                            if (light_sample(s, p, sf, static_cast<int32_t>(depth), dc, 0u, &shD, &shDist, &lc)) {
-                                const f3 Ld = divs(mul(lc, m.kd), static_cast<float>(p.samples_light));
-                                contrib = mul(T, Ld);
-                                ldPos = any_gt0(Ld);
-                                wantShadow = true;
-                            }
-                        }
-                        bool cont = static_cast<int32_t>(depth) <= p.depth_min;
-                        if (!cont) cont = draw_at(p, p.s_rr, dc, kSlotRR, 0, static_cast<int32_t>(depth)) > 0.5f;
-                        if (cont && static_cast<int32_t>(depth) + 1 <= p.depth_max) {
-                            extD = cosine_hemisphere(p, sf.normal, dc, static_cast<int32_t>(depth));
-                            f3 Tn = mul(T, m.kd);
-                            if (static_cast<int32_t>(depth) > p.depth_min) Tn = divs(Tn, 0.5f * 0.5f);
-                            T = Tn;
-                            ++depth;
-                            pendExt = true;
-                        }
-                        vO = sf.point;
-                        vExcl = sf.excl;
-                    }
-                }
-                if (wantShadow) {
-                    ++rc.shadow;
-                    start_walk(vO, shD, vExcl, true, shDist);
-                } else if (pendExt) {
-                    start_walk(vO, extD, vExcl, false, 0.0f);
-                } else {
-                    finish = true;
-                }
-            }
-            if (finish) {
-                // The sample is done (k_wf_regen's result selection and
-                // running mean, Renderer.cpp:101-104).
-                const f3 c = (ends && found) ? accPrefix : accAll;
-                if (A.rad) {
-                    float* r = A.rad + 3ull * ((sample - A.first) * static_cast<uint64_t>(A.pixel_slots) + ps);
-                    r[0] = c.x;
-                    r[1] = c.y;
-                    r[2] = c.z;
-                } else {
-                    const uint32_t bi = y * p.real_width + x;
-                    A.bitmap[bi] = incremental_avg(c, A.bitmap[bi], sample + 1u);
-                    if (A.accum) {
-                        A.accum[3u * pid + 0u] += c.x;
-                        A.accum[3u * pid + 1u] += c.y;
-                        A.accum[3u * pid + 2u] += c.z;
-                    }
-                }
-                ++finished;
-                sample += A.lanes;
-                if (sample < A.spp) {
-                    start_sample();
-                } else {
-                    mode = kMegaIdle;
-                    walk_clear(w);
-                }
+            if (needs && mode != kMegaIdle) {
+                if (MEM) mega_load(mq, lanesAll, m);
+                mega_service(s, A, m, w, finite, mode, occluded, st, rc, finished);
+                if (MEM && mode != kMegaIdle) mega_store(mq, lanesAll, m);
             }
             // New items for idle lanes (every lane of the wave takes part in
             // the fetch's ballot).
@@ -751,12 +809,13 @@ __global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, Stack
                     exhausted = true;
                 } else {
                     const uint32_t k = e / A.pixel_slots;
-                    ps = e - k * A.pixel_slots;
-                    slot_pixel(p, ps, &x, &y);
-                    sample = A.first + k;
-                    if (pixel_covered(p, x, y) && sample < A.spp) {
-                        pid = y * p.width + x;
-                        start_sample();
+                    m.ps = e - k * A.pixel_slots;
+                    slot_pixel(A.p, m.ps, &m.x, &m.y);
+                    m.sample = A.first + k;
+                    if (pixel_covered(A.p, m.x, m.y) && m.sample < A.spp) {
+                        m.pid = m.y * A.p.width + m.x;
+                        mega_start_sample(A.p, m, w, finite, mode, occluded, st);
+                        if (MEM) mega_store(mq, lanesAll, m);
                     }
                 }
             }
@@ -778,7 +837,7 @@ uint32_t mega_blocks(uint32_t num_cu) {
     static int perCu = 0;
     if (perCu == 0) {
         int a = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_mega_p<false>, kBlock, 0) != hipSuccess) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_mega_p<false, true>, kBlock, 0) != hipSuccess) {
             (void)hipGetLastError();
             a = 2;
         }
@@ -791,8 +850,13 @@ hipError_t launch_mega_p(const SceneDev& s, bool count, const MegaArgs& a, uint3
                          unsigned long long* counters, uint32_t* err, hipStream_t stream) {
     if (a.lanes == 0u || a.pixel_slots == 0u || (a.lanes > 1u) != (a.rad != nullptr) || a.fetch_chunk < 64u)
         return hipErrorInvalidValue;
-    if (count) k_mega_p<true><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
-    else k_mega_p<false><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+    if (a.state) {
+        if (count) k_mega_p<true, true><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+        else k_mega_p<false, true><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+    } else {
+        if (count) k_mega_p<true, false><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+        else k_mega_p<false, false><<<blocks, kBlock, 0, stream>>>(s, a, sc, counters, err);
+    }
     return hipGetLastError();
 }
 

@@ -245,6 +245,9 @@ struct MegaArgs {
     uint32_t* fetch;
     uint32_t* done;
     uint32_t service_lanes, refill_lanes, fetch_chunk;
+    // Optional: 7 float4 per lane of the launch (SoA, stride = lanes) holding
+    // each lane's path state between its walks instead of registers.
+    float4* state;
 };
 hipError_t launch_mega_p(const SceneDev& s, bool count, const MegaArgs& a, uint32_t blocks, const StackCfg& sc,
                          unsigned long long* counters, uint32_t* err, hipStream_t stream);

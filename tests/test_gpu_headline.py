@@ -2,12 +2,13 @@
 (990,000 diffuse triangles, 10,000 area lights), 1920x1080, PathTracer at 16
 samples per pixel, depth 6 with Russian roulette from depth 3, per-pixel
 Halton jitter, rendered exactly as the benchmark renders it (AUTO kernel:
-the persistent wavefront with three slot groups, 8 path slots per pixel, walk
-trees written on the device by the GPU BVH build, the light BVH over the 10k
-lights), against the oracle's Renderer::renderFrame restatement
+the persistent megakernel, one launch per frame, 8 sample lanes per pixel,
+walk trees written on the device by the GPU BVH build, the light BVH over
+the 10k lights), against the oracle's Renderer::renderFrame restatement
 (Renderer.cpp:67-116, PathTracer.cpp:25-164, BVH.hpp:158-271) on a fixed
-subset of tiles.  The megakernel and the 128^3 RegularGrid
-(RegularGrid.hpp:157-465) run the same frame at 2 samples per pixel.
+subset of tiles.  The persistent wavefront runs the same 16-spp frame; the
+per-pass megakernel and the 128^3 RegularGrid (RegularGrid.hpp:157-465) run
+the frame at 2 samples per pixel.
 
 The oracle renders only the subset's tiles (its tile mask); the GPU renders
 the whole frame as benchmarked, and the subset's pixels are compared:
@@ -41,14 +42,14 @@ TILES_2SPP = (7 * 16 + 8, 11 * 16 + 6)
 ORACLE_THREADS = 16
 
 
-def bench_desc(cam, spp, kernel=abi.KERNEL_AUTO, tiles=None):
+def bench_desc(cam, spp, kernel=abi.KERNEL_AUTO, tiles=None, persistent=0):
     """The benchmark's descriptor (bench.py make_desc)."""
     return mrt.render_desc(abi.SHADER_PATHTRACER, W, H, spp, cam, kernel=kernel,
                            pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, 5),
                            rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 6),
                            light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 7),
                            hemi=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, 8),
-                           depth_min=2, depth_max=6, tiles=tiles)
+                           depth_min=2, depth_max=6, tiles=tiles, persistent=persistent)
 
 
 def tile_pixels(tiles):
@@ -75,9 +76,31 @@ def soup():
     return data
 
 
-def check_against_oracle(ctx, data, accel, desc_full, desc_sub, tiles, lanes, cam_min=None, cam_max=None):
-    """lanes: path slots per pixel the wavefront frame must run with (its
-    three slot groups included), 0 for a megakernel frame."""
+# Oracle renders of a tile subset, shared by the kernels that render the
+# same frame: (accel, samples per pixel, tiles) -> (bitmap, accum, counts).
+_ORACLE = {}
+
+
+def oracle_render(data, accel, desc_full, tiles, cmin, cmax):
+    key = (accel, desc_full.samples_pixel, tuple(tiles))
+    if key not in _ORACLE:
+        t0 = time.time()
+        orc = oracle_scene(data, accel, cmin, cmax)
+        t1 = time.time()
+        acc_o = np.zeros((H, W, 3), np.float64)
+        bm_o, cnt, secs = orc.render(desc_full, accum=acc_o, threads=ORACLE_THREADS,
+                                     tile_mask=claim_mask(desc_full, tiles))
+        print(f"oracle: build {t1 - t0:.1f} s, render {secs:.1f} s: {int(cnt[0])} camera {int(cnt[1])} extension "
+              f"{int(cnt[2])} shadow rays", flush=True)
+        _ORACLE[key] = (bm_o, acc_o, cnt)
+    return _ORACLE[key]
+
+
+def check_against_oracle(ctx, data, accel, desc_full, desc_sub, tiles, shape, lanes, cam_min=None, cam_max=None):
+    """shape: the kernel the frame must run on: "wavefront" (three slot
+    groups, `lanes` path slots per pixel), "persistent" (the persistent
+    megakernel, `lanes` sample lanes per pixel) or "pass" (the megakernel
+    per sample pass)."""
     cmin = cam_min if cam_min is not None else (0, 0, 0)
     cmax = cam_max if cam_max is not None else (0, 0, 0)
     gpu = mrt.Scene(ctx, data)
@@ -90,15 +113,19 @@ def check_against_oracle(ctx, data, accel, desc_full, desc_sub, tiles, lanes, ca
         bm_full, st_full = gpu.render(desc_full, accum=acc_full)
         print(f"GPU whole frame: {st_full.rays_camera} camera {st_full.rays_extension} extension "
               f"{st_full.rays_shadow} shadow rays, {st_full.ms_render:.1f} ms, {st_full.trace_groups} slot groups, "
-              f"{st_full.slot_lanes} slots per pixel", flush=True)
+              f"{st_full.slot_lanes} slots per pixel, persistent megakernel {st_full.persistent_megakernel}",
+              flush=True)
         acc_sub = np.zeros((H, W, 3), np.float32)
         bm_sub, st_sub = gpu.render(desc_sub, accum=acc_sub)
     finally:
         gpu.close()
-    if lanes:
+    if shape == "wavefront":
         assert st_full.trace_launches > 0 and st_full.trace_groups == 3 and st_full.slot_lanes == lanes
+        assert st_full.persistent_megakernel == 0
+    elif shape == "persistent":
+        assert st_full.trace_launches == 0 and st_full.persistent_megakernel == 1 and st_full.slot_lanes == lanes
     else:
-        assert st_full.trace_launches == 0
+        assert st_full.trace_launches == 0 and st_full.persistent_megakernel == 0
     sel = tile_pixels(tiles)
     assert st_full.covered_pixels == 256 * BSX * BSY
     assert st_sub.covered_pixels == len(tiles) * BSX * BSY
@@ -109,14 +136,7 @@ def check_against_oracle(ctx, data, accel, desc_full, desc_sub, tiles, lanes, ca
     assert np.array_equal(acc_sub[sel].view(np.uint32), acc_full[sel].view(np.uint32))
     assert not bm_sub[~sel].any() and not acc_sub[~sel].any()
 
-    t0 = time.time()
-    orc = oracle_scene(data, accel, cmin, cmax)
-    t1 = time.time()
-    acc_o = np.zeros((H, W, 3), np.float64)
-    bm_o, cnt, secs = orc.render(desc_full, accum=acc_o, threads=ORACLE_THREADS,
-                                 tile_mask=claim_mask(desc_full, tiles))
-    print(f"oracle: build {t1 - t0:.1f} s, render {secs:.1f} s: {int(cnt[0])} camera {int(cnt[1])} extension "
-          f"{int(cnt[2])} shadow rays", flush=True)
+    bm_o, acc_o, cnt = oracle_render(data, accel, desc_full, tiles, cmin, cmax)
     assert (st_sub.rays_camera, st_sub.rays_extension, st_sub.rays_shadow, st_sub.rays_specular) == \
         tuple(int(c) for c in cnt), "ray counts of the tile subset"
     assert int(cnt[0]) == len(tiles) * BSX * BSY * desc_full.samples_pixel
@@ -128,27 +148,38 @@ def check_against_oracle(ctx, data, accel, desc_full, desc_sub, tiles, lanes, ca
     assert d8.max() <= 1, f"bitmap differs by {d8.max()}"
 
 
-def test_headline_frame_bvh_wavefront(gpu_ctx, soup):
-    """The benchmarked frame: 16 spp, AUTO kernel (persistent wavefront, 3
-    slot groups, 8 slots per pixel), GPU-built trees, 10k-light tree."""
+def test_headline_frame_bvh(gpu_ctx, soup):
+    """The benchmarked frame: 16 spp, AUTO kernel (the persistent
+    megakernel, 8 sample lanes per pixel), GPU-built trees, 10k-light tree."""
     cam = soup_camera(W, H)
     check_against_oracle(gpu_ctx, soup, abi.BVH, bench_desc(cam, 16), bench_desc(cam, 16, tiles=TILES_16SPP),
-                         TILES_16SPP, lanes=8)
+                         TILES_16SPP, "persistent", lanes=8)
 
 
-def test_headline_frame_bvh_megakernel(gpu_ctx, soup):
+def test_headline_frame_bvh_wavefront(gpu_ctx, soup):
+    """The same frame on the persistent wavefront (3 slot groups, 8 slots per
+    pixel)."""
+    cam = soup_camera(W, H)
+    k = abi.KERNEL_WAVEFRONT
+    check_against_oracle(gpu_ctx, soup, abi.BVH, bench_desc(cam, 16, k), bench_desc(cam, 16, k, tiles=TILES_16SPP),
+                         TILES_16SPP, "wavefront", lanes=8)
+
+
+def test_headline_frame_bvh_megakernel_per_pass(gpu_ctx, soup):
     cam = soup_camera(W, H)
     k = abi.KERNEL_MEGAKERNEL
-    check_against_oracle(gpu_ctx, soup, abi.BVH, bench_desc(cam, 2, k), bench_desc(cam, 2, k, tiles=TILES_2SPP),
-                         TILES_2SPP, lanes=0)
+    check_against_oracle(gpu_ctx, soup, abi.BVH, bench_desc(cam, 2, k, persistent=2),
+                         bench_desc(cam, 2, k, tiles=TILES_2SPP, persistent=2), TILES_2SPP, "pass", lanes=0)
 
 
 @pytest.mark.parametrize("kernel", [abi.KERNEL_AUTO, abi.KERNEL_MEGAKERNEL])
 def test_headline_frame_regular_grid(gpu_ctx, soup, kernel):
     """RegularGrid 128^3 over the 990k triangles (bounds joined with the
-    camera's, Shader.cpp:52-62)."""
+    camera's, Shader.cpp:52-62): AUTO runs the persistent wavefront's grid
+    walk, MEGAKERNEL the megakernel per sample pass."""
     cam = soup_camera(W, H)
     cmin, cmax = mrt.camera_aabb(cam)
+    auto = kernel == abi.KERNEL_AUTO
     check_against_oracle(gpu_ctx, soup, abi.REGULAR_GRID, bench_desc(cam, 2, kernel),
                          bench_desc(cam, 2, kernel, tiles=TILES_2SPP), TILES_2SPP,
-                         lanes=2 if kernel == abi.KERNEL_AUTO else 0, cam_min=cmin, cam_max=cmax)
+                         "wavefront" if auto else "pass", lanes=2 if auto else 0, cam_min=cmin, cam_max=cmax)
