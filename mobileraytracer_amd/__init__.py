@@ -173,7 +173,8 @@ class Context:
 
     def set_tuning(self, **knobs) -> None:
         """Launch-shape knobs (mrt_tuning): refill_lanes, fetch_chunk,
-        launch_share_pct, max_slot_lanes, max_groups; 0 keeps a default."""
+        launch_share_pct, max_slot_lanes, max_groups, overlap_loads,
+        graph_loop, service_lanes; 0 (or omitted) keeps a default."""
         t = abi.Tuning()
         for k, v in knobs.items():
             setattr(t, k, int(v))
@@ -228,6 +229,7 @@ class Scene:
         p.leaf_size = leaf_size
         p.builder = builder
         check(lib().mrt_scene_build(self.h, accelerator, C.byref(p)), "mrt_scene_build")
+        self.accel = accelerator
 
     def footprint(self) -> int:
         v = C.c_uint64()

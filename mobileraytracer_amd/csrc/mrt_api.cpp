@@ -602,7 +602,8 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
     hipGraphExec_t& exec = s->wf_exec;
     bool graphStop = false;
     uint64_t graphIters = 0;
-    std::vector<hipEvent_t> gev(useGraph ? G + 1 : 0, nullptr);
+    // gev: the captured fork and joins; gev[G + g]: group g's start work.
+    std::vector<hipEvent_t> gev(useGraph ? 2 * G : 0, nullptr);
     struct EventGuard {
         std::vector<hipEvent_t>& e;
         ~EventGuard() {
@@ -611,6 +612,16 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
         }
     } graphGuard{gev};
     for (hipEvent_t& e : gev) MRT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // The graph runs every group's iterations from the first stream, so that
+    // stream waits for the other groups' counter clears and start kernels
+    // (issued on their own streams above): a captured graph carries no
+    // dependency on work queued on a stream before the capture forked it.
+    if (useGraph) {
+        for (uint32_t g = 1; g < G; ++g) {
+            MRT_HIP(hipEventRecord(gev[G + g], st[g]));
+            MRT_HIP(hipStreamWaitEvent(st[0], gev[G + g], 0));
+        }
+    }
     auto graph_key = [&](bool stopFlag) {
         std::vector<uint8_t> k;
         auto put = [&](const void* v, size_t n) {

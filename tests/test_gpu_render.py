@@ -330,12 +330,16 @@ def test_render_with_aovs(gpu_ctx, kernel, spp, first):
     assert 0.05 < (aov2[..., 6] > 0).mean() < 0.95  # the sparse soup: hits and misses
 
 
-@pytest.mark.parametrize("size,spp,generic", [((128, 80), 3, False), ((512, 288), 2, False), ((96, 96), 4, True)])
+@pytest.mark.parametrize("size,spp,generic", [((128, 80), 3, False), ((512, 288), 2, False), ((96, 96), 4, True),
+                                              ((1280, 720), 16, False)])
 def test_graph_loop_equals_launched_loop(size, spp, generic):
     """The wavefront loop replayed as a captured HIP graph (batches of four
     iterations of every slot group) gives the frame the launched loop gives:
     bitmap, FP32 radiance and ray counts identical; 512x288 runs three slot
-    groups forked into the captured batch."""
+    groups forked into the captured batch; 1280x720 at 16 samples runs three
+    groups with two samples per path slot, after uploads and a 177 MB clear
+    that the other groups' start kernels wait for (the graph, launched on the
+    first group's stream, must wait for them too)."""
     W, H = size
     data = mrt.soup_scene(9, 80000, 800)
     if generic:
@@ -360,10 +364,12 @@ def test_graph_loop_equals_launched_loop(size, spp, generic):
         finally:
             sc.close()
             ctx.close()
-        res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow), st.trace_groups))
+        res.append((bm, acc, (st.rays_camera, st.rays_extension, st.rays_shadow), st.trace_groups, st.slot_lanes))
         assert np.array_equal(bm, bm2)
-    (b1, a1, c1, g1), (b2, a2, c2, g2) = res
-    print(f"graph loop {size} {spp} spp: rays {c1} / {c2}, {g1} groups", flush=True)
+    (b1, a1, c1, g1, l1), (b2, a2, c2, g2, l2) = res
+    print(f"graph loop {size} {spp} spp: rays {c1} / {c2}, {g1} groups, {l1} slots per pixel", flush=True)
+    if size == (1280, 720):
+        assert g1 == 3 and l1 * 2 == spp
     assert c1 == c2 and g1 == g2
     assert np.array_equal(b1, b2)
     assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32))
