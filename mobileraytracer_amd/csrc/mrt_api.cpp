@@ -1802,7 +1802,7 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         a.done = ctl + 16;
         a.service_lanes = tune.service_lanes ? tune.service_lanes : kDefaultServiceLanes;
         a.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultRefillLanes;
-        a.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultFetchChunk;
+        a.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultMegaFetchChunk;
         // Path state between walks in device memory rather than registers:
         // the kernel needs about 160 registers either way (3 waves per SIMD),
         // and this way measured 857 against 836 Mrays/s on the 1M-triangle

@@ -772,7 +772,10 @@ __device__ __forceinline__ void mega_service(const SceneDev& s, const MegaArgs& 
 }
 
 template <bool COUNT, bool MEM>
-__global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, StackCfg sc, unsigned long long* counters,
+// Four waves per SIMD (at most 128 VGPRs): the walk waits on dependent node
+// loads, and a fourth wave hides more of that wait than the spills of the
+// shading step cost (1M-triangle 1080p 16-spp frame: 131 -> 122 ms).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_mega_p(SceneDev s, MegaArgs A, StackCfg sc, unsigned long long* counters,
                                                    uint32_t* err) {
     __shared__ uint32_t lds[kStackLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -796,9 +799,14 @@ __global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, Stack
         const unsigned long long svc = __ballot(needs);
         if (svc != 0ull && (static_cast<uint32_t>(__popcll(svc)) >= A.service_lanes || __ballot(walking) == 0ull)) {
             if (needs && mode != kMegaIdle) {
-                if (MEM) mega_load(mq, lanesAll, m);
-                mega_service(s, A, m, w, finite, mode, occluded, st, rc, finished);
-                if (MEM && mode != kMegaIdle) mega_store(mq, lanesAll, m);
+                if (MEM) {
+                    MegaState ml;
+                    mega_load(mq, lanesAll, ml);
+                    mega_service(s, A, ml, w, finite, mode, occluded, st, rc, finished);
+                    if (mode != kMegaIdle) mega_store(mq, lanesAll, ml);
+                } else {
+                    mega_service(s, A, m, w, finite, mode, occluded, st, rc, finished);
+                }
             }
             // New items for idle lanes (every lane of the wave takes part in
             // the fetch's ballot).
@@ -808,14 +816,16 @@ __global__ __launch_bounds__(kBlock) void k_mega_p(SceneDev s, MegaArgs A, Stack
                 if (e == kDeadEntry) {
                     exhausted = true;
                 } else {
+                    MegaState mf{};
+                    MegaState& mm = MEM ? mf : m;
                     const uint32_t k = e / A.pixel_slots;
-                    m.ps = e - k * A.pixel_slots;
-                    slot_pixel(A.p, m.ps, &m.x, &m.y);
-                    m.sample = A.first + k;
-                    if (pixel_covered(A.p, m.x, m.y) && m.sample < A.spp) {
-                        m.pid = m.y * A.p.width + m.x;
-                        mega_start_sample(A.p, m, w, finite, mode, occluded, st);
-                        if (MEM) mega_store(mq, lanesAll, m);
+                    mm.ps = e - k * A.pixel_slots;
+                    slot_pixel(A.p, mm.ps, &mm.x, &mm.y);
+                    mm.sample = A.first + k;
+                    if (pixel_covered(A.p, mm.x, mm.y) && mm.sample < A.spp) {
+                        mm.pid = mm.y * A.p.width + mm.x;
+                        mega_start_sample(A.p, mm, w, finite, mode, occluded, st);
+                        if (MEM) mega_store(mq, lanesAll, mm);
                     }
                 }
             }

@@ -48,9 +48,9 @@ PER_PASS = "per_pass"
 SEARCH_SPACE: dict[str, dict[str, tuple[int, ...]]] = {
     MEGAKERNEL: {
         "max_slot_lanes": (8, 4, 16),
-        "service_lanes": (16, 8, 24, 32),
+        "service_lanes": (32, 16, 24, 40),
         "refill_lanes": (32, 16, 48),
-        "fetch_chunk": (128, 64, 256),
+        "fetch_chunk": (64, 128, 256),
     },
     WAVEFRONT: {
         "max_slot_lanes": (8, 4, 16),
