@@ -380,36 +380,33 @@ def main() -> int:
     elapsed = time.perf_counter() - t_start
     mrays = rays / elapsed / 1e6
 
-    # Roofline of the dominant kernel.  Persistent megakernel (the AUTO
-    # kernel of the BVH frame): one launch per frame, timed with HIP events
-    # around it on the render stream; its bytes are the rays' traversal bytes
-    # plus 256 bytes of shading per closest-hit ray (hit triangle's shading
-    # record, material, light sample, path state).  These algorithmic bytes
-    # count every node and triangle a ray touches; the frame's scene (about
-    # 250 MB) stays in the 256 MB Infinity Cache and the L2s, so they can
-    # exceed what HBM could deliver (frac > 1), and traffic_frac gives the
-    # bytes the counters saw leave L2.  Wavefront: the trace
-    # kernel, timed with HIP events around each of its launches.
-    # Algorithmic bytes per ray (BVH): the oracle's ordered traversal of the
-    # same reference tree over a 1/256 pixel subsample of the frame
-    # (ordered_traffic), independent of this implementation; beside it the
-    # GPU's own visits: 128 bytes per wide node of the persistent walk (64 per
-    # node pair otherwise) + 48 per triangle test (three float4 of tri_geo) +
-    # 48 of ray record in and result out.
-    # Megakernel: the render-pass kernel, one launch per sample pass, plus 256
-    # bytes of shading per closest-hit ray.  Slot groups run their trace
-    # launches concurrently on separate streams, so a launch's duration
-    # includes time shared with the other groups' launches; device_achieved
-    # divides all trace bytes by the time at least one trace launch ran.
+    # Roofline of the dominant kernel.  Algorithmic bytes per ray (BVH): the
+    # oracle's ordered traversal of the same reference tree over a 1/256 pixel
+    # subsample of the frame (ordered_traffic), independent of this
+    # implementation; beside it the GPU's own visits over the compressed walk
+    # tree: 64 bytes per node visit + 64 per triangle test (a 128-byte leaf
+    # record holds the exact leaf box and two triangles) + 48 of ray in and
+    # result out.  These bytes count every node and triangle a ray touches;
+    # the frame's scene (about 250 MB) stays in the 256 MB Infinity Cache and
+    # the L2s, so they can exceed what HBM could deliver (frac > 1), and
+    # traffic_frac gives the bytes the counters saw leave L2.
+    # Wavefront (the AUTO kernel of the 16-spp 1080p frame): the persistent
+    # trace kernel, timed with HIP events around each of its launches on the
+    # stream of its slot group.  Slot groups run their trace launches
+    # concurrently on separate streams, so a launch's duration includes time
+    # shared with the other groups' launches; device_achieved divides all
+    # trace bytes by the time at least one trace launch ran, and
+    # frame_achieved all bytes (trace and 256 bytes of shading per closest-hit
+    # ray) by the frame time.
+    # Persistent megakernel: one launch per frame, timed with HIP events
+    # around it, its bytes the rays' traversal bytes plus the shading bytes.
+    # Per-pass megakernel: one launch per sample pass, likewise.
     # RegularGrid scenes: 8 bytes (a cell's two CSR offsets) per DDA step and
     # 52 per cell primitive test (its 4-byte reference and the triangle);
-    # the light tree's pairs and tests as above.  The persistent walk needs
-    # the BVH's walk tree, so grid scenes trace with the one-ray-per-lane
-    # wavefront kernels.
+    # the light tree's pairs and tests as above.
     wavefront = launches > 0
     persistent = (wavefront or mega_p) and not args.no_persistent and args.accel == "bvh"
-    node_bytes = 128 if persistent else 64
-    bytes_per_ray = 48 + node_bytes * pairs_per_ray + 48 * prims_per_ray
+    bytes_per_ray = 48 + 64 * pairs_per_ray + (64 if persistent else 48) * prims_per_ray
     gpu_bytes_per_ray = bytes_per_ray
     ordered = None
     if args.accel == "bvh" and not args.no_oracle_bytes:
@@ -437,6 +434,7 @@ def main() -> int:
     bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
     device_achieved = bytes_total / (busy_ms / 1000.0) / 1e9 if wavefront and busy_ms > 0 else achieved
+    frame_achieved = (rays * bytes_per_ray + closest * 256) / elapsed / 1e9
     traffic = measured_traffic(kernel_name) if persistent else None
     copy_gbs = copy_bandwidth(torch)
 
@@ -469,6 +467,8 @@ def main() -> int:
                      "avg_launch_ms": round(avg_launch_s * 1000.0, 4),
                      "concurrent_groups": groups, "device_achieved": round(device_achieved, 1),
                      "device_frac": round(device_achieved / HBM_PEAK_GBS, 4),
+                     "frame_achieved": round(frame_achieved, 1),
+                     "frame_frac": round(frame_achieved / HBM_PEAK_GBS, 4),
                      "bytes_per_launch": round(bytes_per_launch), "bytes_per_ray": round(bytes_per_ray, 1),
                      "bytes_per_ray_source": "oracle ordered traversal" if ordered else "GPU visit counts",
                      "oracle_ordered": ordered, "gpu_bytes_per_ray": round(gpu_bytes_per_ray, 1),

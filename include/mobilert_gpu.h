@@ -280,7 +280,8 @@ MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx);
  * speed only, never a pixel).  Zero fields keep the defaults. */
 typedef struct mrt_tuning {
     uint32_t refill_lanes;     /* persistent trace: a wave takes new rays once this
-                                  many of its lanes are idle (1-64; 32)        */
+                                  many of its lanes are idle (1-64; 16, the
+                                  persistent megakernel 32)                    */
     uint32_t fetch_chunk;      /* queue entries a wave reserves per atomic
                                   (64-4096; 128, the persistent megakernel 64) */
     uint32_t launch_share_pct; /* workgroups of one slot group's trace launch, %
@@ -301,7 +302,7 @@ typedef struct mrt_tuning {
                                   trace times).  0, 2: launched iterations    */
     uint32_t service_lanes;    /* persistent megakernel: a wave shades once this
                                   many of its lanes finished their walks (1-64;
-                                  32)                                          */
+                                  16)                                          */
 } mrt_tuning;
 MRT_API int mrt_ctx_set_tuning(mrt_ctx* ctx, const mrt_tuning* tuning);
 MRT_API int mrt_ctx_get_tuning(const mrt_ctx* ctx, mrt_tuning* tuning);

@@ -25,7 +25,29 @@ uint64_t buffers_bytes(const mrt_scene* s) {
                  s->mat.bytes + s->light_geo.bytes + s->light_shade.bytes + s->light_info.bytes;
     for (int i = 0; i < 4; ++i) b += s->nodes[i].bytes + s->perm[i].bytes;
     for (int i = 0; i < 3; ++i) b += s->cell_off[i].bytes + s->cell_ref[i].bytes;
-    return b + s->walk_nodes.bytes + s->wide_nodes.bytes;
+    return b + s->walk_nodes.bytes + s->wide_nodes.bytes + s->cw_nodes.bytes + s->cw_recs.bytes;
+}
+
+// The compressed wide walk tree (mrt_cwt.hpp) of the scene's n_wide wide
+// nodes, written on the device from the wide tree and tri_geo.
+int build_compressed_walk(mrt_scene* s, uint32_t n_wide, hipStream_t st) {
+    s->dev.cw_nodes = nullptr;
+    s->dev.cw_recs = nullptr;
+    const uint64_t slots = s->tris.size() + static_cast<uint64_t>(s->n_light_slots);
+    if (n_wide == 0u || slots == 0u) return MRT_OK;
+    const uint64_t bytes = 64ull * n_wide + 128ull * slots;
+    mrt_ctx* ctx = s->ctx;
+    if (ctx->budget && buffers_bytes(s) + bytes > ctx->budget) return MRT_OK;  // the exact wide tree serves
+    int rc;
+    if ((rc = s->cw_nodes.alloc(64ull * n_wide)) != MRT_OK) return rc;
+    if ((rc = s->cw_recs.alloc(128ull * slots)) != MRT_OK) return rc;
+    // Records no leaf starts at stay zero (never read; zero triangles hit nothing).
+    MRT_HIP(hipMemsetAsync(s->cw_recs.p, 0, 128ull * slots, st));
+    MRT_HIP(gpu_compress_walk(s->wide_nodes.as<float4>(), n_wide, s->tri_geo.as<float4>(),
+                              static_cast<uint32_t>(slots), s->cw_nodes.as<uint4>(), s->cw_recs.as<float4>(), st));
+    s->dev.cw_nodes = s->cw_nodes.as<uint4>();
+    s->dev.cw_recs = s->cw_recs.as<float4>();
+    return MRT_OK;
 }
 
 void pack_geo(const mrt_triangle& t, uint32_t id, float4* g) {
@@ -257,6 +279,7 @@ int build_wide_tree(mrt_scene* s, const std::vector<BNode>& wv, const BvhOut& tr
     MRT_HIP(hipStreamSynchronize(st));  // `out` is released on return
     s->dev.wide_nodes = s->wide_nodes.as<float4>();
     s->dev.wide_light_node = lightNode;
+    if ((rc = build_compressed_walk(s, static_cast<uint32_t>(out.size() / 8), st)) != MRT_OK) return rc;
     // A wide visit pushes at most three children.
     s->depth_need = std::max(s->depth_need, 3u * maxDepth + 1u);
     return MRT_OK;
@@ -429,6 +452,8 @@ int build_tri_bvh_resident(mrt_scene* s, uint32_t leaf, hipStream_t st, BvhOut* 
     s->dev.wide_nodes = wide;
     s->dev.wide_light_node = lightNode;
     s->depth_need = std::max(s->depth_need, 3u * maxDepth + 1u);
+    if ((rc = build_compressed_walk(s, static_cast<uint32_t>(nWide / 8), st)) != MRT_OK) return rc;
+    MRT_HIP(hipStreamSynchronize(st));
     return MRT_OK;
 }
 
@@ -1270,6 +1295,8 @@ MRT_API int mrt_scene_build(mrt_scene* s, int accelerator, const mrt_accel_param
     }
     s->dev.walk_nodes = nullptr;
     s->dev.wide_nodes = nullptr;
+    s->dev.cw_nodes = nullptr;
+    s->dev.cw_recs = nullptr;
     if (accelerator == MRT_BVH) {
         for (int c = 0; c < 3; ++c) {
             BvhOut b;
@@ -1625,17 +1652,19 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
     // Kernel shape.  AUTO: the persistent megakernel for PathTracer frames
-    // of BVH scenes of triangles and lights (one launch, paths in registers;
-    // on the 1M-triangle soup it took 4.5 / 5.0 / 10.2 / 19.2 / 132 ms for
-    // 256x256 x 1 and x 4, 640x480 x 4, 1280x720 x 4 and 1920x1080 x 16
-    // samples against 6.5 / 8.6 / 14.2 / 22.6 / 134 ms for the wavefront),
-    // the wavefront for the other frames it can run and for frames with
-    // auxiliary outputs (it records them while shading), else the
-    // megakernel per sample pass.
+    // of BVH scenes of triangles and lights below 2^24 pixel samples (one
+    // launch, no iterations: on the 1M-triangle soup 3.95 / 9.81 / 17.2 /
+    // 33.2 / 59.9 ms for 256x256 x 1, 640x480 x 4, 1280x720 x 4, 1920x1080 x 4
+    // and 1280x720 x 16 samples against 7.49 / 13.9 / 20.2 / 34.3 / 60.7 ms for
+    // the wavefront), the wavefront above (its trace kernel runs 7 waves per
+    // SIMD against the megakernel's 4: 58.3 / 114.0 / 226 ms for 1920x1080 x
+    // 8, 16 and 32 samples against 62.5 / 125.1 / 245 ms), for the other
+    // frames it can run and for frames with auxiliary outputs (it records them
+    // while shading), else the megakernel per sample pass.
     const bool wfEligible = wavefront_eligible(s, d);
     const Accel& triAcc = s->dev.acc[MRT_CAT_TRIANGLE];
     const bool megaEligible = wfEligible && d->persistent != 2u && !s->generic && triAcc.kind == MRT_BVH &&
-                              s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr;
+                              s->dev.walk_nodes != nullptr && s->dev.cw_nodes != nullptr;
     bool useWavefront = false;
     if (d->kernel == MRT_KERNEL_WAVEFRONT) {
         if (!wfEligible)
@@ -1644,7 +1673,10 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
                                            "65535 samples per pixel");
         useWavefront = true;
     } else if (d->kernel == MRT_KERNEL_AUTO) {
-        useWavefront = wfEligible && (!megaEligible || aov != nullptr);
+        const uint64_t pixelSamples =
+            static_cast<uint64_t>(rendered_tiles(p)) * p.bsx * p.bsy *
+            (d->samples_pixel > d->first_sample ? d->samples_pixel - d->first_sample : 0u);
+        useWavefront = wfEligible && (!megaEligible || aov != nullptr || pixelSamples >= kMegaMaxPixelSamples);
     }
     const uint32_t traceBlocks = cap_blocks(s->ctx) / 2u;
     // Persistent refilling trace: over the combined walk tree for BVH scenes,
@@ -1652,7 +1684,7 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     // only either way).
     const bool persistent =
         useWavefront && d->persistent != 2u && !s->generic &&
-        ((triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.wide_nodes != nullptr) ||
+        ((triAcc.kind == MRT_BVH && s->dev.walk_nodes != nullptr && s->dev.cw_nodes != nullptr) ||
          (triAcc.kind == MRT_REGULAR_GRID && triAcc.count > 0 && triAcc.cell_off != nullptr));
     // The persistent megakernel: megakernel frames the wavefront could run,
     // over the combined walk tree.
@@ -1801,7 +1833,7 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
         a.fetch = ctl;
         a.done = ctl + 16;
         a.service_lanes = tune.service_lanes ? tune.service_lanes : kDefaultServiceLanes;
-        a.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultRefillLanes;
+        a.refill_lanes = tune.refill_lanes ? tune.refill_lanes : kDefaultMegaRefillLanes;
         a.fetch_chunk = tune.fetch_chunk ? tune.fetch_chunk : kDefaultMegaFetchChunk;
         // Path state between walks in device memory rather than registers:
         // the kernel needs about 160 registers either way (3 waves per SIMD),

@@ -114,7 +114,7 @@ struct mrt_scene {
     std::vector<mrt_light> lights;
     uint32_t n_scene_mats = 0;
     mrt::DevBuf tri_geo, tri_shade, sph, sph_mat, pla, mat, light_geo, light_shade, light_info;
-    mrt::DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes, wide_nodes;
+    mrt::DevBuf nodes[4], perm[4], cell_off[3], cell_ref[3], walk_nodes, wide_nodes, cw_nodes, cw_recs;
     mrt::DevBuf spill, counters, err, stage_in, stage_out, stage_aux, tile_rank, wf_state, wf_rad, wf_aov, mega_ctl, mega_state,
         wf_group[mrt::kWfMaxGroups];
     // Host copy of the light tree (pairs relative to its own numbering) for

@@ -92,6 +92,8 @@ struct SceneDev {
     uint32_t walk_light_slot;  // first light slot of tri_geo (= triangle count)
     const float4* wide_nodes;  // wide walk tree, 8 float4 per node
     uint32_t wide_light_node;  // first wide node from a light pair
+    const uint4* cw_nodes;     // compressed wide walk tree (mrt_cwt.hpp), 4 uint4 per node
+    const float4* cw_recs;     // its leaf records, 8 float4 each
 };
 
 // Material fields.

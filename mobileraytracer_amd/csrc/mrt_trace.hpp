@@ -17,6 +17,7 @@
 //  * The grid walk and the naive scan visit in the reference order and use
 //    the reference's strict t < length_ rule.
 #pragma once
+#include "mrt_cwt.hpp"
 #include "mrt_scene.hpp"
 
 namespace mrt {
@@ -682,6 +683,65 @@ __device__ __forceinline__ void walk_visit_wide(const SceneDev& s, Walk& w, cons
     float k2 = (h2 && wd2 == 0u) ? e2 : INFINITY, k3 = (h3 && wd3 == 0u) ? e3 : INFINITY;
     uint32_t n0 = __float_as_uint(ra.x), n1 = __float_as_uint(ra.z);
     uint32_t n2 = __float_as_uint(rb.x), n3 = __float_as_uint(rb.z);
+    cswap(k0, n0, k1, n1);
+    cswap(k2, n2, k3, n3);
+    cswap(k0, n0, k2, n2);
+    cswap(k1, n1, k3, n3);
+    cswap(k1, n1, k2, n2);
+    if (k3 < INFINITY) st.push(n3);
+    if (k2 < INFINITY) st.push(n2);
+    if (k1 < INFINITY) st.push(n1);
+    if (k0 < INFINITY) w.node = n0;
+    else w.node = st.sp == 0u ? kNoNode : st.pop();
+    walk_pull(w);
+}
+
+// One compressed-node visit (layout in mrt_cwt.hpp), the walk_visit_wide of
+// the quantized boxes: open slots are entered at distance 0, reached leaves
+// go to l0..l3 (their exact boxes are tested with their triangles), reached
+// interior slots are ordered by entry distance.
+struct CwNode {
+    uint4 n0, n1, n2, n3;
+};
+__device__ __forceinline__ CwNode load_cw(const SceneDev& s, uint32_t node) {
+    const uint4* p = s.cw_nodes + 4u * node;
+    return CwNode{p[0], p[1], p[2], p[3]};
+}
+
+__device__ __forceinline__ bool cw_slot(const CwNode& c, uint32_t k, const f3& org, const f3& sc, const RayT& r,
+                                        float lim, float* e) {
+    const f3 mn{cw_deq(org.x, cw_byte(c.n1.x, k), sc.x), cw_deq(org.y, cw_byte(c.n1.y, k), sc.y),
+                cw_deq(org.z, cw_byte(c.n1.z, k), sc.z)};
+    const f3 mx{cw_deq(org.x, cw_byte(c.n1.w, k), sc.x), cw_deq(org.y, cw_byte(c.n2.x, k), sc.y),
+                cw_deq(org.z, cw_byte(c.n2.y, k), sc.z)};
+    const uint32_t kind = (c.n0.w >> (24u + 2u * k)) & 3u;
+    const bool open = (c.n3.z >> k) & 1u;
+    const bool hit = slab_finite(mn, mx, r.o, r.inv, e) && *e <= lim;
+    *e = open ? 0.0f : *e;
+    return kind != kCwEmpty && (open || hit);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void cw_visit(const SceneDev& s, Walk& w, const CwNode& c, float lim, Stack& st,
+                                         TraceStats& ts, int kind) {
+    if (COUNT) ++ts.pairs[kind == 2 ? 2 : (w.node >= s.wide_light_node ? 1 : 0)];
+    const f3 org{__uint_as_float(c.n0.x), __uint_as_float(c.n0.y), __uint_as_float(c.n0.z)};
+    const f3 sc{cw_scale(c.n0.w & 0xFFu), cw_scale((c.n0.w >> 8) & 0xFFu), cw_scale((c.n0.w >> 16) & 0xFFu)};
+    float e0, e1, e2, e3;
+    const bool h0 = cw_slot(c, 0u, org, sc, w.r, lim, &e0);
+    const bool h1 = cw_slot(c, 1u, org, sc, w.r, lim, &e1);
+    const bool h2 = cw_slot(c, 2u, org, sc, w.r, lim, &e2);
+    const bool h3 = cw_slot(c, 3u, org, sc, w.r, lim, &e3);
+    const uint32_t kinds = c.n0.w >> 24;
+    const bool f0 = ((kinds >> 0) & 3u) == kCwLeaf, f1 = ((kinds >> 2) & 3u) == kCwLeaf;
+    const bool f2 = ((kinds >> 4) & 3u) == kCwLeaf, f3v = ((kinds >> 6) & 3u) == kCwLeaf;
+    uint32_t n0 = c.n2.z, n1 = c.n2.w, n2 = c.n3.x, n3 = c.n3.y;
+    w.l0 = (h0 && f0) ? n0 : 0u;
+    w.l1 = (h1 && f1) ? n1 : 0u;
+    w.l2 = (h2 && f2) ? n2 : 0u;
+    w.l3 = (h3 && f3v) ? n3 : 0u;
+    float k0 = (h0 && !f0) ? e0 : INFINITY, k1 = (h1 && !f1) ? e1 : INFINITY;
+    float k2 = (h2 && !f2) ? e2 : INFINITY, k3 = (h3 && !f3v) ? e3 : INFINITY;
     cswap(k0, n0, k1, n1);
     cswap(k2, n2, k3, n3);
     cswap(k0, n0, k2, n2);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <hipcub/hipcub.hpp>
 
+#include "mrt_cwt.hpp"
 #include "mrt_grid_reg.hpp"
 
 namespace mrt {
@@ -242,6 +243,118 @@ __global__ __launch_bounds__(kThreads) void k_tri_boxes(const float4* __restrict
     o[5] = bx.mx.z;
 }
 
+// Compressed wide walk tree (layout and exactness in mrt_cwt.hpp): one
+// thread per wide node writes its quantized node and its leaves' records.
+// Slots with a finite box are quantized on a per-node, per-axis grid: origin
+// the lowest bound of the node's finite slots, step a power of two with 255
+// steps reaching the highest bound; each bound is rounded outward on the grid
+// and then moved until the dequantized float contains the exact bound (a
+// slot that cannot be is marked open).  Leaf words that do not fit the slot
+// range write no record.
+__device__ __forceinline__ uint32_t cw_exp_for(float lo, float hi) {
+    const float d = (hi - lo) / 255.0f;
+    uint32_t e = d > 0.0f ? ((__float_as_uint(d) >> 23) & 0xFFu) + 1u : 1u;
+    e = e < 1u ? 1u : e;
+    while (e < 254u && !(cw_deq(lo, 255u, cw_scale(e)) >= hi)) ++e;
+    return e;
+}
+
+__global__ __launch_bounds__(kThreads) void k_cw_build(const float4* __restrict__ wide, uint32_t n_wide,
+                                                       const float4* __restrict__ tri_geo, uint32_t n_slots,
+                                                       uint4* __restrict__ nodes, float4* __restrict__ recs) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n_wide) return;
+    const float4* q = wide + 8ull * i;
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6], q7 = q[7];
+    const float mn[3][4] = {{q0.x, q0.y, q0.z, q0.w}, {q1.x, q1.y, q1.z, q1.w}, {q2.x, q2.y, q2.z, q2.w}};
+    const float mx[3][4] = {{q3.x, q3.y, q3.z, q3.w}, {q4.x, q4.y, q4.z, q4.w}, {q5.x, q5.y, q5.z, q5.w}};
+    const uint32_t ref[4] = {__float_as_uint(q6.x), __float_as_uint(q6.z), __float_as_uint(q7.x),
+                             __float_as_uint(q7.z)};
+    const uint32_t word[4] = {__float_as_uint(q6.y), __float_as_uint(q6.w), __float_as_uint(q7.y),
+                              __float_as_uint(q7.w)};
+    uint32_t kinds = 0u, open = 0u, finite = 0u, outRef[4];
+    for (int k = 0; k < 4; ++k) {
+        bool allNan = true, fin = true;
+        for (int a = 0; a < 3; ++a) {
+            allNan = allNan && isnan(mn[a][k]) && isnan(mx[a][k]);
+            fin = fin && isfinite(mn[a][k]) && isfinite(mx[a][k]);
+        }
+        const uint32_t kind = word[k] != 0u ? kCwLeaf : (allNan ? kCwEmpty : kCwInterior);
+        kinds |= kind << (2 * k);
+        outRef[k] = kind == kCwLeaf ? word[k] : (kind == kCwInterior ? ref[k] : 0u);
+        if (kind != kCwEmpty && fin) finite |= 1u << k;
+        if (kind != kCwEmpty && !fin) open |= 1u << k;
+    }
+    float org[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t ex[3] = {127u, 127u, 127u}, qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+    for (int a = 0; a < 3; ++a) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (int k = 0; k < 4; ++k) {
+            if (!(finite >> k & 1u)) continue;
+            lo = fminf(lo, fminf(mn[a][k], mx[a][k]));
+            hi = fmaxf(hi, fmaxf(mn[a][k], mx[a][k]));
+        }
+        if (!(lo <= hi)) continue;  // no finite slot
+        const uint32_t e = cw_exp_for(lo, hi);
+        const float sc = cw_scale(e);
+        org[a] = lo;
+        ex[a] = e;
+        for (int k = 0; k < 4; ++k) {
+            if (!(finite >> k & 1u)) continue;
+            const float bl = fminf(mn[a][k], mx[a][k]), bh = fmaxf(mn[a][k], mx[a][k]);
+            uint32_t l = static_cast<uint32_t>(fminf(fmaxf(floorf((bl - lo) / sc), 0.0f), 255.0f));
+            while (l > 0u && cw_deq(lo, l, sc) > bl) --l;
+            uint32_t h = static_cast<uint32_t>(fminf(fmaxf(ceilf((bh - lo) / sc), 0.0f), 255.0f));
+            while (h < 255u && cw_deq(lo, h, sc) < bh) ++h;
+            if (!(cw_deq(lo, l, sc) <= bl && cw_deq(lo, h, sc) >= bh)) {
+                finite &= ~(1u << k);
+                open |= 1u << k;
+                continue;
+            }
+            qlo[a] |= l << (8 * k);
+            qhi[a] |= h << (8 * k);
+        }
+    }
+    // Open slots keep zero bytes (their test ignores them).
+    for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 4; ++k)
+            if (open >> k & 1u) {
+                qlo[a] &= ~(0xFFu << (8 * k));
+                qhi[a] &= ~(0xFFu << (8 * k));
+            }
+    uint4* o = nodes + 4ull * i;
+    o[0] = make_uint4(__float_as_uint(org[0]), __float_as_uint(org[1]), __float_as_uint(org[2]),
+                      ex[0] | (ex[1] << 8) | (ex[2] << 16) | (kinds << 24));
+    o[1] = make_uint4(qlo[0], qlo[1], qlo[2], qhi[0]);
+    o[2] = make_uint4(qhi[1], qhi[2], outRef[0], outRef[1]);
+    o[3] = make_uint4(outRef[2], outRef[3], open, 0u);
+    // Leaf records: the exact box (as the reference's leaf has it) and two
+    // triangles per record.
+    for (int k = 0; k < 4; ++k) {
+        if (word[k] == 0u) continue;
+        const uint32_t slot = word[k] & kSlotMask, cnt = word[k] >> kSlotBits;
+        if (cnt == 0u || slot >= n_slots || cnt > n_slots - slot) continue;
+        for (uint32_t j = 0; 2u * j < cnt; ++j) {
+            float4* r = recs + 8ull * (slot + j);
+            r[0] = make_float4(mn[0][k], mn[1][k], mn[2][k], mx[0][k]);
+            r[1] = make_float4(mx[1][k], mx[2][k], 0.0f, 0.0f);
+            const float4* t0 = tri_geo + 3ull * (slot + 2u * j);
+            r[2] = t0[0];
+            r[3] = t0[1];
+            r[4] = t0[2];
+            if (2u * j + 1u < cnt) {
+                r[5] = t0[3];
+                r[6] = t0[4];
+                r[7] = t0[5];
+            } else {
+                r[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                r[6] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                r[7] = make_float4(0.0f, __uint_as_float(MRT_ID_NONE), 0.0f, 0.0f);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_rebase_wide(const float4* __restrict__ src, size_t n,
                                                           float4* __restrict__ dst, uint32_t tri_base,
                                                           uint32_t wide_base) {
@@ -379,6 +492,13 @@ hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_
 hipError_t gpu_pack_triangles(const mrt_triangle* tris, uint32_t n, float4* geo, float4* shade, hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_pack_triangles<<<blocks_for(n), kThreads, 0, st>>>(reinterpret_cast<const float4*>(tris), n, geo, shade);
+    return hipGetLastError();
+}
+
+hipError_t gpu_compress_walk(const float4* wide, uint32_t n_wide, const float4* tri_geo, uint32_t n_slots,
+                             uint4* nodes, float4* recs, hipStream_t st) {
+    if (n_wide == 0u) return hipSuccess;
+    k_cw_build<<<blocks_for(n_wide), kThreads, 0, st>>>(wide, n_wide, tri_geo, n_slots, nodes, recs);
     return hipGetLastError();
 }
 

@@ -68,6 +68,13 @@ hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_
 // nC cA cB cC materialId), the same bits as the host packing.
 hipError_t gpu_pack_triangles(const mrt_triangle* tris, uint32_t n, float4* geo, float4* shade, hipStream_t st);
 
+// The compressed wide walk tree (mrt_cwt.hpp) of the first n_wide nodes of a
+// wide walk tree: n_wide nodes of 4 uint4, and the leaf records (8 float4
+// each, indexed by a leaf's first slot) of every leaf the nodes reference;
+// tri_geo holds n_slots slots.
+hipError_t gpu_compress_walk(const float4* wide, uint32_t n_wide, const float4* tri_geo, uint32_t n_slots,
+                             uint4* nodes, float4* recs, hipStream_t st);
+
 // Boxes (6 floats: min xyz, max xyz) of the first n triangles of a tri_geo
 // array (3 float4 per triangle: A.xyz AB.x | AB.yz AC.xy | AC.z id), the
 // same arithmetic as the host's triangle_box.

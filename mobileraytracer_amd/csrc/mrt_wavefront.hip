@@ -299,6 +299,54 @@ __device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, const TriG
     return blocked;
 }
 
+// The leaf test over the compressed tree's leaf records (mrt_cwt.hpp): one
+// record per step, its exact leaf box tested first (lanes with a finite
+// inverse direction reached the leaf through quantized boxes; the others
+// through the pair tree's exact boxes, so they skip it), then its one or two
+// triangles as walk_test tests them.  A box the ray misses drops the leaf.
+template <bool COUNT>
+__device__ __forceinline__ bool cw_test(const SceneDev& s, Walk& w, bool shadow, bool finite, TraceStats& ts) {
+    const uint32_t rec = w.cur & kSlotMask;
+    const uint32_t cnt = w.cur >> kSlotBits;
+    const float4* p = s.cw_recs + 8u * rec;
+    const float4 b0 = p[0], b1 = p[1];
+    const float4 a0 = p[2], a1 = p[3], a2 = p[4];
+    const float4 c0 = p[5], c1 = p[6], c2 = p[7];
+    float eb;
+    const bool boxOk = !finite || (slab_finite(f3{b0.x, b0.y, b0.z}, f3{b0.w, b1.x, b1.y}, w.r.o, w.r.inv, &eb) &&
+                                   eb <= cull_limit(w.h.t));
+    if (!boxOk) {
+        w.cur = 0u;
+        walk_pull(w);
+        return false;
+    }
+    const TriG ga{f3{a0.x, a0.y, a0.z}, f3{a0.w, a1.x, a1.y}, f3{a1.z, a1.w, a2.x}};
+    const TriG gb{f3{c0.x, c0.y, c0.z}, f3{c0.w, c1.x, c1.y}, f3{c1.z, c1.w, c2.x}};
+    const uint32_t ida = __float_as_uint(a2.y), idb = __float_as_uint(c2.y);
+    const bool two = cnt > 1u;
+    if (COUNT) ts.prims[shadow ? 2 : (rec >= s.walk_light_slot ? 1 : 0)] += two ? 2u : 1u;
+    float t, u, v;
+    bool hit = tri_test_flat(ga, w.r, &t, &u, &v) && ida != w.r.excl;
+    bool better = !shadow && hit && key_better(t, ida, w.h);
+    bool blocked = shadow && hit && t > kEpsilon && t < w.h.t;
+    w.h.t = better ? t : w.h.t;
+    w.h.id = better ? ida : w.h.id;
+    w.h.u = better ? u : w.h.u;
+    w.h.v = better ? v : w.h.v;
+    hit = two && tri_test_flat(gb, w.r, &t, &u, &v) && idb != w.r.excl;
+    better = !shadow && hit && key_better(t, idb, w.h);
+    blocked = blocked || (shadow && hit && t > kEpsilon && t < w.h.t);
+    w.h.t = better ? t : w.h.t;
+    w.h.id = better ? idb : w.h.id;
+    w.h.u = better ? u : w.h.u;
+    w.h.v = better ? v : w.h.v;
+    // Next record of a longer leaf (count - 2, record + 1), else the next leaf.
+    w.cur = cnt > 2u ? ((cnt - 2u) << kSlotBits) | (rec + 1u) : 0u;
+    walk_pull(w);
+    if (blocked) walk_clear(w);
+    return blocked;
+}
+
 // One unit of walk work for the wave: node visits when at least as many
 // lanes can visit as can test, else one triangle test per lane with a
 // pending leaf.  Lanes with a finite inverse direction walk the wide tree,
@@ -312,12 +360,36 @@ __device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, const TriG
 // ray's walk latency sets the launch time) and loses on full launches (818
 // against 844 Mrays/s on the 1M-triangle 1080p frame), so both are built and
 // the host picks by queue size.
-template <bool COUNT, bool OVERLAP>
+// CW: over the compressed wide tree and its leaf records (finite lanes) and
+// the pair tree with the leaf records (the others).
+template <bool COUNT, bool OVERLAP, bool CW>
 __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, bool shadow, bool finite, Stack& st,
                                           TraceStats& ts) {
     const bool canVisit = has && walk_can_visit(w);
     const bool canTest = has && w.cur != 0u;
     bool occluded = false;
+    if (CW) {
+        if (OVERLAP) {
+            CwNode cn{};
+            if (canVisit && finite) cn = load_cw(s, w.node);
+            occluded = canTest && cw_test<COUNT>(s, w, shadow, finite, ts);
+            if (canVisit && !occluded) {
+                const float lim = cull_limit(w.h.t);
+                const int kind = shadow ? 2 : 0;
+                if (finite) cw_visit<COUNT>(s, w, cn, lim, st, ts, kind);
+                else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
+            }
+            return occluded;
+        }
+        if (canTest) occluded = cw_test<COUNT>(s, w, shadow, finite, ts);
+        if (canVisit && !occluded) {
+            const float lim = cull_limit(w.h.t);
+            const int kind = shadow ? 2 : 0;
+            if (finite) cw_visit<COUNT>(s, w, load_cw(s, w.node), lim, st, ts, kind);
+            else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
+        }
+        return occluded;
+    }
     if (OVERLAP) {
         TriG g{};
         uint32_t id = 0;
@@ -461,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
             ++rc.ts.lane_slots;
             if (has) ++rc.ts.lane_iters;
         }
-        const bool occluded = walk_work<COUNT, OVERLAP>(s, w, has, shadow, finite, st, rc.ts);
+        const bool occluded = walk_work<COUNT, OVERLAP, !MULTI>(s, w, has, shadow, finite, st, rc.ts);
         if (has && walk_done(w)) {
             if (!shadow) hits[idx] = make_float4(w.h.t, __uint_as_float(w.h.id), w.h.u, w.h.v);
             else if (!occluded) apply_light(b, b.shadow + 4u * idx);
@@ -835,7 +907,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
             ++rc.ts.lane_slots;
             if (walking) ++rc.ts.lane_iters;
         }
-        if (walk_work<COUNT, false>(s, w, walking, mode == kMegaShadow, finite, st, rc.ts)) occluded = true;
+        if (walk_work<COUNT, false, true>(s, w, walking, mode == kMegaShadow, finite, st, rc.ts)) occluded = true;
     }
     const uint32_t total = wave_sum(finished);
     if ((threadIdx.x & 63u) == 0u && total) atomicAdd(A.done, total);

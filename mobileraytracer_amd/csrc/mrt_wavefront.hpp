@@ -82,7 +82,7 @@ inline bool wavefront_overlap(uint64_t rays, uint32_t blocks, uint32_t mode) {
 }
 
 // Defaults of the launch-shape knobs (mrt_tuning).
-constexpr uint32_t kDefaultRefillLanes = 32;
+constexpr uint32_t kDefaultRefillLanes = 16;
 constexpr uint32_t kDefaultFetchChunk = 128;
 constexpr uint32_t kDefaultSlotLanes = 8;
 
@@ -253,10 +253,13 @@ hipError_t launch_mega_p(const SceneDev& s, bool count, const MegaArgs& a, uint3
                          unsigned long long* counters, uint32_t* err, hipStream_t stream);
 // Workgroups of k_mega_p the chip holds at once.
 uint32_t mega_blocks(uint32_t num_cu);
-// Persistent megakernel defaults: shade once half the wave's lanes wait
-// (1M-triangle 1080p 16-spp frame: 16 lanes 128.2 ms, 24 124.2, 32 123.2,
-// 48 133.4), reserve 64 items per fetch.
-constexpr uint32_t kDefaultServiceLanes = 32;
+// Persistent megakernel defaults: shade once a quarter of the wave's lanes
+// wait, refill at half, reserve 64 items per fetch (the fastest on the
+// 1M-triangle soup from 256x256 x 1 to 1920x1080 x 16 samples).
+constexpr uint32_t kDefaultServiceLanes = 16;
+// AUTO renders frames of at least this many pixel samples with the wavefront.
+constexpr uint64_t kMegaMaxPixelSamples = 1ull << 24;
+constexpr uint32_t kDefaultMegaRefillLanes = 32;
 constexpr uint32_t kDefaultMegaFetchChunk = 64;
 
 // The host-side bookkeeping of one iteration (queue and counter roles,

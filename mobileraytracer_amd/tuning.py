@@ -48,7 +48,7 @@ PER_PASS = "per_pass"
 SEARCH_SPACE: dict[str, dict[str, tuple[int, ...]]] = {
     MEGAKERNEL: {
         "max_slot_lanes": (8, 4, 16),
-        "service_lanes": (32, 16, 24, 40),
+        "service_lanes": (16, 8, 24, 32),
         "refill_lanes": (32, 16, 48),
         "fetch_chunk": (64, 128, 256),
     },
@@ -56,7 +56,7 @@ SEARCH_SPACE: dict[str, dict[str, tuple[int, ...]]] = {
         "max_slot_lanes": (8, 4, 16),
         "max_groups": (3, 1, 2),
         "launch_share_pct": (0, 40, 100),
-        "refill_lanes": (32, 16, 48),
+        "refill_lanes": (16, 8, 32),
         "fetch_chunk": (128, 64, 256),
         "overlap_loads": (0, 1, 2),
         "graph_loop": (0, 1),
