@@ -405,8 +405,8 @@ def main() -> int:
     # 52 per cell primitive test (its 4-byte reference and the triangle);
     # the light tree's pairs and tests as above.
     wavefront = launches > 0
-    persistent = (wavefront or mega_p) and not args.no_persistent and args.accel == "bvh"
-    bytes_per_ray = 48 + 64 * pairs_per_ray + (64 if persistent else 48) * prims_per_ray
+    persistent = (wavefront or mega_p) and not args.no_persistent
+    bytes_per_ray = 48 + 64 * pairs_per_ray + (64 if persistent and args.accel == "bvh" else 48) * prims_per_ray
     gpu_bytes_per_ray = bytes_per_ray
     ordered = None
     if args.accel == "bvh" and not args.no_oracle_bytes:
@@ -423,7 +423,8 @@ def main() -> int:
         bytes_total = rays * bytes_per_ray + closest * 256
         avg_launch_s = kernel_ms / 1000.0 / launches
     elif wavefront:
-        kernel_name = "k_wf_trace_p" if persistent else "k_wf_extend + k_wf_shadow"
+        kernel_name = ("k_wf_extend + k_wf_shadow" if not persistent else
+                       "k_wf_trace_grid_p" if args.accel == "grid" else "k_wf_trace_p")
         bytes_total = rays * bytes_per_ray
         avg_launch_s = trace_ms / 1000.0 / launches
     else:
@@ -435,7 +436,7 @@ def main() -> int:
     achieved = bytes_per_launch / avg_launch_s / 1e9
     device_achieved = bytes_total / (busy_ms / 1000.0) / 1e9 if wavefront and busy_ms > 0 else achieved
     frame_achieved = (rays * bytes_per_ray + closest * 256) / elapsed / 1e9
-    traffic = measured_traffic(kernel_name) if persistent else None
+    traffic = measured_traffic(kernel_name) if persistent and args.accel == "bvh" else None
     copy_gbs = copy_bandwidth(torch)
 
     out = {
