@@ -3,10 +3,10 @@
 // The reference renders one frame per Renderer::renderFrame call
 // (Renderer.cpp:31-59); a server runs many requests at once through the
 // wavefront pipeline of mrt_wavefront.hip:
-//   * scenes are made resident once: their walk trees (pair tree, wide tree
-//     and triangle slots) are copied into one arena with node and slot
-//     references rebased, so one persistent trace launch walks rays of every
-//     scene, each from its scene's roots;
+//   * scenes are made resident once: their walk trees (pair tree, compressed
+//     wide tree, leaf records and triangle slots) are copied into one arena
+//     with node and slot references rebased, so one persistent trace launch
+//     walks rays of every scene, each from its scene's roots;
 //   * the path-slot pool is cut into pipelines (own queues and stream), and
 //     the pool's 64-slot waves are handed to jobs as they are admitted; each
 //     wave records its job, so the regen and shade kernels of a pipeline serve
@@ -168,7 +168,7 @@ struct mrt_server {
     hipEvent_t chk[2][kWfMaxGroups] = {};
     // Device state.
     DevBuf state, group[kWfMaxGroups], jobs_dev, scenes_dev, waves_dev, finished_dev, counters, err, spill, job_mem;
-    DevBuf arena_wide, arena_walk, arena_geo;
+    DevBuf arena_cw, arena_recs, arena_walk, arena_geo;
     uint64_t wide_used = 0, walk_used = 0, geo_used = 0;  // in nodes / pairs / slots
     uint32_t depth_need = 0;
     WfBuffers wb[kWfMaxGroups]{};
@@ -225,7 +225,8 @@ uint32_t* snap_of(const mrt_server* s, uint32_t snap, uint32_t g) {
 SceneDev arena_scene(const mrt_server* s) {
     SceneDev a{};
     a.walk_nodes = s->arena_walk.as<BNode>();
-    a.wide_nodes = s->arena_wide.as<float4>();
+    a.cw_nodes = s->arena_cw.as<uint4>();
+    a.cw_recs = s->arena_recs.as<float4>();
     a.tri_geo = s->arena_geo.as<float4>();
     return a;
 }
@@ -624,42 +625,49 @@ void fill_info(const Job* j, mrt_job_info* info) {
 // (rebased on the device, on the transfer stream: the pipelines keep running,
 // and the new nodes land past every node they read).
 int add_to_arena(mrt_server* s, mrt_scene* sc, Resident* out) {
-    const size_t nWide = sc->wide_nodes.bytes / (8 * sizeof(float4));
+    const size_t nWide = sc->cw_nodes.bytes / (4 * sizeof(uint4));
     const size_t nWalk = sc->walk_nodes.bytes / sizeof(BNode);
     const size_t nGeo = sc->tri_geo.bytes / (3 * sizeof(float4));
+    const size_t nRec = std::min(nGeo, sc->cw_recs.bytes / (8 * sizeof(float4)));
     if (s->geo_used + nGeo >= (1ull << kSlotBits) || s->wide_used + nWide >= 0xFFFFFFF0ull ||
         s->walk_used + nWalk >= 0xFFFFFFF0ull)
         return fail(MRT_E_BUDGET, "the scene arena's 27-bit triangle slots are used up");
     const uint32_t wideBase = static_cast<uint32_t>(s->wide_used);
     const uint32_t walkBase = static_cast<uint32_t>(s->walk_used);
     const uint32_t triBase = static_cast<uint32_t>(s->geo_used);
-    const size_t wideBytes = (s->wide_used + nWide) * 8 * sizeof(float4);
+    const size_t cwBytes = (s->wide_used + nWide) * 4 * sizeof(uint4);
+    const size_t recBytes = (s->geo_used + nGeo) * 8 * sizeof(float4);
     const size_t walkBytes = (s->walk_used + nWalk) * sizeof(BNode);
     const size_t geoBytes = (s->geo_used + nGeo) * 3 * sizeof(float4);
-    const uint64_t total = wideBytes + walkBytes + geoBytes;
+    const uint64_t total = cwBytes + recBytes + walkBytes + geoBytes;
     if (s->scene_memory && total > s->scene_memory)
         return fail(MRT_E_BUDGET, "scene arena needs " + std::to_string(total) + " bytes, limit " +
                                       std::to_string(s->scene_memory));
     int rc;
     auto cap = [](size_t need, size_t have) { return need <= have ? have : std::max(need, have + have / 2); };
-    if ((rc = grow(s, s->arena_wide, cap(wideBytes, s->arena_wide.bytes), s->wide_used * 8 * sizeof(float4))) != MRT_OK)
+    if ((rc = grow(s, s->arena_cw, cap(cwBytes, s->arena_cw.bytes), s->wide_used * 4 * sizeof(uint4))) != MRT_OK)
+        return rc;
+    if ((rc = grow(s, s->arena_recs, cap(recBytes, s->arena_recs.bytes), s->geo_used * 8 * sizeof(float4))) != MRT_OK)
         return rc;
     if ((rc = grow(s, s->arena_walk, cap(walkBytes, s->arena_walk.bytes), s->walk_used * sizeof(BNode))) != MRT_OK)
         return rc;
     if ((rc = grow(s, s->arena_geo, cap(geoBytes, s->arena_geo.bytes), s->geo_used * 3 * sizeof(float4))) != MRT_OK)
         return rc;
-    // Wide nodes: 4 (ref, word) slots; a leaf word carries the slot in its low
-    // bits, an interior slot (word 0, finite box) a wide node.  Walk pairs:
-    // leaf children (count != 0) move with the slots, interior ones (finite
-    // box) with the pairs.
+    // Compressed wide nodes: leaf words carry the slot in their low bits,
+    // interior slots a wide node.  Walk pairs: leaf children (count != 0)
+    // move with the slots, interior ones (finite box) with the pairs.  Leaf
+    // records and triangle slots hold no references and move as they are.
     const hipStream_t st = s->xfer;
-    MRT_HIP(gpu_rebase_wide(sc->wide_nodes.as<float4>(), nWide, s->arena_wide.as<float4>() + 8 * s->wide_used, triBase,
-                            wideBase, st));
+    MRT_HIP(gpu_rebase_cw(sc->cw_nodes.as<uint4>(), nWide, s->arena_cw.as<uint4>() + 4 * s->wide_used, triBase,
+                          wideBase, st));
     MRT_HIP(gpu_rebase_walk(sc->walk_nodes.as<BNode>(), nWalk, s->arena_walk.as<BNode>() + s->walk_used, triBase,
                             walkBase, st));
     if (nGeo)
         MRT_HIP(hipMemcpyAsync(s->arena_geo.as<float4>() + 3 * s->geo_used, sc->tri_geo.p,
                                nGeo * 3 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    if (nRec)
+        MRT_HIP(hipMemcpyAsync(s->arena_recs.as<float4>() + 8 * s->geo_used, sc->cw_recs.p,
+                               nRec * 8 * sizeof(float4), hipMemcpyDeviceToDevice, st));
     MRT_HIP(hipStreamSynchronize(st));
     s->wide_used += nWide;
     s->walk_used += nWalk;
@@ -828,7 +836,7 @@ MRT_API int mrt_server_add_scene(mrt_server* s, mrt_scene* scene, uint32_t* scen
     if (!s || !scene || !scene_id) return fail(MRT_E_INVALID, "null argument");
     if (scene->ctx != s->ctx) return fail(MRT_E_INVALID, "scene belongs to another context");
     if (!scene->built || scene->generic || scene->dev.acc[MRT_CAT_TRIANGLE].kind != MRT_BVH ||
-        !scene->dev.walk_nodes || !scene->dev.wide_nodes)
+        !scene->dev.walk_nodes || !scene->dev.cw_nodes)
         return fail(MRT_E_UNSUPPORTED, "render servers take scenes of triangles and lights built with the BVH");
     std::lock_guard<std::mutex> lock(s->mu);
     if (s->scenes.size() >= s->max_scenes) return fail(MRT_E_BUDGET, "render server scene table is full");
@@ -974,7 +982,8 @@ MRT_API int mrt_server_stats(mrt_server* s, mrt_serving_stats* st) {
         uint64_t freeWaves = 0;
         for (uint32_t g = 0; g < s->G; ++g) freeWaves += s->waves_free[g].free_total();
         st->slots_used = static_cast<uint32_t>((s->pool_waves - freeWaves) * 64u);
-        st->scene_bytes = s->wide_used * 8 * sizeof(float4) + s->walk_used * sizeof(BNode) +
+        st->scene_bytes = s->wide_used * 4 * sizeof(uint4) + s->geo_used * 8 * sizeof(float4) +
+                          s->walk_used * sizeof(BNode) +
                           s->geo_used * 3 * sizeof(float4);
         st->job_bytes_used = (s->mem_free.total() - s->mem_free.free_total()) * kAlign;
         st->batches = s->batches;

@@ -513,8 +513,9 @@ __device__ __forceinline__ HitT closest_hit(const SceneDev& s, const RayT& r, fl
 // ----------------------------------------------------------- combined walk --
 // Persistent kernels walk one tree that joins the triangle BVH and the light
 // BVH under a virtual root pair (layout in mrt_scene.hpp), one unit of work
-// per lane and call: a node pair visit or a single triangle test.  Leaves a
-// visit reaches are queued (at most three ranges per lane) and tested later,
+// per lane and call: a node visit and / or one leaf record's test (its exact
+// box and up to two triangles).  Leaves a visit reaches are queued (at most
+// four per lane) and tested later,
 // so a wave runs visits and tests as separate, mostly convergent phases and
 // a lane can take a new ray the moment its ray is done.  The slab predicate on
 // the reference's own boxes (the virtual root tests the triangle root box as
@@ -574,13 +575,6 @@ __device__ __forceinline__ void walk_pull(Walk& w) {
     w.l1 = t1 ? 0u : w.l1;
     w.l2 = t2 ? 0u : w.l2;
     w.l3 = t3 ? 0u : w.l3;
-}
-
-// Drops the triangle just tested.
-__device__ __forceinline__ void walk_pop_prim(Walk& w) {
-    const bool more = (w.cur >> kSlotBits) > 1u;
-    w.cur = more ? w.cur - kSlotMask : 0u;  // slot + 1, count - 1
-    walk_pull(w);
 }
 
 // Slab test for rays whose inverse direction is finite in every axis: then no
@@ -646,56 +640,6 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& ra, float& kb, uint32
     ra = r;
 }
 
-// One wide-node visit (rays with a finite inverse direction, wide tree):
-// reached leaves go to l0..l3, reached interior slots are ordered by entry
-// distance; the nearest is visited next and the others are pushed, farthest
-// first.
-struct WideNode {
-    float4 mnx, mny, mnz, mxx, mxy, mxz, ra, rb;
-};
-__device__ __forceinline__ WideNode load_wide(const SceneDev& s, uint32_t node) {
-    const float4* nd = s.wide_nodes + 8u * node;
-    return WideNode{nd[0], nd[1], nd[2], nd[3], nd[4], nd[5], nd[6], nd[7]};
-}
-
-// The visit of node w.node with its record already loaded (so that callers
-// can issue the load before other work).
-template <bool COUNT>
-__device__ __forceinline__ void walk_visit_wide(const SceneDev& s, Walk& w, const WideNode& wn, float lim, Stack& st,
-                                                TraceStats& ts, int kind) {
-    const float4 mnx = wn.mnx, mny = wn.mny, mnz = wn.mnz, mxx = wn.mxx, mxy = wn.mxy, mxz = wn.mxz;
-    const float4 ra = wn.ra, rb = wn.rb;
-    if (COUNT) ++ts.pairs[kind == 2 ? 2 : (w.node >= s.wide_light_node ? 1 : 0)];
-    float e0, e1, e2, e3;
-    const bool h0 = slab_finite(f3{mnx.x, mny.x, mnz.x}, f3{mxx.x, mxy.x, mxz.x}, w.r.o, w.r.inv, &e0) && e0 <= lim;
-    const bool h1 = slab_finite(f3{mnx.y, mny.y, mnz.y}, f3{mxx.y, mxy.y, mxz.y}, w.r.o, w.r.inv, &e1) && e1 <= lim;
-    const bool h2 = slab_finite(f3{mnx.z, mny.z, mnz.z}, f3{mxx.z, mxy.z, mxz.z}, w.r.o, w.r.inv, &e2) && e2 <= lim;
-    const bool h3 = slab_finite(f3{mnx.w, mny.w, mnz.w}, f3{mxx.w, mxy.w, mxz.w}, w.r.o, w.r.inv, &e3) && e3 <= lim;
-    const uint32_t wd0 = __float_as_uint(ra.y), wd1 = __float_as_uint(ra.w);
-    const uint32_t wd2 = __float_as_uint(rb.y), wd3 = __float_as_uint(rb.w);
-    w.l0 = (h0 && wd0 != 0u) ? wd0 : 0u;
-    w.l1 = (h1 && wd1 != 0u) ? wd1 : 0u;
-    w.l2 = (h2 && wd2 != 0u) ? wd2 : 0u;
-    w.l3 = (h3 && wd3 != 0u) ? wd3 : 0u;
-    // Interior slots keyed by entry; +inf marks "not reached" (a reached
-    // slot's entry is at most lim, which is finite).
-    float k0 = (h0 && wd0 == 0u) ? e0 : INFINITY, k1 = (h1 && wd1 == 0u) ? e1 : INFINITY;
-    float k2 = (h2 && wd2 == 0u) ? e2 : INFINITY, k3 = (h3 && wd3 == 0u) ? e3 : INFINITY;
-    uint32_t n0 = __float_as_uint(ra.x), n1 = __float_as_uint(ra.z);
-    uint32_t n2 = __float_as_uint(rb.x), n3 = __float_as_uint(rb.z);
-    cswap(k0, n0, k1, n1);
-    cswap(k2, n2, k3, n3);
-    cswap(k0, n0, k2, n2);
-    cswap(k1, n1, k3, n3);
-    cswap(k1, n1, k2, n2);
-    if (k3 < INFINITY) st.push(n3);
-    if (k2 < INFINITY) st.push(n2);
-    if (k1 < INFINITY) st.push(n1);
-    if (k0 < INFINITY) w.node = n0;
-    else w.node = st.sp == 0u ? kNoNode : st.pop();
-    walk_pull(w);
-}
-
 // One compressed-node visit (layout in mrt_cwt.hpp), the walk_visit_wide of
 // the quantized boxes: open slots are entered at distance 0, reached leaves
 // go to l0..l3 (their exact boxes are tested with their triangles), reached
@@ -755,16 +699,6 @@ __device__ __forceinline__ void cw_visit(const SceneDev& s, Walk& w, const CwNod
     walk_pull(w);
 }
 
-__device__ __forceinline__ TriG walk_tri(const SceneDev& s, uint32_t slot, uint32_t* id) {
-    const float4* g = s.tri_geo + 3u * slot;
-    const float4 g0 = g[0], g1 = g[1], g2 = g[2];
-    *id = __float_as_uint(g2.y);
-    TriG t;
-    t.a = f3{g0.x, g0.y, g0.z};
-    t.ab = f3{g0.w, g1.x, g1.y};
-    t.ac = f3{g1.z, g1.w, g2.x};
-    return t;
-}
 
 // tri_test without early exits (the same operations; lanes of a wave rarely
 // agree on an exit, so the branches would only cost mask bookkeeping).

@@ -355,31 +355,32 @@ __global__ __launch_bounds__(kThreads) void k_cw_build(const float4* __restrict_
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_rebase_wide(const float4* __restrict__ src, size_t n,
-                                                          float4* __restrict__ dst, uint32_t tri_base,
-                                                          uint32_t wide_base) {
+// A compressed node copied into a render server's arena: leaf words + tri_base
+// (their slot field), interior references + wide_base.
+__global__ __launch_bounds__(kThreads) void k_rebase_cw(const uint4* __restrict__ src, size_t n,
+                                                        uint4* __restrict__ dst, uint32_t tri_base,
+                                                        uint32_t wide_base) {
     const size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i >= n) return;
-    const float4* q = src + 8 * i;
-    float4* o = dst + 8 * i;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) o[j] = q[j];
-    const float mnx[4] = {q[0].x, q[0].y, q[0].z, q[0].w};
-    uint32_t r[8];
-    const float4 a = q[6], b = q[7];
-    r[0] = __float_as_uint(a.x), r[1] = __float_as_uint(a.y), r[2] = __float_as_uint(a.z), r[3] = __float_as_uint(a.w);
-    r[4] = __float_as_uint(b.x), r[5] = __float_as_uint(b.y), r[6] = __float_as_uint(b.z), r[7] = __float_as_uint(b.w);
+    const uint4* q = src + 4 * i;
+    uint4* o = dst + 4 * i;
+    const uint4 n0 = q[0], n1 = q[1];
+    uint4 n2 = q[2], n3 = q[3];
+    const uint32_t kinds = n0.w >> 24;
+    uint32_t r[4] = {n2.z, n2.w, n3.x, n3.y};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (r[2 * k + 1] != 0u) {
-            r[2 * k] += tri_base;
-            r[2 * k + 1] += tri_base;
-        } else if (!isnan(mnx[k])) {
-            r[2 * k] += wide_base;
-        }
+        const uint32_t kind = (kinds >> (2 * k)) & 3u;
+        r[k] = kind == kCwLeaf ? r[k] + tri_base : (kind == kCwInterior ? r[k] + wide_base : 0u);
     }
-    o[6] = make_float4(__uint_as_float(r[0]), __uint_as_float(r[1]), __uint_as_float(r[2]), __uint_as_float(r[3]));
-    o[7] = make_float4(__uint_as_float(r[4]), __uint_as_float(r[5]), __uint_as_float(r[6]), __uint_as_float(r[7]));
+    n2.z = r[0];
+    n2.w = r[1];
+    n3.x = r[2];
+    n3.y = r[3];
+    o[0] = n0;
+    o[1] = n1;
+    o[2] = n2;
+    o[3] = n3;
 }
 
 __global__ __launch_bounds__(kThreads) void k_rebase_walk(const BNode* __restrict__ src, size_t n,
@@ -475,10 +476,10 @@ hipError_t gpu_tri_tree_write(const RefNode* ref, uint32_t cnt, const WalkGpuScr
     return hipGetLastError();
 }
 
-hipError_t gpu_rebase_wide(const float4* src, size_t n, float4* dst, uint32_t tri_base, uint32_t wide_base,
-                           hipStream_t st) {
+hipError_t gpu_rebase_cw(const uint4* src, size_t n, uint4* dst, uint32_t tri_base, uint32_t wide_base,
+                         hipStream_t st) {
     if (n == 0) return hipSuccess;
-    k_rebase_wide<<<blocks_for(n), kThreads, 0, st>>>(src, n, dst, tri_base, wide_base);
+    k_rebase_cw<<<blocks_for(n), kThreads, 0, st>>>(src, n, dst, tri_base, wide_base);
     return hipGetLastError();
 }
 

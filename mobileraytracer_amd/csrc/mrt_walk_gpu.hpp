@@ -53,12 +53,12 @@ hipError_t gpu_tri_tree_write(const RefNode* ref, uint32_t cnt, const WalkGpuScr
                               hipStream_t st);
 
 // Copies of walk trees with their references rebased (a render server's
-// scene arena): wide nodes (8 float4 each) with leaf slots and leaf words +
-// tri_base and interior slots (word 0, finite box) + wide_base; walk pairs
-// with leaf references and words + tri_base and interior references (count 0,
-// finite box) + walk_base.  src and dst may not overlap.
-hipError_t gpu_rebase_wide(const float4* src, size_t n, float4* dst, uint32_t tri_base, uint32_t wide_base,
-                           hipStream_t st);
+// scene arena): compressed wide nodes (4 uint4 each) with leaf words +
+// tri_base and interior references + wide_base; walk pairs with leaf
+// references and words + tri_base and interior references (count 0, finite
+// box) + walk_base.  src and dst may not overlap.
+hipError_t gpu_rebase_cw(const uint4* src, size_t n, uint4* dst, uint32_t tri_base, uint32_t wide_base,
+                         hipStream_t st);
 hipError_t gpu_rebase_walk(const BNode* src, size_t n, BNode* dst, uint32_t tri_base, uint32_t walk_base,
                            hipStream_t st);
 

@@ -277,28 +277,6 @@ __device__ __forceinline__ bool wave_wants_refill(bool need, bool has, uint32_t 
     return needMask != 0ull && (static_cast<uint32_t>(__popcll(needMask)) >= refill_lanes || __ballot(has) == 0ull);
 }
 
-// Test of the first queued triangle for a closest-hit lane (the (t, id) key)
-// or a shadow lane (Triangle::intersect(Ray, dist), Triangle.cpp:105-136, with
-// dist held in h.t); a blocker ends the shadow lane's walk.  Returns true when
-// a shadow lane is occluded.
-template <bool COUNT>
-__device__ __forceinline__ bool walk_test(const SceneDev& s, Walk& w, const TriG& g, uint32_t id, bool shadow,
-                                          TraceStats& ts) {
-    const uint32_t slot = w.cur & kSlotMask;
-    if (COUNT) ++ts.prims[shadow ? 2 : (slot >= s.walk_light_slot ? 1 : 0)];
-    float t, u, v;
-    const bool hit = tri_test_flat(g, w.r, &t, &u, &v) && id != w.r.excl;
-    const bool better = !shadow && hit && key_better(t, id, w.h);
-    const bool blocked = shadow && hit && t > kEpsilon && t < w.h.t;
-    w.h.t = better ? t : w.h.t;
-    w.h.id = better ? id : w.h.id;
-    w.h.u = better ? u : w.h.u;
-    w.h.v = better ? v : w.h.v;
-    walk_pop_prim(w);
-    if (blocked) walk_clear(w);
-    return blocked;
-}
-
 // The leaf test over the compressed tree's leaf records (mrt_cwt.hpp): one
 // record per step, its exact leaf box tested first (lanes with a finite
 // inverse direction reached the leaf through quantized boxes; the others
@@ -347,73 +325,31 @@ __device__ __forceinline__ bool cw_test(const SceneDev& s, Walk& w, bool shadow,
     return blocked;
 }
 
-// One unit of walk work for the wave: node visits when at least as many
-// lanes can visit as can test, else one triangle test per lane with a
-// pending leaf.  Lanes with a finite inverse direction walk the wide tree,
-// the others the pair tree with the reference slab test.  Returns true when
-// a shadow lane found a blocker.
-// OVERLAP: both loads of the iteration go out before either result is used
-// (the node a lane visits does not depend on its test), so a ray's two
-// memory round trips per iteration overlap; the node's 32 registers are then
-// held across the test (86 instead of 68 registers, 5 instead of 7 waves per
-// SIMD).  That wins when a launch holds too few rays to fill the chip (a
-// ray's walk latency sets the launch time) and loses on full launches (818
-// against 844 Mrays/s on the 1M-triangle 1080p frame), so both are built and
-// the host picks by queue size.
-// CW: over the compressed wide tree and its leaf records (finite lanes) and
-// the pair tree with the leaf records (the others).
-template <bool COUNT, bool OVERLAP, bool CW>
+// One unit of walk work per lane: the test of one leaf record (when a leaf
+// is pending) and a node visit (when none is queued).  Lanes with a finite
+// inverse direction walk the compressed wide tree, the others the pair tree
+// with the reference slab test; both test the leaf records.  Returns true
+// when a shadow lane found a blocker.
+// OVERLAP: both loads of the step go out before either result is used (the
+// node a lane visits does not depend on its test), so a ray's two memory
+// round trips per step overlap; the node's registers are then held across
+// the test (80 instead of 67 registers, 6 instead of 7 waves per SIMD).  That
+// wins when a launch holds too few rays to fill the chip (a ray's walk
+// latency sets the launch time) and loses on full launches, so both are built
+// and the host picks by queue size.
+template <bool COUNT, bool OVERLAP>
 __device__ __forceinline__ bool walk_work(const SceneDev& s, Walk& w, bool has, bool shadow, bool finite, Stack& st,
                                           TraceStats& ts) {
     const bool canVisit = has && walk_can_visit(w);
     const bool canTest = has && w.cur != 0u;
     bool occluded = false;
-    if (CW) {
-        if (OVERLAP) {
-            CwNode cn{};
-            if (canVisit && finite) cn = load_cw(s, w.node);
-            occluded = canTest && cw_test<COUNT>(s, w, shadow, finite, ts);
-            if (canVisit && !occluded) {
-                const float lim = cull_limit(w.h.t);
-                const int kind = shadow ? 2 : 0;
-                if (finite) cw_visit<COUNT>(s, w, cn, lim, st, ts, kind);
-                else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
-            }
-            return occluded;
-        }
-        if (canTest) occluded = cw_test<COUNT>(s, w, shadow, finite, ts);
-        if (canVisit && !occluded) {
-            const float lim = cull_limit(w.h.t);
-            const int kind = shadow ? 2 : 0;
-            if (finite) cw_visit<COUNT>(s, w, load_cw(s, w.node), lim, st, ts, kind);
-            else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
-        }
-        return occluded;
-    }
-    if (OVERLAP) {
-        TriG g{};
-        uint32_t id = 0;
-        if (canTest) g = walk_tri(s, w.cur & kSlotMask, &id);
-        WideNode wn{};
-        if (canVisit && finite) wn = load_wide(s, w.node);
-        occluded = canTest && walk_test<COUNT>(s, w, g, id, shadow, ts);
-        if (canVisit && !occluded) {
-            const float lim = cull_limit(w.h.t);
-            const int kind = shadow ? 2 : 0;
-            if (finite) walk_visit_wide<COUNT>(s, w, wn, lim, st, ts, kind);
-            else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
-        }
-        return occluded;
-    }
-    if (canTest) {
-        uint32_t id;
-        const TriG g = walk_tri(s, w.cur & kSlotMask, &id);
-        occluded = walk_test<COUNT>(s, w, g, id, shadow, ts);
-    }
+    CwNode cn{};
+    if (OVERLAP && canVisit && finite) cn = load_cw(s, w.node);
+    if (canTest) occluded = cw_test<COUNT>(s, w, shadow, finite, ts);
     if (canVisit && !occluded) {
         const float lim = cull_limit(w.h.t);
         const int kind = shadow ? 2 : 0;
-        if (finite) walk_visit_wide<COUNT>(s, w, load_wide(s, w.node), lim, st, ts, kind);
+        if (finite) cw_visit<COUNT>(s, w, OVERLAP ? cn : load_cw(s, w.node), lim, st, ts, kind);
         else walk_visit<false, COUNT>(s, w, lim, st, ts, kind);
     }
     return occluded;
@@ -533,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
             ++rc.ts.lane_slots;
             if (has) ++rc.ts.lane_iters;
         }
-        const bool occluded = walk_work<COUNT, OVERLAP, !MULTI>(s, w, has, shadow, finite, st, rc.ts);
+        const bool occluded = walk_work<COUNT, OVERLAP>(s, w, has, shadow, finite, st, rc.ts);
         if (has && walk_done(w)) {
             if (!shadow) hits[idx] = make_float4(w.h.t, __uint_as_float(w.h.id), w.h.u, w.h.v);
             else if (!occluded) apply_light(b, b.shadow + 4u * idx);
@@ -907,7 +843,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
             ++rc.ts.lane_slots;
             if (walking) ++rc.ts.lane_iters;
         }
-        if (walk_work<COUNT, false, true>(s, w, walking, mode == kMegaShadow, finite, st, rc.ts)) occluded = true;
+        if (walk_work<COUNT, false>(s, w, walking, mode == kMegaShadow, finite, st, rc.ts)) occluded = true;
     }
     const uint32_t total = wave_sum(finished);
     if ((threadIdx.x & 63u) == 0u && total) atomicAdd(A.done, total);
