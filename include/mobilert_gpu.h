@@ -286,7 +286,7 @@ typedef struct mrt_tuning {
                                   (64-4096; 128, the persistent megakernel 64) */
     uint32_t launch_share_pct; /* workgroups of one slot group's trace launch, %
                                   of those the chip holds at once (1-100; 100
-                                  for one group, 180 / groups for more)        */
+                                  for one group, 120 / groups for more)        */
     uint32_t max_slot_lanes;   /* path slots per pixel at most (1, 2, 4, 8, 16; 8) */
     uint32_t max_groups;       /* slot groups of a frame at most (1-3; 3)      */
     uint32_t overlap_loads;    /* persistent trace issues each step's triangle and

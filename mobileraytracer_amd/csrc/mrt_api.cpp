@@ -1652,15 +1652,15 @@ static int render_impl(mrt_scene* s, const mrt_render_desc* d, uint32_t* bitmap,
     }
     if ((rc = prepare_aux(s)) != MRT_OK) return rc;
     // Kernel shape.  AUTO: the persistent megakernel for PathTracer frames
-    // of BVH scenes of triangles and lights below 2^24 pixel samples (one
-    // launch, no iterations: on the 1M-triangle soup 3.95 / 9.81 / 17.2 /
-    // 33.2 / 59.9 ms for 256x256 x 1, 640x480 x 4, 1280x720 x 4, 1920x1080 x 4
-    // and 1280x720 x 16 samples against 7.49 / 13.9 / 20.2 / 34.3 / 60.7 ms for
-    // the wavefront), the wavefront above (its trace kernel runs 7 waves per
-    // SIMD against the megakernel's 4: 58.3 / 114.0 / 226 ms for 1920x1080 x
-    // 8, 16 and 32 samples against 62.5 / 125.1 / 245 ms), for the other
-    // frames it can run and for frames with auxiliary outputs (it records them
-    // while shading), else the megakernel per sample pass.
+    // of BVH scenes of triangles and lights below 6 x 2^20 pixel samples (one
+    // launch, no iterations: on the 1M-triangle soup 3.95 / 9.81 / 17.2 / 29.4
+    // ms for 256x256 x 1, 640x480 x 4, 1280x720 x 4 and 640x480 x 16 samples
+    // against 7.49 / 13.9 / 20.2 / 34.3 ms for the wavefront), the wavefront
+    // above (its trace kernel runs 7 waves per SIMD against the megakernel's
+    // 4: 29.4 / 31.9 / 53.9 / 106.3 ms for 1280x720 x 8, 1920x1080 x 4, 8 and
+    // 16 samples against 30.4 / 33.2 / 62.3 / 124.8 ms), for the other frames
+    // it can run and for frames with auxiliary outputs (it records them while
+    // shading), else the megakernel per sample pass.
     const bool wfEligible = wavefront_eligible(s, d);
     const Accel& triAcc = s->dev.acc[MRT_CAT_TRIANGLE];
     const bool megaEligible = wfEligible && d->persistent != 2u && !s->generic && triAcc.kind == MRT_BVH &&

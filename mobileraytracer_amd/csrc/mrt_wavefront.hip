@@ -76,6 +76,62 @@ __device__ __forceinline__ void block_range(uint32_t n, uint32_t* begin, uint32_
     *end = min(n, *begin + chunk);
 }
 
+// Reserves `want[k] ? 1 : 0` entries of queue k per thread with one atomic
+// per queue and workgroup: the waves' counts meet in LDS, K threads reserve
+// the workgroup's entries, and each wave takes its range after the waves
+// before it.  Same-address atomics serialise device-wide (about 10 ns each),
+// so one per wave made the shading kernels wait on their queue counters most
+// of their time.  Every thread of the workgroup calls it, the same number of
+// times.  Returns each thread's entries in out (valid where want is true).
+template <int K>
+__device__ __forceinline__ void block_append(uint32_t* const (&counter)[K], const bool (&want)[K],
+                                             uint32_t (&out)[K]) {
+    constexpr uint32_t kWaves = kBlock / 64u;
+    __shared__ uint32_t tot[K][kWaves];
+    __shared__ uint32_t base[K];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long mask[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) mask[k] = __ballot(want[k]);
+    __syncthreads();  // the previous call's readers are done with tot / base
+    if (lane == 0u) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) tot[k][wave] = static_cast<uint32_t>(__popcll(mask[k]));
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<uint32_t>(K)) {
+        const uint32_t k = threadIdx.x;
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) sum += tot[k][w];
+        base[k] = sum ? atomicAdd(counter[k], sum) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uint32_t off = base[k];
+        for (uint32_t w = 0; w < wave; ++w) off += tot[k][w];
+        out[k] = off + static_cast<uint32_t>(__popcll(mask[k] & below));
+    }
+}
+
+// The ray counters of a workgroup, summed in LDS and added with one atomic
+// each (every thread of the workgroup calls it).
+__device__ __forceinline__ void block_flush_rays(const RayCounts& rc, unsigned long long* counters) {
+    __shared__ uint32_t part[4][kBlock / 64u];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t v[4] = {wave_sum(rc.camera), wave_sum(rc.extension), wave_sum(rc.shadow), wave_sum(rc.specular)};
+    __syncthreads();
+    if (lane == 0u)
+        for (int k = 0; k < 4; ++k) part[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 4u) {
+        unsigned long long sum = 0;
+        for (uint32_t w = 0; w < kBlock / 64u; ++w) sum += part[threadIdx.x][w];
+        if (sum) atomicAdd(counters + threadIdx.x, sum);
+    }
+}
+
 // Reserves `want ? 1 : 0` slots of a queue per lane with one atomic per wave;
 // returns the lane's slot (valid only where want is true).
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
@@ -970,8 +1026,11 @@ __global__ __launch_bounds__(kBlock) void k_wf_regen(RenderParams p0, uint32_t s
                 last = true;
             }
         }
-        const uint32_t qe = wave_append(count_out, again);
-        if (again) put_ray(b, queue_out, qe, o, d, MRT_ID_NONE, slot);
+        uint32_t* const cq[1] = {count_out};
+        const bool wq[1] = {again};
+        uint32_t qe[1];
+        block_append<1>(cq, wq, qe);
+        if (again) put_ray(b, queue_out, qe[0], o, d, MRT_ID_NONE, slot);
         if (MULTI) count_finished(bd, job, last);
     }
     const uint32_t total = wave_sum(finished);
@@ -1080,9 +1139,12 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s0, RenderParams p
                 wantEnd = !wantExt;
             }
         }
-        const uint32_t qe = wave_append(count_out, wantExt);
+        uint32_t* const cq[3] = {count_out, count_shadow, count_ended};
+        const bool wq[3] = {wantExt, wantShadow, wantEnd};
+        uint32_t q[3];
+        block_append<3>(cq, wq, q);
+        const uint32_t qe = q[0], qs = q[1], qd = q[2];
         if (wantExt) put_ray(b, queue_out, qe, nextO, nextD, excl, slot);
-        const uint32_t qs = wave_append(count_shadow, wantShadow);
         if (wantShadow) {
             float4* e = b.shadow + 4u * qs;
             e[0] = make_float4(nextO.x, nextO.y, nextO.z, shD.x);
@@ -1090,10 +1152,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(SceneDev s0, RenderParams p
             e[2] = make_float4(Ld.x, Ld.y, Ld.z, __uint_as_float(slot));
             e[3] = make_float4(Tv.x, Tv.y, Tv.z, __uint_as_float(job));
         }
-        const uint32_t qd = wave_append(count_ended, wantEnd);
         if (wantEnd) ended[qd] = slot;
     }
-    flush_counts<false>(rc, counters, 0u, nullptr);
+    block_flush_rays(rc, counters);
 }
 
 // ------------------------------------------------------------- reorder --
@@ -1219,7 +1280,7 @@ uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t gro
     }
     const uint32_t resident = static_cast<uint32_t>(perCu) * num_cu;
     const uint32_t share = share_pct ? std::max(1u, resident * std::min(share_pct, 100u) / 100u)
-                           : groups > 1u ? std::max(1u, resident * 9u / (5u * groups)) : resident;
+                           : groups > 1u ? std::max(1u, resident * 6u / (5u * groups)) : resident;
     return std::min({cap, resident, share});
 }
 

@@ -172,15 +172,15 @@ inline uint32_t wavefront_lanes(uint32_t pixel_slots, uint32_t samples, uint32_t
 }
 
 // Workgroups of one persistent trace launch when `groups` launches run
-// concurrently: 9 / (5 groups) of those that can be resident at once on
+// concurrently: 6 / (5 groups) of those that can be resident at once on
 // `num_cu` compute units (all of them for one group), at most `cap`.  A launch
-// carries a few rays per lane only if it does not claim the whole chip; on the
-// 1M-triangle 1080p frame with three groups and one slot per pixel, launches of
-// 25 / 34 / 40 / 46 / 67 / 100 % of the resident workgroups took 205 / 193 /
-// 187 / 190 / 204 / 206 ms per frame; with 8 slots per pixel, 40 / 50 / 60 / 70
-// % took 148 / 140 / 140 / 153 ms when waves refilled at 8 idle lanes, and 50
-// / 60 / 65 / 70 % took 138.7 / 133.9 / 140.1 / 147.0 ms with the refill at 32
-// idle lanes.
+// carries a few rays per lane only if it does not claim the whole chip, and
+// the regen and shade launches of the other groups need room beside it; on
+// the 1M-triangle 1080p frame with three groups and one slot per pixel,
+// launches of 25 / 34 / 40 / 46 / 67 / 100 % of the resident workgroups took
+// 205 / 193 / 187 / 190 / 204 / 206 ms per frame; with 8 slots per pixel, the
+// compressed tree and workgroup-aggregated queue appends, 30 / 40 / 50 / 60 /
+// 100 % took 106.0 / 103.9 / 105.0 / 106.4 / 110.7 ms.
 uint32_t wavefront_persistent_blocks(uint32_t num_cu, uint32_t cap, uint32_t groups, uint32_t share_pct = 0);
 
 // Launch sizes of one slot group's iteration when at most `slots` of its
@@ -258,7 +258,7 @@ uint32_t mega_blocks(uint32_t num_cu);
 // 1M-triangle soup from 256x256 x 1 to 1920x1080 x 16 samples).
 constexpr uint32_t kDefaultServiceLanes = 16;
 // AUTO renders frames of at least this many pixel samples with the wavefront.
-constexpr uint64_t kMegaMaxPixelSamples = 1ull << 24;
+constexpr uint64_t kMegaMaxPixelSamples = 6ull << 20;
 constexpr uint32_t kDefaultMegaRefillLanes = 32;
 constexpr uint32_t kDefaultMegaFetchChunk = 64;
 
