@@ -527,6 +527,77 @@ MRT_API int mrt_denoiser_destroy(mrt_denoiser* dn);
 MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, uint32_t width, uint32_t cin,
                              const uint16_t* w, const float* bias, uint32_t cout, int relu, uint16_t* y, int mem);
 
+/* ---- denoiser training ----------------------------------------------------- */
+/* No counterpart in the reference.  Trains the network above on pairs the
+ * renderer produced: input = the packed network input of a low-sample frame
+ * (10 FP32 per pixel: log(1 + max(radiance, 0)) (3), albedo (3), normal (3),
+ * depth * depth_scale (1)), target = log(1 + max(radiance, 0)) of an
+ * independent high-sample frame (3 FP32 per pixel).  A step takes `batch`
+ * square crops of side `crop` (frame, position and left-right / top-bottom
+ * mirroring drawn from the seed and the step index), runs the network in
+ * BF16 (weights and activations rounded to BF16 as mrt_denoiser_run does,
+ * the output y rounded to BF16 too), the loss mean((y - target)^2) over the
+ * batch's pixels and 3 channels, the gradients (BF16 activation gradients,
+ * FP32 accumulation, FP32 weight gradients; weights carried straight through
+ * their BF16 rounding) and one Adam update of FP32 master weights
+ * (torch.optim.Adam's arithmetic with its double-precision hyperparameters
+ * rounded where torch rounds them, no weight decay).  Kernels: BF16 MFMA for
+ * every convolution's forward, backward-data and weight-gradient products. */
+typedef struct mrt_denoiser_trainer mrt_denoiser_trainer;
+
+typedef struct mrt_denoiser_train_desc {
+    uint32_t batch;   /* crops per step, 1..64 (0: 8)                                    */
+    uint32_t crop;    /* crop side, a multiple of 8 * 2^(levels - 1), at most 4096 (0: 256) */
+    double beta1;     /* Adam (0: 0.9)                                                   */
+    double beta2;     /* (0: 0.999)                                                      */
+    double eps;       /* (0: 1e-8)                                                       */
+    uint64_t seed;    /* crop choice                                                     */
+    uint32_t reserved[8];
+} mrt_denoiser_train_desc;
+
+typedef struct mrt_denoiser_train_stats {
+    uint64_t steps;        /* Adam steps taken                                         */
+    double last_loss;      /* mean loss of the last mrt_denoiser_trainer_step call     */
+    float last_ms;         /* device milliseconds of that call (events around it)      */
+    float ms_per_step;     /* last_ms / its steps                                      */
+    double flops_per_step; /* algorithmic FLOP of one step: forward, backward-data
+                              (all layers but the first) and weight-gradient products,
+                              2 * 9 * cin * cout per pixel, logical channels           */
+    uint32_t reserved[8];
+} mrt_denoiser_train_stats;
+
+/* weights: the initial FP32 weights in mrt_denoiser_create's layout. */
+MRT_API int mrt_denoiser_trainer_create(mrt_ctx* ctx, const mrt_denoiser_desc* net, const float* weights,
+                                        uint64_t count, const mrt_denoiser_train_desc* desc,
+                                        mrt_denoiser_trainer** out);
+/* The training pairs: count frames of width x height pixels, inputs (10 FP32
+ * per pixel) and targets (3 FP32 per pixel), copied to the device (host or
+ * device memory per mem).  Frames must be at least crop x crop. */
+MRT_API int mrt_denoiser_trainer_set_pairs(mrt_denoiser_trainer* tr, const float* inputs, const float* targets,
+                                           uint32_t count, uint32_t width, uint32_t height, int mem);
+/* The crops step `step` takes: batch x 5 values (frame, top, left, mirrored
+ * left-right, mirrored top-bottom).  Crop i of step s: with mix(z) = the
+ * splitmix64 finaliser of z + 0x9E3779B97F4A7C15 and k = s * batch + i,
+ * r1 = mix(seed ^ mix(2k)), r2 = mix(seed ^ mix(2k + 1)); frame = r1 mod
+ * count, top = (r1 >> 32) mod (height - crop + 1), left = r2 mod (width -
+ * crop + 1), mirrored left-right = bit 40 of r2, top-bottom = bit 41. */
+MRT_API int mrt_denoiser_trainer_crops(const mrt_denoiser_trainer* tr, uint64_t step, uint32_t* crops);
+/* `steps` training steps at learning rate lr; *loss (may be NULL) = the mean
+ * of their losses. */
+MRT_API int mrt_denoiser_trainer_step(mrt_denoiser_trainer* tr, uint32_t steps, double lr, double* loss);
+/* One forward and backward pass over a given batch (batch <= the trainer's,
+ * crops of the trainer's side: inputs batch x crop x crop x 10, targets
+ * batch x crop x crop x 3, host or device memory per mem): *loss (may be
+ * NULL) = its loss, grad (may be NULL, count entries) = the gradient in the
+ * weights' layout; with lr > 0 an Adam update follows (a step). */
+MRT_API int mrt_denoiser_trainer_gradient(mrt_denoiser_trainer* tr, const float* inputs, const float* targets,
+                                          uint32_t batch, double lr, double* loss, float* grad, uint64_t count,
+                                          int mem);
+/* The FP32 master weights in mrt_denoiser_create's layout. */
+MRT_API int mrt_denoiser_trainer_weights(mrt_denoiser_trainer* tr, float* weights, uint64_t count);
+MRT_API int mrt_denoiser_trainer_stats(const mrt_denoiser_trainer* tr, mrt_denoiser_train_stats* stats);
+MRT_API int mrt_denoiser_trainer_destroy(mrt_denoiser_trainer* tr);
+
 /* ---- host-side construction helpers (no device work) ------------------- */
 /* Triangle constructors (Triangle.cpp:12-55): normals/colors may be NULL. */
 MRT_API int mrt_triangle_make(const float a[3], const float b[3], const float c[3], int32_t material_id,

@@ -438,6 +438,95 @@ class Denoiser:
         return v.value
 
 
+def denoiser_train_desc(batch: int = 8, crop: int = 256, seed: int = 0, beta1: float = 0.9, beta2: float = 0.999,
+                        eps: float = 1e-8) -> abi.DenoiserTrainDesc:
+    d = abi.DenoiserTrainDesc()
+    d.batch, d.crop, d.seed = batch, crop, seed
+    d.beta1, d.beta2, d.eps = beta1, beta2, eps
+    return d
+
+
+def splitmix64(z: int) -> int:
+    """The splitmix64 finaliser of z + 0x9E3779B97F4A7C15 (the trainer's crop
+    schedule)."""
+    m = (1 << 64) - 1
+    z = (z + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+class DenoiserTrainer:
+    """mrt_denoiser_trainer_*: trains the U-Net on the GPU (BF16 MFMA forward,
+    backward-data and weight-gradient kernels, FP32 master weights, Adam).
+    Pairs: packed inputs (n, h, w, 10) and log-space targets (n, h, w, 3),
+    FP32."""
+
+    def __init__(self, ctx: Context, weights: np.ndarray, desc: abi.DenoiserDesc | None = None,
+                 train: abi.DenoiserTrainDesc | None = None):
+        self.ctx = ctx
+        self.desc = desc if desc is not None else denoiser_desc()
+        self.train = train if train is not None else denoiser_train_desc()
+        w = np.ascontiguousarray(weights, np.float32).ravel()
+        self.count = w.size
+        h = C.c_void_p()
+        check(lib().mrt_denoiser_trainer_create(ctx.h, C.byref(self.desc), w.ctypes.data, w.size,
+                                                C.byref(self.train), C.byref(h)), "mrt_denoiser_trainer_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mrt_denoiser_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_pairs(self, inputs: np.ndarray, targets: np.ndarray) -> None:
+        x = np.ascontiguousarray(inputs, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        if x.ndim != 4 or x.shape[3] != 10 or t.shape != x.shape[:3] + (3,):
+            raise ValueError("inputs must be (n, h, w, 10) and targets (n, h, w, 3)")
+        n, h, w = x.shape[:3]
+        check(lib().mrt_denoiser_trainer_set_pairs(self.h, x.ctypes.data, t.ctypes.data, n, w, h, MEM_HOST),
+              "mrt_denoiser_trainer_set_pairs")
+
+    def crops(self, step: int) -> np.ndarray:
+        out = np.zeros((self.train.batch, 5), np.uint32)
+        check(lib().mrt_denoiser_trainer_crops(self.h, step, out.ctypes.data), "mrt_denoiser_trainer_crops")
+        return out
+
+    def step(self, steps: int, lr: float) -> float:
+        loss = C.c_double()
+        check(lib().mrt_denoiser_trainer_step(self.h, steps, lr, C.byref(loss)), "mrt_denoiser_trainer_step")
+        return loss.value
+
+    def gradient(self, inputs: np.ndarray, targets: np.ndarray, lr: float = 0.0) -> tuple[float, np.ndarray]:
+        """(loss, flat gradient) of one batch (b, crop, crop, 10) / (b, crop,
+        crop, 3); an Adam step follows when lr > 0."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        b = x.shape[0]
+        g = np.zeros(self.count, np.float32)
+        loss = C.c_double()
+        check(lib().mrt_denoiser_trainer_gradient(self.h, x.ctypes.data, t.ctypes.data, b, lr, C.byref(loss),
+                                                  g.ctypes.data, g.size, MEM_HOST), "mrt_denoiser_trainer_gradient")
+        return loss.value, g
+
+    def weights(self) -> np.ndarray:
+        w = np.zeros(self.count, np.float32)
+        check(lib().mrt_denoiser_trainer_weights(self.h, w.ctypes.data, w.size), "mrt_denoiser_trainer_weights")
+        return w
+
+    def stats(self) -> abi.DenoiserTrainStats:
+        st = abi.DenoiserTrainStats()
+        check(lib().mrt_denoiser_trainer_stats(self.h, C.byref(st)), "mrt_denoiser_trainer_stats")
+        return st
+
+
 def conv3x3_bf16(ctx: Context, x: np.ndarray, w: np.ndarray, bias: np.ndarray | None, relu: bool) -> np.ndarray:
     """One 3x3 convolution layer on the GPU: x (h, w, cin) and w (3, 3, cout,
     cin) as BF16 bit patterns (uint16), bias FP32; returns (h, w, cout) BF16

@@ -148,6 +148,16 @@ class DenoiserDesc(C.Structure):
                 ("reserved", C.c_uint32 * 5)]
 
 
+class DenoiserTrainDesc(C.Structure):
+    _fields_ = [("batch", C.c_uint32), ("crop", C.c_uint32), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double), ("seed", C.c_uint64), ("reserved", C.c_uint32 * 8)]
+
+
+class DenoiserTrainStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("last_loss", C.c_double), ("last_ms", C.c_float),
+                ("ms_per_step", C.c_float), ("flops_per_step", C.c_double), ("reserved", C.c_uint32 * 8)]
+
+
 class AccelParams(C.Structure):
     _fields_ = [("camera_min", F3), ("camera_max", F3), ("leaf_size", C.c_uint32), ("builder", C.c_uint32),
                 ("reserved", C.c_uint32 * 2)]
@@ -219,6 +229,15 @@ def _proto(lib):
         "mrt_denoiser_last_ms": (C.c_int, [P, C.POINTER(f32)]),
         "mrt_denoiser_destroy": (C.c_int, [P]),
         "mrt_conv3x3_bf16": (C.c_int, [P, P, u32, u32, u32, P, P, u32, C.c_int, P, C.c_int]),
+        "mrt_denoiser_trainer_create": (C.c_int, [P, C.POINTER(DenoiserDesc), P, u64, C.POINTER(DenoiserTrainDesc),
+                                                  C.POINTER(P)]),
+        "mrt_denoiser_trainer_set_pairs": (C.c_int, [P, P, P, u32, u32, u32, C.c_int]),
+        "mrt_denoiser_trainer_crops": (C.c_int, [P, u64, P]),
+        "mrt_denoiser_trainer_step": (C.c_int, [P, u32, C.c_double, C.POINTER(C.c_double)]),
+        "mrt_denoiser_trainer_gradient": (C.c_int, [P, P, P, u32, C.c_double, C.POINTER(C.c_double), P, u64, C.c_int]),
+        "mrt_denoiser_trainer_weights": (C.c_int, [P, P, u64]),
+        "mrt_denoiser_trainer_stats": (C.c_int, [P, C.POINTER(DenoiserTrainStats)]),
+        "mrt_denoiser_trainer_destroy": (C.c_int, [P]),
         "mrt_triangle_make": (C.c_int, [P, P, P, i32, P, P, P]),
         "mrt_camera_make": (C.c_int, [C.c_int, P, P, P, f32, f32, C.POINTER(Camera)]),
         "mrt_camera_aabb": (C.c_int, [C.POINTER(Camera), P]),

@@ -28,10 +28,10 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-GPU_DEVICE_SOURCES = ["mrt_kernels.hip", "mrt_wavefront.hip", "mrt_denoise.hip", "mrt_bvh_gpu.hip",
-                      "mrt_grid_gpu.hip", "mrt_walk_gpu.hip"]
+GPU_DEVICE_SOURCES = ["mrt_kernels.hip", "mrt_wavefront.hip", "mrt_denoise.hip", "mrt_denoise_train.hip",
+                      "mrt_bvh_gpu.hip", "mrt_grid_gpu.hip", "mrt_walk_gpu.hip"]
 GPU_HOST_SOURCES = ["mrt_api.cpp", "mrt_build.cpp", "mrt_scenes.cpp", "mrt_server.cpp", "mrt_obj.cpp",
-                    "mrt_denoise_api.cpp"]
+                    "mrt_denoise_api.cpp", "mrt_denoise_train_api.cpp"]
 HOST_API_SOURCES = ["host/mobilert_api.cpp", "host/c_wrapper.cpp"]
 
 LIB_GPU = HERE / "libmrt_gpu.so"

@@ -72,8 +72,9 @@ constexpr uint32_t kTileRows = 4, kTileW = 64, kHaloW = kTileW + 2, kRowQuads = 
 constexpr uint32_t kTileQuads = (kTileRows + 2) * kHaloW * kRowQuads;
 
 // What a convolution reads and writes: BF16 tensors; the packed network input
-// (first layer); FP32 radiance out (last layer).
-enum : int { kModeTensor = 0, kModePackedIn = 1, kModeRadOut = 2 };
+// (first layer); FP32 radiance out (last layer); BF16 tensors with the output
+// masked by a ReLU's output (training's backward-data convolutions).
+enum : int { kModeTensor = 0, kModePackedIn = 1, kModeRadOut = 2, kModeMasked = 3 };
 
 // Stages the packed network input (32 channels, 0..9 used) of the tile rows
 // and halo: log(1 + max(rad * rad_scale, 0)), aov[0..5], aov[6] *
@@ -135,8 +136,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = uniform(tid >> 6);
     const uint32_t li = lane & 15u, lg = lane >> 4;
     const int x0 = static_cast<int>(blockIdx.x * kTileW), y0 = static_cast<int>(blockIdx.y * kTileRows);
-    const uint32_t n0 = blockIdx.z * NT;
+    // blockIdx.z = frame * (cout / NT) + channel block.
+    const uint32_t nblk = uniform(a.cout / NT);
+    const uint32_t img = uniform(blockIdx.z / nblk);
+    const uint32_t n0 = (blockIdx.z - img * nblk) * NT;
     const int H = static_cast<int>(a.H), W = static_cast<int>(a.W);
+    const size_t plane = static_cast<size_t>(a.H) * a.W;
+    const uint16_t* x_img = a.x + img * plane * a.x_stride;
+    const uint16_t* xu_img = a.xu + img * (plane / 4u) * a.cu;
     // Accumulators start at the bias: output channels n0 + 16 j + 4 lg + v.
     floatx4 acc[NJ][4];
 #pragma unroll
@@ -173,7 +180,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
     uint4 sv[kRows + 1];
     auto load_input = [&](uint32_t kc) {
         const bool up = kc < cu;
-        const uint16_t* xb = up ? a.xu + kc : a.x + a.x_off + (kc - cu);
+        const uint16_t* xb = up ? xu_img + kc : x_img + a.x_off + (kc - cu);
         const size_t row_el = up ? static_cast<size_t>(a.W >> 1) * cu : static_cast<size_t>(a.W) * xs;
         const uint32_t o_m = up ? oum : om;
 #pragma unroll
@@ -275,7 +282,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
 #pragma unroll
                     for (int v = 0; v < 4; ++v) acc[j][m][v] = fmaxf(acc[j][m][v], 0.0f);
         }
-        uint16_t* yb = a.y + static_cast<size_t>(y) * a.W * a.y_stride + a.y_off + n0;
+        if constexpr (MODE == kModeMasked) {
+            // Zero where the ReLU's output is not above zero (BF16 bits as a
+            // signed 16-bit integer > 0).
+            const uint16_t* mb =
+                a.mask + (img * plane + static_cast<size_t>(y) * a.W) * a.mask_stride + a.mask_off + n0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t x = static_cast<uint32_t>(x0) + 16u * m + li;
+                if (x >= a.W) continue;
+                const uint32_t o = x * a.mask_stride + 4u * lg;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    const uint2 mv = *reinterpret_cast<const uint2*>(mb + o + 16u * j);
+                    const uint32_t w4[2] = {mv.x, mv.y};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int16_t bits = static_cast<int16_t>(w4[v >> 1] >> (16 * (v & 1)));
+                        if (bits <= 0) acc[j][m][v] = 0.0f;
+                    }
+                }
+            }
+        }
+        uint16_t* yb = a.y + (img * plane + static_cast<size_t>(y) * a.W) * a.y_stride + a.y_off + n0;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             const uint32_t x = static_cast<uint32_t>(x0) + 16u * m + li;
@@ -322,6 +351,11 @@ void conv_launch(const ConvArgs& a, dim3 grid, hipStream_t stream) {
 hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t stream) {
     if (a.cin == 0 || a.cin % 32u || a.cout % 32u || a.cu % 32u || a.cu > a.cin || a.x_off % 8u || a.x_stride % 8u)
         return hipErrorInvalidValue;
+    const uint32_t batch = a.batch ? a.batch : 1u;
+    if (batch > 1u && (a.rad || a.rad_out)) return hipErrorInvalidValue;
+    if (a.mask && (a.rad_out || a.relu || a.mask_off % 4u || a.mask_stride % 4u || a.mask_off + a.cout > a.mask_stride ||
+                   (reinterpret_cast<uintptr_t>(a.mask) & 7u)))
+        return hipErrorInvalidValue;
     if (a.rad ? (a.cin != 32u || a.cu != 0u || !a.aov || a.fw > a.W || a.fh > a.H)
               : (a.cin > a.cu && (!a.x || a.x_off + (a.cin - a.cu) > a.x_stride)))
         return hipErrorInvalidValue;
@@ -334,13 +368,18 @@ hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t st
     if (a.rad_out ? (a.fw > a.W || a.fh > a.H)
                   : (!a.y || a.y_off % 4u || a.y_stride % 4u || a.y_off + a.cout > a.y_stride))
         return hipErrorInvalidValue;
-    const uint32_t tiles = ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileRows - 1) / kTileRows);
+    const uint32_t tiles = ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileRows - 1) / kTileRows) * batch;
     const uint32_t nt = (a.cout % 64u == 0 && tiles * (a.cout / 64u) >= min_blocks) ? 64u : 32u;
-    dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileRows - 1) / kTileRows, a.cout / nt);
+    if (static_cast<uint64_t>(a.cout / nt) * batch > 65535u) return hipErrorInvalidValue;
+    dim3 grid((a.W + kTileW - 1) / kTileW, (a.H + kTileRows - 1) / kTileRows, a.cout / nt * batch);
     if (a.rad)
         conv_launch<32, kModePackedIn>(a, dim3(grid.x, grid.y, a.cout / 32u), stream);
     else if (a.rad_out)
         conv_launch<32, kModeRadOut>(a, dim3(grid.x, grid.y, a.cout / 32u), stream);
+    else if (a.mask && nt == 64u)
+        conv_launch<64, kModeMasked>(a, grid, stream);
+    else if (a.mask)
+        conv_launch<32, kModeMasked>(a, grid, stream);
     else if (nt == 64u)
         conv_launch<64, kModeTensor>(a, grid, stream);
     else

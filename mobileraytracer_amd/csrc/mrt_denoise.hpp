@@ -27,6 +27,14 @@ namespace mrt {
 //     channels 0..2 for the pixels of the fw x fh frame (3 floats per pixel,
 //     row length fw); the other channels are dropped.
 // Offsets and strides are multiples of 8.
+//
+// Batches (training): batch > 1 frames of H x W pixels, each tensor's frames
+// one after the other (x: H W x_stride elements apart, xu: H W cu / 4, y and
+// mask: H W times their strides); only for tensor inputs and outputs.
+//
+// Masked output (training, the gradient through a ReLU): when mask is set,
+// output channel k of a pixel is zeroed unless channel mask_off + k of the
+// same pixel of mask (mask_stride channels per pixel, BF16) is above zero.
 struct ConvArgs {
     const uint16_t* x;
     uint32_t x_stride, x_off, cin;
@@ -44,13 +52,128 @@ struct ConvArgs {
     uint32_t fw, fh;
     uint32_t H, W;
     uint32_t relu;
+    uint32_t batch;  // frames (0 or 1: one)
+    const uint16_t* mask;
+    uint32_t mask_stride, mask_off;
 };
 
 // min_blocks: the channel block per workgroup (32 or 64 output channels) is
 // the widest that still gives this many workgroups.
 hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t stream);
 // 2x2 max pooling: x (H x W, c channels per pixel, H and W even, c a
-// multiple of 8) into y (H/2 x W/2, c channels per pixel).
+// multiple of 8) into y (H/2 x W/2, c channels per pixel).  A batch of frames
+// of even height pools as one frame of the summed height.
 hipError_t launch_pool2(const uint16_t* x, uint32_t c, uint32_t H, uint32_t W, uint16_t* y, hipStream_t stream);
+
+// ---- training (mrt_denoise_train.hip) --------------------------------------
+//
+// Octet layout, the operand layout of the weight-gradient kernel: a tensor of
+// B frames, R rows, W pixels per row (W a multiple of 8) and C channels, with
+// element (b, r, x, c) at (((b R + r) (W / 8) + x / 8) C + c) 8 + x % 8: eight
+// consecutive pixels of one channel are 16 contiguous bytes.
+
+// What a convolution reads, for the octet transpose: channels [0, cu) of xu
+// upsampled 2x (nearest), then channels [x_off, x_off + c - cu) of x.
+struct TensorIn {
+    const uint16_t* x;
+    uint32_t x_stride, x_off;
+    const uint16_t* xu;
+    uint32_t cu;
+    uint32_t c;  // channels (multiple of 8)
+};
+
+// Plain octet copy (shifted = 0: R = H rows) or the three column-shifted,
+// row-padded copies of a convolution input (shifted = 1: copy dx in 0..2 holds
+// input pixel (r - 1, x + dx - 1) at row r of H + 2, zero outside the frame).
+hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t W, int shifted, uint16_t* out,
+                        hipStream_t stream);
+
+// Weight gradients of one 3x3 convolution: partial sums per K split of
+// dW[tap][co][ci] = sum over pixels of dZ(pixel, co) X(pixel + tap offset, ci)
+// and db[co] = sum of dZ(pixel, co), from dZ in octet layout (B x H x W, cout
+// channels) and X as the three shifted octet copies (cin channels); FP32
+// accumulation on MFMA.  part_w: [splits][9][cout][cin], part_b:
+// [splits][cout]; `splits` is chosen here (at most max_splits) and returned.
+// K steps of 4 segments per wave and the number of splits launch_wgrad uses:
+// about target_waves waves, at least 32 K steps (1024 pixels) per wave, at
+// most max_splits splits.
+inline uint32_t wgrad_splits(uint32_t batch, uint32_t H, uint32_t W, uint32_t cin, uint32_t cout, uint32_t max_splits,
+                             uint32_t target_waves, uint32_t* kps) {
+    const uint32_t nseg = batch * H * (W / 8u), ksteps = (nseg + 3u) / 4u;
+    const uint32_t per_split = 3u * (cin / 32u) * (cout / 32u);
+    uint32_t s = (target_waves + per_split - 1) / per_split;
+    if (s > max_splits) s = max_splits;
+    if (s > (ksteps + 31u) / 32u) s = (ksteps + 31u) / 32u;
+    if (s < 1u) s = 1u;
+    const uint32_t k = (ksteps + s - 1) / s;
+    if (kps) *kps = k;
+    return (ksteps + k - 1) / k;
+}
+hipError_t launch_wgrad(const uint16_t* dzt, const uint16_t* xt, uint32_t batch, uint32_t H, uint32_t W, uint32_t cin,
+                        uint32_t cout, uint32_t max_splits, uint32_t target_waves, float* part_w, float* part_b,
+                        uint32_t* splits, hipStream_t stream);
+// out[i] = sum over s < splits of part[s * n + i] (in order of s).
+hipError_t launch_sum_splits(const float* part, uint32_t splits, size_t n, float* out, hipStream_t stream);
+
+// Loss of a batch: y (n pixels x 32 BF16 channels, 0..2 used), target t (n x
+// 3 FP32): loss_sum += sum of (y - t)^2 (FP64 accumulation), dy = BF16((y -
+// t) scale) in channels 0..2 of 32 (the others zero).
+hipError_t launch_mse(const uint16_t* y, const float* t, size_t n, float scale, uint16_t* dy, double* loss_sum,
+                      hipStream_t stream);
+
+// Gradient of an encoder output e (n_rows x W pixels, c channels; rows in
+// pairs) that feeds both a 2x2 max pooling and a skip connection:
+// dz = BF16((dskip + routed dp) * (e > 0)), dp (pooled grid, c channels)
+// going to the first position of its window, in row-major order, that holds
+// the window's maximum; dskip channels at dskip_off of dskip_stride per pixel.
+hipError_t launch_pool2_back(const uint16_t* e, const uint16_t* dp, const uint16_t* dskip, uint32_t dskip_stride,
+                             uint32_t dskip_off, uint32_t c, uint32_t n_rows, uint32_t W, uint16_t* dz,
+                             hipStream_t stream);
+// Gradient of the coarse tensor of a decoder's 2x upsampling:
+// dz(y, x) = BF16((g(2y, 2x) + g(2y, 2x + 1) + g(2y + 1, 2x) + g(2y + 1, 2x + 1)) * (coarse(y, x) > 0)),
+// g = channels [0, c) of dcat (dcat_stride per pixel, n_rows x W fine grid).
+hipError_t launch_up2_back(const uint16_t* dcat, uint32_t dcat_stride, const uint16_t* coarse, uint32_t c,
+                           uint32_t n_rows, uint32_t W, uint16_t* dz, hipStream_t stream);
+
+// One crop of a training batch: frame n of the data set, top-left (oy, ox),
+// mirrored left-right (fx) and/or top-bottom (fy).
+struct Crop {
+    uint32_t n, oy, ox, fx, fy;
+};
+constexpr uint32_t kMaxCrops = 64;
+struct CropTable {
+    Crop c[kMaxCrops];
+};
+// Gathers `batch` crops of side S from a data set of frames (fw x fh pixels):
+// packed inputs xin (10 FP32 per pixel) to x (S x S x 32 BF16 per crop,
+// channels 10..31 zero) and targets tin (3 FP32 per pixel) to t.
+hipError_t launch_gather(const float* xin, const float* tin, uint32_t fw, uint32_t fh, const CropTable& crops,
+                         uint32_t batch, uint32_t S, uint16_t* x, float* t, hipStream_t stream);
+
+// Adam over a flat FP32 parameter vector (n_w convolution weights in the
+// BF16 weight layout [tap][cout][cin] per layer, then the biases): the update
+// of torch.optim.Adam (m += omb1 (g - m); v = v b2 + (omb2 g) g;
+// p += -step_size (m / (sqrt(v) / bc2_sqrt + eps)), omb = 1 - beta and the
+// others rounded to FP32 from double as torch rounds them), then the BF16 forward
+// weights wq and the transposed, tap-flipped weights wt of the
+// backward-data convolutions ([8 - tap][cin][cout] per layer).
+struct AdamLayer {
+    uint32_t w_off, cin, cout;
+};
+constexpr uint32_t kMaxLayers = 19;
+struct AdamArgs {
+    float* p;
+    const float* g;
+    float* m;
+    float* v;
+    uint16_t* wq;
+    uint16_t* wt;
+    size_t n, n_w;
+    float omb1, b2, omb2, eps, step_size, bc2_sqrt;
+    uint32_t update;  // 0: only write wq and wt from p
+    uint32_t n_layers;
+    AdamLayer layer[kMaxLayers];
+};
+hipError_t launch_adam(const AdamArgs& a, hipStream_t stream);
 
 }  // namespace mrt

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "mrt_denoise.hpp"
+#include "mrt_denoise_net.hpp"
 #include "mrt_internal.hpp"
 
 using mrt::DevBuf;
@@ -14,75 +15,12 @@ using mrt::fail;
 
 namespace {
 
-constexpr uint32_t kInChannels = 10, kOutChannels = 3, kPad = 32, kMaxChannels = 256;
-
-uint16_t bf16_rne(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-    return static_cast<uint16_t>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
-struct Net {
-    uint32_t levels = 5;
-    uint32_t c[MRT_DENOISER_MAX_LEVELS] = {32, 64, 96, 128, 128};
-    float depth_scale = 1.0f;
-};
-
-int resolve(const mrt_denoiser_desc* d, Net* n) {
-    if (!d) return MRT_OK;
-    if (d->levels > MRT_DENOISER_MAX_LEVELS) return fail(MRT_E_INVALID, "denoiser levels must be 1..5");
-    if (d->levels) n->levels = d->levels;
-    bool any = false;
-    for (uint32_t l = 0; l < MRT_DENOISER_MAX_LEVELS; ++l) any = any || d->channels[l];
-    if (any) {
-        for (uint32_t l = 0; l < n->levels; ++l) {
-            const uint32_t c = d->channels[l];
-            if (c == 0 || c % kPad || c > kMaxChannels)
-                return fail(MRT_E_INVALID, "denoiser channels must be multiples of 32 in 32..256");
-            n->c[l] = c;
-        }
-    }
-    if (d->depth_scale != 0.0f) {
-        if (!std::isfinite(d->depth_scale)) return fail(MRT_E_INVALID, "depth_scale must be finite");
-        n->depth_scale = d->depth_scale;
-    }
-    return MRT_OK;
-}
-
-struct LayerShape {
-    uint32_t cin, cout;  // logical
-};
-
-// Layers in weight order: encoder 0..L-1 (two each), decoder L-2..0 (two
-// each), output.
-std::vector<LayerShape> layer_shapes(const Net& n) {
-    std::vector<LayerShape> v;
-    const uint32_t L = n.levels;
-    for (uint32_t l = 0; l < L; ++l) {
-        v.push_back({l == 0 ? kInChannels : n.c[l - 1], n.c[l]});
-        v.push_back({n.c[l], n.c[l]});
-    }
-    for (int l = static_cast<int>(L) - 2; l >= 0; --l) {
-        v.push_back({n.c[l + 1] + n.c[l], n.c[l]});
-        v.push_back({n.c[l], n.c[l]});
-    }
-    v.push_back({n.c[0], kOutChannels});
-    return v;
-}
-
-uint32_t pad32(uint32_t c) { return (c + kPad - 1) / kPad * kPad; }
-
-uint64_t weight_count(const Net& n) {
-    uint64_t k = 0;
-    for (const LayerShape& s : layer_shapes(n)) k += static_cast<uint64_t>(s.cout) * s.cin * 9 + s.cout;
-    return k;
-}
-
-struct DevLayer {
-    uint32_t cin, cout;  // padded
-    size_t w_off, b_off; // elements into the weight / bias buffers
-};
+using mrt::unet::DevLayer;
+using mrt::unet::kPad;
+using mrt::unet::LayerShape;
+using mrt::unet::Net;
+using mrt::unet::resolve;
+using mrt::unet::weight_count;
 
 // Byte offsets of a frame's activations (H x W padded frame): per level l,
 // t = first conv's output, s = encoder output (the skip), p = pooled input
@@ -230,24 +168,12 @@ MRT_API int mrt_denoiser_create(mrt_ctx* ctx, const mrt_denoiser_desc* desc, con
                                        std::to_string(weight_count(n)));
     for (uint64_t i = 0; i < count; ++i)
         if (!std::isfinite(weights[i])) return fail(MRT_E_INVALID, "non-finite denoiser weight");
-    std::vector<uint16_t> wdev;
-    std::vector<float> bdev;
-    std::vector<DevLayer> layers;
-    const float* src = weights;
-    for (const LayerShape& s : layer_shapes(n)) {
-        DevLayer d{pad32(s.cin), pad32(s.cout), wdev.size(), bdev.size()};
-        wdev.resize(wdev.size() + static_cast<size_t>(9) * d.cout * d.cin, 0);
-        bdev.resize(bdev.size() + d.cout, 0.0f);
-        for (uint32_t o = 0; o < s.cout; ++o)
-            for (uint32_t i = 0; i < s.cin; ++i)
-                for (uint32_t tap = 0; tap < 9; ++tap)
-                    wdev[d.w_off + (static_cast<size_t>(tap) * d.cout + o) * d.cin + i] =
-                        bf16_rne(src[(static_cast<size_t>(o) * s.cin + i) * 9 + tap]);
-        src += static_cast<size_t>(s.cout) * s.cin * 9;
-        for (uint32_t o = 0; o < s.cout; ++o) bdev[d.b_off + o] = src[o];
-        src += s.cout;
-        layers.push_back(d);
-    }
+    size_t n_w = 0, n_b = 0;
+    std::vector<DevLayer> layers = mrt::unet::device_layers(n, &n_w, &n_b);
+    std::vector<float> wf(n_w, 0.0f), bdev(n_b, 0.0f);
+    mrt::unet::to_device_layout(layers, weights, wf.data(), bdev.data());
+    std::vector<uint16_t> wdev(n_w);
+    for (size_t i = 0; i < n_w; ++i) wdev[i] = mrt::unet::bf16_rne(wf[i]);
     std::lock_guard<std::mutex> lock(ctx->mu);
     MRT_HIP(hipSetDevice(ctx->device));
     auto dn = std::make_unique<mrt_denoiser>();

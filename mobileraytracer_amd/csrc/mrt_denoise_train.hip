@@ -1,0 +1,475 @@
+// Training kernels of the denoiser's U-Net: weight gradients on MFMA (BF16
+// operands, FP32 accumulation), the octet transposes that feed them, the
+// loss, the gradients through pooling and upsampling, the crop gather and
+// Adam.  The backward-data convolutions are the forward kernel
+// (mrt_denoise.hip) over the transposed, tap-flipped weights with its masked
+// output mode.
+//
+// Weight gradient = GEMM with the pixels as the reduction: dW[tap][co][ci] =
+// sum_p dZ(p, co) X(p + offset(tap), ci).  An MFMA operand lane holds 8
+// consecutive K values, here 8 consecutive pixels of one channel, which NHWC
+// does not store together; the octet layout does (16 bytes per channel and
+// 8-pixel segment), so both operands are single 16-byte loads.  A K step of
+// v_mfma_f32_16x16x32_bf16 is 4 segments of 8 pixels (lane group l / 16 takes
+// segment 4 k + l / 16), anywhere in the batch, so frames of any width that
+// is a multiple of 8 fill every step.  The three column shifts of a tap row
+// are three octet copies of the input, each holding input column x + dx - 1 at
+// x, with a zero row above and below every frame: a tap's operand is then the
+// same segment of another copy and row, with no bounds checks in the loop.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mrt_denoise.hpp"
+
+namespace mrt {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+    const bf16x2 b = __builtin_convertvector(floatx2{lo, hi}, bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &b, sizeof(u));
+    return u;
+}
+
+__device__ __forceinline__ float bf16f(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
+    bf16x8 r;
+    __builtin_memcpy(&r, &v, sizeof(r));
+    return r;
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Element e (0..7) of 8 BF16 values held as a uint4.
+__device__ __forceinline__ uint32_t elem(const uint4& q, int e) {
+    const uint32_t w = (&q.x)[e >> 1];
+    return (e & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+// ---------------------------------------------------------------- octet --
+// Thread: one 8-channel group of one 8-pixel segment of one row.  It reads
+// the segment's 8 pixels (plus one on each side when shifted) as 16-byte
+// channel quads and writes 8 (or 3 x 8) 16-byte channel rows.
+template <bool SHIFTED>
+__global__ __launch_bounds__(256) void k_octet(TensorIn in, uint32_t batch, uint32_t H, uint32_t W, uint16_t* out) {
+    const uint32_t W8 = W / 8u, cg = in.c / 8u;
+    const uint32_t R = SHIFTED ? H + 2u : H;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t total = static_cast<size_t>(batch) * R * W8 * cg;
+    if (i >= total) return;
+    const uint32_t g = static_cast<uint32_t>(i % cg);
+    const size_t seg = i / cg;  // (b R + r) W8 + xo
+    const uint32_t xo = static_cast<uint32_t>(seg % W8);
+    const size_t br = seg / W8;
+    const uint32_t r = static_cast<uint32_t>(br % R), b = static_cast<uint32_t>(br / R);
+    const int y = SHIFTED ? static_cast<int>(r) - 1 : static_cast<int>(r);
+    constexpr int kPx = SHIFTED ? 10 : 8;
+    const int xbase = static_cast<int>(xo * 8u) - (SHIFTED ? 1 : 0);
+    const uint32_t c0 = g * 8u;
+    const bool up = c0 < in.cu;
+    const size_t plane = static_cast<size_t>(H) * W;
+    uint4 px[kPx];
+#pragma unroll
+    for (int k = 0; k < kPx; ++k) {
+        const int x = xbase + k;
+        px[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (y < 0 || y >= static_cast<int>(H) || x < 0 || x >= static_cast<int>(W)) continue;
+        const uint16_t* src;
+        if (up)
+            src = in.xu + (b * (plane / 4u) + static_cast<size_t>(y >> 1) * (W >> 1) + (x >> 1)) * in.cu + c0;
+        else
+            src = in.x + (b * plane + static_cast<size_t>(y) * W + x) * in.x_stride + in.x_off + (c0 - in.cu);
+        px[k] = *reinterpret_cast<const uint4*>(src);
+    }
+    constexpr int kCopies = SHIFTED ? 3 : 1;
+    const size_t copy_elems = static_cast<size_t>(batch) * R * W8 * in.c * 8u;
+#pragma unroll
+    for (int dx = 0; dx < kCopies; ++dx) {
+        uint16_t* o = out + dx * copy_elems + (seg * in.c + c0) * 8u;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            // Channel c0 + e of pixels xo * 8 + j + dx - 1 (shifted) or xo * 8 + j.
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                w[j] = elem(px[2 * j + dx], e) | (elem(px[2 * j + 1 + dx], e) << 16);
+            *reinterpret_cast<uint4*>(o + e * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- wgrad --
+// Wave task (co block of 32, ci block of 32, tap row dy, K split): 2 x 2
+// MFMA tiles for each of the 3 column shifts, 12 accumulators; the waves of
+// ci block 0 and dy = 1 also sum dZ against a ones operand (the bias
+// gradient).  Lane (li = l % 16, lg = l / 16) loads, per K step, channel
+// co0 + 16 jA + li of segment 4 k + lg (A) and channel ci0 + 16 jB + li of
+// the same segment, row + dy, copy dx (B); its results are C[co = 4 lg + v]
+// [ci = li] of each tile.
+struct WgradArgs {
+    const uint16_t* dzt;
+    const uint16_t* xt;
+    uint32_t batch, H, W8, cin, cout;
+    uint32_t nseg, ksteps, kps, tasks;
+    float* part_w;
+    float* part_b;
+};
+
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t task = uniform(blockIdx.x * 4u + (threadIdx.x >> 6));
+    if (task >= a.tasks) return;
+    const uint32_t cob_n = a.cout / 32u, cib_n = a.cin / 32u;
+    const uint32_t cob = task % cob_n;
+    uint32_t rest = task / cob_n;
+    const uint32_t cib = rest % cib_n;
+    rest /= cib_n;
+    const uint32_t dy = rest % 3u, split = rest / 3u;
+    const uint32_t li = lane & 15u, lg = lane >> 4;
+    const uint32_t co0 = cob * 32u, ci0 = cib * 32u;
+    const bool bias = cib == 0u && dy == 1u;
+    const uint32_t k0 = split * a.kps;
+    const uint32_t k1 = min(a.ksteps, k0 + a.kps);
+    const size_t copy_segs = static_cast<size_t>(a.batch) * (a.H + 2u) * a.W8;  // segments per shifted copy
+    const uint32_t frame_segs = a.H * a.W8;
+    floatx4 acc[3][2][2];
+    floatx4 accb[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[d][i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    accb[0] = accb[1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // BF16 1.0 in all 8 K slots.
+    const uint4 ones = make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u);
+    uint4 av[2], bv[3][2];
+    auto load = [&](uint32_t ks) {
+        const uint32_t s = ks * 4u + lg;
+        const bool ok = s < a.nseg;
+        const uint32_t ss = ok ? s : 0u;
+        const uint32_t b = ss / frame_segs;
+        // Row-padded segment index of segment s, tap row dy, in a shifted copy.
+        const size_t sx = ss + static_cast<size_t>(2u * b + dy) * a.W8;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            av[j] = ok ? *reinterpret_cast<const uint4*>(a.dzt + (static_cast<size_t>(ss) * a.cout + co0 + 16u * j + li) * 8u)
+                       : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                bv[d][j] = ok ? *reinterpret_cast<const uint4*>(
+                                    a.xt + ((d * copy_segs + sx) * a.cin + ci0 + 16u * j + li) * 8u)
+                              : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    for (uint32_t ks = k0; ks < k1; ++ks) {
+        load(ks);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[d][i][j] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[i]), as_bf16x8(bv[d][j]), acc[d][i][j], 0, 0, 0);
+        if (bias) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                accb[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[i]), as_bf16x8(ones), accb[i], 0, 0, 0);
+        }
+    }
+    const size_t layer = static_cast<size_t>(a.cout) * a.cin;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float* pw = a.part_w + (static_cast<size_t>(split) * 9u + dy * 3u + d) * layer;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    pw[static_cast<size_t>(co0 + 16u * i + 4u * lg + v) * a.cin + ci0 + 16u * j + li] = acc[d][i][j][v];
+    }
+    if (bias && li == 0u) {
+        float* pb = a.part_b + static_cast<size_t>(split) * a.cout + co0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pb[16u * i + 4u * lg + v] = accb[i][v];
+    }
+}
+
+__global__ void k_sum_splits(const float* __restrict__ part, uint32_t splits, size_t n, float* __restrict__ out) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < splits; ++k) s += part[k * n + i];
+    out[i] = s;
+}
+
+// ----------------------------------------------------------------- loss --
+__global__ __launch_bounds__(256) void k_mse(const uint16_t* __restrict__ y, const float* __restrict__ t, size_t n,
+                                             float scale, uint16_t* __restrict__ dy, double* loss_sum) {
+    __shared__ float red[256];
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float sq = 0.0f;
+    if (i < n) {
+        const uint2 yv = *reinterpret_cast<const uint2*>(y + i * 32u);
+        float d[3];
+        d[0] = bf16f(yv.x & 0xFFFFu) - t[3 * i];
+        d[1] = bf16f(yv.x >> 16) - t[3 * i + 1];
+        d[2] = bf16f(yv.y & 0xFFFFu) - t[3 * i + 2];
+        sq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        uint4* o = reinterpret_cast<uint4*>(dy + i * 32u);
+        o[0] = make_uint4(pack2(d[0] * scale, d[1] * scale), pack2(d[2] * scale, 0.0f), 0u, 0u);
+        o[1] = o[2] = o[3] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss_sum, static_cast<double>(red[0]));
+}
+
+// ------------------------------------------------------- pool / upsample --
+__device__ __forceinline__ bool positive(uint32_t bits16) { return static_cast<int16_t>(bits16) > 0; }
+
+__global__ __launch_bounds__(256) void k_pool2_back(const uint16_t* __restrict__ e, const uint16_t* __restrict__ dp,
+                                                    const uint16_t* __restrict__ dskip, uint32_t dskip_stride,
+                                                    uint32_t dskip_off, uint32_t c, uint32_t n_rows, uint32_t W,
+                                                    uint16_t* __restrict__ dz) {
+    const uint32_t W2 = W / 2u, cq = c / 8u;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<size_t>(n_rows / 2u) * W2 * cq) return;
+    const uint32_t q = static_cast<uint32_t>(i % cq);
+    const size_t p = i / cq;
+    const uint32_t px = static_cast<uint32_t>(p % W2);
+    const size_t py = p / W2;
+    size_t pix[4];
+    uint4 ev[4], sv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        pix[k] = (2u * py + (k >> 1)) * W + 2u * px + (k & 1);
+        ev[k] = *reinterpret_cast<const uint4*>(e + pix[k] * c + q * 8u);
+        sv[k] = *reinterpret_cast<const uint4*>(dskip + pix[k] * dskip_stride + dskip_off + q * 8u);
+    }
+    const uint4 dv = *reinterpret_cast<const uint4*>(dp + p * c + q * 8u);
+    uint32_t o[4][4];
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) {
+        float mx = bf16f(elem(ev[0], ch));
+        int arg = 0;
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            const float v = bf16f(elem(ev[k], ch));
+            if (v > mx) {
+                mx = v;
+                arg = k;
+            }
+        }
+        const float g = bf16f(elem(dv, ch));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float s = bf16f(elem(sv[k], ch));
+            if (k == arg) s = s + g;
+            if (!positive(elem(ev[k], ch))) s = 0.0f;
+            const uint32_t b = pack2(s, 0.0f) & 0xFFFFu;
+            if (ch & 1)
+                o[k][ch >> 1] |= b << 16;
+            else
+                o[k][ch >> 1] = b;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        *reinterpret_cast<uint4*>(dz + pix[k] * c + q * 8u) = make_uint4(o[k][0], o[k][1], o[k][2], o[k][3]);
+}
+
+__global__ __launch_bounds__(256) void k_up2_back(const uint16_t* __restrict__ dcat, uint32_t dcat_stride,
+                                                  const uint16_t* __restrict__ coarse, uint32_t c, uint32_t n_rows,
+                                                  uint32_t W, uint16_t* __restrict__ dz) {
+    const uint32_t W2 = W / 2u, cq = c / 8u;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<size_t>(n_rows / 2u) * W2 * cq) return;
+    const uint32_t q = static_cast<uint32_t>(i % cq);
+    const size_t p = i / cq;
+    const uint32_t px = static_cast<uint32_t>(p % W2);
+    const size_t py = p / W2;
+    uint4 gv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        gv[k] = *reinterpret_cast<const uint4*>(dcat + ((2u * py + (k >> 1)) * W + 2u * px + (k & 1)) * dcat_stride +
+                                                q * 8u);
+    const uint4 cv = *reinterpret_cast<const uint4*>(coarse + p * c + q * 8u);
+    uint32_t o[4];
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) {
+        float s = bf16f(elem(gv[0], ch));
+        s = s + bf16f(elem(gv[1], ch));
+        s = s + bf16f(elem(gv[2], ch));
+        s = s + bf16f(elem(gv[3], ch));
+        if (!positive(elem(cv, ch))) s = 0.0f;
+        const uint32_t b = pack2(s, 0.0f) & 0xFFFFu;
+        if (ch & 1)
+            o[ch >> 1] |= b << 16;
+        else
+            o[ch >> 1] = b;
+    }
+    *reinterpret_cast<uint4*>(dz + p * c + q * 8u) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// --------------------------------------------------------------- gather --
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ xin, const float* __restrict__ tin,
+                                                uint32_t fw, uint32_t fh, CropTable crops, uint32_t S,
+                                                uint16_t* __restrict__ x, float* __restrict__ t) {
+    const uint32_t b = blockIdx.y;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<size_t>(S) * S) return;
+    const uint32_t yy = static_cast<uint32_t>(i / S), xx = static_cast<uint32_t>(i % S);
+    const Crop c = crops.c[b];
+    const uint32_t sy = c.oy + (c.fy ? S - 1u - yy : yy), sx = c.ox + (c.fx ? S - 1u - xx : xx);
+    const size_t src = (static_cast<size_t>(c.n) * fh + sy) * fw + sx;
+    const float* v = xin + src * 10u;
+    const size_t dst = static_cast<size_t>(b) * S * S + i;
+    uint4* o = reinterpret_cast<uint4*>(x + dst * 32u);
+    o[0] = make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+    o[1] = make_uint4(pack2(v[8], v[9]), 0u, 0u, 0u);
+    o[2] = o[3] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[dst * 3u + k] = tin[src * 3u + k];
+}
+
+// ----------------------------------------------------------------- adam --
+__global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    float p = a.p[i];
+    if (a.update) {
+        const float g = a.g[i];
+        float m = a.m[i], v = a.v[i];
+        m = m + a.omb1 * (g - m);
+        v = v * a.b2 + (a.omb2 * g) * g;
+        const float denom = sqrtf(v) / a.bc2_sqrt + a.eps;
+        p = p + (-a.step_size) * (m / denom);
+        a.m[i] = m;
+        a.v[i] = v;
+        a.p[i] = p;
+    }
+    if (i >= a.n_w) return;
+    const uint16_t q = static_cast<uint16_t>(pack2(p, 0.0f) & 0xFFFFu);
+    a.wq[i] = q;
+    uint32_t k = 0;
+    while (k + 1 < a.n_layers && i >= a.layer[k + 1].w_off) ++k;
+    const AdamLayer L = a.layer[k];
+    const uint32_t local = static_cast<uint32_t>(i - L.w_off);
+    const uint32_t per_tap = L.cout * L.cin;
+    const uint32_t tap = local / per_tap, r = local % per_tap;
+    const uint32_t co = r / L.cin, ci = r % L.cin;
+    a.wt[L.w_off + (static_cast<size_t>(8u - tap) * L.cin + ci) * L.cout + co] = q;
+}
+
+uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t W, int shifted, uint16_t* out,
+                        hipStream_t stream) {
+    if (W % 8u || in.c % 8u || in.cu % 8u || in.cu > in.c || in.x_stride % 8u || in.x_off % 8u || !out)
+        return hipErrorInvalidValue;
+    if ((in.cu && (!in.xu || H % 2u || W % 2u)) || (in.c > in.cu && (!in.x || in.x_off + in.c - in.cu > in.x_stride)))
+        return hipErrorInvalidValue;
+    const size_t n = static_cast<size_t>(batch) * (shifted ? H + 2u : H) * (W / 8u) * (in.c / 8u);
+    if (n == 0) return hipSuccess;
+    if (shifted)
+        k_octet<true><<<blocks_for(n), 256, 0, stream>>>(in, batch, H, W, out);
+    else
+        k_octet<false><<<blocks_for(n), 256, 0, stream>>>(in, batch, H, W, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wgrad(const uint16_t* dzt, const uint16_t* xt, uint32_t batch, uint32_t H, uint32_t W, uint32_t cin,
+                        uint32_t cout, uint32_t max_splits, uint32_t target_waves, float* part_w, float* part_b,
+                        uint32_t* splits, hipStream_t stream) {
+    if (cin % 32u || cout % 32u || W % 8u || !batch || !H || !max_splits) return hipErrorInvalidValue;
+    WgradArgs a{};
+    a.dzt = dzt;
+    a.xt = xt;
+    a.batch = batch;
+    a.H = H;
+    a.W8 = W / 8u;
+    a.cin = cin;
+    a.cout = cout;
+    a.nseg = batch * H * a.W8;
+    a.ksteps = (a.nseg + 3u) / 4u;
+    const uint32_t per_split = 3u * (cin / 32u) * (cout / 32u);
+    const uint32_t s = wgrad_splits(batch, H, W, cin, cout, max_splits, target_waves, &a.kps);
+    a.tasks = s * per_split;
+    a.part_w = part_w;
+    a.part_b = part_b;
+    // Every split has the (co block, ci block 0, dy 1) task of each co block,
+    // so every bias partial is written.
+    k_wgrad<<<(a.tasks + 3u) / 4u, 256, 0, stream>>>(a);
+    *splits = s;
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_splits(const float* part, uint32_t splits, size_t n, float* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_sum_splits<<<blocks_for(n), 256, 0, stream>>>(part, splits, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mse(const uint16_t* y, const float* t, size_t n, float scale, uint16_t* dy, double* loss_sum,
+                      hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_mse<<<blocks_for(n), 256, 0, stream>>>(y, t, n, scale, dy, loss_sum);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool2_back(const uint16_t* e, const uint16_t* dp, const uint16_t* dskip, uint32_t dskip_stride,
+                             uint32_t dskip_off, uint32_t c, uint32_t n_rows, uint32_t W, uint16_t* dz,
+                             hipStream_t stream) {
+    if (c % 8u || n_rows % 2u || W % 2u || dskip_stride % 8u || dskip_off % 8u || dskip_off + c > dskip_stride)
+        return hipErrorInvalidValue;
+    const size_t n = static_cast<size_t>(n_rows / 2u) * (W / 2u) * (c / 8u);
+    if (n == 0) return hipSuccess;
+    k_pool2_back<<<blocks_for(n), 256, 0, stream>>>(e, dp, dskip, dskip_stride, dskip_off, c, n_rows, W, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_up2_back(const uint16_t* dcat, uint32_t dcat_stride, const uint16_t* coarse, uint32_t c,
+                           uint32_t n_rows, uint32_t W, uint16_t* dz, hipStream_t stream) {
+    if (c % 8u || n_rows % 2u || W % 2u || dcat_stride % 8u || c > dcat_stride) return hipErrorInvalidValue;
+    const size_t n = static_cast<size_t>(n_rows / 2u) * (W / 2u) * (c / 8u);
+    if (n == 0) return hipSuccess;
+    k_up2_back<<<blocks_for(n), 256, 0, stream>>>(dcat, dcat_stride, coarse, c, n_rows, W, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const float* xin, const float* tin, uint32_t fw, uint32_t fh, const CropTable& crops,
+                         uint32_t batch, uint32_t S, uint16_t* x, float* t, hipStream_t stream) {
+    if (batch == 0 || batch > kMaxCrops || S == 0 || S > fw || S > fh) return hipErrorInvalidValue;
+    for (uint32_t b = 0; b < batch; ++b)
+        if (crops.c[b].oy + S > fh || crops.c[b].ox + S > fw) return hipErrorInvalidValue;
+    k_gather<<<dim3(blocks_for(static_cast<size_t>(S) * S), batch), 256, 0, stream>>>(xin, tin, fw, fh, crops, S, x, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(const AdamArgs& a, hipStream_t stream) {
+    if (a.n_layers == 0 || a.n_layers > kMaxLayers || a.n_w > a.n || a.layer[0].w_off != 0) return hipErrorInvalidValue;
+    if (a.n == 0) return hipSuccess;
+    k_adam<<<blocks_for(a.n), 256, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace mrt

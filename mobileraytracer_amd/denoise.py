@@ -121,6 +121,79 @@ def make_unet(levels: int = 5, channels=DEFAULT_CHANNELS):
     return UNet()
 
 
+def training_output(model, x, rounding: bool = True):
+    """The trainer's forward pass as an autograd graph (mrt_denoiser_trainer_*):
+    x (N, 10, H, W) packed inputs, H and W multiples of 8 * 2^(levels-1);
+    returns y (N, 3, H, W).  With `rounding`, BF16 rounding (nearest even) at
+    the trainer's points, both ways: the input, the weights (straight
+    through: their gradient is not rounded), every ReLU output, the pooled
+    tensors, the decoder's concatenated input and the output y are rounded
+    in the forward pass, and the gradient arriving at each of them (but the
+    input and the weights) is rounded in the backward pass, as the trainer
+    stores it in BF16.  Without it, the plain FP32 (or FP64) network."""
+    import torch
+    import torch.nn.functional as F
+
+    class RoundBoth(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, t):
+            return t.to(torch.bfloat16).to(t.dtype)
+
+        @staticmethod
+        def backward(ctx, g):
+            return g.to(torch.bfloat16).to(g.dtype)
+
+    class RoundForward(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, t):
+            return t.to(torch.bfloat16).to(t.dtype)
+
+        @staticmethod
+        def backward(ctx, g):
+            return g
+
+    q = RoundBoth.apply if rounding else (lambda t: t)
+    qw = RoundForward.apply if rounding else (lambda t: t)
+
+    def conv(k, t):
+        m = model.convs[k]
+        return F.conv2d(t, qw(m.weight), m.bias, padding=1)
+
+    L = model.levels
+    k = 0
+    skips = []
+    h = qw(x)
+    for l in range(L):
+        if l > 0:
+            h = q(F.max_pool2d(skips[-1], 2))
+        h = q(F.relu(conv(k, h)))
+        h = q(F.relu(conv(k + 1, h)))
+        k += 2
+        skips.append(h)
+    d = skips[-1]
+    for l in range(L - 2, -1, -1):
+        u = F.interpolate(d, scale_factor=2, mode="nearest")
+        h = q(F.relu(conv(k, q(torch.cat([u, skips[l]], 1)))))
+        d = q(F.relu(conv(k + 1, h)))
+        k += 2
+    return q(conv(k, d))
+
+
+def adam_step(p, g, m, v, step: int, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    """One update of the trainer's Adam in FP32 numpy, torch.optim.Adam's
+    arithmetic: the hyperparameters are doubles, rounded to FP32 where they
+    meet the tensors (1 - b1, b2, 1 - b2, eps, lr / (1 - b1^t),
+    sqrt(1 - b2^t)); returns (p, m, v); step counts from 1."""
+    f = np.float32
+    p, g, m, v = (np.asarray(a, np.float32) for a in (p, g, m, v))
+    m = (m + f(1.0 - b1) * (g - m)).astype(np.float32)
+    v = (v * f(b2) + (f(1.0 - b2) * g) * g).astype(np.float32)
+    step_size, bc2_sqrt = f(lr / (1.0 - b1 ** step)), f(np.sqrt(1.0 - b2 ** step))
+    denom = (np.sqrt(v) / bc2_sqrt + f(eps)).astype(np.float32)
+    p = (p + (-step_size) * (m / denom)).astype(np.float32)
+    return p, m, v
+
+
 def flat_weights(model) -> np.ndarray:
     """The library's flat FP32 weight vector of a UNet."""
     parts = []
