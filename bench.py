@@ -26,7 +26,9 @@ and AOVs resident in HBM; device time per frame and the MFMA roofline of the
 network (algorithmic FLOPs / time) go into the line's "denoiser" object.
 --denoise runs only this leg, plus the whole denoised frame of the benchmark
 scene (4-sample render with its AOVs, then the network), and prints its own
-line.
+line.  The denoiser object also carries "training": Adam steps of the
+network on the GPU (8 crops of 256x256 per step, synthetic pairs in HBM),
+ms per step and TFLOP/s; --denoise-train runs only that leg.
 
 Prints one JSON line (see the repository's benchmark contract).
 """
@@ -252,6 +254,9 @@ def main() -> int:
                     help="BVH builder of the scene (auto: GPU from 65536 triangles)")
     ap.add_argument("--denoise", action="store_true", help="run only the denoiser leg (1920x1080, trained weights)")
     ap.add_argument("--denoise-reps", type=int, default=20)
+    ap.add_argument("--denoise-train", action="store_true",
+                    help="run only the denoiser training leg (Adam steps, 8 crops of 256x256)")
+    ap.add_argument("--denoise-train-steps", type=int, default=40)
     ap.add_argument("--batched", action="store_true",
                     help="run only the batched workload: 64 scenes (10k-4M triangles), one request each, all at once")
     args = ap.parse_args()
@@ -269,6 +274,14 @@ def main() -> int:
 
     if args.batched:
         return batched(mrt, abi)
+
+    if args.denoise_train:
+        ctx = mrt.Context(0)
+        res = denoiser_train_leg(mrt, ctx, args.denoise_train_steps)
+        print(json.dumps({"metric": "denoiser training ms per step (BF16 U-Net on MFMA, 8 crops of 256x256)",
+                          "value": res["ms_per_step"], "unit": "ms", "higher_is_better": False, "training": res}),
+              flush=True)
+        return 0
 
     if args.denoise:
         ctx = mrt.Context(0)
@@ -485,6 +498,7 @@ def main() -> int:
     # The network alone (synthetic inputs: no trace launch besides the timed
     # frames'); the denoised-frame pipeline is measured by --denoise.
     out["denoiser"] = denoiser_leg(mrt, abi, ctx, torch, args.denoise_reps)
+    out["denoiser"]["training"] = denoiser_train_leg(mrt, ctx, args.denoise_train_steps)
     if not args.no_serve:
         scene.close()
         del bitmap, accum
@@ -602,6 +616,50 @@ def denoiser_leg(mrt, abi, ctx, torch, reps: int, scene=None, desc4=None) -> dic
                          "frac": round(tflops / BF16_PEAK_TFLOPS, 4), "traffic": None,
                          "flops_per_frame": flops, "kernel": "whole network (k_conv3x3 + pack/pool/up/unpack)"},
             "pipeline": pipeline}
+
+
+def denoiser_train_leg(mrt, ctx, steps: int, batch: int = 8, crop: int = 256) -> dict:
+    """Training steps of the U-Net on the GPU (mrt_denoiser_trainer_step:
+    gather of `batch` crops, forward, backward-data and weight-gradient
+    products on BF16 MFMA, Adam) over synthetic pairs resident in HBM (16
+    frames of 512x512); device time from the library's events, algorithmic
+    FLOPs per step from the network's shapes (forward, weight gradient and,
+    but for the first layer, backward-data: 2 * 9 * cin * cout per pixel
+    each)."""
+    from mobileraytracer_amd import denoise
+
+    rng = np.random.default_rng(0)
+    n, side = 16, 512
+    x = np.zeros((n, side, side, 10), np.float32)
+    x[..., 0:3] = np.log1p(rng.exponential(0.5, size=(n, side, side, 3)))
+    x[..., 3:9] = rng.uniform(-1, 1, size=(n, side, side, 6))
+    x[..., 9] = rng.uniform(0, 3, size=(n, side, side))
+    t = np.log1p(rng.exponential(0.5, size=(n, side, side, 3))).astype(np.float32)
+    flat, levels, channels, ds = denoise.load_weights()
+    tr = mrt.DenoiserTrainer(ctx, flat, mrt.denoiser_desc(levels, channels, ds),
+                             mrt.denoiser_train_desc(batch=batch, crop=crop, seed=1))
+    tr.set_pairs(x, t)
+    del x, t
+    tr.step(3, 1e-4)
+    per = []
+    chunk = max(1, steps // 4)
+    for _ in range(4):
+        tr.step(chunk, 1e-4)
+        per.append(tr.stats().ms_per_step)
+    st = tr.stats()
+    tr.close()
+    ms = float(np.median(per))
+    tflops = st.flops_per_step / (ms / 1000.0) / 1e12
+    log(f"denoiser training: {ms:.3f} ms per step ({batch} crops of {crop}^2), {tflops:.1f} TFLOP/s algorithmic")
+    return {"ms_per_step": round(ms, 4), "steps": 4 * chunk, "batch": batch, "crop": crop,
+            "workload": f"U-Net levels={levels} channels={list(channels)}: Adam steps on {batch} crops of "
+                        f"{crop}x{crop} from 16 synthetic 512x512 pairs in HBM; BF16 MFMA forward, backward-data "
+                        "and weight-gradient kernels, FP32 master weights",
+            "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "flops_per_step": st.flops_per_step,
+                         "kernel": "whole step (k_conv3x3, k_octet, k_wgrad, k_sum_splits, pool/upsample "
+                                   "gradients, k_mse, k_gather, k_adam)"}}
 
 
 def batched(mrt, abi) -> int:

@@ -82,35 +82,35 @@ struct TensorIn {
     uint32_t c;  // channels (multiple of 8)
 };
 
-// Plain octet copy (shifted = 0: R = H rows) or the three column-shifted,
-// row-padded copies of a convolution input (shifted = 1: copy dx in 0..2 holds
-// input pixel (r - 1, x + dx - 1) at row r of H + 2, zero outside the frame).
-hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t W, int shifted, uint16_t* out,
+// Octet copy of a tensor (rowpad = 0: R = H rows) or of a convolution input
+// with a zero row above and below each frame (rowpad = 1: R = H + 2 rows, row
+// r holding input row r - 1).
+hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t W, int rowpad, uint16_t* out,
                         hipStream_t stream);
 
 // Weight gradients of one 3x3 convolution: partial sums per K split of
 // dW[tap][co][ci] = sum over pixels of dZ(pixel, co) X(pixel + tap offset, ci)
 // and db[co] = sum of dZ(pixel, co), from dZ in octet layout (B x H x W, cout
-// channels) and X as the three shifted octet copies (cin channels); FP32
+// channels) and X as the row-padded octet copy (cin channels); FP32
 // accumulation on MFMA.  part_w: [splits][9][cout][cin], part_b:
 // [splits][cout]; `splits` is chosen here (at most max_splits) and returned.
-// K steps of 4 segments per wave and the number of splits launch_wgrad uses:
-// about target_waves waves, at least 32 K steps (1024 pixels) per wave, at
-// most max_splits splits.
+// K steps of 4 segments per split and the number of splits launch_wgrad
+// uses: about target_groups workgroups (of 4 waves), at least 32 K steps
+// (1024 pixels) per wave, at most max_splits splits.
 inline uint32_t wgrad_splits(uint32_t batch, uint32_t H, uint32_t W, uint32_t cin, uint32_t cout, uint32_t max_splits,
-                             uint32_t target_waves, uint32_t* kps) {
+                             uint32_t target_groups, uint32_t* kps) {
     const uint32_t nseg = batch * H * (W / 8u), ksteps = (nseg + 3u) / 4u;
     const uint32_t per_split = 3u * (cin / 32u) * (cout / 32u);
-    uint32_t s = (target_waves + per_split - 1) / per_split;
+    uint32_t s = (target_groups + per_split - 1) / per_split;
     if (s > max_splits) s = max_splits;
-    if (s > (ksteps + 31u) / 32u) s = (ksteps + 31u) / 32u;
+    if (s > (ksteps + 127u) / 128u) s = (ksteps + 127u) / 128u;
     if (s < 1u) s = 1u;
     const uint32_t k = (ksteps + s - 1) / s;
     if (kps) *kps = k;
     return (ksteps + k - 1) / k;
 }
 hipError_t launch_wgrad(const uint16_t* dzt, const uint16_t* xt, uint32_t batch, uint32_t H, uint32_t W, uint32_t cin,
-                        uint32_t cout, uint32_t max_splits, uint32_t target_waves, float* part_w, float* part_b,
+                        uint32_t cout, uint32_t max_splits, uint32_t target_groups, float* part_w, float* part_b,
                         uint32_t* splits, hipStream_t stream);
 // out[i] = sum over s < splits of part[s * n + i] (in order of s).
 hipError_t launch_sum_splits(const float* part, uint32_t splits, size_t n, float* out, hipStream_t stream);

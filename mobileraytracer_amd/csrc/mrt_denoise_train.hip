@@ -12,10 +12,12 @@
 // 8-pixel segment), so both operands are single 16-byte loads.  A K step of
 // v_mfma_f32_16x16x32_bf16 is 4 segments of 8 pixels (lane group l / 16 takes
 // segment 4 k + l / 16), anywhere in the batch, so frames of any width that
-// is a multiple of 8 fill every step.  The three column shifts of a tap row
-// are three octet copies of the input, each holding input column x + dx - 1 at
-// x, with a zero row above and below every frame: a tap's operand is then the
-// same segment of another copy and row, with no bounds checks in the loop.
+// is a multiple of 8 fill every step.  The input's octet copy has a zero row
+// above and below every frame, so a tap row is the same segment one row up or
+// down; the column shifts dx = 0 and 2 are the segment's 16-byte operand
+// shifted by one element in registers (v_alignbit), the element shifted in
+// taken from the neighbouring segment of the same row (zero at the frame's
+// edges).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -56,12 +58,12 @@ __device__ __forceinline__ uint32_t elem(const uint4& q, int e) {
 
 // ---------------------------------------------------------------- octet --
 // Thread: one 8-channel group of one 8-pixel segment of one row.  It reads
-// the segment's 8 pixels (plus one on each side when shifted) as 16-byte
-// channel quads and writes 8 (or 3 x 8) 16-byte channel rows.
-template <bool SHIFTED>
+// the segment's 8 pixels as 16-byte channel quads and writes 8 16-byte
+// channel rows.
+template <bool ROWPAD>
 __global__ __launch_bounds__(256) void k_octet(TensorIn in, uint32_t batch, uint32_t H, uint32_t W, uint16_t* out) {
     const uint32_t W8 = W / 8u, cg = in.c / 8u;
-    const uint32_t R = SHIFTED ? H + 2u : H;
+    const uint32_t R = ROWPAD ? H + 2u : H;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t total = static_cast<size_t>(batch) * R * W8 * cg;
     if (i >= total) return;
@@ -70,9 +72,9 @@ __global__ __launch_bounds__(256) void k_octet(TensorIn in, uint32_t batch, uint
     const uint32_t xo = static_cast<uint32_t>(seg % W8);
     const size_t br = seg / W8;
     const uint32_t r = static_cast<uint32_t>(br % R), b = static_cast<uint32_t>(br / R);
-    const int y = SHIFTED ? static_cast<int>(r) - 1 : static_cast<int>(r);
-    constexpr int kPx = SHIFTED ? 10 : 8;
-    const int xbase = static_cast<int>(xo * 8u) - (SHIFTED ? 1 : 0);
+    const int y = ROWPAD ? static_cast<int>(r) - 1 : static_cast<int>(r);
+    constexpr int kPx = 8;
+    const int xbase = static_cast<int>(xo * 8u);
     const uint32_t c0 = g * 8u;
     const bool up = c0 < in.cu;
     const size_t plane = static_cast<size_t>(H) * W;
@@ -89,31 +91,27 @@ __global__ __launch_bounds__(256) void k_octet(TensorIn in, uint32_t batch, uint
             src = in.x + (b * plane + static_cast<size_t>(y) * W + x) * in.x_stride + in.x_off + (c0 - in.cu);
         px[k] = *reinterpret_cast<const uint4*>(src);
     }
-    constexpr int kCopies = SHIFTED ? 3 : 1;
-    const size_t copy_elems = static_cast<size_t>(batch) * R * W8 * in.c * 8u;
+    uint16_t* o = out + (seg * in.c + c0) * 8u;
 #pragma unroll
-    for (int dx = 0; dx < kCopies; ++dx) {
-        uint16_t* o = out + dx * copy_elems + (seg * in.c + c0) * 8u;
+    for (int e = 0; e < 8; ++e) {
+        // Channel c0 + e of pixels xo * 8 .. + 7.
+        uint32_t w[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            // Channel c0 + e of pixels xo * 8 + j + dx - 1 (shifted) or xo * 8 + j.
-            uint32_t w[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                w[j] = elem(px[2 * j + dx], e) | (elem(px[2 * j + 1 + dx], e) << 16);
-            *reinterpret_cast<uint4*>(o + e * 8) = make_uint4(w[0], w[1], w[2], w[3]);
-        }
+        for (int j = 0; j < 4; ++j) w[j] = elem(px[2 * j], e) | (elem(px[2 * j + 1], e) << 16);
+        *reinterpret_cast<uint4*>(o + e * 8) = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
 // ---------------------------------------------------------------- wgrad --
-// Wave task (co block of 32, ci block of 32, tap row dy, K split): 2 x 2
-// MFMA tiles for each of the 3 column shifts, 12 accumulators; the waves of
-// ci block 0 and dy = 1 also sum dZ against a ones operand (the bias
-// gradient).  Lane (li = l % 16, lg = l / 16) loads, per K step, channel
-// co0 + 16 jA + li of segment 4 k + lg (A) and channel ci0 + 16 jB + li of
-// the same segment, row + dy, copy dx (B); its results are C[co = 4 lg + v]
-// [ci = li] of each tile.
+// Workgroup task (co block of 32, ci block of 32, tap row dy, K split): its 4
+// waves take the split's K steps in turn (wave w: steps k0 + w, k0 + w + 4,
+// ...), each with 2 x 2 MFMA tiles for each of the 3 column shifts (12
+// accumulators); the tasks of ci block 0 and dy = 1 also sum dZ against a
+// ones operand (the bias gradient).  Lane (li = l % 16, lg = l / 16) loads,
+// per K step, channel co0 + 16 jA + li of segment 4 k + lg (A) and channel
+// ci0 + 16 jB + li of the same segment one row padded by dy (B); its results
+// are C[co = 4 lg + v][ci = li] of each tile.  The waves' sums meet in LDS
+// and are added in wave order (deterministic), one partial sum per split.
 struct WgradArgs {
     const uint16_t* dzt;
     const uint16_t* xt;
@@ -123,10 +121,12 @@ struct WgradArgs {
     float* part_b;
 };
 
+constexpr int kWgradAcc = 56;  // 12 tiles x 4 + 2 bias tiles x 4 floats per lane
+
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t task = uniform(blockIdx.x * 4u + (threadIdx.x >> 6));
-    if (task >= a.tasks) return;
+    __shared__ float red[3][kWgradAcc][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = uniform(threadIdx.x >> 6);
+    const uint32_t task = blockIdx.x;
     const uint32_t cob_n = a.cout / 32u, cib_n = a.cin / 32u;
     const uint32_t cob = task % cob_n;
     uint32_t rest = task / cob_n;
@@ -138,54 +138,93 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
     const bool bias = cib == 0u && dy == 1u;
     const uint32_t k0 = split * a.kps;
     const uint32_t k1 = min(a.ksteps, k0 + a.kps);
-    const size_t copy_segs = static_cast<size_t>(a.batch) * (a.H + 2u) * a.W8;  // segments per shifted copy
     const uint32_t frame_segs = a.H * a.W8;
-    floatx4 acc[3][2][2];
-    floatx4 accb[2];
+    const size_t seg_el = static_cast<size_t>(a.cin) * 8u;  // elements per segment of the input copy
+    floatx4 acc[kWgradAcc / 4];  // tile (d, i, j) at (d * 2 + i) * 2 + j, bias tiles at 12 + i
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[d][i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    accb[0] = accb[1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int r = 0; r < kWgradAcc / 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     // BF16 1.0 in all 8 K slots.
     const uint4 ones = make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u);
-    uint4 av[2], bv[3][2];
-    auto load = [&](uint32_t ks) {
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    // Operands of one K step as loaded: dZ (A), the input segment (B for dx
+    // = 1) and the neighbouring elements the shifted operands take.  The
+    // next step's loads are issued before this step's MFMAs.
+    struct Frag {
+        uint4 av[2], c[2];
+        uint32_t pv[2], nx[2];
+    };
+    auto load = [&](uint32_t ks, Frag& f) {
         const uint32_t s = ks * 4u + lg;
-        const bool ok = s < a.nseg;
+        const bool ok = ks < k1 && s < a.nseg;
         const uint32_t ss = ok ? s : 0u;
         const uint32_t b = ss / frame_segs;
-        // Row-padded segment index of segment s, tap row dy, in a shifted copy.
+        const uint32_t xo = ss % a.W8;
+        const bool has_prev = ok && xo > 0u, has_next = ok && xo + 1u < a.W8;
+        // Segment s in the row-padded copy, tap row dy.
         const size_t sx = ss + static_cast<size_t>(2u * b + dy) * a.W8;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            av[j] = ok ? *reinterpret_cast<const uint4*>(a.dzt + (static_cast<size_t>(ss) * a.cout + co0 + 16u * j + li) * 8u)
-                       : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-                bv[d][j] = ok ? *reinterpret_cast<const uint4*>(
-                                    a.xt + ((d * copy_segs + sx) * a.cin + ci0 + 16u * j + li) * 8u)
-                              : make_uint4(0u, 0u, 0u, 0u);
+            // Loads under branches (a select between a global address and a
+            // zero constant would become a flat load that may read private
+            // memory).
+            f.av[j] = zero;
+            f.c[j] = zero;
+            f.pv[j] = 0u;
+            f.nx[j] = 0u;
+            const uint16_t* xc = a.xt + sx * seg_el + (ci0 + 16u * j + li) * 8u;
+            if (ok) {
+                f.av[j] = *reinterpret_cast<const uint4*>(a.dzt + (static_cast<size_t>(ss) * a.cout + co0 + 16u * j + li) * 8u);
+                f.c[j] = *reinterpret_cast<const uint4*>(xc);
+            }
+            // Last two elements of the previous segment, first two of the next.
+            if (has_prev) f.pv[j] = *reinterpret_cast<const uint32_t*>(xc - seg_el + 6);
+            if (has_next) f.nx[j] = *reinterpret_cast<const uint32_t*>(xc + seg_el);
         }
     };
-    for (uint32_t ks = k0; ks < k1; ++ks) {
-        load(ks);
+    Frag cur, nxt;
+    if (k0 + wv < k1) load(k0 + wv, cur);
+    for (uint32_t ks = k0 + wv; ks < k1; ks += 4u) {
+        load(ks + 4u, nxt);
+        uint4 bv[3][2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint4 c = cur.c[j];
+            // dx = 0: pixels x - 1; dx = 1: x; dx = 2: x + 1.
+            bv[0][j] = make_uint4(__builtin_amdgcn_alignbit(c.x, cur.pv[j], 16), __builtin_amdgcn_alignbit(c.y, c.x, 16),
+                                  __builtin_amdgcn_alignbit(c.z, c.y, 16), __builtin_amdgcn_alignbit(c.w, c.z, 16));
+            bv[1][j] = c;
+            bv[2][j] = make_uint4(__builtin_amdgcn_alignbit(c.y, c.x, 16), __builtin_amdgcn_alignbit(c.z, c.y, 16),
+                                  __builtin_amdgcn_alignbit(c.w, c.z, 16), __builtin_amdgcn_alignbit(cur.nx[j], c.w, 16));
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[d][i][j] =
-                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[i]), as_bf16x8(bv[d][j]), acc[d][i][j], 0, 0, 0);
+                    acc[(d * 2 + i) * 2 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        as_bf16x8(cur.av[i]), as_bf16x8(bv[d][j]), acc[(d * 2 + i) * 2 + j], 0, 0, 0);
         if (bias) {
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                accb[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[i]), as_bf16x8(ones), accb[i], 0, 0, 0);
+                acc[12 + i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(cur.av[i]), as_bf16x8(ones), acc[12 + i],
+                                                                      0, 0, 0);
         }
+        cur = nxt;
     }
+    if (wv > 0u) {
+#pragma unroll
+        for (int r = 0; r < kWgradAcc / 4; ++r)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[wv - 1u][4 * r + v][lane] = acc[r][v];
+    }
+    __syncthreads();
+    if (wv != 0u) return;
+#pragma unroll
+    for (int r = 0; r < kWgradAcc / 4; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            acc[r][v] = ((acc[r][v] + red[0][4 * r + v][lane]) + red[1][4 * r + v][lane]) + red[2][4 * r + v][lane];
     const size_t layer = static_cast<size_t>(a.cout) * a.cin;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -196,23 +235,41 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
             for (int j = 0; j < 2; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    pw[static_cast<size_t>(co0 + 16u * i + 4u * lg + v) * a.cin + ci0 + 16u * j + li] = acc[d][i][j][v];
+                    pw[static_cast<size_t>(co0 + 16u * i + 4u * lg + v) * a.cin + ci0 + 16u * j + li] =
+                        acc[(d * 2 + i) * 2 + j][v];
     }
     if (bias && li == 0u) {
         float* pb = a.part_b + static_cast<size_t>(split) * a.cout + co0;
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) pb[16u * i + 4u * lg + v] = accb[i][v];
+            for (int v = 0; v < 4; ++v) pb[16u * i + 4u * lg + v] = acc[12 + i][v];
     }
 }
 
-__global__ void k_sum_splits(const float* __restrict__ part, uint32_t splits, size_t n, float* __restrict__ out) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// out[i] = sum over s of part[s n + i]: 64 outputs per workgroup, wave w
+// summing splits w, w + 4, ... in order, the four sums added in wave order.
+__global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ part, uint32_t splits, size_t n,
+                                                    float* __restrict__ out) {
+    __shared__ float red[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const size_t i = static_cast<size_t>(blockIdx.x) * 64u + lane;
     float s = 0.0f;
-    for (uint32_t k = 0; k < splits; ++k) s += part[k * n + i];
-    out[i] = s;
+    if (i < n) {
+        uint32_t k = wv;
+        for (; k + 12u < splits; k += 16u) {
+            const float v0 = part[k * n + i], v1 = part[(k + 4u) * n + i], v2 = part[(k + 8u) * n + i],
+                        v3 = part[(k + 12u) * n + i];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; k < splits; k += 4u) s += part[k * n + i];
+    }
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0u && i < n) out[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // ----------------------------------------------------------------- loss --
@@ -382,15 +439,15 @@ uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + 255) / 256); }
 
 }  // namespace
 
-hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t W, int shifted, uint16_t* out,
+hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t W, int rowpad, uint16_t* out,
                         hipStream_t stream) {
     if (W % 8u || in.c % 8u || in.cu % 8u || in.cu > in.c || in.x_stride % 8u || in.x_off % 8u || !out)
         return hipErrorInvalidValue;
     if ((in.cu && (!in.xu || H % 2u || W % 2u)) || (in.c > in.cu && (!in.x || in.x_off + in.c - in.cu > in.x_stride)))
         return hipErrorInvalidValue;
-    const size_t n = static_cast<size_t>(batch) * (shifted ? H + 2u : H) * (W / 8u) * (in.c / 8u);
+    const size_t n = static_cast<size_t>(batch) * (rowpad ? H + 2u : H) * (W / 8u) * (in.c / 8u);
     if (n == 0) return hipSuccess;
-    if (shifted)
+    if (rowpad)
         k_octet<true><<<blocks_for(n), 256, 0, stream>>>(in, batch, H, W, out);
     else
         k_octet<false><<<blocks_for(n), 256, 0, stream>>>(in, batch, H, W, out);
@@ -398,7 +455,7 @@ hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t
 }
 
 hipError_t launch_wgrad(const uint16_t* dzt, const uint16_t* xt, uint32_t batch, uint32_t H, uint32_t W, uint32_t cin,
-                        uint32_t cout, uint32_t max_splits, uint32_t target_waves, float* part_w, float* part_b,
+                        uint32_t cout, uint32_t max_splits, uint32_t target_groups, float* part_w, float* part_b,
                         uint32_t* splits, hipStream_t stream) {
     if (cin % 32u || cout % 32u || W % 8u || !batch || !H || !max_splits) return hipErrorInvalidValue;
     WgradArgs a{};
@@ -412,20 +469,21 @@ hipError_t launch_wgrad(const uint16_t* dzt, const uint16_t* xt, uint32_t batch,
     a.nseg = batch * H * a.W8;
     a.ksteps = (a.nseg + 3u) / 4u;
     const uint32_t per_split = 3u * (cin / 32u) * (cout / 32u);
-    const uint32_t s = wgrad_splits(batch, H, W, cin, cout, max_splits, target_waves, &a.kps);
+    const uint32_t s = wgrad_splits(batch, H, W, cin, cout, max_splits, target_groups, &a.kps);
     a.tasks = s * per_split;
     a.part_w = part_w;
     a.part_b = part_b;
     // Every split has the (co block, ci block 0, dy 1) task of each co block,
     // so every bias partial is written.
-    k_wgrad<<<(a.tasks + 3u) / 4u, 256, 0, stream>>>(a);
+    k_wgrad<<<a.tasks, 256, 0, stream>>>(a);
     *splits = s;
     return hipGetLastError();
 }
 
 hipError_t launch_sum_splits(const float* part, uint32_t splits, size_t n, float* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    k_sum_splits<<<blocks_for(n), 256, 0, stream>>>(part, splits, n, out);
+    if (n > (static_cast<size_t>(1) << 36)) return hipErrorInvalidValue;
+    k_sum_splits<<<static_cast<uint32_t>((n + 63) / 64), 256, 0, stream>>>(part, splits, n, out);
     return hipGetLastError();
 }
 

@@ -54,7 +54,7 @@ struct mrt_denoiser_trainer {
     uint64_t seed = 0;
     std::vector<DevLayer> layers;
     size_t n_w = 0, n_b = 0;
-    uint32_t target_waves = 0;
+    uint32_t target_groups = 0;
     TrainArena ar;
     DevBuf params, grads, adam_m, adam_v, wq, wt, arena, data_x, data_t, stage_x, stage_t;
     uint32_t n_pairs = 0, fw = 0, fh = 0;
@@ -144,9 +144,9 @@ TrainArena plan_arena(const mrt_denoiser_trainer* tr) {
         else
             lvl = 0;
         const uint32_t s = S >> lvl;
-        octx = std::max(octx, static_cast<size_t>(3) * B * (s + 2) * s * d.cin * 2);
+        octx = std::max(octx, static_cast<size_t>(B) * (s + 2) * s * d.cin * 2);
         octdz = std::max(octdz, px(lvl) * d.cout * 2);
-        const uint32_t sp = mrt::wgrad_splits(B, s, s, d.cin, d.cout, kMaxSplits, tr->target_waves, nullptr);
+        const uint32_t sp = mrt::wgrad_splits(B, s, s, d.cin, d.cout, kMaxSplits, tr->target_groups, nullptr);
         pw = std::max(pw, static_cast<size_t>(sp) * 9 * d.cout * d.cin * 4);
         pb = std::max(pb, static_cast<size_t>(sp) * d.cout * 4);
     }
@@ -249,7 +249,7 @@ int layer_wgrad(mrt_denoiser_trainer* tr, size_t k, const LayerIo& io, const uin
     MRT_HIP(mrt::launch_octet(mrt::TensorIn{dz, d.cout, 0u, nullptr, 0u, d.cout}, B, s, s, 0, oct_dz, st));
     MRT_HIP(mrt::launch_octet(io.in, B, s, s, 1, oct_x, st));
     uint32_t splits = 0;
-    MRT_HIP(mrt::launch_wgrad(oct_dz, oct_x, B, s, s, d.cin, d.cout, kMaxSplits, tr->target_waves, pw, pb, &splits, st));
+    MRT_HIP(mrt::launch_wgrad(oct_dz, oct_x, B, s, s, d.cin, d.cout, kMaxSplits, tr->target_groups, pw, pb, &splits, st));
     MRT_HIP(mrt::launch_sum_splits(pw, splits, static_cast<size_t>(9) * d.cout * d.cin, tr->grads.as<float>() + d.w_off,
                                    st));
     MRT_HIP(mrt::launch_sum_splits(pb, splits, d.cout, tr->grads.as<float>() + tr->n_w + d.b_off, st));
@@ -409,7 +409,7 @@ MRT_API int mrt_denoiser_trainer_create(mrt_ctx* ctx, const mrt_denoiser_desc* n
     if (!(tr->b1 >= 0.0 && tr->b1 < 1.0 && tr->b2 >= 0.0 && tr->b2 < 1.0 && tr->eps > 0.0 && std::isfinite(tr->eps)))
         return fail(MRT_E_INVALID, "Adam needs 0 <= beta < 1 and eps > 0");
     tr->layers = mrt::unet::device_layers(tr->net, &tr->n_w, &tr->n_b);
-    tr->target_waves = 16u * ctx->num_cu;
+    tr->target_groups = 4u * ctx->num_cu;
     tr->ar = plan_arena(tr.get());
     std::vector<float> p(tr->n_w + tr->n_b, 0.0f);
     mrt::unet::to_device_layout(tr->layers, weights, p.data(), p.data() + tr->n_w);

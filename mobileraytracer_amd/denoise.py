@@ -121,6 +121,19 @@ def make_unet(levels: int = 5, channels=DEFAULT_CHANNELS):
     return UNet()
 
 
+def init_weights(levels: int = 5, channels=DEFAULT_CHANNELS, seed: int = 0) -> np.ndarray:
+    """Initial flat FP32 weights: every weight and bias uniform in
+    +-1/sqrt(9 cin) (the bound of torch.nn.Conv2d's default initialisation),
+    from a seeded numpy generator."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for ci, co in layer_shapes(levels, channels):
+        bound = 1.0 / np.sqrt(9.0 * ci)
+        parts.append(rng.uniform(-bound, bound, size=co * ci * 9))
+        parts.append(rng.uniform(-bound, bound, size=co))
+    return np.concatenate(parts).astype(np.float32)
+
+
 def training_output(model, x, rounding: bool = True):
     """The trainer's forward pass as an autograd graph (mrt_denoiser_trainer_*):
     x (N, 10, H, W) packed inputs, H and W multiples of 8 * 2^(levels-1);

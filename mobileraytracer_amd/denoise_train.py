@@ -2,15 +2,19 @@
 
     python -m mobileraytracer_amd.denoise_train --out weights.npz --eval-json eval.json
 
-Pairs: 1920x1080 PathTracer frames (maximum depth 6, Russian roulette from
-depth 3, Halton pixel jitter) of seeded triangle soups and displaced
-icosphere clusters (1% of the triangles emissive) from random poses around
-the unit cube.  Input: the mean of samples [0, 4) and the AOVs of the same
-camera rays; target: the mean of samples [4, 4 + T), independent of the
-input, so the L2 optimum on noisy targets is the clean one.  Training: Adam
-on random 256x256 crops with flips, loss = mean squared error of
-log(1 + radiance), BF16 autocast, for a fixed wall-clock budget with a
-cosine learning-rate decay.
+Pairs: regions of 1920x1080 PathTracer frames (maximum depth 6, Russian
+roulette from depth 3, Halton pixel jitter) of seeded triangle soups and
+displaced icosphere clusters (1% of the triangles emissive) from random poses
+around the unit cube.  A pair is a block of 3 x 4 of the renderer's 16 x 16
+tiles (360 x 268 pixels) at a random place in the frame, rendered alone
+(tile_skip), so its pixels see the scene at the 1080p frame's angular
+resolution and jitter.  Input: the mean of samples [0, 4) and the AOVs of
+the same camera rays; target: the mean of samples [4, 4 + T) (T = 1024 by
+default), independent of the input.  Training: mrt_denoiser_trainer_* on the
+GPU (BF16 MFMA forward, backward-data and weight-gradient kernels, FP32
+master weights, Adam) on random 256x256 crops with flips, loss = mean squared
+error of log(1 + radiance), for a fixed wall-clock budget with a cosine
+learning-rate decay.
 
 Evaluation, on scenes and poses not in the training set (the 1M-triangle
 soup of the benchmark with its camera, and an icosphere cluster): relative
@@ -18,29 +22,22 @@ MSE, mean over pixels and channels of (x - ref)^2 / (ref^2 + 0.01), against
 the mean of 1024 further samples, for the plain 4-sample frame, the denoised
 4-sample frame (mrt_denoiser_run, BF16 on MFMA) and the plain 64-sample
 frame, with the time each took.
-
-torch is initialised before the library is loaded: its wheel carries a HIP
-runtime of the same soname as the system one, and the first one loaded
-serves both.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import math
-import os
 import sys
 import time
 
 import numpy as np
 
-os.environ.setdefault("MIOPEN_FIND_MODE", "FAST")
-import torch  # noqa: E402
-
-from . import denoise  # noqa: E402
+from . import denoise
 
 W_FULL, H_FULL = 1920, 1080
 SPP_IN = 4
+REGION_TILES = (3, 4)  # tiles across, down
 
 
 def log(msg: str) -> None:
@@ -64,12 +61,12 @@ def pose_camera(mrt, abi, rng, w, h):
     return mrt.make_camera(abi.CAMERA_PERSPECTIVE, tuple(pos.tolist()), tuple(at.tolist()), (0, 1, 0), fh, fv)
 
 
-def frame_desc(mrt, abi, cam, w, h, first, end, seed):
+def frame_desc(mrt, abi, cam, w, h, first, end, seed, tiles=None):
     return mrt.render_desc(abi.SHADER_PATHTRACER, w, h, end, cam, pixel=mrt.sampler(abi.SAMPLER_HALTON, 0.0, seed),
                            rr=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, seed + 1),
                            light=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, seed + 2),
                            hemi=mrt.sampler(abi.SAMPLER_RANDOM, 0.0, seed + 3), depth_min=2, depth_max=6,
-                           kernel=abi.KERNEL_WAVEFRONT, first_sample=first)
+                           kernel=abi.KERNEL_WAVEFRONT, first_sample=first, tiles=tiles)
 
 
 def render_mean(scene, desc) -> tuple[np.ndarray, float]:
@@ -88,72 +85,100 @@ def make_scene(mrt, kind, seed, n):
     return mrt.icosphere_scene(seed, spheres, sub, max(1, spheres * 20 * 4 ** sub // 100))
 
 
-def render_pairs(mrt, abi, ctx, n_scenes, poses, spp_tgt, seed, tri_lo, tri_hi):
-    """(inputs (N, 10, H, W), targets (N, 3, H, W)) as FP32 numpy arrays."""
+def render_pairs(mrt, abi, ctx, n_pairs, n_scenes, spp_tgt, seed, tri_lo, tri_hi, budget_s):
+    """(inputs (N, 268, 360, 10), log-space targets (N, 268, 360, 3)), FP32:
+    up to n_pairs regions over n_scenes scenes, stopping early when the
+    rendering time passes budget_s."""
     rng = np.random.default_rng(seed)
-    xs, ts = [], []
+    tw, th = W_FULL // 16, H_FULL // 16
+    ta, td = REGION_TILES
+    rw, rh = ta * tw, td * th
+    xs = np.zeros((n_pairs, rh, rw, 10), np.float32)
+    ts = np.zeros((n_pairs, rh, rw, 3), np.float32)
+    # Frame buffers in device memory (torch allocates them; the renders
+    # write them in place); only each region is copied to the host.
+    import torch
+
+    dev = torch.device("cuda:0")
+    bm = torch.zeros((H_FULL, W_FULL), dtype=torch.int32, device=dev)
+    acc = torch.zeros((H_FULL, W_FULL, 3), dtype=torch.float32, device=dev)
+    aov = torch.zeros((H_FULL, W_FULL, 7), dtype=torch.float32, device=dev)
+    per_scene = -(-n_pairs // n_scenes)
+    done = 0
+    t_start = time.perf_counter()
     for i in range(n_scenes):
+        if done >= n_pairs or time.perf_counter() - t_start > budget_s:
+            break
         kind = "soup" if i % 2 == 0 else "icospheres"
         n = int(round(math.exp(rng.uniform(math.log(tri_lo), math.log(tri_hi)))))
         t0 = time.perf_counter()
         scene = mrt.Scene(ctx, make_scene(mrt, kind, 1000 + seed + i, n))
         scene.build(abi.BVH)
-        for k in range(poses):
+        k0 = done
+        for _ in range(per_scene):
+            if done >= n_pairs or time.perf_counter() - t_start > budget_s:
+                break
             cam = pose_camera(mrt, abi, rng, W_FULL, H_FULL)
             s = int(rng.integers(1 << 20)) * 16
-            d_in = frame_desc(mrt, abi, cam, W_FULL, H_FULL, 0, SPP_IN, s)
-            rad, _ = render_mean(scene, d_in)
-            aov = scene.aovs(d_in)
-            tgt, _ = render_mean(scene, frame_desc(mrt, abi, cam, W_FULL, H_FULL, SPP_IN, SPP_IN + spp_tgt, s))
-            xs.append(denoise.pack_input(rad, aov, 1.0, 1.0, 5)[:, :H_FULL, :W_FULL])
-            ts.append(tgt.transpose(2, 0, 1).copy())
+            cx, cy = int(rng.integers(0, 16 - ta + 1)), int(rng.integers(0, 16 - td + 1))
+            tiles = [(cy + r) * 16 + cx + c for r in range(td) for c in range(ta)]
+            x0, y0 = cx * tw, cy * th
+            acc.zero_()
+            bm.zero_()
+            d_in = frame_desc(mrt, abi, cam, W_FULL, H_FULL, 0, SPP_IN, s, tiles)
+            scene.render_with_aovs(d_in, bitmap=bm, accum=acc, aov=aov, mem=abi.MEM_DEVICE)
+            rad = acc[y0:y0 + rh, x0:x0 + rw].cpu().numpy() / np.float32(SPP_IN)
+            a = aov[y0:y0 + rh, x0:x0 + rw].cpu().numpy()
+            xs[done, ..., 0:3] = np.log1p(np.maximum(rad, 0))
+            xs[done, ..., 3:9] = a[..., 0:6]
+            xs[done, ..., 9] = a[..., 6]
+            acc.zero_()
+            # The target in passes of up to 64 samples, each adding to acc:
+            # a frame's per-sample radiance (12 bytes per pixel and sample)
+            # must stay within 2 GiB for sample-parallel slots.
+            for f in range(SPP_IN, SPP_IN + spp_tgt, 64):
+                bm.zero_()
+                scene.render(frame_desc(mrt, abi, cam, W_FULL, H_FULL, f, min(f + 64, SPP_IN + spp_tgt), s, tiles),
+                             bitmap=bm, accum=acc, mem=abi.MEM_DEVICE)
+            tgt = acc[y0:y0 + rh, x0:x0 + rw].cpu().numpy()
+            ts[done] = np.log1p(np.maximum(tgt / np.float32(spp_tgt), 0))
+            done += 1
         scene.close()
-        log(f"scene {i + 1}/{n_scenes} ({kind}, {n} triangles): {poses} pairs in {time.perf_counter() - t0:.1f} s")
-    return np.stack(xs), np.stack(ts)
+        log(f"scene {i + 1}/{n_scenes} ({kind}, {n} triangles): {done - k0} pairs in {time.perf_counter() - t0:.1f} s "
+            f"({done} of {n_pairs})")
+    return xs[:done], ts[:done]
 
 
-def train(x_all, t_all, seconds, batch, crop, lr, seed, levels=5, channels=denoise.DEFAULT_CHANNELS,
-          device="cuda:0"):
-    dev = torch.device(device)
-    torch.manual_seed(seed)
-    model = denoise.make_unet(levels, channels).to(dev)
-    opt = torch.optim.Adam(model.parameters(), lr=lr)
-    x_all = torch.from_numpy(x_all).to(dev)
-    t_all = torch.from_numpy(np.log1p(np.maximum(t_all, 0))).to(dev)
-    N, _, H, W = x_all.shape
-    g = torch.Generator(device="cpu").manual_seed(seed)
+def train(mrt, ctx, x_all, t_all, seconds, batch, crop, lr, seed, levels=5, channels=denoise.DEFAULT_CHANNELS):
+    """Adam steps of mrt_denoiser_trainer on the pairs for `seconds` of wall
+    clock, learning rate lr with a cosine decay to 2%; returns (flat FP32
+    weights, steps, last loss, ms per step, algorithmic TFLOP/s)."""
+    w0 = denoise.init_weights(levels, channels, seed)
+    tr = mrt.DenoiserTrainer(ctx, w0, mrt.denoiser_desc(levels, channels),
+                             mrt.denoiser_train_desc(batch=batch, crop=crop, seed=seed))
+    tr.set_pairs(x_all, t_all)
     t0 = time.perf_counter()
     last = t0
-    it = 0
+    loss = float("nan")
     ema = None
     while True:
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-        for pg in opt.param_groups:
-            pg["lr"] = lr * (0.02 + 0.98 * 0.5 * (1 + math.cos(math.pi * min(1.0, el / seconds))))
-        idx = torch.randint(N, (batch,), generator=g)
-        oy = torch.randint(H - crop + 1, (batch,), generator=g)
-        ox = torch.randint(W - crop + 1, (batch,), generator=g)
-        xb = torch.stack([x_all[i, :, y:y + crop, x:x + crop] for i, y, x in zip(idx, oy, ox)])
-        tb = torch.stack([t_all[i, :, y:y + crop, x:x + crop] for i, y, x in zip(idx, oy, ox)])
-        if torch.rand(1, generator=g).item() < 0.5:
-            xb, tb = xb.flip(3), tb.flip(3)
-        if torch.rand(1, generator=g).item() < 0.5:
-            xb, tb = xb.flip(2), tb.flip(2)
-        with torch.autocast(dev.type, dtype=torch.bfloat16):
-            y = model(xb)
-        loss = torch.mean((y.float() - tb) ** 2)
-        opt.zero_grad(set_to_none=True)
-        loss.backward()
-        opt.step()
-        it += 1
-        ema = loss.item() if ema is None else 0.98 * ema + 0.02 * loss.item()
+        cur = lr * (0.02 + 0.98 * 0.5 * (1 + math.cos(math.pi * min(1.0, el / seconds))))
+        loss = tr.step(50, cur)
+        ema = loss if ema is None else 0.9 * ema + 0.1 * loss
         if time.perf_counter() - last > 15:
             last = time.perf_counter()
-            log(f"iteration {it}, {el:.0f} s, loss {ema:.5f}")
-    log(f"trained {it} iterations in {time.perf_counter() - t0:.1f} s, final loss {ema:.5f}")
-    return model.cpu().float(), it, ema
+            st = tr.stats()
+            log(f"step {st.steps}, {el:.0f} s, loss {ema:.5f}, {st.ms_per_step:.2f} ms per step")
+    st = tr.stats()
+    w = tr.weights()
+    tr.close()
+    tflops = st.flops_per_step / (st.ms_per_step / 1e3) / 1e12
+    log(f"trained {st.steps} steps in {time.perf_counter() - t0:.1f} s, loss {ema:.5f}, "
+        f"{st.ms_per_step:.2f} ms per step ({tflops:.0f} TFLOP/s)")
+    return w, int(st.steps), float(ema), float(st.ms_per_step), float(tflops)
 
 
 def rel_mse(x, ref):
@@ -214,9 +239,11 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", required=True, help="weights file (.npz) to write")
     ap.add_argument("--eval-json", default=None, help="write the evaluation here")
-    ap.add_argument("--scenes", type=int, default=10)
-    ap.add_argument("--poses", type=int, default=4)
-    ap.add_argument("--spp-target", type=int, default=128)
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--scenes", type=int, default=20)
+    ap.add_argument("--spp-target", type=int, default=1024)
+    ap.add_argument("--render-budget", type=float, default=660.0,
+                    help="seconds of pair rendering at most (fewer pairs if it runs out)")
     ap.add_argument("--spp-ref", type=int, default=1024)
     ap.add_argument("--tri-lo", type=int, default=50_000)
     ap.add_argument("--tri-hi", type=int, default=1_000_000)
@@ -228,6 +255,11 @@ def main() -> int:
     ap.add_argument("--eval-only", default=None, metavar="WEIGHTS",
                     help="skip training: evaluate this weights file (--out is ignored)")
     args = ap.parse_args()
+
+    # torch (device buffers of the pair renders) is initialised before the
+    # library is loaded: its wheel carries a HIP runtime of the same soname
+    # as the system one, and the first one loaded serves both.
+    import torch
 
     torch.cuda.init()
     import mobileraytracer_amd as mrt
@@ -247,16 +279,21 @@ def main() -> int:
         ctx.close()
         return 0
     t0 = time.perf_counter()
-    x_all, t_all = render_pairs(mrt, abi, ctx, args.scenes, args.poses, args.spp_target, args.seed, args.tri_lo,
-                                args.tri_hi)
-    log(f"{len(x_all)} pairs rendered in {time.perf_counter() - t0:.1f} s")
-    model, iters, loss = train(x_all, t_all, args.train_seconds, args.batch, args.crop, args.lr, args.seed)
+    x_all, t_all = render_pairs(mrt, abi, ctx, args.pairs, args.scenes, args.spp_target, args.seed, args.tri_lo,
+                                args.tri_hi, args.render_budget)
+    render_s = time.perf_counter() - t0
+    log(f"{len(x_all)} pairs rendered in {render_s:.1f} s")
+    flat, steps, loss, ms_step, tflops = train(mrt, ctx, x_all, t_all, args.train_seconds, args.batch, args.crop,
+                                               args.lr, args.seed)
+    n_pairs = len(x_all)
     del x_all, t_all
-    torch.cuda.empty_cache()
-    flat = denoise.flat_weights(model)
-    meta = json.dumps({"pairs": args.scenes * args.poses, "spp_in": SPP_IN, "spp_target": args.spp_target,
-                       "iterations": iters, "loss": loss, "batch": args.batch, "crop": args.crop,
-                       "train_seconds": args.train_seconds, "seed": args.seed})
+    meta = json.dumps({"pairs": n_pairs, "pair_region": [REGION_TILES[0] * (W_FULL // 16),
+                                                          REGION_TILES[1] * (H_FULL // 16)],
+                       "frame": [W_FULL, H_FULL], "spp_in": SPP_IN, "spp_target": args.spp_target,
+                       "render_seconds": round(render_s, 1), "steps": steps, "loss": loss, "batch": args.batch,
+                       "crop": args.crop, "train_seconds": args.train_seconds, "ms_per_step": round(ms_step, 3),
+                       "tflops": round(tflops, 1), "trainer": "mrt_denoiser_trainer (BF16 MFMA kernels)",
+                       "seed": args.seed})
     denoise.save_weights(args.out, flat, 5, denoise.DEFAULT_CHANNELS, 1.0, meta)
     log(f"weights written to {args.out}")
     flat_bf16, levels, channels, ds = denoise.load_weights(args.out)
