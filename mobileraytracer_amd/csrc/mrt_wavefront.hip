@@ -539,8 +539,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
 // ------------------------------------------- trace (persistent, grid) --
 // Persistent trace of both queues for RegularGrid scenes of triangles and
 // lights: the refilling lanes of k_wf_trace_p over a resumable 3D-DDA.  A
-// lane's unit of work is one primitive test of its current cell's list or
-// one DDA step to the next cell (RegularGrid<T>::trace / shadowTrace,
+// lane's unit of work is up to kGridBatch primitive tests of its current
+// cell's list or one DDA step to the next cell (RegularGrid<T>::trace / shadowTrace,
 // RegularGrid.hpp:157-465, in their order).  Closest hits: phase 1 tests the
 // cell's list until a primitive hits; the rest of that cell's list is then
 // tested in phase 2 (the reference's phase 2 re-tests the whole list, but a
@@ -549,6 +549,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_p(SceneDev s, const float4*
 // nearest t is not below the next wall.  The area lights are tested after the
 // walk (Scene::traceLights after the accelerators, Shader.cpp:144-149).  Any
 // hits return at the first blocker.
+// Primitives tested per step: their loads go out together (1M-soup 1080p
+// 16-sample grid frame with 1 / 2 / 4: 141.7 / 181.4 / 190.6 Mrays/s).
+constexpr uint32_t kGridBatch = 4;
+
 struct GridWalk {
     Dda g;
     uint32_t pos, end;  // untested part of the current cell's list
@@ -625,13 +629,33 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace_grid_p(SceneDev s, const fl
         }
         bool occluded = false;
         if (has && w.pos < w.end) {
-            const uint32_t pi = a.cell_ref[w.pos++];
-            if (COUNT) ++rc.ts.prims[shadow ? 2 : 0];
+            // Up to kGridBatch primitives of the cell's list per step, all
+            // loaded before any is tested and tested in list order (the strict
+            // t < best rule keeps the first of equal hits, as one at a time).
+            const uint32_t m = min(w.end - w.pos, kGridBatch);
+            uint32_t pi[kGridBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kGridBatch; ++k) {
+                pi[k] = 0u;
+                if (k < m) pi[k] = a.cell_ref[w.pos + k];
+            }
+            w.pos += m;
+            TriG g[kGridBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kGridBatch; ++k) g[k] = load_tri(s.tri_geo, k < m ? pi[k] : pi[0]);
+            if (COUNT) rc.ts.prims[shadow ? 2 : 0] += m;
             if (shadow) {
-                occluded = prim_any<MRT_CAT_TRIANGLE>(s, pi, r, h.t);
+#pragma unroll
+                for (uint32_t k = 0; k < kGridBatch; ++k)
+                    occluded = occluded ||
+                               (k < m && MRT_MAKE_ID(MRT_CAT_TRIANGLE, pi[k]) != r.excl && tri_any(g[k], r, h.t));
                 w.done = occluded;
-            } else if (prim_closest<MRT_CAT_TRIANGLE, true>(s, pi, r, h)) {
-                w.phase2 = true;
+            } else {
+                bool hit = false;
+#pragma unroll
+                for (uint32_t k = 0; k < kGridBatch; ++k)
+                    if (k < m) hit = tri_closest_strict(g[k], pi[k], r, h) || hit;
+                if (hit) w.phase2 = true;
             }
         } else if (has) {
             if (COUNT) ++rc.ts.pairs[shadow ? 2 : 0];
