@@ -414,10 +414,6 @@ __device__ __forceinline__ bool dda_step(Dda& g, bool useLimit, float limit) {
     return true;
 }
 
-// Triangles a per-ray grid walk tests per batch (their loads go out
-// together); two keep the per-ray kernels' registers close to the BVH walk's.
-constexpr uint32_t kGridRayBatch = 2;
-
 // RegularGrid<T>::trace, RegularGrid.hpp:157-337.
 template <int CAT, bool COUNT>
 __device__ void grid_closest(const SceneDev& s, const Accel& a, const RayT& r, HitT& h, TraceStats& ts) {
@@ -427,32 +423,6 @@ __device__ void grid_closest(const SceneDev& s, const Accel& a, const RayT& r, H
     while (true) {
         const uint32_t c = dda_cell(a, g);
         const uint32_t b = a.cell_off[c], e = a.cell_off[c + 1u];
-        if (CAT == MRT_CAT_TRIANGLE) {
-            // Triangles: the cell's whole list in batches of kGridRayBatch
-            // (loads together, tests in list order).  The reference's phase 1 stops
-            // at the first hit and its phase 2 then tests the whole list
-            // again; under the strict t < best rule that is the list tested
-            // once in order, and the step that follows is the same.
-            for (uint32_t k = b; k < e; k += kGridRayBatch) {
-                const uint32_t m = min(e - k, kGridRayBatch);
-                uint32_t pi[kGridRayBatch];
-#pragma unroll
-                for (uint32_t j = 0; j < kGridRayBatch; ++j) {
-                    pi[j] = 0u;
-                    if (j < m) pi[j] = a.cell_ref[k + j];
-                }
-                TriG tg[kGridRayBatch];
-#pragma unroll
-                for (uint32_t j = 0; j < kGridRayBatch; ++j) tg[j] = load_tri(s.tri_geo, j < m ? pi[j] : pi[0]);
-                if (COUNT) ts.prims[0] += m;
-#pragma unroll
-                for (uint32_t j = 0; j < kGridRayBatch; ++j)
-                    if (j < m && tri_closest_strict(tg[j], pi[j], r, h)) phase2 = true;
-            }
-            if (COUNT) ++ts.pairs[0];
-            if (!dda_step(g, phase2, h.t)) return;
-            continue;
-        }
         if (!phase2) {
             bool hit = false;
             for (uint32_t k = b; k < e; ++k) {
@@ -481,31 +451,6 @@ __device__ bool grid_any(const SceneDev& s, const Accel& a, const RayT& r, float
     while (true) {
         const uint32_t c = dda_cell(a, g);
         const uint32_t b = a.cell_off[c], e = a.cell_off[c + 1u];
-        if (CAT == MRT_CAT_TRIANGLE) {
-            // Triangles in batches of kGridRayBatch (loads together).
-            for (uint32_t k = b; k < e; k += kGridRayBatch) {
-                const uint32_t m = min(e - k, kGridRayBatch);
-                uint32_t pi[kGridRayBatch];
-#pragma unroll
-                for (uint32_t j = 0; j < kGridRayBatch; ++j) {
-                    pi[j] = 0u;
-                    if (j < m) pi[j] = a.cell_ref[k + j];
-                }
-                TriG tg[kGridRayBatch];
-#pragma unroll
-                for (uint32_t j = 0; j < kGridRayBatch; ++j) tg[j] = load_tri(s.tri_geo, j < m ? pi[j] : pi[0]);
-                if (COUNT) ts.prims[2] += m;
-                bool blocked = false;
-#pragma unroll
-                for (uint32_t j = 0; j < kGridRayBatch; ++j)
-                    blocked = blocked || (j < m && MRT_MAKE_ID(MRT_CAT_TRIANGLE, pi[j]) != r.excl &&
-                                          tri_any(tg[j], r, dist));
-                if (blocked) return true;
-            }
-            if (COUNT) ++ts.pairs[2];
-            if (!dda_step(g, false, 0.0f)) return false;
-            continue;
-        }
         for (uint32_t k = b; k < e; ++k) {
             if (COUNT) ++ts.prims[2];
             if (prim_any<CAT>(s, a.cell_ref[k], r, dist)) return true;
