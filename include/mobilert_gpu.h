@@ -283,7 +283,7 @@ typedef struct mrt_tuning {
                                   many of its lanes are idle (1-64; 16, the
                                   persistent megakernel 32)                    */
     uint32_t fetch_chunk;      /* queue entries a wave reserves per atomic
-                                  (64-4096; 128, the persistent megakernel 64) */
+                                  (64-4096; 64)                                */
     uint32_t launch_share_pct; /* workgroups of one slot group's trace launch, %
                                   of those the chip holds at once (1-100; 100
                                   for one group, 120 / groups for more)        */

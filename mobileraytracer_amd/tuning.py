@@ -57,7 +57,7 @@ SEARCH_SPACE: dict[str, dict[str, tuple[int, ...]]] = {
         "max_groups": (3, 1, 2),
         "launch_share_pct": (0, 30, 60, 100),
         "refill_lanes": (16, 8, 32),
-        "fetch_chunk": (128, 64, 256),
+        "fetch_chunk": (64, 128, 256),
         "overlap_loads": (0, 1, 2),
         "graph_loop": (0, 1),
     },
