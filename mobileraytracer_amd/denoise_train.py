@@ -19,9 +19,11 @@ learning-rate decay.
 Evaluation, on scenes and poses not in the training set (the 1M-triangle
 soup of the benchmark with its camera, and an icosphere cluster): relative
 MSE, mean over pixels and channels of (x - ref)^2 / (ref^2 + 0.01), against
-the mean of 1024 further samples, for the plain 4-sample frame, the denoised
-4-sample frame (mrt_denoiser_run, BF16 on MFMA) and the plain 64-sample
-frame, with the time each took.
+the mean of 1024 independent samples (from sample 1024), for the plain
+frames of 4, 64, 128, 256 and 512 samples and the denoised 4-sample frame
+(mrt_denoiser_run, BF16 on MFMA), with the time each took, and the plain
+render's equal-quality point (the samples and time at which it matches the
+denoised frame's relMSE).
 """
 from __future__ import annotations
 
@@ -185,10 +187,42 @@ def rel_mse(x, ref):
     return float(np.mean((x.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 0.01)))
 
 
+PLAIN_SPP = (4, 64, 128, 256, 512)
+REF_FIRST = 1024  # plain frames use samples below 512; 1920 x 1080 x 2048 < 2^32
+
+
+def render_plain(scene, mrt, abi, cam, spp, seed):
+    """Mean of samples [0, spp) rendered in passes of up to 64 samples (a
+    frame's per-sample radiance must stay within 2 GiB for sample-parallel
+    slots, so one 512-sample call would run one slot per pixel), and the
+    wall-clock milliseconds of the passes (host buffers)."""
+    acc = np.zeros((H_FULL, W_FULL, 3), np.float32)
+    t0 = time.perf_counter()
+    for f in range(0, spp, 64):
+        scene.render(frame_desc(mrt, abi, cam, W_FULL, H_FULL, f, min(f + 64, spp), seed), accum=acc)
+    return acc / np.float32(spp), (time.perf_counter() - t0) * 1e3
+
+
+def equal_quality(spps, rels, times, target):
+    """Samples and milliseconds at which the plain render reaches relMSE
+    `target`: log-log interpolation of relMSE between the two measured sample
+    counts that bracket it, time linear in samples; None above the largest."""
+    for k in range(len(spps) - 1):
+        r1, r2 = rels[k], rels[k + 1]
+        if r1 >= target >= r2 and r1 > r2:
+            f = (math.log(target) - math.log(r1)) / (math.log(r2) - math.log(r1))
+            spp = math.exp(math.log(spps[k]) + f * (math.log(spps[k + 1]) - math.log(spps[k])))
+            ms = times[k] + (spp - spps[k]) * (times[k + 1] - times[k]) / (spps[k + 1] - spps[k])
+            return spp, ms
+    return None, None
+
+
 def evaluate(mrt, abi, ctx, dn, spp_ref, seed):
-    """relMSE of the plain 4-sample, denoised 4-sample and plain 64-sample
-    frames against 1024 further samples, for the benchmark soup and an
-    icosphere cluster."""
+    """relMSE of the plain frames of 4 to 512 samples and of the denoised
+    4-sample frame against `spp_ref` further, independent samples (from
+    sample 1024), for the benchmark soup and an icosphere cluster, and the
+    plain render's equal-quality point: the samples and time it needs to
+    match the denoised frame's relMSE."""
     out = []
     cases = [("soup 1M (benchmark scene and camera)", "soup", 2, 1_000_000, None),
              ("icosphere cluster 1.6M", "icospheres", 5, 1_638_400, 77)]
@@ -222,14 +256,30 @@ def evaluate(mrt, abi, ctx, dn, spp_ref, seed):
         den = dn.run(x4, aov)
         ms_dn_host = (time.perf_counter() - t0) * 1e3
         ms_dn = dn.last_ms()
-        x64, ms64 = render_mean(scene, frame_desc(mrt, abi, cam, W_FULL, H_FULL, 0, 64, s))
-        ref, ms_ref = render_mean(scene, frame_desc(mrt, abi, cam, W_FULL, H_FULL, 64, 64 + spp_ref, s))
+        ref = np.zeros((H_FULL, W_FULL, 3), np.float32)
+        t0 = time.perf_counter()
+        for f in range(REF_FIRST, REF_FIRST + spp_ref, 64):
+            scene.render(frame_desc(mrt, abi, cam, W_FULL, H_FULL, f, min(f + 64, REF_FIRST + spp_ref), s), accum=ref)
+        ms_ref = (time.perf_counter() - t0) * 1e3
+        ref /= np.float32(spp_ref)
+        plain_rel, plain_ms = [], []
+        for k in PLAIN_SPP:
+            xk, msk = render_plain(scene, mrt, abi, cam, k, s)
+            plain_rel.append(rel_mse(xk, ref))
+            plain_ms.append(round(msk, 2))
         scene.close()
-        r = {"scene": name, "triangles": n, "relmse_4spp": rel_mse(x4, ref), "relmse_denoised_4spp": rel_mse(den, ref),
-             "relmse_64spp": rel_mse(x64, ref), "ms_render_4spp": round(ms4, 2), "ms_aov": round(ms_aov, 2),
-             "ms_render_4spp_with_aovs": round(ms4_aov, 2),
-             "ms_denoise_device": round(ms_dn, 3), "ms_denoise_call": round(ms_dn_host, 2),
-             "ms_render_64spp": round(ms64, 2), "ms_render_ref": round(ms_ref, 1), "spp_ref": spp_ref}
+        rel_den = rel_mse(den, ref)
+        ms_den_frame = ms4_aov + ms_dn
+        eq_spp, eq_ms = equal_quality(PLAIN_SPP, plain_rel, plain_ms, rel_den)
+        r = {"scene": name, "triangles": n, "relmse_4spp": plain_rel[0], "relmse_denoised_4spp": rel_den,
+             "relmse_64spp": plain_rel[1], "plain_spp": list(PLAIN_SPP), "plain_relmse": plain_rel,
+             "plain_ms": plain_ms, "ms_render_4spp": round(ms4, 2), "ms_aov": round(ms_aov, 2),
+             "ms_render_4spp_with_aovs": round(ms4_aov, 2), "ms_denoise_device": round(ms_dn, 3),
+             "ms_denoise_call": round(ms_dn_host, 2), "ms_denoised_frame": round(ms_den_frame, 2),
+             "ms_render_64spp": plain_ms[1], "equal_quality_spp": None if eq_spp is None else round(eq_spp, 1),
+             "equal_quality_ms": None if eq_ms is None else round(eq_ms, 1),
+             "equal_quality_speedup": None if eq_ms is None else round(eq_ms / ms_den_frame, 1),
+             "ms_render_ref": round(ms_ref, 1), "spp_ref": spp_ref, "ref_first_sample": REF_FIRST}
         log(json.dumps(r))
         out.append(r)
     return out
