@@ -17,7 +17,9 @@ holding a resident pool of 16 seeded scenes (50k-8M triangles, soups and
 displaced-icosphere clusters, 1% emissive) receives a Poisson stream of
 requests (random camera pose, 256x256 to 1920x1080, 1-64 spp) for a fixed
 window; p50/p99 request latency and the window's Mrays/s go into the line's
-"serving" object.  --serve-sweep runs only the serving workload over a list of
+"serving" object, beside ("plain") the same stream on the plain GPU path:
+each request rendered alone with mrt_render, and the latencies of serving
+them one at a time in arrival order with those times.  --serve-sweep runs only the serving workload over a list of
 rates and prints one JSON line per rate.
 
 Denoiser leg (after the timed frames, outside their timing): the BF16 U-Net
@@ -303,7 +305,7 @@ def main() -> int:
             # Bounded request count at high rates (the host buffers of queued
             # requests are held until they are served).
             dur = min(args.serve_duration, 600.0 / rate)
-            res = serving_run(mrt, srv, pool, rate, dur)
+            res = serving_run(mrt, srv, pool, rate, dur, plain=False)
             print(json.dumps({"metric": "render-request latency (serving sweep)", "serving": res}), flush=True)
         srv.close()
         return 0
@@ -726,16 +728,20 @@ def serving_setup(mrt, ctx, args):
     return srv, pool
 
 
-def serving_run(mrt, srv, pool, rate, duration):
+def serving_run(mrt, srv, pool, rate, duration, plain=True):
     from mobileraytracer_amd import serving
 
     t0 = time.time()
-    res = serving.run_stream(srv, pool, rate, duration, seed=4)
+    res = serving.run_stream(srv, pool, rate, duration, seed=4, plain=plain)
     res["config"] = (f"{len(pool)} resident scenes {min(p.triangles for p in pool)}-"
                      f"{max(p.triangles for p in pool)} triangles; Poisson arrivals; 256x256..1920x1080; "
                      f"1-64 spp; PathTracer depth 6 RR 3; FP32 accumulation")
     log(f"serving at {rate} req/s: {res['done']} of {res['requests']} requests in {time.time() - t0:.1f} s, "
         f"p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms, {res['mrays_per_s']} Mrays/s")
+    if "plain" in res:
+        pl = res["plain"]
+        log(f"plain GPU path, one request at a time in arrival order: p50 {pl['fifo_p50_ms']} ms p99 "
+            f"{pl['fifo_p99_ms']} ms (alone p50 {pl['alone_p50_ms']} ms p99 {pl['alone_p99_ms']} ms)")
     return res
 
 

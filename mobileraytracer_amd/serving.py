@@ -102,11 +102,36 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else float("nan")
 
 
+def plain_fifo(pool: list[PoolScene], arrivals: list[float], reqs: list, accum: bool = True) -> dict:
+    """The plain GPU path for the same stream: each request rendered alone
+    (mrt_render, AUTO kernel, host buffers, wall clock) on an idle GPU, and
+    the latencies of serving them one at a time in arrival order with those
+    service times (first in, first out, no overlap)."""
+    alone = []
+    for si, desc in reqs:
+        d = abi.RenderDesc.from_buffer_copy(desc)
+        d.kernel = abi.KERNEL_AUTO
+        bm = np.zeros((d.height, d.width), np.uint32)
+        acc = np.zeros((d.height, d.width, 3), np.float32) if accum else None
+        t0 = time.perf_counter()
+        pool[si].scene.render(d, bitmap=bm, accum=acc)
+        alone.append((time.perf_counter() - t0) * 1e3)
+    free, lat = 0.0, []
+    for ta, ms in zip(arrivals, alone):
+        start = max(ta * 1e3, free)
+        free = start + ms
+        lat.append(free - ta * 1e3)
+    return {"alone_p50_ms": round(pct(alone, 50), 2), "alone_p99_ms": round(pct(alone, 99), 2),
+            "fifo_p50_ms": round(pct(lat, 50), 2), "fifo_p99_ms": round(pct(lat, 99), 2),
+            "busy_s": round(sum(alone) / 1e3, 2)}
+
+
 def run_stream(srv: RenderServer, pool: list[PoolScene], rate: float, duration: float, seed: int,
-               accum: bool = True, max_requests: int = 100000) -> dict:
+               accum: bool = True, max_requests: int = 100000, plain: bool = False) -> dict:
     """Submits Poisson(rate) arrivals for `duration` seconds, waits for all of
     them and returns latency percentiles (ms) and the throughput of the
-    window from the first submission to the last result."""
+    window from the first submission to the last result; with `plain`, also
+    the same stream on the plain GPU path (plain_fifo)."""
     rng = np.random.default_rng(seed)
     arrivals = []
     t = rng.exponential(1.0 / rate)
@@ -154,14 +179,17 @@ def run_stream(srv: RenderServer, pool: list[PoolScene], rate: float, duration: 
     wait = [(a - s) / 1000.0 for (_, s, a, _, _) in infos]
     window = (max(d for (*_, d) in infos) - min(s for (_, s, *_) in infos)) / 1e6 if infos else float("nan")
     px_samples = sum(m[1] * m[2] * m[3] for (m, *_) in infos)
-    return {"rate_rps": rate, "duration_s": duration, "requests": len(arrivals), "done": len(infos),
-            "failed": len(errors), "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
-            "mean_ms": round(float(np.mean(lat)) if lat else float("nan"), 2),
-            "max_ms": round(max(lat) if lat else float("nan"), 2),
-            "queue_p50_ms": round(pct(wait, 50), 2), "queue_p99_ms": round(pct(wait, 99), 2),
-            "window_s": round(window, 3), "achieved_rps": round(len(infos) / window, 2) if infos else 0.0,
-            "mrays_per_s": round(rays / window / 1e6, 2) if infos else 0.0, "rays": int(rays),
-            "pixel_samples": int(px_samples), "errors": errors[:3]}
+    out = {"rate_rps": rate, "duration_s": duration, "requests": len(arrivals), "done": len(infos),
+           "failed": len(errors), "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+           "mean_ms": round(float(np.mean(lat)) if lat else float("nan"), 2),
+           "max_ms": round(max(lat) if lat else float("nan"), 2),
+           "queue_p50_ms": round(pct(wait, 50), 2), "queue_p99_ms": round(pct(wait, 99), 2),
+           "window_s": round(window, 3), "achieved_rps": round(len(infos) / window, 2) if infos else 0.0,
+           "mrays_per_s": round(rays / window / 1e6, 2) if infos else 0.0, "rays": int(rays),
+           "pixel_samples": int(px_samples), "errors": errors[:3]}
+    if plain:
+        out["plain"] = plain_fifo(pool, arrivals, reqs, accum)
+    return out
 
 
 # ---------------------------------------------------------------- batched --
