@@ -293,11 +293,12 @@ typedef struct mrt_tuning {
                                   node loads together: 0 when a launch cannot
                                   give each lane more than one ray, 1 always,
                                   2 never                                      */
-    uint32_t graph_loop;       /* 1: mrt_render's wavefront loop as a captured
-                                  HIP graph of 4 iterations of every slot
-                                  group, replayed until the frame drains; the
-                                  scene keeps it, and a frame with the same
-                                  launch arguments replays it without a new
+    uint32_t graph_loop;       /* 1: mrt_render's wavefront loop as captured
+                                  HIP graphs, one per slot group (4 of its
+                                  iterations, replayed on the group's stream
+                                  until the group drains); the scene keeps
+                                  them, and a frame with the same launch
+                                  arguments replays them without a new
                                   capture (graph frames report no per-launch
                                   trace times).  0, 2: launched iterations    */
     uint32_t service_lanes;    /* persistent megakernel: a wave shades once this

@@ -613,40 +613,20 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
     uint32_t checks = 0;
     bool stopReq = false;
     size_t ev = 0;
-    // Graph loop: one batch (kBatch iterations of every group, the groups
-    // forked from and joined back into the first stream) is captured once and
-    // replayed until the frame drains.  An even number of iterations returns
-    // every queue and list to its starting role, so each replay repeats the
-    // captured launches exactly; a stop request recaptures with the stop
-    // flag set.  The counter snapshots and check events stay outside.
-    // A later frame whose launch arguments are all the same (same scene,
-    // descriptor, buffers and launch sizes: a repeated or progressive view)
-    // replays the scene's kept graph without capturing it again.
+    // Graph loop: each slot group's batch (kBatch iterations) is captured
+    // once on the group's own stream and replayed there until the group
+    // drains, so the groups overlap as launched iterations do.  An even
+    // number of iterations returns every queue and list to its starting
+    // role, so each replay repeats the captured launches exactly; a stop
+    // request recaptures with the stop flag set.  The counter snapshots and
+    // check events stay outside.  A later frame whose launch arguments are
+    // all the same (same scene, descriptor, buffers and launch sizes: a
+    // repeated or progressive view) replays the scene's kept graphs without
+    // capturing them again.  Each graph is launched on its group's stream
+    // behind that group's counter clear and start kernel.
     const bool useGraph = s->ctx->tuning.graph_loop == 1u;
-    hipGraph_t& graph = s->wf_graph;
-    hipGraphExec_t& exec = s->wf_exec;
     bool graphStop = false;
     uint64_t graphIters = 0;
-    // gev: the captured fork and joins; gev[G + g]: group g's start work.
-    std::vector<hipEvent_t> gev(useGraph ? 2 * G : 0, nullptr);
-    struct EventGuard {
-        std::vector<hipEvent_t>& e;
-        ~EventGuard() {
-            for (hipEvent_t v : e)
-                if (v) (void)hipEventDestroy(v);
-        }
-    } graphGuard{gev};
-    for (hipEvent_t& e : gev) MRT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // The graph runs every group's iterations from the first stream, so that
-    // stream waits for the other groups' counter clears and start kernels
-    // (issued on their own streams above): a captured graph carries no
-    // dependency on work queued on a stream before the capture forked it.
-    if (useGraph) {
-        for (uint32_t g = 1; g < G; ++g) {
-            MRT_HIP(hipEventRecord(gev[G + g], st[g]));
-            MRT_HIP(hipStreamWaitEvent(st[0], gev[G + g], 0));
-        }
-    }
     auto graph_key = [&](bool stopFlag) {
         std::vector<uint8_t> k;
         auto put = [&](const void* v, size_t n) {
@@ -671,38 +651,36 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
     };
     auto capture = [&]() -> int {
         std::vector<uint8_t> key = graph_key(stopReq);
-        if (exec && key == s->wf_key) {
-            // Same arguments: replay the kept batch (the loop state advances
-            // as the captured launches would have advanced it).
+        if (s->wf_exec[0] && key == s->wf_key) {
+            // Same arguments: replay the kept batches (the loop state
+            // advances as the captured launches would have advanced it).
             for (uint32_t k = 0; k < kBatch; ++k)
                 for (uint32_t g = 0; g < G; ++g) wavefront_loop_advance(&L[g], wbs[g].sort != 0u);
             graphStop = stopReq;
             return MRT_OK;
         }
         s->wf_key.clear();
-        if (exec) MRT_HIP(hipGraphExecDestroy(exec));
-        exec = nullptr;
-        if (graph) MRT_HIP(hipGraphDestroy(graph));
-        graph = nullptr;
-        MRT_HIP(hipStreamBeginCapture(st[0], hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipEventRecord(gev[0], st[0]);
-        for (uint32_t g = 1; g < G && e == hipSuccess; ++g) e = hipStreamWaitEvent(st[g], gev[0], 0);
-        for (uint32_t k = 0; k < kBatch && e == hipSuccess; ++k)
-            for (uint32_t g = 0; g < G && e == hipSuccess; ++g)
+        for (uint32_t g = 0; g < kWfMaxGroups; ++g) {
+            if (s->wf_exec[g]) MRT_HIP(hipGraphExecDestroy(s->wf_exec[g]));
+            s->wf_exec[g] = nullptr;
+            if (s->wf_graph[g]) MRT_HIP(hipGraphDestroy(s->wf_graph[g]));
+            s->wf_graph[g] = nullptr;
+        }
+        for (uint32_t g = 0; g < G; ++g) {
+            MRT_HIP(hipStreamBeginCapture(st[g], hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (uint32_t k = 0; k < kBatch && e == hipSuccess; ++k)
                 e = launch_wavefront_iteration(s->dev, s->generic, persistent, d->instrument != 0u, p,
                                                d->samples_pixel, stopReq, dbm, dacc, wbs[g], &L[g],
                                                sized_blocks(trace_blocks, wbs[g].slot_count),
                                                sized_trace_blocks(persist_blocks, wbs[g].slot_count), scs[g],
                                                s->counters.as<unsigned long long>(), s->err.as<uint32_t>(),
                                                nullptr, nullptr, st[g]);
-        for (uint32_t g = 1; g < G && e == hipSuccess; ++g) {
-            e = hipEventRecord(gev[g], st[g]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(st[0], gev[g], 0);
+            const hipError_t ec = hipStreamEndCapture(st[g], &s->wf_graph[g]);
+            MRT_HIP(e);
+            MRT_HIP(ec);
+            MRT_HIP(hipGraphInstantiate(&s->wf_exec[g], s->wf_graph[g], nullptr, nullptr, 0));
         }
-        const hipError_t ec = hipStreamEndCapture(st[0], &graph);
-        MRT_HIP(e);
-        MRT_HIP(ec);
-        MRT_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         s->wf_key = std::move(key);
         graphStop = stopReq;
         return MRT_OK;
@@ -717,16 +695,13 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
                 graphReady = true;
             } else {
                 for (uint32_t k = 0; k < kBatch; ++k)  // the replay repeats the captured batch
-                    for (uint32_t g = 0; g < G; ++g) wavefront_loop_advance(&L[g], wbs[g].sort != 0u);
+                    for (uint32_t g = 0; g < G; ++g)
+                        if (!done[g]) wavefront_loop_advance(&L[g], wbs[g].sort != 0u);
             }
-            MRT_HIP(hipGraphLaunch(exec, st[0]));
-            graphIters += kBatch * G;
-            const uint32_t slot = checks & 1u;
             for (uint32_t g = 0; g < G; ++g) {
-                MRT_HIP(hipMemcpyAsync(s->counts_host + (2 * g + slot) * kCntWords, wbs[g].counts,
-                                       kCntWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st[0]));
-                MRT_HIP(hipEventRecord(chk(g, slot), st[0]));
-                std::copy(L[g].pending, L[g].pending + 3, pend[g][slot]);
+                if (done[g]) continue;
+                MRT_HIP(hipGraphLaunch(s->wf_exec[g], st[g]));
+                graphIters += kBatch;
             }
         }
         for (uint32_t k = 0; k < kBatch && !useGraph; ++k) {
@@ -748,7 +723,7 @@ int run_wavefront(mrt_scene* s, const mrt_render_desc* d, const RenderParams& p,
             }
         }
         const uint32_t slot = checks & 1u;
-        for (uint32_t g = 0; g < G && !useGraph; ++g) {
+        for (uint32_t g = 0; g < G; ++g) {
             if (done[g]) continue;
             MRT_HIP(hipMemcpyAsync(s->counts_host + (2 * g + slot) * kCntWords, wbs[g].counts,
                                    kCntWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st[g]));

@@ -125,15 +125,18 @@ struct mrt_scene {
     // one batch behind the launches) and event pairs around trace launches.
     uint32_t* counts_host = nullptr;
     std::vector<hipEvent_t> trace_events;
-    // Graph loop: the last captured batch and the launch arguments it was
-    // captured with (a frame with the same arguments replays it).
-    hipGraph_t wf_graph = nullptr;
-    hipGraphExec_t wf_exec = nullptr;
+    // Graph loop: the last captured batch of each slot group and the launch
+    // arguments they were captured with (a frame with the same arguments
+    // replays them).
+    hipGraph_t wf_graph[mrt::kWfMaxGroups] = {};
+    hipGraphExec_t wf_exec[mrt::kWfMaxGroups] = {};
     std::vector<uint8_t> wf_key;
     mrt::SceneDev dev{};
     ~mrt_scene() {
-        if (wf_exec) (void)hipGraphExecDestroy(wf_exec);
-        if (wf_graph) (void)hipGraphDestroy(wf_graph);
+        for (uint32_t g = 0; g < mrt::kWfMaxGroups; ++g) {
+            if (wf_exec[g]) (void)hipGraphExecDestroy(wf_exec[g]);
+            if (wf_graph[g]) (void)hipGraphDestroy(wf_graph[g]);
+        }
         if (counts_host) (void)hipHostFree(counts_host);
         for (hipEvent_t e : trace_events) (void)hipEventDestroy(e);
         if (counted && ctx) --ctx->children;

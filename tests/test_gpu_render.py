@@ -333,13 +333,13 @@ def test_render_with_aovs(gpu_ctx, kernel, spp, first):
 @pytest.mark.parametrize("size,spp,generic", [((128, 80), 3, False), ((512, 288), 2, False), ((96, 96), 4, True),
                                               ((1280, 720), 16, False)])
 def test_graph_loop_equals_launched_loop(size, spp, generic):
-    """The wavefront loop replayed as a captured HIP graph (batches of four
-    iterations of every slot group) gives the frame the launched loop gives:
-    bitmap, FP32 radiance and ray counts identical; 512x288 runs three slot
-    groups forked into the captured batch; 1280x720 at 16 samples runs three
-    groups with two samples per path slot, after uploads and a 177 MB clear
-    that the other groups' start kernels wait for (the graph, launched on the
-    first group's stream, must wait for them too)."""
+    """The wavefront loop replayed as captured HIP graphs (each slot group's
+    batch of four iterations, captured and replayed on the group's own
+    stream) gives the frame the launched loop gives: bitmap, FP32 radiance
+    and ray counts identical; 512x288 runs three slot groups; 1280x720 at 16
+    samples runs three groups with two samples per path slot, after uploads
+    and a 177 MB clear that each group's graph must wait for behind its
+    group's counter clear and start kernel."""
     W, H = size
     data = mrt.soup_scene(9, 80000, 800)
     if generic:
