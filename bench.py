@@ -610,13 +610,20 @@ def denoiser_leg(mrt, abi, ctx, torch, reps: int, scene=None, desc4=None) -> dic
     ms = float(np.median(times))
     flops = denoise.algorithmic_flops(H, W, levels, channels)
     tflops = flops / (ms / 1000.0) / 1e12
-    log(f"denoiser: {ms:.3f} ms per {W}x{H} frame (median of {reps}), {tflops:.1f} TFLOP/s algorithmic")
+    # Each layer at its own bound (the full-resolution layers move more bytes
+    # than MFMA time covers): the network's time at the layer-wise roofline.
+    roof_ms = denoise.layer_roofline_s(H, W, HBM_PEAK_GBS * 1e9, BF16_PEAK_TFLOPS * 1e12, levels, channels) * 1e3
+    log(f"denoiser: {ms:.3f} ms per {W}x{H} frame (median of {reps}), {tflops:.1f} TFLOP/s algorithmic; "
+        f"layer-wise roofline {roof_ms:.3f} ms ({roof_ms / ms:.3f} of it)")
     return {"ms_per_frame": round(ms, 4), "ms_min": round(min(times), 4), "reps": reps,
             "workload": f"U-Net levels={levels} channels={list(channels)} {W}x{H}, BF16 weights/activations, FP32 "
                         "accumulation, inputs resident in HBM",
             "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / BF16_PEAK_TFLOPS, 4), "traffic": None,
-                         "flops_per_frame": flops, "kernel": "whole network (k_conv3x3 + pack/pool/up/unpack)"},
+                         "flops_per_frame": flops, "kernel": "whole network (k_conv3x3 + pack/pool/up/unpack)",
+                         "layer_roofline_ms": round(roof_ms, 4), "frac_of_layer_roofline": round(roof_ms / ms, 4),
+                         "layer_roofline": "sum over the network's kernels of max(least HBM bytes / 8 TB/s, "
+                                           "FLOPs / 2.5 PFLOP/s)"},
             "pipeline": pipeline}
 
 

@@ -57,6 +57,49 @@ def algorithmic_flops(height: int, width: int, levels: int = 5, channels=DEFAULT
     return sum(2 * 9 * ci * co * (H >> l) * (W >> l) for (ci, co), l in zip(shapes, lvl))
 
 
+def layer_bounds(height: int, width: int, levels: int = 5, channels=DEFAULT_CHANNELS) -> list[dict]:
+    """Per kernel of one network run (convolutions and 2x2 max pools, in run
+    order): algorithmic FLOPs (logical channels, as algorithmic_flops) and the
+    least HBM bytes the kernel can move: every tensor it reads once and every
+    tensor it writes once, BF16 activations (2 bytes per channel), the FP32
+    network input (radiance 3 + AOVs 7 floats per frame pixel) and FP32
+    radiance out, weights and biases.  The decoder's first convolution reads
+    the coarser level's tensor at its own size (the upsampling happens while
+    staging)."""
+    m = 1 << (levels - 1)
+    H, W = (height + m - 1) // m * m, (width + m - 1) // m * m
+    c = list(channels)[:levels]
+    frame = height * width
+    px = [(H >> l) * (W >> l) for l in range(levels)]
+    out = []
+
+    def conv(name, l, ci, co, rd):
+        wb = 9 * ci * co * 2 + 4 * co
+        out.append({"kernel": name, "flops": 2 * 9 * ci * co * px[l], "bytes": rd + wb + 2 * co * px[l]})
+
+    for l in range(levels):
+        ci = IN_CHANNELS if l == 0 else c[l - 1]
+        conv(f"enc{l}a", l, ci, c[l], 40 * frame if l == 0 else 2 * ci * px[l])
+        conv(f"enc{l}b", l, c[l], c[l], 2 * c[l] * px[l])
+        if l + 1 < levels:
+            out.append({"kernel": f"pool{l}", "flops": 0, "bytes": 2 * c[l] * (px[l] + px[l + 1])})
+    for l in range(levels - 2, -1, -1):
+        conv(f"dec{l}a", l, c[l + 1] + c[l], c[l], 2 * c[l + 1] * px[l + 1] + 2 * c[l] * px[l])
+        conv(f"dec{l}b", l, c[l], c[l], 2 * c[l] * px[l])
+    wb = 9 * c[0] * OUT_CHANNELS * 2 + 4 * OUT_CHANNELS
+    out.append({"kernel": "out", "flops": 2 * 9 * c[0] * OUT_CHANNELS * px[0],
+                "bytes": 2 * c[0] * px[0] + wb + 12 * frame})
+    return out
+
+
+def layer_roofline_s(height: int, width: int, hbm_bytes_per_s: float, flops_per_s: float, levels: int = 5,
+                     channels=DEFAULT_CHANNELS) -> float:
+    """Seconds one network run takes at the roofline of each of its kernels:
+    the sum over layer_bounds of max(bytes / HBM bandwidth, FLOPs / peak)."""
+    return sum(max(k["bytes"] / hbm_bytes_per_s, k["flops"] / flops_per_s)
+               for k in layer_bounds(height, width, levels, channels))
+
+
 def bf16_round(a: np.ndarray) -> np.ndarray:
     """FP32 -> nearest-even BF16, returned as FP32 (finite inputs)."""
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)

@@ -128,3 +128,29 @@ def test_pack_input_known_answer():
     y = np.zeros((3, 4, 4), np.float32)
     y[:, 0, 0] = [np.log(3.0), -1.0, 0.0]
     np.testing.assert_allclose(denoise.unpack_output(y, 1, 1)[0, 0], [2.0, 0.0, 0.0], rtol=1e-6)
+
+
+def test_layer_bounds_known_answer():
+    """Per-kernel FLOPs and least HBM bytes of a 2-level network on a 30x32
+    frame (padded to 30x32: even sides), written out by hand."""
+    ks = denoise.layer_bounds(30, 32, 2, (32, 64))
+    assert [k["kernel"] for k in ks] == ["enc0a", "enc0b", "pool0", "enc1a", "enc1b", "dec0a", "dec0b", "out"]
+    p0, p1 = 30 * 32, 15 * 16
+    wb = lambda ci, co: 9 * ci * co * 2 + 4 * co  # noqa: E731
+    expect = {
+        "enc0a": (2 * 9 * 10 * 32 * p0, 40 * p0 + wb(10, 32) + 64 * p0),
+        "enc0b": (2 * 9 * 32 * 32 * p0, 64 * p0 + wb(32, 32) + 64 * p0),
+        "pool0": (0, 64 * p0 + 64 * p1),
+        "enc1a": (2 * 9 * 32 * 64 * p1, 64 * p1 + wb(32, 64) + 128 * p1),
+        "enc1b": (2 * 9 * 64 * 64 * p1, 128 * p1 + wb(64, 64) + 128 * p1),
+        "dec0a": (2 * 9 * 96 * 32 * p0, 128 * p1 + 64 * p0 + wb(96, 32) + 64 * p0),
+        "dec0b": (2 * 9 * 32 * 32 * p0, 64 * p0 + wb(32, 32) + 64 * p0),
+        "out": (2 * 9 * 32 * 3 * p0, 64 * p0 + wb(32, 3) + 12 * p0),
+    }
+    for k in ks:
+        assert (k["flops"], k["bytes"]) == expect[k["kernel"]], k["kernel"]
+    # The FLOPs are the network's algorithmic count; 1080 rows pad to 1088.
+    for h, w in [(1080, 1920), (64, 48)]:
+        assert sum(k["flops"] for k in denoise.layer_bounds(h, w)) == denoise.algorithmic_flops(h, w)
+    t = denoise.layer_roofline_s(30, 32, 1e12, 1e15, 2, (32, 64))
+    assert t == pytest.approx(sum(max(b / 1e12, f / 1e15) for f, b in expect.values()))
