@@ -409,7 +409,8 @@ MRT_API int mrt_render_with_aovs(mrt_scene* scene, const mrt_render_desc* desc, 
 typedef struct mrt_server mrt_server;
 
 typedef struct mrt_server_desc {
-    uint32_t max_slots;        /* path slots (0: 25165824, twelve 1920x1080 frames) */
+    uint32_t max_slots;        /* path slots (0: 33554432, sixteen 1920x1080
+                                  frames)                                      */
     uint32_t max_jobs;         /* jobs admitted at once (0: 256)               */
     uint32_t max_scenes;       /* resident scenes (0: 64)                      */
     uint32_t pipelines;        /* concurrent pipelines, 1..3 (0: 3)            */
@@ -420,7 +421,18 @@ typedef struct mrt_server_desc {
                                   beyond the context budget)                   */
     uint64_t sample_memory;    /* device bytes for the per-sample radiance of
                                   sample-parallel jobs (0: 8 GiB)              */
-    uint32_t reserved[2];
+    uint32_t idle_direct;      /* 1: a job that reaches an idle server (no job
+                                  admitted or queued beside it) and that
+                                  mrt_render's AUTO kernel would give the
+                                  one-launch persistent megakernel (below
+                                  6 x 2^20 pixel samples) renders alone
+                                  through mrt_render on the server's worker,
+                                  with the same results; jobs submitted
+                                  meanwhile wait for it (a cancel lets it
+                                  complete, then it fails with
+                                  MRT_E_STOPPED).  0 (default): every job runs
+                                  on the pipelines                             */
+    uint32_t reserved;
 } mrt_server_desc;
 
 enum mrt_job_state { MRT_JOB_QUEUED = 0, MRT_JOB_RUNNING = 1, MRT_JOB_DONE = 2, MRT_JOB_FAILED = 3 };
@@ -435,7 +447,8 @@ typedef struct mrt_job_info {
     double t_done_us;          /* result in the caller's buffers               */
     uint32_t lanes;            /* slots per pixel the job ran with (1 while
                                   queued; > 1: sample-parallel)               */
-    uint32_t reserved;
+    uint32_t direct;           /* 1: rendered alone by an idle server
+                                  (mrt_server_desc.idle_direct)               */
 } mrt_job_info;
 
 typedef struct mrt_serving_stats {

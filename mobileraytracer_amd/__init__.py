@@ -307,14 +307,18 @@ class RenderServer:
     """Continuous-batching render server over resident scenes (mrt_server_*).
 
     submit() queues a render and returns a job id; wait() blocks until its
-    bitmap (and accumulation) are in the caller's buffers."""
+    bitmap (and accumulation) are in the caller's buffers.  idle_direct=True
+    lets an idle server render a lone small job through mrt_render's
+    one-launch megakernel instead of its pipelines."""
 
     def __init__(self, ctx: Context, max_slots: int = 0, max_jobs: int = 0, max_scenes: int = 0,
-                 pipelines: int = 0, job_memory: int = 0, scene_memory: int = 0, sample_memory: int = 0):
+                 pipelines: int = 0, job_memory: int = 0, scene_memory: int = 0, sample_memory: int = 0,
+                 idle_direct: bool = False):
         self.ctx = ctx
         d = abi.ServerDesc()
         d.max_slots, d.max_jobs, d.max_scenes, d.pipelines = max_slots, max_jobs, max_scenes, pipelines
         d.job_memory, d.scene_memory, d.sample_memory = job_memory, scene_memory, sample_memory
+        d.idle_direct = 1 if idle_direct else 0
         h = C.c_void_p()
         check(lib().mrt_server_create(ctx.h, C.byref(d), C.byref(h)), "mrt_server_create")
         self.h = h

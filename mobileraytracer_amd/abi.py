@@ -126,13 +126,13 @@ class Tuning(C.Structure):
 class ServerDesc(C.Structure):
     _fields_ = [("max_slots", C.c_uint32), ("max_jobs", C.c_uint32), ("max_scenes", C.c_uint32),
                 ("pipelines", C.c_uint32), ("job_memory", C.c_uint64), ("scene_memory", C.c_uint64),
-                ("sample_memory", C.c_uint64), ("reserved", C.c_uint32 * 2)]
+                ("sample_memory", C.c_uint64), ("idle_direct", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class JobInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("t_submit_us", C.c_double),
                 ("t_admit_us", C.c_double), ("t_gpu_done_us", C.c_double), ("t_done_us", C.c_double),
-                ("lanes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("lanes", C.c_uint32), ("direct", C.c_uint32)]
 
 
 class ServingStats(C.Structure):

@@ -43,7 +43,7 @@ using namespace mrt;
 namespace {
 
 constexpr uint32_t kBatch = 4;  // iterations per pipeline between counter snapshots
-constexpr uint32_t kDefaultSlots = 25165824u;
+constexpr uint32_t kDefaultSlots = 33554432u;
 constexpr uint32_t kDefaultJobs = 256u;
 constexpr uint32_t kDefaultScenes = 64u;
 constexpr uint64_t kDefaultJobMemory = 1ull << 30;
@@ -147,6 +147,8 @@ struct Job {
     uint64_t off_bm = 0, off_acc = 0;  // inside the job memory block
     std::vector<Run> runs;
     uint32_t pipes = 0;               // bit g: the job has waves on pipeline g
+    uint32_t stride = 0;              // caller's bitmap row stride (bytes)
+    bool direct = false;              // rendered alone through mrt_render by an idle server
     hipEvent_t done_ev = nullptr;
     bool cancel = false;
     double t_submit = 0, t_admit = 0, t_gpu = 0, t_done = 0;
@@ -198,6 +200,11 @@ struct mrt_server {
     uint64_t submitted = 0, done = 0, failed = 0, batches = 0;
     bool shutdown = false;
     bool broken = false;
+    // Idle-direct renders: on, the job being rendered (0: none), and the rays
+    // of those renders (camera, extension, shadow).
+    bool idle_direct = false;
+    uint64_t direct_job = 0;
+    uint64_t direct_rays[3] = {};
     std::string broken_msg;
     std::thread worker;
 };
@@ -541,6 +548,62 @@ int retire(mrt_server* s) {
     return MRT_OK;
 }
 
+// A job an idle server renders alone: the only job in the server (none
+// admitted, none queued behind it), and one that mrt_render's AUTO gives the
+// one-launch persistent megakernel (below kMegaMaxPixelSamples pixel samples),
+// which finishes a small frame sooner than the pipelines' iterations.  Jobs
+// that queue up together go to the pipelines.
+bool direct_fits(const mrt_server* s, const Job* j) {
+    if (!s->idle_direct || j->desc.persistent != 0u) return false;
+    const uint64_t px = 256ull * j->p.bsx * j->p.bsy;  // every tile (servers refuse tile_skip)
+    return px * (j->desc.samples_pixel - j->desc.first_sample) < kMegaMaxPixelSamples;
+}
+
+// Renders the queue's head alone through mrt_render, without the server's
+// lock (submissions, polls and cancels go on meanwhile; the pipelines are
+// idle).  Returns a device error that ends the server, else MRT_OK (a job
+// that failed otherwise has failed on its own).
+int run_direct(mrt_server* s, Job* j, std::unique_lock<std::mutex>& lk) {
+    s->queue.pop_front();
+    j->phase = kRunning;
+    j->direct = true;
+    j->lanes = 1u;
+    j->t_admit = now_us();
+    s->direct_job = j->id;
+    mrt_scene* scene = s->scenes[j->scene].scene;
+    mrt_render_desc d = j->desc;
+    d.kernel = MRT_KERNEL_AUTO;
+    mrt_render_stats st{};
+    lk.unlock();
+    const int rc = mrt_render(scene, &d, j->bitmap, j->stride, j->accum, j->mem, &st);
+    const std::string err = rc == MRT_OK ? std::string() : g_error;
+    lk.lock();
+    s->direct_job = 0;
+    j->t_gpu = now_us();
+    if (rc != MRT_OK) {
+        fail_job(s, j, rc, err);
+        if (rc == MRT_E_HIP) {
+            g_error = err;
+            return rc;
+        }
+        return MRT_OK;
+    }
+    s->direct_rays[0] += st.rays_camera;
+    s->direct_rays[1] += st.rays_extension;
+    s->direct_rays[2] += st.rays_shadow;
+    if (j->cancel) {
+        // Stopped while rendering: every sample is in the caller's buffers.
+        fail_job(s, j, MRT_E_STOPPED, "render stopped");
+        return MRT_OK;
+    }
+    j->phase = kDone;
+    j->status = MRT_OK;
+    j->t_done = now_us();
+    ++s->done;
+    s->done_cv.notify_all();
+    return MRT_OK;
+}
+
 void worker_main(mrt_server* s) {
     (void)hipSetDevice(s->ctx->device);
     uint32_t snap = 0;
@@ -548,6 +611,11 @@ void worker_main(mrt_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         int rc = retire(s);
+        if (rc == MRT_OK && !pending && s->running.empty() && s->queue.size() == 1u && !s->broken &&
+            direct_fits(s, s->jobs[s->queue.front()].get())) {
+            rc = run_direct(s, s->jobs[s->queue.front()].get(), lk);
+            if (rc == MRT_OK) continue;
+        }
         if (rc == MRT_OK) rc = admit(s);
         bool issued = false;
         if (rc == MRT_OK) rc = issue_batch(s, snap, &issued);
@@ -618,7 +686,7 @@ void fill_info(const Job* j, mrt_job_info* info) {
     info->t_gpu_done_us = j->t_gpu;
     info->t_done_us = j->t_done;
     info->lanes = j->lanes;
-    info->reserved = 0;
+    info->direct = j->direct ? 1u : 0u;
 }
 
 // Copies a resident scene's walk trees into the arena with rebased references
@@ -715,6 +783,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     if (slots % 64u != 0u || slots / 64u < G) return fail(MRT_E_INVALID, "max_slots must be a multiple of 64 per pipeline");
     const uint32_t maxJobs = d.max_jobs ? d.max_jobs : kDefaultJobs;
     const uint32_t maxScenes = d.max_scenes ? d.max_scenes : kDefaultScenes;
+    if (d.idle_direct > 1u) return fail(MRT_E_INVALID, "idle_direct must be 0 or 1");
     const uint64_t jobMem = align_up(d.job_memory ? d.job_memory : kDefaultJobMemory);
     const uint64_t sampleMem = align_up(d.sample_memory ? d.sample_memory : kDefaultSampleMemory);
     if (maxJobs > 65536u || maxScenes > 65536u) return fail(MRT_E_INVALID, "max_jobs and max_scenes must be <= 65536");
@@ -732,6 +801,7 @@ MRT_API int mrt_server_create(mrt_ctx* ctx, const mrt_server_desc* desc, mrt_ser
     s->max_jobs = maxJobs;
     s->max_scenes = maxScenes;
     s->pool_waves = slots / 64u;
+    s->idle_direct = d.idle_direct == 1u;
     s->job_memory = jobMem;
     s->scene_memory = d.scene_memory;
     auto bail = [&](int rc) {
@@ -879,6 +949,7 @@ MRT_API int mrt_server_submit(mrt_server* s, uint32_t scene_id, const mrt_render
         if (d->tile_skip[k] != 0u) return fail(MRT_E_UNSUPPORTED, "render servers render whole frames (tile_skip)");
     j->scene = scene_id;
     j->desc = *d;
+    j->stride = stride_bytes;
     j->bitmap = bitmap;
     j->accum = accum;
     j->mem = mem;
@@ -953,6 +1024,11 @@ MRT_API int mrt_server_cancel(mrt_server* s, uint64_t job_id) {
         fail_job(s, j, MRT_E_STOPPED, "render stopped");
         return MRT_OK;
     }
+    if (j->direct) {
+        // A one-launch render: it completes, then fails with MRT_E_STOPPED.
+        if (j->phase == kRunning) j->cancel = true;
+        return MRT_OK;
+    }
     if (j->phase == kRunning && !j->cancel) {
         j->cancel = true;
         s->jobs_host[j->slot].stop = 1u;
@@ -977,7 +1053,7 @@ MRT_API int mrt_server_stats(mrt_server* s, mrt_serving_stats* st) {
         st->jobs_done = s->done;
         st->jobs_failed = s->failed;
         st->jobs_queued = static_cast<uint32_t>(s->queue.size());
-        st->jobs_running = static_cast<uint32_t>(s->running.size());
+        st->jobs_running = static_cast<uint32_t>(s->running.size()) + (s->direct_job ? 1u : 0u);
         st->slots_total = s->pool_waves * 64u;
         uint64_t freeWaves = 0;
         for (uint32_t g = 0; g < s->G; ++g) freeWaves += s->waves_free[g].free_total();
@@ -989,10 +1065,15 @@ MRT_API int mrt_server_stats(mrt_server* s, mrt_serving_stats* st) {
         st->batches = s->batches;
     }
     // The counters are read while kernels may be running: a snapshot.
+    uint64_t direct[3];
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        std::copy(s->direct_rays, s->direct_rays + 3, direct);
+    }
     MRT_HIP(hipMemcpy(cnt, s->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-    st->rays_camera = cnt[0];
-    st->rays_extension = cnt[1];
-    st->rays_shadow = cnt[2];
+    st->rays_camera = cnt[0] + direct[0];
+    st->rays_extension = cnt[1] + direct[1];
+    st->rays_shadow = cnt[2] + direct[2];
     return MRT_OK;
 }
 

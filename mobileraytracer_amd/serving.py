@@ -241,7 +241,7 @@ def tune_server(ctx, pool: list[PoolScene], reqs: list, candidates=None) -> tupl
     result."""
     if candidates is None:
         candidates = [dict(pipelines=p, max_slots=s) for p in (1, 2, 3)
-                      for s in (2_097_152, 6_291_456, 12_582_912, 25_165_824)]
+                      for s in (2_097_152, 6_291_456, 12_582_912, 25_165_824, 33_554_432)]
     probe = []
     for si, d in reqs:
         q = abi.RenderDesc.from_buffer_copy(d)

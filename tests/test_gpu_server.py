@@ -6,7 +6,9 @@ oracle): a pixel's samples take the same draws and the same operations
 whatever else shares the launches.  Mixes cover several scenes (light tree and
 linear light scan), resolutions, sample counts and progressive resumes, jobs
 admitted together and while others run, and a slot pool small enough that
-jobs wait for slots to free up.
+jobs wait for slots to free up; these run every job on the pipelines.
+The idle-direct option (a lone small job rendered through mrt_render by an
+idle server) is checked the same way.
 """
 from __future__ import annotations
 
@@ -73,7 +75,8 @@ MIX = [(0, 128, 80, 4, 5), (1, 256, 144, 2, 9), (2, 64, 64, 8, 13), (0, 64, 48, 
 def test_server_mix_equals_render_alone(gpu_ctx, scenes, pipelines, max_slots):
     """All jobs submitted at once; with the small pools most of them wait
     for waves that finished jobs free up."""
-    srv = mrt.RenderServer(gpu_ctx, max_slots=max_slots, pipelines=pipelines, job_memory=64 << 20)
+    srv = mrt.RenderServer(gpu_ctx, max_slots=max_slots, pipelines=pipelines, job_memory=64 << 20,
+                           idle_direct=False)
     try:
         ids = [srv.add_scene(s) for s in scenes]
         jobs = []
@@ -110,7 +113,7 @@ def test_server_staggered_and_progressive(gpu_ctx, scenes):
     """Jobs arriving while others run, and a frame rendered as two
     progressive requests (samples 0-1, then 2-4 on the same buffers), equal
     the frame rendered alone in one request."""
-    srv = mrt.RenderServer(gpu_ctx, max_slots=64 * 3 * 300, job_memory=64 << 20)
+    srv = mrt.RenderServer(gpu_ctx, max_slots=64 * 3 * 300, job_memory=64 << 20, idle_direct=False)
     try:
         ids = [srv.add_scene(s) for s in scenes]
         pending = []
@@ -139,7 +142,7 @@ def test_server_staggered_and_progressive(gpu_ctx, scenes):
 
 def test_server_device_buffers(gpu_ctx, scenes):
     """Jobs writing into caller device memory give the host-memory result."""
-    srv = mrt.RenderServer(gpu_ctx, job_memory=16 << 20)
+    srv = mrt.RenderServer(gpu_ctx, job_memory=16 << 20, idle_direct=False)
     dbm = dacc = None
     try:
         sid = srv.add_scene(scenes[1])
@@ -164,7 +167,7 @@ def test_server_device_buffers(gpu_ctx, scenes):
 
 
 def test_server_cancel_and_errors(gpu_ctx, scenes):
-    srv = mrt.RenderServer(gpu_ctx, max_slots=64 * 3 * 100, job_memory=64 << 20)
+    srv = mrt.RenderServer(gpu_ctx, max_slots=64 * 3 * 100, job_memory=64 << 20, idle_direct=False)
     try:
         sid = srv.add_scene(scenes[0])
         # Unknown scene, a Whitted request, a frame larger than the pool.
@@ -201,3 +204,73 @@ def test_server_cancel_and_errors(gpu_ctx, scenes):
         assert st.jobs_failed == 2 and st.jobs_running == 0 and st.jobs_queued == 0
     finally:
         srv.close()
+
+
+def test_server_idle_direct(gpu_ctx, scenes):
+    """With idle_direct on: a lone small job at an idle server
+    is rendered alone through mrt_render (info.direct), host and device
+    buffers; a job at or above 6 x 2^20 pixel samples, and jobs queued
+    together, run on the pipelines; every result and the server's ray counts
+    equal mrt_render alone; a cancel during a direct render completes it and
+    fails it with MRT_E_STOPPED."""
+    srv = mrt.RenderServer(gpu_ctx, job_memory=256 << 20, idle_direct=True)
+    dbm = None
+    try:
+        ids = [srv.add_scene(s) for s in scenes]
+        rays = np.zeros(3, np.int64)
+        # Lone jobs, one at a time: direct.
+        for k, (si, w, h, spp, seed) in enumerate(MIX[:4]):
+            desc = make_desc(si, w, h, spp, seed + 200)
+            bm, acc, info = srv.wait(srv.submit(ids[si], desc, accum=np.zeros((h, w, 3), np.float32)))
+            assert info.state == abi.JOB_DONE and info.direct == 1 and info.lanes == 1
+            assert info.t_submit_us <= info.t_admit_us <= info.t_gpu_done_us <= info.t_done_us
+            wb, wa, st = alone(scenes[si], desc)
+            rays += (st.rays_camera, st.rays_extension, st.rays_shadow)
+            check_same(f"direct job {k}", bm, acc, wb, wa)
+        # Device buffers, direct.
+        desc = make_desc(1, 256, 144, 3, 251)
+        host_bm = np.zeros((144, 256), np.uint32)
+        dbm = DeviceArray(host_bm.nbytes)
+        dbm.upload(host_bm)
+        info = srv.wait(srv.submit(ids[1], desc, bitmap=dbm, mem=abi.MEM_DEVICE))[2]
+        assert info.direct == 1
+        wb, _, st = alone(scenes[1], desc)
+        rays += (st.rays_camera, st.rays_extension, st.rays_shadow)
+        assert np.array_equal(dbm.download(host_bm), wb)
+        # 2048x1024 at 3 samples is 6.3M pixel samples: the pipelines.
+        big = make_desc(0, 2048, 1024, 3, 261)
+        bm, acc, info = srv.wait(srv.submit(ids[0], big, accum=np.zeros((1024, 2048, 3), np.float32)))
+        assert info.direct == 0
+        wb, wa, st = alone(scenes[0], big)
+        rays += (st.rays_camera, st.rays_extension, st.rays_shadow)
+        check_same("above the megakernel size", bm, acc, wb, wa)
+        # A burst: the jobs that queue together go to the pipelines.
+        jobs = []
+        for (si, w, h, spp, seed) in MIX:
+            desc = make_desc(si, w, h, spp, seed + 300)
+            jobs.append((si, desc, srv.submit(ids[si], desc, accum=np.zeros((h, w, 3), np.float32))))
+        direct = []
+        for k, (si, desc, jid) in enumerate(jobs):
+            bm, acc, info = srv.wait(jid)
+            direct.append(info.direct)
+            wb, wa, st = alone(scenes[si], desc)
+            rays += (st.rays_camera, st.rays_extension, st.rays_shadow)
+            check_same(f"burst job {k}", bm, acc, wb, wa)
+        print(f"burst: direct flags {direct}", flush=True)
+        assert sum(direct) < len(direct)
+        stt = srv.stats()
+        assert (stt.rays_camera, stt.rays_extension, stt.rays_shadow) == tuple(int(r) for r in rays)
+        # Cancel while a direct render runs: it completes, then fails.
+        desc = make_desc(0, 256, 144, 160, 271)  # 5.9M pixel samples
+        jid = srv.submit(ids[0], desc, accum=np.zeros((144, 256, 3), np.float32))
+        time.sleep(0.002)
+        srv.cancel(jid)
+        with pytest.raises(abi.MrtError) as e:
+            srv.wait(jid, timeout_us=60_000_000)
+        assert e.value.code == abi.MRT_E_STOPPED
+        st = srv.stats()
+        assert st.jobs_running == 0 and st.jobs_queued == 0 and st.slots_used == 0
+    finally:
+        srv.close()
+        if dbm is not None:
+            dbm.free()
