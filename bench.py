@@ -17,10 +17,13 @@ holding a resident pool of 16 seeded scenes (50k-8M triangles, soups and
 displaced-icosphere clusters, 1% emissive) receives a Poisson stream of
 requests (random camera pose, 256x256 to 1920x1080, 1-64 spp) for a fixed
 window; p50/p99 request latency and the window's Mrays/s go into the line's
-"serving" object, beside ("plain") the same stream on the plain GPU path:
-each request rendered alone with mrt_render, and the latencies of serving
-them one at a time in arrival order with those times.  --serve-sweep runs only the serving workload over a list of
-rates and prints one JSON line per rate.
+"serving" object.  --serve-plain adds ("plain") the same stream on the plain
+GPU path: each request rendered alone with mrt_render, and the latencies of
+serving them one at a time in arrival order with those times (off by
+default: those renders run the trace kernel of the timed frames at other
+sizes, and the profile of the default command then mixes them into that
+kernel's average).  --serve-sweep runs only the serving workload over a
+list of rates and prints one JSON line per rate.
 
 Denoiser leg (after the timed frames, outside their timing): the BF16 U-Net
 (mrt_denoiser_run, trained weights) over a 1920x1080 frame of FP32 radiance
@@ -245,6 +248,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=5.0, help="serving leg: requests per second")
     ap.add_argument("--serve-duration", type=float, default=20.0, help="serving leg: arrival window, seconds")
+    ap.add_argument("--serve-plain", action="store_true",
+                    help="serving leg: also the same stream on the plain path (each request alone, FIFO)")
     ap.add_argument("--serve-scenes", type=int, default=16)
     ap.add_argument("--serve-min-tris", type=int, default=50_000)
     ap.add_argument("--serve-max-tris", type=int, default=8_000_000)
@@ -305,7 +310,7 @@ def main() -> int:
             # Bounded request count at high rates (the host buffers of queued
             # requests are held until they are served).
             dur = min(args.serve_duration, 600.0 / rate)
-            res = serving_run(mrt, srv, pool, rate, dur, plain=False)
+            res = serving_run(mrt, srv, pool, rate, dur)
             print(json.dumps({"metric": "render-request latency (serving sweep)", "serving": res}), flush=True)
         srv.close()
         return 0
@@ -505,7 +510,7 @@ def main() -> int:
         scene.close()
         del bitmap, accum
         srv, pool = serving_setup(mrt, ctx, args)
-        out["serving"] = serving_run(mrt, srv, pool, args.serve_rate, args.serve_duration)
+        out["serving"] = serving_run(mrt, srv, pool, args.serve_rate, args.serve_duration, plain=args.serve_plain)
         srv.close()
     print(json.dumps(out), flush=True)
     return 0
@@ -735,7 +740,7 @@ def serving_setup(mrt, ctx, args):
     return srv, pool
 
 
-def serving_run(mrt, srv, pool, rate, duration, plain=True):
+def serving_run(mrt, srv, pool, rate, duration, plain=False):
     from mobileraytracer_amd import serving
 
     t0 = time.time()
