@@ -416,7 +416,10 @@ MRT_API int mrt_denoiser_trainer_create(mrt_ctx* ctx, const mrt_denoiser_desc* n
     std::lock_guard<std::mutex> lock(ctx->mu);
     MRT_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
+    // The trainer holds one of the context's children from here on: its
+    // destructor gives it back, also when creation fails below.
     tr->ctx = ctx;
+    ++ctx->children;
     for (hipEvent_t& e : tr->ev) MRT_HIP(hipEventCreate(&e));
     const size_t np = p.size() * sizeof(float);
     if ((rc = tr->params.upload(p.data(), np, st)) != MRT_OK) return rc;
@@ -429,7 +432,6 @@ MRT_API int mrt_denoiser_trainer_create(mrt_ctx* ctx, const mrt_denoiser_desc* n
     MRT_HIP(hipMemsetAsync(tr->adam_v.p, 0, np, st));
     if ((rc = adam_update(tr.get(), 0.0, false, st)) != MRT_OK) return rc;
     MRT_HIP(hipStreamSynchronize(st));
-    ++ctx->children;
     *out = tr.release();
     return MRT_OK;
 }
