@@ -196,19 +196,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
         if (e_in)
             sv[kRows] = ld16(xb + static_cast<size_t>(up ? ye_c >> 1 : ye_c) * row_el, up ? oue : oe);
     };
-    // Weights of a chunk.  NT = 64: LDS, row tap * 64 + n of the chunk from
-    // thread quads (tap = round, n = t >> 2); NT = 32: registers, lane (li,
-    // lg) holding cin 8 lg .. + 7 of output channel n0 + 16 j + li per tap.
+    // Weights of a chunk, each 16-channel block of a tap one contiguous 1 KB
+    // of the fragment-order copy.  NT = 64: LDS, wave w staging block w of
+    // every tap (lane (li, lg) into row tap * 64 + 16 w + li, quad lg);
+    // NT = 32: registers, lane (li, lg) holding cin 8 lg .. + 7 of output
+    // channel n0 + 16 j + li per tap.
     const uint32_t cin = a.cin, cout = a.cout;
+    const uint16_t* wfb = a.wf + static_cast<size_t>(n0 >> 4) * (cin >> 5) * 512u;
+    const size_t wf_tap = static_cast<size_t>(cout >> 4) * (cin >> 5) * 512u, wf_blk = (cin >> 5) * 512u;
     constexpr uint32_t kWIters = kWeightsLds ? 9u : 1u;
     uint4 wq[kWIters];
-    const uint32_t ow = 2u * (cm * cin + 8u * q);
     auto load_weights_lds = [&](uint32_t kc) {
 #pragma unroll
         for (uint32_t tap = 0; tap < kWIters; ++tap)
-            wq[tap] = ld16(a.w + kc + static_cast<size_t>(tap * cout + n0) * cin, ow);
+            wq[tap] = ld16(wfb + tap * wf_tap + wv * wf_blk + (kc >> 5) * 512u, 16u * lane);
     };
-    const uint32_t owr = 2u * (li * cin + 8u * lg);
     if constexpr (MODE != kModePackedIn) load_input(0);
     if constexpr (kWeightsLds) load_weights_lds(0);
     for (uint32_t kc = 0; kc < cin; kc += 32u) {
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
             for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j)
-                    wr[tap][j] = ld16(a.w + kc + static_cast<size_t>(tap * cout + n0 + 16u * j) * cin, owr);
+                    wr[tap][j] = ld16(wfb + tap * wf_tap + j * wf_blk + (kc >> 5) * 512u, 16u * lane);
         }
         __syncthreads();
         if constexpr (MODE == kModePackedIn) {
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
         }
         if constexpr (kWeightsLds) {
 #pragma unroll
-            for (uint32_t tap = 0; tap < kWIters; ++tap) wts[(tap * NT + cm) * kRowQuads + q] = wq[tap];
+            for (uint32_t tap = 0; tap < kWIters; ++tap) wts[(tap * NT + 16u * wv + li) * kRowQuads + lg] = wq[tap];
         }
         __syncthreads();
         if (kc + 32u < cin) {
@@ -318,6 +320,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
     }
 }
 
+// Canonical [9][cout][cin] weights into fragment order, one element per
+// thread.
+__global__ void k_frag_weights(const uint16_t* __restrict__ w, uint32_t cin, uint32_t cout, uint16_t* __restrict__ wf) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t per_tap = static_cast<size_t>(cout) * cin;
+    if (i >= 9u * per_tap) return;
+    const uint32_t tap = static_cast<uint32_t>(i / per_tap);
+    const uint32_t r = static_cast<uint32_t>(i % per_tap);
+    wf[frag_index(tap, r / cin, r % cin, cin, cout)] = w[i];
+}
+
 // 2x2 max pooling, 8 channels (16 bytes) per thread.
 __global__ void k_pool2(const uint16_t* __restrict__ x, uint32_t c, uint32_t H, uint32_t W, uint16_t* __restrict__ y) {
     const uint32_t W2 = W / 2u, H2 = H / 2u, cq = c / 8u;
@@ -362,7 +375,7 @@ hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t st
     if (a.cu && (a.H % 2u || a.W % 2u || !a.xu)) return hipErrorInvalidValue;
     // 16-byte loads of inputs and weights, 8-byte stores of outputs.
     const auto misaligned = [](const void* p, uintptr_t n) { return (reinterpret_cast<uintptr_t>(p) & (n - 1)) != 0; };
-    if (misaligned(a.x, 16) || misaligned(a.xu, 16) || misaligned(a.w, 16) || misaligned(a.y, 8) ||
+    if (!a.wf || misaligned(a.x, 16) || misaligned(a.xu, 16) || misaligned(a.wf, 16) || misaligned(a.y, 8) ||
         misaligned(a.bias, 4))
         return hipErrorInvalidValue;
     if (a.rad_out ? (a.fw > a.W || a.fh > a.H)
@@ -384,6 +397,13 @@ hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t st
         conv_launch<64, kModeTensor>(a, grid, stream);
     else
         conv_launch<32, kModeTensor>(a, grid, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_frag_weights(const uint16_t* w, uint32_t cin, uint32_t cout, uint16_t* wf, hipStream_t stream) {
+    if (!w || !wf || cin == 0 || cout == 0 || cin % 32u || cout % 32u) return hipErrorInvalidValue;
+    const size_t n = 9ull * cout * cin;
+    k_frag_weights<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, stream>>>(w, cin, cout, wf);
     return hipGetLastError();
 }
 

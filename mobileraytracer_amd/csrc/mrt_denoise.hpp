@@ -18,8 +18,12 @@ namespace mrt {
 //     frame of fw x fh pixels at the top left of the H x W frame, zero
 //     elsewhere: log(1 + max(rad * rad_scale, 0)) (3 channels), aov[0..5],
 //     aov[6] * depth_scale, then 22 zero channels.
-// Weights w[tap][cout][cin] BF16 (tap = 3 * dy + dx for offsets dy - 1,
-// dx - 1); FP32 bias (or null); ReLU when relu is set.
+// Weights W[tap][cout][cin] BF16 (tap = 3 * dy + dx for offsets dy - 1,
+// dx - 1) in fragment order, wf (frag_index; launch_frag_weights reorders
+// [9][cout][cin] weights): each 16-output-channel x 32-input-channel block of
+// a tap is one contiguous 1 KB, which a wave loads in one coalesced access
+// (rows cin apart, the loads took up to 45 % longer on the coarse layers);
+// FP32 bias (or null); ReLU when relu is set.
 //
 // Output (cout a multiple of 32):
 //   * BF16 channels [y_off, y_off + cout) of y (y_stride per pixel);
@@ -35,6 +39,16 @@ namespace mrt {
 // Masked output (training, the gradient through a ReLU): when mask is set,
 // output channel k of a pixel is zeroed unless channel mask_off + k of the
 // same pixel of mask (mask_stride channels per pixel, BF16) is above zero.
+// Position of weight (tap, co, ci) of a cin x cout layer in fragment order:
+// tap, then 16-channel output block, then 32-channel input chunk, each block
+// 64 lanes x 8 channels, lane 16 lg + li holding output channel 16 n + li and
+// input channels 32 k + 8 lg .. + 7 (the B operand of
+// v_mfma_f32_16x16x32_bf16).
+__host__ __device__ inline size_t frag_index(uint32_t tap, uint32_t co, uint32_t ci, uint32_t cin, uint32_t cout) {
+    return ((static_cast<size_t>(tap) * (cout >> 4) + (co >> 4)) * (cin >> 5) + (ci >> 5)) * 512u +
+           (((ci >> 3) & 3u) * 16u + (co & 15u)) * 8u + (ci & 7u);
+}
+
 struct ConvArgs {
     const uint16_t* x;
     uint32_t x_stride, x_off, cin;
@@ -43,7 +57,7 @@ struct ConvArgs {
     const float* rad;
     const float* aov;
     float rad_scale, depth_scale;
-    const uint16_t* w;
+    const uint16_t* wf;  // weights in fragment order
     const float* bias;
     uint32_t cout;
     uint16_t* y;
@@ -60,6 +74,8 @@ struct ConvArgs {
 // min_blocks: the channel block per workgroup (32 or 64 output channels) is
 // the widest that still gives this many workgroups.
 hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t stream);
+// w ([9][cout][cin] BF16) into fragment order (frag_index) in wf.
+hipError_t launch_frag_weights(const uint16_t* w, uint32_t cin, uint32_t cout, uint16_t* wf, hipStream_t stream);
 // 2x2 max pooling: x (H x W, c channels per pixel, H and W even, c a
 // multiple of 8) into y (H/2 x W/2, c channels per pixel).  A batch of frames
 // of even height pools as one frame of the summed height.
@@ -156,7 +172,8 @@ hipError_t launch_gather(const float* xin, const float* tin, uint32_t fw, uint32
 // p += -step_size (m / (sqrt(v) / bc2_sqrt + eps)), omb = 1 - beta and the
 // others rounded to FP32 from double as torch rounds them), then the BF16 forward
 // weights wq and the transposed, tap-flipped weights wt of the
-// backward-data convolutions ([8 - tap][cin][cout] per layer).
+// backward-data convolutions ([8 - tap][cin][cout] per layer), both in
+// fragment order (wqf, wtf).
 struct AdamLayer {
     uint32_t w_off, cin, cout;
 };
@@ -166,11 +183,11 @@ struct AdamArgs {
     const float* g;
     float* m;
     float* v;
-    uint16_t* wq;
-    uint16_t* wt;
+    uint16_t* wqf;
+    uint16_t* wtf;
     size_t n, n_w;
     float omb1, b2, omb2, eps, step_size, bc2_sqrt;
-    uint32_t update;  // 0: only write wq and wt from p
+    uint32_t update;  // 0: only write wqf and wtf from p
     uint32_t n_layers;
     AdamLayer layer[kMaxLayers];
 };

@@ -60,7 +60,7 @@ struct mrt_denoiser {
     mrt_ctx* ctx = nullptr;
     Net net;
     std::vector<DevLayer> layers;
-    DevBuf weights, biases, act, stage_rad, stage_aov, stage_out;
+    DevBuf weights_f, biases, act, stage_rad, stage_aov, stage_out;  // weights_f: fragment order
     hipEvent_t ev[2] = {};
     float last_ms = 0.0f;
     ~mrt_denoiser() {
@@ -83,7 +83,7 @@ mrt::ConvArgs layer_args(const mrt_denoiser* dn, size_t layer, const uint16_t* x
     a.cin = L.cin;
     a.xu = xu;
     a.cu = cu;
-    a.w = dn->weights.as<uint16_t>() + L.w_off;
+    a.wf = dn->weights_f.as<uint16_t>() + L.w_off;
     a.bias = dn->biases.as<float>() + L.b_off;
     a.cout = L.cout;
     a.y = y;
@@ -172,15 +172,23 @@ MRT_API int mrt_denoiser_create(mrt_ctx* ctx, const mrt_denoiser_desc* desc, con
     std::vector<DevLayer> layers = mrt::unet::device_layers(n, &n_w, &n_b);
     std::vector<float> wf(n_w, 0.0f), bdev(n_b, 0.0f);
     mrt::unet::to_device_layout(layers, weights, wf.data(), bdev.data());
+    // BF16 weights in fragment order, per layer.
     std::vector<uint16_t> wdev(n_w);
-    for (size_t i = 0; i < n_w; ++i) wdev[i] = mrt::unet::bf16_rne(wf[i]);
+    for (const DevLayer& L : layers) {
+        const size_t per_tap = static_cast<size_t>(L.cout) * L.cin;
+        for (size_t i = 0; i < 9 * per_tap; ++i) {
+            const uint32_t tap = static_cast<uint32_t>(i / per_tap), r = static_cast<uint32_t>(i % per_tap);
+            wdev[L.w_off + mrt::frag_index(tap, r / L.cin, r % L.cin, L.cin, L.cout)] =
+                mrt::unet::bf16_rne(wf[L.w_off + i]);
+        }
+    }
     std::lock_guard<std::mutex> lock(ctx->mu);
     MRT_HIP(hipSetDevice(ctx->device));
     auto dn = std::make_unique<mrt_denoiser>();
     dn->net = n;
     dn->layers = std::move(layers);
     for (hipEvent_t& e : dn->ev) MRT_HIP(hipEventCreate(&e));
-    if ((rc = dn->weights.upload(wdev.data(), wdev.size() * 2, ctx->stream)) != MRT_OK) return rc;
+    if ((rc = dn->weights_f.upload(wdev.data(), wdev.size() * 2, ctx->stream)) != MRT_OK) return rc;
     if ((rc = dn->biases.upload(bdev.data(), bdev.size() * 4, ctx->stream)) != MRT_OK) return rc;
     MRT_HIP(hipStreamSynchronize(ctx->stream));
     dn->ctx = ctx;
@@ -250,7 +258,7 @@ MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, u
     MRT_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
     const size_t px = static_cast<size_t>(width) * height;
-    DevBuf bx, bw, bb, by;
+    DevBuf bx, bw, bb, by, bwf;
     const uint16_t *dx = x, *dw = w;
     const float* db = bias;
     uint16_t* dy = y;
@@ -265,11 +273,13 @@ MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, u
         db = bias ? bb.as<float>() : nullptr;
         dy = by.as<uint16_t>();
     }
+    if ((rc = bwf.alloc(static_cast<size_t>(9) * cout * cin * 2)) != MRT_OK) return rc;
+    MRT_HIP(mrt::launch_frag_weights(dw, cin, cout, bwf.as<uint16_t>(), st));
     mrt::ConvArgs a{};
     a.x = dx;
     a.x_stride = cin;
     a.cin = cin;
-    a.w = dw;
+    a.wf = bwf.as<uint16_t>();
     a.bias = db;
     a.cout = cout;
     a.y = dy;

@@ -88,8 +88,10 @@ struct DevLayer {
     size_t w_off, b_off;  // elements into the weight / bias buffers
 };
 
-// Device layout of every layer: BF16 weights [tap][cout][cin] (padded
-// channels zero) one layer after the other, FP32 biases [cout] likewise.
+// Layout of every layer's parameters: weights [tap][cout][cin] (padded
+// channels zero) one layer after the other (the convolutions read them in
+// fragment order, mrt::frag_index, from the same offsets), FP32 biases
+// [cout] likewise.
 inline std::vector<DevLayer> device_layers(const Net& n, size_t* n_w, size_t* n_b) {
     std::vector<DevLayer> v;
     size_t w = 0, b = 0;

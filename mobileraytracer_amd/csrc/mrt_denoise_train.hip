@@ -424,7 +424,6 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
     }
     if (i >= a.n_w) return;
     const uint16_t q = static_cast<uint16_t>(pack2(p, 0.0f) & 0xFFFFu);
-    a.wq[i] = q;
     uint32_t k = 0;
     while (k + 1 < a.n_layers && i >= a.layer[k + 1].w_off) ++k;
     const AdamLayer L = a.layer[k];
@@ -432,7 +431,8 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
     const uint32_t per_tap = L.cout * L.cin;
     const uint32_t tap = local / per_tap, r = local % per_tap;
     const uint32_t co = r / L.cin, ci = r % L.cin;
-    a.wt[L.w_off + (static_cast<size_t>(8u - tap) * L.cin + ci) * L.cout + co] = q;
+    a.wqf[L.w_off + frag_index(tap, co, ci, L.cin, L.cout)] = q;
+    a.wtf[L.w_off + frag_index(8u - tap, ci, co, L.cout, L.cin)] = q;
 }
 
 uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + 255) / 256); }

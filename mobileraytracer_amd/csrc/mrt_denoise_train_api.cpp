@@ -56,7 +56,7 @@ struct mrt_denoiser_trainer {
     size_t n_w = 0, n_b = 0;
     uint32_t target_groups = 0;
     TrainArena ar;
-    DevBuf params, grads, adam_m, adam_v, wq, wt, arena, data_x, data_t, stage_x, stage_t;
+    DevBuf params, grads, adam_m, adam_v, wqf, wtf, arena, data_x, data_t, stage_x, stage_t;
     uint32_t n_pairs = 0, fw = 0, fh = 0;
     uint64_t steps = 0;
     double last_loss = 0.0;
@@ -166,8 +166,8 @@ int adam_update(mrt_denoiser_trainer* tr, double lr, bool update, hipStream_t st
     a.g = tr->grads.as<float>();
     a.m = tr->adam_m.as<float>();
     a.v = tr->adam_v.as<float>();
-    a.wq = tr->wq.as<uint16_t>();
-    a.wt = tr->wt.as<uint16_t>();
+    a.wqf = tr->wqf.as<uint16_t>();
+    a.wtf = tr->wtf.as<uint16_t>();
     a.n = tr->n_w + tr->n_b;
     a.n_w = tr->n_w;
     a.omb1 = static_cast<float>(1.0 - tr->b1);
@@ -200,7 +200,7 @@ mrt::ConvArgs conv_args(const mrt_denoiser_trainer* tr, size_t k, const mrt::Ten
     a.cin = L.cin;
     a.xu = in.xu;
     a.cu = in.cu;
-    a.w = tr->wq.as<uint16_t>() + L.w_off;
+    a.wf = tr->wqf.as<uint16_t>() + L.w_off;
     a.bias = tr->params.as<float>() + tr->n_w + L.b_off;
     a.cout = L.cout;
     a.y = y;
@@ -267,7 +267,7 @@ int layer_dgrad(mrt_denoiser_trainer* tr, size_t k, const LayerIo& io, const uin
     a.x = dz;
     a.x_stride = d.cout;
     a.cin = d.cout;
-    a.w = tr->wt.as<uint16_t>() + d.w_off;
+    a.wf = tr->wtf.as<uint16_t>() + d.w_off;
     a.cout = d.cin;
     a.y = out;
     a.y_stride = d.cin;
@@ -424,8 +424,8 @@ MRT_API int mrt_denoiser_trainer_create(mrt_ctx* ctx, const mrt_denoiser_desc* n
     const size_t np = p.size() * sizeof(float);
     if ((rc = tr->params.upload(p.data(), np, st)) != MRT_OK) return rc;
     if ((rc = tr->grads.alloc(np)) != MRT_OK || (rc = tr->adam_m.alloc(np)) != MRT_OK ||
-        (rc = tr->adam_v.alloc(np)) != MRT_OK || (rc = tr->wq.alloc(tr->n_w * 2)) != MRT_OK ||
-        (rc = tr->wt.alloc(tr->n_w * 2)) != MRT_OK || (rc = tr->arena.alloc(tr->ar.total)) != MRT_OK)
+        (rc = tr->adam_v.alloc(np)) != MRT_OK || (rc = tr->wqf.alloc(tr->n_w * 2)) != MRT_OK ||
+        (rc = tr->wtf.alloc(tr->n_w * 2)) != MRT_OK || (rc = tr->arena.alloc(tr->ar.total)) != MRT_OK)
         return rc;
     MRT_HIP(hipMemsetAsync(tr->grads.p, 0, np, st));
     MRT_HIP(hipMemsetAsync(tr->adam_m.p, 0, np, st));
