@@ -111,15 +111,15 @@ hipError_t launch_octet(const TensorIn& in, uint32_t batch, uint32_t H, uint32_t
 // accumulation on MFMA.  part_w: [splits][9][cout][cin], part_b:
 // [splits][cout]; `splits` is chosen here (at most max_splits) and returned.
 // K steps of 4 segments per split and the number of splits launch_wgrad
-// uses: about target_groups workgroups (of 4 waves), at least 32 K steps
-// (1024 pixels) per wave, at most max_splits splits.
+// uses: about target_groups workgroups (of 4 waves), at least 8 K steps
+// (256 pixels) per wave, at most max_splits splits.
 inline uint32_t wgrad_splits(uint32_t batch, uint32_t H, uint32_t W, uint32_t cin, uint32_t cout, uint32_t max_splits,
                              uint32_t target_groups, uint32_t* kps) {
     const uint32_t nseg = batch * H * (W / 8u), ksteps = (nseg + 3u) / 4u;
     const uint32_t per_split = 3u * (cin / 32u) * (cout / 32u);
     uint32_t s = (target_groups + per_split - 1) / per_split;
     if (s > max_splits) s = max_splits;
-    if (s > (ksteps + 127u) / 128u) s = (ksteps + 127u) / 128u;
+    if (s > (ksteps + 31u) / 32u) s = (ksteps + 31u) / 32u;
     if (s < 1u) s = 1u;
     const uint32_t k = (ksteps + s - 1) / s;
     if (kps) *kps = k;
