@@ -280,7 +280,7 @@ MRT_API int mrt_ctx_synchronize(mrt_ctx* ctx);
  * speed only, never a pixel).  Zero fields keep the defaults. */
 typedef struct mrt_tuning {
     uint32_t refill_lanes;     /* persistent trace: a wave takes new rays once this
-                                  many of its lanes are idle (1-64; 16, the
+                                  many of its lanes are idle (1-64; 12, the
                                   persistent megakernel 32)                    */
     uint32_t fetch_chunk;      /* queue entries a wave reserves per atomic
                                   (64-4096; 64)                                */

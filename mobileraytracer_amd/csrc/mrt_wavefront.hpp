@@ -81,8 +81,10 @@ inline bool wavefront_overlap(uint64_t rays, uint32_t blocks, uint32_t mode) {
     return rays <= static_cast<uint64_t>(blocks) * 256u;
 }
 
-// Defaults of the launch-shape knobs (mrt_tuning).
-constexpr uint32_t kDefaultRefillLanes = 16;
+// Defaults of the launch-shape knobs (mrt_tuning).  Refill at 12 idle lanes:
+// 1M soup, 1080p x 16 spp, 64-entry fetch runs, 8 / 10 / 12 / 14 / 16 / 24:
+// 102.1 / 102.0 / 101.7 / 102.5 / 102.6 / 105.8 ms per frame.
+constexpr uint32_t kDefaultRefillLanes = 12;
 // 64 entries: one per lane of the wave's first refill, so a short queue is
 // spread over as many waves as it has rays / 64 (256x256 x 1-sample frame
 // 7.9 -> 6.7 ms against 128) and the 1080p x 16 frame is no slower (128 / 96 /

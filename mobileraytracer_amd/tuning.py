@@ -56,7 +56,7 @@ SEARCH_SPACE: dict[str, dict[str, tuple[int, ...]]] = {
         "max_slot_lanes": (8, 4, 16),
         "max_groups": (3, 1, 2),
         "launch_share_pct": (0, 30, 60, 100),
-        "refill_lanes": (16, 8, 32),
+        "refill_lanes": (12, 8, 16, 32),
         "fetch_chunk": (64, 128, 256),
         "overlap_loads": (0, 1, 2),
         "graph_loop": (0, 1),
