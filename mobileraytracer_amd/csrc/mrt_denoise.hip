@@ -260,6 +260,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 32 ? 
             }
         }
     }
+    if constexpr (MODE == kModeTensor) {
+        if (a.pool) {
+            // The 2x2 max pooling of this (ReLU) output into a.pool, as
+            // k_pool2 would compute it from the stored tensor (the maximum
+            // commutes with the BF16 rounding): pixel pairs from lanes li and
+            // li ^ 1, row pairs from waves 2k + 1 through the LDS tile buffer.
+            static_assert(NJ * 4 * 128 * sizeof(floatx4) <= sizeof(smem), "pooling exchange fits the LDS");
+            floatx4 hp[NJ][4];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const float r = a.relu ? fmaxf(acc[j][m][v], 0.0f) : acc[j][m][v];
+                        hp[j][m][v] = fmaxf(r, __shfl_xor(r, 1));
+                    }
+            floatx4* xch = reinterpret_cast<floatx4*>(smem);
+            const uint32_t slot = (wv >> 1) * 64u + lane;
+            __syncthreads();  // every wave has finished reading the tile
+            if (wv & 1u) {
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) xch[(j * 4 + m) * 128 + slot] = hp[j][m];
+            }
+            __syncthreads();
+            const int yr = y0 + static_cast<int>(wv);  // H is even: row yr + 1 exists when yr does
+            if (!(wv & 1u) && yr < H && !(li & 1u)) {
+                uint16_t* pb = a.pool + (img * (plane / 4u) + static_cast<size_t>(yr >> 1) * (a.W >> 1)) * a.cout + n0;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t x = static_cast<uint32_t>(x0) + 16u * m + li;
+                    if (x >= a.W) continue;
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) {
+                        const floatx4 o = xch[(j * 4 + m) * 128 + slot];
+                        *reinterpret_cast<uint2*>(pb + (x >> 1) * a.cout + 16u * j + 4u * lg) =
+                            make_uint2(pack2(fmaxf(hp[j][m][0], o[0]), fmaxf(hp[j][m][1], o[1])),
+                                       pack2(fmaxf(hp[j][m][2], o[2]), fmaxf(hp[j][m][3], o[3])));
+                    }
+                }
+            }
+        }
+    }
     const int y = y0 + static_cast<int>(wv);
     if (y >= H) return;
     if constexpr (MODE == kModeRadOut) {
@@ -373,6 +418,9 @@ hipError_t launch_conv3x3(const ConvArgs& a, uint32_t min_blocks, hipStream_t st
               : (a.cin > a.cu && (!a.x || a.x_off + (a.cin - a.cu) > a.x_stride)))
         return hipErrorInvalidValue;
     if (a.cu && (a.H % 2u || a.W % 2u || !a.xu)) return hipErrorInvalidValue;
+    if (a.pool && (a.rad || a.rad_out || a.mask || a.H % 2u || a.W % 2u ||
+                   (reinterpret_cast<uintptr_t>(a.pool) & 7u)))
+        return hipErrorInvalidValue;
     // 16-byte loads of inputs and weights, 8-byte stores of outputs.
     const auto misaligned = [](const void* p, uintptr_t n) { return (reinterpret_cast<uintptr_t>(p) & (n - 1)) != 0; };
     if (!a.wf || misaligned(a.x, 16) || misaligned(a.xu, 16) || misaligned(a.wf, 16) || misaligned(a.y, 8) ||

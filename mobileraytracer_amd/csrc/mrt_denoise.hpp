@@ -67,6 +67,10 @@ struct ConvArgs {
     uint32_t H, W;
     uint32_t relu;
     uint32_t batch;  // frames (0 or 1: one)
+    // Tensor outputs only: also the 2x2 max pooling of the output (H/2 x W/2
+    // pixels of cout channels per frame; H and W even), what launch_pool2
+    // would write from y.
+    uint16_t* pool;
     const uint16_t* mask;
     uint32_t mask_stride, mask_off;
 };

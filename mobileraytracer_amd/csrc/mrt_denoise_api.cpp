@@ -121,9 +121,9 @@ int run_net(mrt_denoiser* dn, const float* rad, float scale, const float* aov, u
         }
         MRT_HIP(mrt::launch_conv3x3(c, min_blocks, st));
         uint16_t* e = l + 1 < L ? bf(a.s[l]) : bf(a.e_last);
-        MRT_HIP(mrt::launch_conv3x3(layer_args(dn, layer++, bf(a.t[l]), n.c[l], nullptr, 0, e, Hl, Wl), min_blocks,
-                                    st));
-        if (l + 1 < L) MRT_HIP(mrt::launch_pool2(e, n.c[l], Hl, Wl, bf(a.p[l + 1]), st));
+        mrt::ConvArgs cb = layer_args(dn, layer++, bf(a.t[l]), n.c[l], nullptr, 0, e, Hl, Wl);
+        if (l + 1 < L) cb.pool = bf(a.p[l + 1]);  // the next level's input, pooled in the epilogue
+        MRT_HIP(mrt::launch_conv3x3(cb, min_blocks, st));
     }
     for (int l = static_cast<int>(L) - 2; l >= 0; --l) {
         const uint32_t Hl = H >> l, Wl = W >> l;
