@@ -537,7 +537,9 @@ MRT_API int mrt_denoiser_destroy(mrt_denoiser* dn);
  * (dy, dx) major) and bias (cout FP32, or NULL), ReLU when relu is set, into
  * y (height x width x cout BF16).  cin and cout multiples of 32.  Device
  * buffers (mem = MRT_MEM_DEVICE): x and w 16-byte aligned, y 8-byte aligned,
- * bias 4-byte aligned (MRT_E_INVALID otherwise). */
+ * bias 4-byte aligned (MRT_E_INVALID otherwise).  Each call reorders w into
+ * the kernel's fragment order in a device buffer of the same size that it
+ * allocates and frees. */
 MRT_API int mrt_conv3x3_bf16(mrt_ctx* ctx, const uint16_t* x, uint32_t height, uint32_t width, uint32_t cin,
                              const uint16_t* w, const float* bias, uint32_t cout, int relu, uint16_t* y, int mem);
 
