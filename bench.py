@@ -140,7 +140,7 @@ ONE_THREAD_TILE = 7 * 16 + 7
 
 def host_threads() -> int:
     """Host threads of the CPU baseline: the logical CPUs this process may
-    run on, at most 16 (one GPU's share of the GPU box's host)."""
+    run on, at most 16 (one GPU's share of an MI355X host)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -200,7 +200,7 @@ def cpu_baseline(data, cam, abi, mrt, accel: str = "bvh", runs: int = 3) -> dict
             "runs_mrays_per_s": [round(r, 4) for r in rates], "cpu_model": cpu_model(),
             "one_thread": {"value": round(one, 4), "unit": "Mrays/s",
                            "sample": f"tile {ONE_THREAD_TILE} x 2 samples, {int(c1.sum())} rays in {s1:.2f} s"},
-            "build": "g++ -O3 -march=x86-64-v3 (built off the GPU box)"}
+            "build": "g++ -O3 -march=x86-64-v3 (built on another host than the one it runs on)"}
 
 
 def ordered_traffic(data, cam, abi, mrt) -> dict:
