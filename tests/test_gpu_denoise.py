@@ -96,6 +96,29 @@ def test_conv3x3_device_buffers_checked_for_alignment(gpu_ctx):
             b.free()
 
 
+@pytest.mark.parametrize("h,w,cin,cout", [(9, 70, 64, 96), (136, 512, 96, 128)])
+def test_conv3x3_device_buffers_equal_host(gpu_ctx, h, w, cin, cout):
+    """The same layer on device buffers (the weights reordered on the
+    device) and on host buffers: bit-identical outputs."""
+    rng = np.random.default_rng(h + cin)
+    x = denoise.bf16_bits(rng.normal(size=(h, w, cin)).astype(np.float32) * 0.5)
+    wt = denoise.bf16_bits(rng.normal(size=(3, 3, cout, cin)).astype(np.float32) * 0.05)
+    b = rng.normal(size=cout).astype(np.float32) * 0.1
+    host = mrt.conv3x3_bf16(gpu_ctx, x, wt, b, True)
+    bufs = [DeviceArray(x.nbytes), DeviceArray(wt.nbytes), DeviceArray(b.nbytes), DeviceArray(h * w * cout * 2)]
+    try:
+        for buf, a in zip(bufs, (x, wt, b)):
+            buf.upload(np.ascontiguousarray(a))
+        rc = abi.lib().mrt_conv3x3_bf16(gpu_ctx.h, bufs[0].data_ptr(), h, w, cin, bufs[1].data_ptr(),
+                                        bufs[2].data_ptr(), cout, 1, bufs[3].data_ptr(), abi.MEM_DEVICE)
+        assert rc == abi.MRT_OK
+        dev = bufs[3].download(np.zeros((h, w, cout), np.uint16))
+    finally:
+        for buf in bufs:
+            buf.free()
+    assert np.array_equal(dev, host)
+
+
 def random_net(levels, channels, seed):
     torch = _torch()
     torch.manual_seed(seed)
