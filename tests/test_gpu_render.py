@@ -179,6 +179,53 @@ def test_render_stop_and_progress(gpu_ctx):
         assert 2 <= gpu.progress() < 64
 
 
+def test_graph_loop_stop_then_frames(gpu_ctx):
+    """A stop request during a graph-loop wavefront frame (the groups' graphs
+    re-captured with the stop flag) ends it with its completed samples; the
+    scene's next frames, replayed from graphs again, equal the launched
+    loop's."""
+    data = mrt.soup_scene(4, 60000, 600)
+    cam = soup_camera(1024, 576)
+    desc = mrt.render_desc(abi.SHADER_PATHTRACER, 1024, 576, 256, cam, pixel=rnd(1), rr=rnd(2), light=rnd(3),
+                           hemi=rnd(4), depth_min=2, depth_max=6, kernel=abi.KERNEL_WAVEFRONT)
+    ctx = mrt.Context(0)
+    ctx.set_tuning(graph_loop=1)
+    gpu = mrt.Scene(ctx, data)
+    try:
+        gpu.build(abi.BVH)
+        result = {}
+
+        def run():
+            try:
+                gpu.render(desc)
+                result["rc"] = 0
+            except abi.MrtError as e:
+                result["rc"] = e.code
+
+        t = threading.Thread(target=run)
+        t.start()
+        while gpu.progress() < 2 and t.is_alive():
+            pass
+        gpu.stop()
+        t.join(timeout=60)
+        assert not t.is_alive()
+        print(f"graph-loop frame: rc {result['rc']}, {gpu.progress()} of 256 samples", flush=True)
+        assert result["rc"] in (abi.MRT_E_STOPPED, 0)
+        if result["rc"] == abi.MRT_E_STOPPED:
+            assert 2 <= gpu.progress() < 256
+        short = mrt.render_desc(abi.SHADER_PATHTRACER, 1024, 576, 3, cam, pixel=rnd(1), rr=rnd(2), light=rnd(3),
+                                hemi=rnd(4), depth_min=2, depth_max=6, kernel=abi.KERNEL_WAVEFRONT)
+        frames = [gpu.render(short, accum=np.zeros((576, 1024, 3), np.float32)) for _ in range(2)]
+        ctx.set_tuning(graph_loop=2)
+        acc = np.zeros((576, 1024, 3), np.float32)
+        launched, _ = gpu.render(short, accum=acc)
+        for bm, _ in frames:
+            assert np.array_equal(bm, launched)
+    finally:
+        gpu.close()
+        ctx.close()
+
+
 def test_render_deterministic(gpu_ctx):
     data = mrt.soup_scene(4, 60000, 600)
     cam = soup_camera(128, 128)
